@@ -1,0 +1,397 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/liborc.so).
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and
+bench.py's ``cpu_baseline`` leg may import this module, and only as the
+checker (or, for the baseline, as the timed CPU port).  The product package
+``inq_amd`` never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE], stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        P = C.c_void_p
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        cp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        sig = {
+            "orc_version": (C.c_int, []),
+            "orc_set_threads": (None, [C.c_int]),
+            "orc_good_size": (C.c_int, [C.c_int]),
+            "orc_fft3d": (None, [ip, C.c_int, cp, C.c_int]),
+            "orc_ham_create": (P, [dp, ip]),
+            "orc_ham_destroy": (None, [P]),
+            "orc_ham_volume": (C.c_double, [P]),
+            "orc_ham_set_potential": (None, [P, dp]),
+            "orc_sphere_build": (C.c_long, [P, dp, C.c_double, P, P, P, C.c_long]),
+            "orc_ham_add_projector_gauss": (C.c_int, [P, dp, C.c_double, C.c_int, C.c_double, dp]),
+            "orc_ham_add_projector_matrix": (C.c_int, [P, dp, C.c_double, C.c_int, dp, dp]),
+            "orc_ham_num_projectors": (C.c_int, [P]),
+            "orc_ham_projector_size": (C.c_long, [P, C.c_int]),
+            "orc_ham_projector_points": (None, [P, C.c_int, ip, fp, fp, dp]),
+            "orc_ham_apply_rs": (None, [P, C.c_int, dp, cp, cp]),
+            "orc_ham_apply_fs": (None, [P, C.c_int, dp, cp, cp]),
+            "orc_ham_apply_sphere": (None, [P, C.c_int, dp, C.c_long, ip, cp, cp]),
+            "orc_ham_projections": (None, [P, C.c_int, dp, cp, C.c_int, cp]),
+            "orc_ham_nonlocal_energy": (C.c_double, [P, C.c_int, dp, cp, dp]),
+            "orc_gsphere": (C.c_long, [P, dp, C.c_double, P, C.c_long]),
+            "orc_gsphere_kinetic": (None, [P, dp, C.c_long, ip, dp]),
+            "orc_sphere_to_cube": (None, [P, C.c_int, C.c_long, ip, cp, cp]),
+            "orc_cube_to_sphere": (None, [P, C.c_int, C.c_long, ip, cp, cp]),
+            "orc_to_real": (None, [P, C.c_int, cp, cp, C.c_int]),
+            "orc_to_fourier": (None, [P, C.c_int, cp, cp]),
+            "orc_overlap": (None, [C.c_long, C.c_int, C.c_int, C.c_double, C.c_int, cp, cp, cp]),
+            "orc_cholesky": (C.c_int, [C.c_int, cp]),
+            "orc_invert_lower": (None, [C.c_int, cp]),
+            "orc_rotate_h": (None, [C.c_long, C.c_int, C.c_int, cp, cp]),
+            "orc_orthogonalize": (C.c_int, [C.c_long, C.c_int, C.c_double, C.c_int, cp]),
+            "orc_eigh": (C.c_int, [C.c_int, cp, dp]),
+            "orc_subspace_diag_sphere": (C.c_int, [P, C.c_int, dp, C.c_long, ip, cp, dp]),
+            "orc_etrs_step": (None, [P, C.c_int, dp, C.c_long, ip, C.c_double, cp]),
+            "orc_energy_sphere": (None, [P, C.c_int, dp, C.c_long, ip, cp, dp, dp]),
+            "orc_density_dipole_sphere": (None, [P, C.c_int, C.c_long, ip, cp, dp, P, dp]),
+            "orc_poisson_in_place": (None, [P, C.c_int, cp, dp, C.c_double]),
+            "orc_sing_auxiliary": (C.c_double, [dp, dp]),
+            "orc_sing_correction": (None, [dp, C.c_int, dp, dp, dp, dp]),
+            "orc_ace_build": (C.c_double, [P, C.c_int, cp, dp, C.c_double, cp]),
+            "orc_exchange_direct": (None, [P, C.c_int, cp, C.c_int, cp, dp, C.c_double, C.c_double, cp]),
+            "orc_ace_apply": (None, [P, C.c_int, cp, cp, cp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        # complex-by-value signatures
+        class _C(C.Structure):
+            _fields_ = [("re", C.c_double), ("im", C.c_double)]
+        L._cplx = _C
+        for name, args in {
+            "orc_exponential_in_place": [P, C.c_int, dp, C.c_long, ip, _C, C.c_int, cp],
+            "orc_exponential_2_for_1": [P, C.c_int, dp, C.c_long, ip, _C, _C, C.c_int, cp, cp],
+            "orc_rotate5": [C.c_long, C.c_int, C.c_int, cp, cp, cp, _C, _C],
+        }.items():
+            f = getattr(L, name)
+            f.restype = None
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _c(z):
+    z = complex(z)
+    return lib()._cplx(z.real, z.imag)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def fft3d(n, data, sign):
+    """In-place batched 3-D FFT of a [n0,n1,n2,nst] complex128 array."""
+    data = _c128(data)
+    nst = data.size // (n[0] * n[1] * n[2])
+    lib().orc_fft3d(np.asarray(n, dtype=np.int32), int(nst), data, int(sign))
+    return data
+
+
+class Ham:
+    """Oracle Hamiltonian: grid + local potential + KB projectors."""
+
+    def __init__(self, lattice, n):
+        self.lattice = _f64(lattice).reshape(3, 3)
+        self.n = tuple(int(x) for x in n)
+        self.N = self.n[0] * self.n[1] * self.n[2]
+        self.h = lib().orc_ham_create(self.lattice.ravel(), np.asarray(self.n, dtype=np.int32))
+        self.projs = []
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.orc_ham_destroy(self.h)
+            self.h = None
+
+    @property
+    def volume(self):
+        return lib().orc_ham_volume(self.h)
+
+    def set_potential(self, v):
+        self.v = _f64(v).ravel()
+        lib().orc_ham_set_potential(self.h, self.v)
+
+    def sphere(self, center, radius):
+        L = lib()
+        c = _f64(center)
+        cnt = L.orc_sphere_build(self.h, c, float(radius), None, None, None, 0)
+        coords = np.zeros((cnt, 3), dtype=np.int32)
+        dist = np.zeros(cnt, dtype=np.float32)
+        rel = np.zeros((cnt, 3), dtype=np.float32)
+        L.orc_sphere_build(self.h, c, float(radius), coords.ctypes.data, dist.ctypes.data, rel.ctypes.data, cnt)
+        return coords, dist, rel
+
+    def add_projector_gauss(self, center, radius, nlm, width, coeff):
+        i = lib().orc_ham_add_projector_gauss(self.h, _f64(center), float(radius), int(nlm), float(width),
+                                              _f64(coeff).ravel())
+        self.projs.append(i)
+        return i
+
+    def add_projector_matrix(self, center, radius, nlm, matrix, coeff):
+        return lib().orc_ham_add_projector_matrix(self.h, _f64(center), float(radius), int(nlm),
+                                                  _f64(matrix).ravel(), _f64(coeff).ravel())
+
+    def projector_points(self, i, nlm):
+        L = lib()
+        n = L.orc_ham_projector_size(self.h, i)
+        coords = np.zeros((n, 3), dtype=np.int32)
+        dist = np.zeros(n, dtype=np.float32)
+        rel = np.zeros((n, 3), dtype=np.float32)
+        mat = np.zeros((nlm, n), dtype=np.float64)
+        L.orc_ham_projector_points(self.h, i, coords, dist, rel, mat)
+        return coords, dist, rel, mat
+
+    # ---- H ----
+    def apply_rs(self, phi, k=(0, 0, 0)):
+        phi = _c128(phi)
+        nst = phi.size // self.N
+        out = np.zeros_like(phi)
+        lib().orc_ham_apply_rs(self.h, nst, _f64(k), phi, out)
+        return out
+
+    def apply_fs(self, phi, k=(0, 0, 0)):
+        phi = _c128(phi)
+        nst = phi.size // self.N
+        out = np.zeros_like(phi)
+        lib().orc_ham_apply_fs(self.h, nst, _f64(k), phi, out)
+        return out
+
+    def gsphere(self, k, ecut):
+        L = lib()
+        kk = _f64(k)
+        cnt = L.orc_gsphere(self.h, kk, float(ecut), None, 0)
+        g = np.zeros(cnt, dtype=np.int32)
+        L.orc_gsphere(self.h, kk, float(ecut), g.ctypes.data, cnt)
+        return g
+
+    def gsphere_kinetic(self, k, gidx):
+        out = np.zeros(len(gidx), dtype=np.float64)
+        lib().orc_gsphere_kinetic(self.h, _f64(k), len(gidx), np.ascontiguousarray(gidx, np.int32), out)
+        return out
+
+    def apply_sphere(self, phi, gidx, k=(0, 0, 0)):
+        phi = _c128(phi)
+        npw = len(gidx)
+        nst = phi.size // npw
+        out = np.zeros_like(phi)
+        lib().orc_ham_apply_sphere(self.h, nst, _f64(k), npw, np.ascontiguousarray(gidx, np.int32), phi, out)
+        return out
+
+    def projections(self, phi_rs, k=(0, 0, 0), apply_coeff=True, maxnlm=4):
+        phi_rs = _c128(phi_rs)
+        nst = phi_rs.size // self.N
+        out = np.zeros((len(self.projs), maxnlm, nst), dtype=np.complex128)
+        lib().orc_ham_projections(self.h, nst, _f64(k), phi_rs, int(apply_coeff), out)
+        return out
+
+    def nonlocal_energy(self, phi_rs, occ, k=(0, 0, 0)):
+        phi_rs = _c128(phi_rs)
+        nst = phi_rs.size // self.N
+        return lib().orc_ham_nonlocal_energy(self.h, nst, _f64(k), phi_rs, _f64(occ))
+
+    def to_real(self, fs, normalize=True):
+        fs = _c128(fs)
+        out = np.zeros_like(fs)
+        lib().orc_to_real(self.h, fs.size // self.N, fs, out, int(normalize))
+        return out
+
+    def to_fourier(self, rs):
+        rs = _c128(rs)
+        out = np.zeros_like(rs)
+        lib().orc_to_fourier(self.h, rs.size // self.N, rs, out)
+        return out
+
+    def sphere_to_cube(self, phi, gidx):
+        phi = _c128(phi)
+        npw = len(gidx)
+        nst = phi.size // npw
+        out = np.zeros((self.N, nst), dtype=np.complex128)
+        lib().orc_sphere_to_cube(self.h, nst, npw, np.ascontiguousarray(gidx, np.int32), phi, out)
+        return out
+
+    def cube_to_sphere(self, cube, gidx):
+        cube = _c128(cube)
+        nst = cube.size // self.N
+        out = np.zeros((nst, len(gidx)), dtype=np.complex128)
+        lib().orc_cube_to_sphere(self.h, nst, len(gidx), np.ascontiguousarray(gidx, np.int32), cube, out)
+        return out
+
+    # ---- propagation / energies on the sphere ----
+    def subspace_diag_sphere(self, phi, gidx, k=(0, 0, 0)):
+        phi = _c128(phi).copy()
+        npw = len(gidx)
+        nst = phi.size // npw
+        ev = np.zeros(nst)
+        lib().orc_subspace_diag_sphere(self.h, nst, _f64(k), npw, np.ascontiguousarray(gidx, np.int32), phi, ev)
+        return phi, ev
+
+    def exponential_in_place(self, phi, gidx, factor, order=4, k=(0, 0, 0)):
+        phi = _c128(phi).copy()
+        npw = len(gidx)
+        lib().orc_exponential_in_place(self.h, phi.size // npw, _f64(k), npw, np.ascontiguousarray(gidx, np.int32),
+                                       _c(factor), int(order), phi)
+        return phi
+
+    def exponential_2_for_1(self, phi, gidx, f1, f2, order=4, k=(0, 0, 0)):
+        phi = _c128(phi).copy()
+        npw = len(gidx)
+        exp = np.zeros_like(phi)
+        lib().orc_exponential_2_for_1(self.h, phi.size // npw, _f64(k), npw, np.ascontiguousarray(gidx, np.int32),
+                                      _c(f1), _c(f2), int(order), phi, exp)
+        return phi, exp
+
+    def etrs_step(self, phi, gidx, dt, k=(0, 0, 0)):
+        phi = _c128(phi).copy()
+        npw = len(gidx)
+        lib().orc_etrs_step(self.h, phi.size // npw, _f64(k), npw, np.ascontiguousarray(gidx, np.int32),
+                            float(dt), phi)
+        return phi
+
+    def energy_sphere(self, phi, gidx, occ, k=(0, 0, 0)):
+        phi = _c128(phi)
+        npw = len(gidx)
+        out = np.zeros(4)
+        lib().orc_energy_sphere(self.h, phi.size // npw, _f64(k), npw, np.ascontiguousarray(gidx, np.int32), phi,
+                                _f64(occ), out)
+        return out
+
+    def density_dipole_sphere(self, phi, gidx, occ):
+        phi = _c128(phi)
+        npw = len(gidx)
+        rho = np.zeros(self.N)
+        dip = np.zeros(3)
+        lib().orc_density_dipole_sphere(self.h, phi.size // npw, npw, np.ascontiguousarray(gidx, np.int32), phi,
+                                        _f64(occ), rho.ctypes.data, dip)
+        return rho, dip
+
+    def poisson_in_place(self, rho, gshift=(0, 0, 0), zeroterm=0.0):
+        rho = _c128(rho).copy()
+        lib().orc_poisson_in_place(self.h, rho.size // self.N, rho, _f64(gshift), float(zeroterm))
+        return rho
+
+    def ace_build(self, phi_rs, occ, exx_coeff=1.0):
+        phi_rs = _c128(phi_rs)
+        nst = phi_rs.size // self.N
+        xi = np.zeros_like(phi_rs)
+        ex = lib().orc_ace_build(self.h, nst, phi_rs, _f64(occ), float(exx_coeff), xi)
+        return xi, ex
+
+    def exchange_direct(self, phi_rs, hf_rs, occ, exx_coeff=1.0, scale=1.0):
+        phi_rs = _c128(phi_rs)
+        hf_rs = _c128(hf_rs)
+        nst = phi_rs.size // self.N
+        nhf = hf_rs.size // self.N
+        out = np.zeros_like(phi_rs)
+        lib().orc_exchange_direct(self.h, nst, phi_rs, nhf, hf_rs, _f64(occ), float(exx_coeff), float(scale), out)
+        return out
+
+    def ace_apply(self, xi, phi_rs):
+        xi = _c128(xi)
+        phi_rs = _c128(phi_rs)
+        out = np.zeros_like(phi_rs)
+        lib().orc_ace_apply(self.h, phi_rs.size // self.N, xi, phi_rs, out)
+        return out
+
+
+# ---- dense linear algebra ----
+def overlap(phi1, phi2, vol_el, band_major):
+    phi1 = _c128(phi1)
+    phi2 = _c128(phi2)
+    if band_major:
+        n1, npnt = phi1.shape
+        n2 = phi2.shape[0]
+    else:
+        npnt, n1 = phi1.shape
+        n2 = phi2.shape[1]
+    out = np.zeros((n1, n2), dtype=np.complex128)
+    lib().orc_overlap(npnt, n1, n2, float(vol_el), int(band_major), phi1, phi2, out)
+    return out
+
+
+def cholesky(m):
+    m = _c128(m).copy()
+    info = lib().orc_cholesky(m.shape[0], m)
+    return m, info
+
+
+def invert_lower(m):
+    m = _c128(m).copy()
+    lib().orc_invert_lower(m.shape[0], m)
+    return m
+
+
+def eigh(m):
+    m = _c128(m).copy()
+    w = np.zeros(m.shape[0])
+    lib().orc_eigh(m.shape[0], m, w)
+    return w, m
+
+
+def orthogonalize(phi, vol_el, band_major):
+    phi = _c128(phi).copy()
+    if band_major:
+        nst, npnt = phi.shape
+    else:
+        npnt, nst = phi.shape
+    info = lib().orc_orthogonalize(npnt, nst, float(vol_el), int(band_major), phi)
+    return phi, info
+
+
+def rotate_h(rot, phi, band_major):
+    phi = _c128(phi).copy()
+    nst = rot.shape[0]
+    npnt = phi.size // nst
+    lib().orc_rotate_h(npnt, nst, int(band_major), _c128(rot), phi)
+    return phi
+
+
+def rotate5(rot, phi, rotphi, alpha, beta, band_major):
+    rotphi = _c128(rotphi).copy()
+    nst = rot.shape[0]
+    npnt = phi.size // nst
+    lib().orc_rotate5(npnt, nst, int(band_major), _c128(rot), _c128(phi), rotphi, _c(alpha), _c(beta))
+    return rotphi
+
+
+def sing_auxiliary(lattice, q):
+    return lib().orc_sing_auxiliary(_f64(lattice).ravel(), _f64(q))
+
+
+def sing_correction(lattice, kcov, kweight):
+    kcov = _f64(kcov).reshape(-1, 3)
+    nk = kcov.shape[0]
+    fk = np.zeros(nk)
+    fz = np.zeros(1)
+    lib().orc_sing_correction(_f64(lattice).ravel(), nk, kcov.ravel(), _f64(kweight), fk, fz)
+    return fk, fz[0]
