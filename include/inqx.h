@@ -1,0 +1,140 @@
+/*
+ * inqx.h -- C ABI of the MI355X-native Kohn-Sham Hamiltonian engine.
+ *
+ * Drop-in boundary for LLNL/inq's hot path (paths relative to the reference
+ * tree).  The reference exposes this path as header-only C++ with duck-typed
+ * operators; each entry point below names the member or free function it
+ * replaces.  Plain C types only: device pointers are `void *` to complex
+ * arrays (complex128 for INQX_FP64, complex64 for INQX_FP32 precision), host
+ * pointers are plain `double *`/`int *`.
+ *
+ * Error convention: every call returns an int status (INQX_OK == 0, negative
+ * on error) and never aborts or throws across the boundary;
+ * inqx_last_error() gives the message.  The reference aborts on GPU errors
+ * (external_libs/gpurun/include/gpu/host.hpp:42-59) and on Cholesky
+ * info < 0 (src/matrix/cholesky.hpp:77-82); here both become status codes.
+ *
+ * Streams: every compute call is enqueued on the context's stream with no
+ * hidden host synchronisation (the reference synchronises after every
+ * kernel, external_libs/gpurun/include/gpu/run.hpp:77,117,154,201), so
+ * apply/exponential/etrs sequences can be captured in a hipGraph once their
+ * workspaces exist (inqx_ham_prepare).  A context is not thread-safe.
+ *
+ * State layouts:
+ *   sphere ("band-major"): [nst][npw] G-sphere coefficients of one block,
+ *     in the reference's unnormalised forward-FFT convention
+ *     (src/operations/transform.hpp:79); sphere order is the ascending
+ *     flattened FFT index (i0*n1 + i1)*n2 + i2 returned by
+ *     inqx_basis_gindex.
+ *   real space / full Fourier cube: [i0][i1][i2][ist] with states fastest,
+ *     the reference's field_set layout (src/basis/field_set.hpp:52,62,232-238).
+ */
+#ifndef INQX_H
+#define INQX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct inqx_ctx_s *inqx_ctx;
+typedef struct inqx_ham_s *inqx_ham;
+typedef struct inqx_basis_s *inqx_basis;
+
+enum {
+  INQX_OK = 0,
+  INQX_ERR_ARG = -1,         /* bad argument / shape */
+  INQX_ERR_HIP = -2,         /* HIP runtime failure */
+  INQX_ERR_UNSUPPORTED = -3, /* grid size or option not built */
+  INQX_ERR_NOT_PD = -4,      /* Cholesky: matrix not positive definite */
+  INQX_ERR_STATE = -5        /* call order (e.g. no potential set) */
+};
+
+enum { INQX_FP64 = 0, INQX_FP32 = 1 };
+
+/* ------------------------------------------------------------ context */
+/* Environment/device selection the reference leaves to input::environment
+ * (src/input/environment.hpp:27-92) and scripts/gpubind. */
+int inqx_ctx_create(int device, void *stream, inqx_ctx *out);
+int inqx_ctx_destroy(inqx_ctx ctx);
+int inqx_ctx_set_stream(inqx_ctx ctx, void *stream);
+int inqx_sync(inqx_ctx ctx);
+const char *inqx_last_error(inqx_ctx ctx);
+int inqx_version(void);
+
+/* per-kernel timing with HIP events on the context stream (replaces the
+ * reference's caliper / kalimotxo scope timers, src/utils/profiling.hpp,
+ * external_libs/kalimotxo/cali.h:74-168).  inqx_prof_get enumerates the
+ * accumulated (kernel name, launch count, total milliseconds) after
+ * synchronising the stream. */
+int inqx_prof_enable(inqx_ctx ctx, int on);
+int inqx_prof_reset(inqx_ctx ctx);
+int inqx_prof_count(inqx_ctx ctx, int *nkernels);
+int inqx_prof_get(inqx_ctx ctx, int index, char *name, int name_cap, long *launches, double *total_ms);
+
+/* device memory helpers, for callers without their own allocator
+ * (replaces the gpu::array caching pool, external_libs/gpurun/include/gpu/array.hpp:65-119) */
+int inqx_malloc(inqx_ctx ctx, size_t bytes, void **ptr);
+int inqx_free(inqx_ctx ctx, void *ptr);
+int inqx_memcpy_h2d(inqx_ctx ctx, void *dst, const void *src, size_t bytes);
+int inqx_memcpy_d2h(inqx_ctx ctx, void *dst, const void *src, size_t bytes);
+int inqx_memcpy_d2d(inqx_ctx ctx, void *dst, const void *src, size_t bytes);
+
+/* ------------------------------------------------------------ Hamiltonian */
+/* hamiltonian::ks_hamiltonian constructor (src/hamiltonian/ks_hamiltonian.hpp:92-102):
+ * lattice rows are the cell vectors in bohr (src/systems/cell.hpp:36-52), n the
+ * real-space grid (src/basis/real_space.hpp:39-50). */
+int inqx_ham_create(inqx_ctx ctx, const double lattice[9], const int n[3], inqx_ham *out);
+int inqx_ham_destroy(inqx_ham ham);
+/* ks_hamiltonian::scalar_potential() (ks_hamiltonian.hpp:241-243): host array
+ * [N][ncomp] in Hartree; spin index `spin` of a basis selects the column
+ * (src/hamiltonian/scalar_potential.hpp:31-34). */
+int inqx_ham_set_potential(inqx_ham ham, const double *v, int ncomp);
+/* ks_hamiltonian::uniform_vector_potential() (ks_hamiltonian.hpp:257-262):
+ * covariant A added to every block's k-point (velocity-gauge kick,
+ * src/perturbations/kick.hpp:36-48). */
+int inqx_ham_set_vector_potential(inqx_ham ham, const double a_cov[3]);
+/* one projector of projector_all (src/hamiltonian/projector.hpp:102-113,
+ * projector_all.hpp:40-109).  The sphere is built as
+ * basis::spherical_grid(grid, center, radius) (src/basis/spherical_grid.hpp:59-158).
+ * matrix: host [nlm][npts] projector values on that sphere, or NULL for the
+ * Gaussian s+p form exp(-r^2/(2 width^2)) Y_lm (l=0, then l=1 with m = -1,0,1
+ * mapped to y, z, x).  kb_coeff: host [nlm][nlm] (projector.hpp:85-93).
+ * Returns the projector index (>= 0) or a negative status. */
+int inqx_ham_add_projector(inqx_ham ham, const double center[3], double radius, int nlm,
+                           const double *matrix, const double *kb_coeff, double width);
+int inqx_ham_projector_size(inqx_ham ham, int iproj, long *npts);
+/* sphere points of a projector: coords [npts][3] grid indices, dist [npts]
+ * (float, spherical_grid.hpp:40-43), relpos [npts][3] contravariant (float) */
+int inqx_ham_projector_points(inqx_ham ham, int iproj, int *coords, float *dist, float *relpos);
+
+/* ------------------------------------------------------------ basis (one k-point/spin block) */
+/* G sphere 1/2|G+k|^2 < ecut of orbital_set(k, spin) (src/states/orbital_set.hpp:18-192) */
+int inqx_basis_create(inqx_ham ham, const double kcov[3], double ecut, int spin, inqx_basis *out);
+int inqx_basis_destroy(inqx_basis basis);
+int inqx_basis_size(inqx_basis basis, long *npw);
+int inqx_basis_gindex(inqx_basis basis, int *gidx);
+/* 1/2 |G+k+A|^2 per sphere point (host, FP64) */
+int inqx_basis_kinetic(inqx_basis basis, double *kin);
+/* allocate the workspaces apply needs for nst states at a precision, so that
+ * later calls allocate nothing (required before hipGraph capture) */
+int inqx_ham_prepare(inqx_ham ham, inqx_basis basis, int nst, int precision);
+
+/* ------------------------------------------------------------ H |psi> */
+/* ks_hamiltonian::operator()(orbital_set<fourier_space>) (ks_hamiltonian.hpp:160-180)
+ * restricted to the G sphere: hpsi = P H P psi, psi/hpsi band-major device arrays. */
+int inqx_ham_apply(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *psi, void *hpsi);
+/* the same for many blocks (k-points x spins) in one launch sequence */
+int inqx_ham_apply_batch(inqx_ham ham, int nblocks, const inqx_basis *bases, const int *nst, int precision,
+                         const void *const *psi, void *const *hpsi);
+
+/* ------------------------------------------------------------ sphere <-> grids */
+/* operations::transform::to_real (transform.hpp:220-249) of sphere states into
+ * the reference real-space layout [N][nst] (normalised by 1/N) */
+int inqx_sphere_to_real(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *psi, void *psi_rs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

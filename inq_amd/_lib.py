@@ -1,0 +1,85 @@
+"""Loads the HIP library libinqx.so through its C ABI (include/inqx.h).
+
+The product path has no CPU fallback: if the library is missing or fails to
+load, every call raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libinqx.so")
+_LIB = None
+
+INQX_OK = 0
+INQX_FP64 = 0
+INQX_FP32 = 1
+
+# name -> (restype, argtypes); every symbol include/inqx.h declares
+P = C.c_void_p
+PP = C.POINTER(C.c_void_p)
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+FP = C.POINTER(C.c_float)
+LP = C.POINTER(C.c_long)
+SIGNATURES = {
+    "inqx_version": (C.c_int, []),
+    "inqx_ctx_create": (C.c_int, [C.c_int, P, PP]),
+    "inqx_ctx_destroy": (C.c_int, [P]),
+    "inqx_ctx_set_stream": (C.c_int, [P, P]),
+    "inqx_sync": (C.c_int, [P]),
+    "inqx_last_error": (C.c_char_p, [P]),
+    "inqx_prof_enable": (C.c_int, [P, C.c_int]),
+    "inqx_prof_reset": (C.c_int, [P]),
+    "inqx_prof_count": (C.c_int, [P, IP]),
+    "inqx_prof_get": (C.c_int, [P, C.c_int, C.c_char_p, C.c_int, LP, DP]),
+    "inqx_malloc": (C.c_int, [P, C.c_size_t, PP]),
+    "inqx_free": (C.c_int, [P, P]),
+    "inqx_memcpy_h2d": (C.c_int, [P, P, P, C.c_size_t]),
+    "inqx_memcpy_d2h": (C.c_int, [P, P, P, C.c_size_t]),
+    "inqx_memcpy_d2d": (C.c_int, [P, P, P, C.c_size_t]),
+    "inqx_ham_create": (C.c_int, [P, DP, IP, PP]),
+    "inqx_ham_destroy": (C.c_int, [P]),
+    "inqx_ham_set_potential": (C.c_int, [P, DP, C.c_int]),
+    "inqx_ham_set_vector_potential": (C.c_int, [P, DP]),
+    "inqx_ham_add_projector": (C.c_int, [P, DP, C.c_double, C.c_int, DP, DP, C.c_double]),
+    "inqx_ham_projector_size": (C.c_int, [P, C.c_int, LP]),
+    "inqx_ham_projector_points": (C.c_int, [P, C.c_int, IP, FP, FP]),
+    "inqx_basis_create": (C.c_int, [P, DP, C.c_double, C.c_int, PP]),
+    "inqx_basis_destroy": (C.c_int, [P]),
+    "inqx_basis_size": (C.c_int, [P, LP]),
+    "inqx_basis_gindex": (C.c_int, [P, IP]),
+    "inqx_basis_kinetic": (C.c_int, [P, DP]),
+    "inqx_ham_prepare": (C.c_int, [P, P, C.c_int, C.c_int]),
+    "inqx_ham_apply": (C.c_int, [P, P, C.c_int, C.c_int, P, P]),
+    "inqx_ham_apply_batch": (C.c_int, [P, C.c_int, PP, IP, C.c_int, PP, PP]),
+    "inqx_sphere_to_real": (C.c_int, [P, P, C.c_int, C.c_int, P, P]),
+}
+
+
+class InqxError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library; raises if it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise InqxError(f"{LIB_PATH} is not built: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(status, ctx=None, what=""):
+    if status < 0:
+        msg = ""
+        if ctx is not None:
+            m = lib().inqx_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise InqxError(f"{what} failed with status {status}: {msg}")
+    return status

@@ -1,0 +1,560 @@
+// api.cpp -- the inqx C ABI: contexts, Hamiltonian set-up, G-sphere bases and
+// the H|psi> entry points.  Every entry point catches inqx::Error and turns it
+// into a status code; nothing throws or aborts across the boundary.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+
+using namespace inqx;
+
+namespace inqx {
+
+// ---------------------------------------------------------------- cell metric
+// systems::cell (src/systems/cell.hpp:36-52) and cell_metric (:146-224)
+void Grid::init(const double *lat, const int *nn) {
+  for (int i = 0; i < 3; ++i) {
+    n[i] = nn[i];
+    for (int j = 0; j < 3; ++j) a[i][j] = lat[3 * i + j];
+  }
+  N = (long)n[0] * n[1] * n[2];
+  auto cross = [](const double *u, const double *v, double *w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+  };
+  double c[3][3];
+  cross(a[1], a[2], c[0]);
+  cross(a[2], a[0], c[1]);
+  cross(a[0], a[1], c[2]);
+  vol = a[0][0] * c[0][0] + a[0][1] * c[0][1] + a[0][2] * c[0][2];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      b[i][j] = 2.0 * M_PI / vol * c[i][j];
+      rl[i][j] = b[i][j] / (2.0 * M_PI);
+    }
+}
+void Grid::cov_to_cart(const double *v, double *c) const {
+  for (int j = 0; j < 3; ++j) c[j] = rl[0][j] * v[0] + rl[1][j] * v[1] + rl[2][j] * v[2];
+}
+void Grid::contra_to_cart(const double *v, double *c) const {
+  for (int j = 0; j < 3; ++j) c[j] = a[0][j] * v[0] + a[1][j] * v[1] + a[2][j] * v[2];
+}
+void Grid::cart_to_contra(const double *c, double *v) const {
+  for (int i = 0; i < 3; ++i) v[i] = rl[i][0] * c[0] + rl[i][1] * c[1] + rl[i][2] * c[2];
+}
+
+static double dot3(const double *u, const double *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; }
+
+hipEvent_t Ctx::get_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  INQX_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void Ctx::flush_prof() {
+  if (host_only || pending.empty()) return;
+  INQX_HIP(hipStreamSynchronize(stream));
+  for (auto &p : pending) {
+    float ms = 0.0f;
+    INQX_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    KStat &k = stats[p.first];
+    k.count += 1;
+    k.total_ms += ms;
+    event_pool.push_back(p.second.first);
+    event_pool.push_back(p.second.second);
+  }
+  pending.clear();
+}
+
+// ---------------------------------------------------------------- spherical grid
+// basis::spherical_grid::initialize (src/basis/spherical_grid.hpp:59-158) with
+// ionic::periodic_replicas (src/ionic/periodic_replicas.hpp:25-50) and
+// basis::containing_cube (src/basis/containing_cube.hpp:20-45).  The point
+// order (replica, then cube order i0,i1,i2) and the float rounding of the
+// distance and of the contravariant relative position follow the reference,
+// because both feed the projector values and the KB phases.
+static void build_sphere(const Grid &g, const double *center, double radius, Projector &P) {
+  // position_in_cell (cell.hpp:238-250)
+  double cc[3], p0[3];
+  g.cart_to_contra(center, cc);
+  for (int d = 0; d < 3; ++d) {
+    cc[d] -= std::floor(cc[d]);
+    if (cc[d] >= 0.5) cc[d] -= 1.0;
+  }
+  g.contra_to_cart(cc, p0);
+  const double range = std::sqrt(dot3(g.a[0], g.a[0]) + dot3(g.a[1], g.a[1]) + dot3(g.a[2], g.a[2]));
+  int nm[3];
+  for (int d = 0; d < 3; ++d) nm[d] = 1 + (int)std::lround(range / std::sqrt(dot3(g.a[d], g.a[d])));
+  std::vector<std::array<double, 3>> reps;
+  reps.push_back({p0[0], p0[1], p0[2]});
+  for (int ix = -nm[0]; ix <= nm[0]; ++ix)
+    for (int iy = -nm[1]; iy <= nm[1]; ++iy)
+      for (int iz = -nm[2]; iz <= nm[2]; ++iz) {
+        if (ix == 0 && iy == 0 && iz == 0) continue;
+        std::array<double, 3> r;
+        for (int j = 0; j < 3; ++j) r[j] = p0[j] + ix * g.a[0][j] + iy * g.a[1][j] + iz * g.a[2][j];
+        reps.push_back(r);
+      }
+  for (auto &re : reps) {
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+      const double *rec = g.b[d];
+      const double len = std::sqrt(dot3(rec, rec));
+      double li[3], hh[3], cl[3], ch[3];
+      for (int j = 0; j < 3; ++j) {
+        li[j] = re[j] - radius / len * rec[j];
+        hh[j] = re[j] + radius / len * rec[j];
+      }
+      g.cart_to_contra(li, cl);
+      g.cart_to_contra(hh, ch);
+      const double sp = 1.0 / g.n[d];
+      lo[d] = (int)std::lround(std::floor(cl[d] / sp));
+      hi[d] = (int)std::lround(std::ceil(ch[d] / sp)) + 1;
+      const int rb = -g.n[d] / 2, rend = g.n[d] / 2 + g.n[d] % 2;
+      lo[d] = std::min(std::max(lo[d], rb), rend);
+      hi[d] = std::min(std::max(hi[d], rb), rend);
+    }
+    for (int ix = lo[0]; ix < hi[0]; ++ix)
+      for (int iy = lo[1]; iy < hi[1]; ++iy)
+        for (int iz = lo[2]; iz < hi[2]; ++iz) {
+          int ii[3] = {ix, iy, iz};
+          for (int d = 0; d < 3; ++d)
+            if (ii[d] < 0) ii[d] += g.n[d];
+          double f[3] = {(double)Grid::sym(ii[0], g.n[0]) / g.n[0], (double)Grid::sym(ii[1], g.n[1]) / g.n[1],
+                         (double)Grid::sym(ii[2], g.n[2]) / g.n[2]};
+          double rp[3];
+          g.contra_to_cart(f, rp);
+          double dv[3] = {rp[0] - re[0], rp[1] - re[1], rp[2] - re[2]};
+          const double n2 = dot3(dv, dv);
+          if (n2 > radius * radius) continue;
+          double c[3];
+          g.cart_to_contra(dv, c);
+          for (int d = 0; d < 3; ++d) P.coords.push_back(ii[d]);
+          P.dist.push_back((float)std::sqrt(n2));
+          for (int d = 0; d < 3; ++d) P.relpos.push_back((float)c[d]);
+        }
+  }
+}
+
+// Gaussian s + p projector values on the float sphere tables (the
+// reference's projector::build, src/hamiltonian/projector.hpp:38-66, with an
+// analytic radial function in place of the pseudopotential spline)
+static void gauss_matrix(const Grid &g, Projector &P, double width) {
+  const long np = P.npts();
+  P.matrix.assign((size_t)P.nlm * np, 0.0);
+  const double y00 = 0.5 / std::sqrt(M_PI), y1 = std::sqrt(3.0 / (4.0 * M_PI));
+  for (long ip = 0; ip < np; ++ip) {
+    const double rr = (double)P.dist[ip];
+    const double radial = std::exp(-rr * rr / (2.0 * width * width));
+    double c[3] = {(double)P.relpos[3 * ip], (double)P.relpos[3 * ip + 1], (double)P.relpos[3 * ip + 2]};
+    double v[3];
+    g.contra_to_cart(c, v);
+    const double r = std::sqrt(dot3(v, v));
+    for (int l = 0; l < P.nlm; ++l) {
+      double y;
+      if (l == 0) y = y00;
+      else if (r == 0.0) y = 0.0;
+      else y = y1 * (l == 1 ? v[1] : (l == 2 ? v[2] : v[0])) / r;
+      P.matrix[(size_t)l * np + ip] = radial * y;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- G sphere of a block
+static void build_basis(const Grid &g, Basis &b) {
+  b.gidx.clear();
+  for (int i0 = 0; i0 < g.n[0]; ++i0)
+    for (int i1 = 0; i1 < g.n[1]; ++i1)
+      for (int i2 = 0; i2 < g.n[2]; ++i2) {
+        double v[3] = {2.0 * M_PI * Grid::sym(i0, g.n[0]) + b.kcov[0], 2.0 * M_PI * Grid::sym(i1, g.n[1]) + b.kcov[1],
+                       2.0 * M_PI * Grid::sym(i2, g.n[2]) + b.kcov[2]};
+        double c[3];
+        g.cov_to_cart(v, c);
+        if (0.5 * dot3(c, c) < b.ecut) b.gidx.push_back((i0 * g.n[1] + i1) * g.n[2] + i2);
+      }
+  b.npw = (long)b.gidx.size();
+  b.slot_i0.clear();
+  b.seg_off.clear();
+  b.row_off.clear();
+  b.rows.clear();
+  b.pw_yz.resize(b.npw);
+  b.slot_of.assign(g.n[0], -1);
+  const int n12 = g.n[1] * g.n[2];
+  for (long i = 0; i < b.npw; ++i) {
+    const int f = b.gidx[i];
+    const int i0 = f / n12, yz = f - i0 * n12, i1 = yz / g.n[2];
+    b.pw_yz[i] = yz;
+    if (b.slot_i0.empty() || b.slot_i0.back() != i0) {
+      b.slot_of[i0] = (int)b.slot_i0.size();
+      b.slot_i0.push_back(i0);
+      b.seg_off.push_back((int)i);
+      b.row_off.push_back((int)b.rows.size());
+      b.rows.push_back(i1);
+    } else if (b.rows.back() != i1) {
+      b.rows.push_back(i1);
+    }
+  }
+  b.seg_off.push_back((int)b.npw);
+  b.row_off.push_back((int)b.rows.size());
+}
+
+}  // namespace inqx
+
+// ---------------------------------------------------------------- error plumbing
+#define API_BEGIN try {
+#define API_END(ctxp)                                    \
+  }                                                      \
+  catch (const inqx::Error &e) {                         \
+    if (ctxp) (ctxp)->c.err = e.msg;                     \
+    return e.code;                                       \
+  }                                                      \
+  catch (const std::exception &e) {                      \
+    if (ctxp) (ctxp)->c.err = e.what();                  \
+    return INQX_ERR_ARG;                                 \
+  }                                                      \
+  return INQX_OK;
+
+extern "C" {
+
+int inqx_version(void) { return 1; }
+
+int inqx_ctx_create(int device, void *stream, inqx_ctx *out) {
+  inqx_ctx ctx = nullptr;
+  API_BEGIN
+  INQX_CHECK(out != nullptr, INQX_ERR_ARG, "null output");
+  ctx = new inqx_ctx_s();
+  ctx->c.device = device;
+  if (device < 0) {  // host-only context: set-up and queries, no device work
+    ctx->c.host_only = true;
+    *out = ctx;
+    return INQX_OK;
+  }
+  INQX_HIP(hipSetDevice(device));
+  ctx->c.stream = reinterpret_cast<hipStream_t>(stream);
+  hipDeviceProp_t prop;
+  INQX_HIP(hipGetDeviceProperties(&prop, device));
+  ctx->c.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  *out = ctx;
+  API_END(ctx)
+}
+
+int inqx_ctx_destroy(inqx_ctx ctx) {
+  if (!ctx) return INQX_OK;
+  if (!ctx->c.host_only) {
+    (void)hipStreamSynchronize(ctx->c.stream);
+    for (auto &p : ctx->c.pending) {
+      ctx->c.event_pool.push_back(p.second.first);
+      ctx->c.event_pool.push_back(p.second.second);
+    }
+    for (auto e : ctx->c.event_pool) (void)hipEventDestroy(e);
+  }
+  delete ctx;
+  return INQX_OK;
+}
+
+int inqx_prof_enable(inqx_ctx ctx, int on) {
+  if (!ctx) return INQX_ERR_ARG;
+  ctx->c.prof = on != 0;
+  return INQX_OK;
+}
+
+int inqx_prof_reset(inqx_ctx ctx) {
+  API_BEGIN
+  INQX_CHECK(ctx, INQX_ERR_ARG, "null context");
+  ctx->c.flush_prof();
+  ctx->c.stats.clear();
+  API_END(ctx)
+}
+
+int inqx_prof_count(inqx_ctx ctx, int *n) {
+  API_BEGIN
+  INQX_CHECK(ctx && n, INQX_ERR_ARG, "null argument");
+  ctx->c.flush_prof();
+  *n = (int)ctx->c.stats.size();
+  API_END(ctx)
+}
+
+int inqx_prof_get(inqx_ctx ctx, int index, char *name, int cap, long *launches, double *total_ms) {
+  API_BEGIN
+  INQX_CHECK(ctx && name && cap > 0, INQX_ERR_ARG, "null argument");
+  ctx->c.flush_prof();
+  INQX_CHECK(index >= 0 && index < (int)ctx->c.stats.size(), INQX_ERR_ARG, "index out of range");
+  auto it = ctx->c.stats.begin();
+  std::advance(it, index);
+  std::snprintf(name, cap, "%s", it->first.c_str());
+  if (launches) *launches = it->second.count;
+  if (total_ms) *total_ms = it->second.total_ms;
+  API_END(ctx)
+}
+
+int inqx_ctx_set_stream(inqx_ctx ctx, void *stream) {
+  if (!ctx) return INQX_ERR_ARG;
+  ctx->c.stream = reinterpret_cast<hipStream_t>(stream);
+  return INQX_OK;
+}
+
+int inqx_sync(inqx_ctx ctx) {
+  API_BEGIN
+  INQX_CHECK(ctx, INQX_ERR_ARG, "null context");
+  if (ctx->c.host_only) return INQX_OK;
+  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END(ctx)
+}
+
+const char *inqx_last_error(inqx_ctx ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int inqx_malloc(inqx_ctx ctx, size_t bytes, void **ptr) {
+  API_BEGIN
+  INQX_CHECK(ctx && ptr, INQX_ERR_ARG, "null argument");
+  INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_HIP(hipMalloc(ptr, bytes ? bytes : 16));
+  API_END(ctx)
+}
+
+int inqx_free(inqx_ctx ctx, void *ptr) {
+  API_BEGIN
+  INQX_CHECK(ctx, INQX_ERR_ARG, "null context");
+  if (ptr) INQX_HIP(hipFree(ptr));
+  API_END(ctx)
+}
+
+int inqx_memcpy_h2d(inqx_ctx ctx, void *dst, const void *src, size_t bytes) {
+  API_BEGIN
+  INQX_CHECK(ctx, INQX_ERR_ARG, "null context");
+  INQX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END(ctx)
+}
+
+int inqx_memcpy_d2h(inqx_ctx ctx, void *dst, const void *src, size_t bytes) {
+  API_BEGIN
+  INQX_CHECK(ctx, INQX_ERR_ARG, "null context");
+  INQX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END(ctx)
+}
+
+int inqx_memcpy_d2d(inqx_ctx ctx, void *dst, const void *src, size_t bytes) {
+  API_BEGIN
+  INQX_CHECK(ctx, INQX_ERR_ARG, "null context");
+  INQX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->c.stream));
+  API_END(ctx)
+}
+
+// ---------------------------------------------------------------- Hamiltonian
+int inqx_ham_create(inqx_ctx ctx, const double lattice[9], const int n[3], inqx_ham *out) {
+  API_BEGIN
+  INQX_CHECK(ctx && lattice && n && out, INQX_ERR_ARG, "null argument");
+  for (int d = 0; d < 3; ++d) INQX_CHECK(n[d] > 0 && n[d] <= 4096, INQX_ERR_ARG, "bad grid size");
+  inqx_ham h = new inqx_ham_s();
+  h->ctx = ctx;
+  h->g.init(lattice, n);
+  if (!(h->g.vol > 1e-10)) {
+    delete h;
+    INQX_CHECK(false, INQX_ERR_ARG, "the lattice volume is not positive");
+  }
+  *out = h;
+  API_END(ctx)
+}
+
+int inqx_ham_destroy(inqx_ham h) {
+  if (!h) return INQX_OK;
+  invalidate_launch_cache(h);
+  for (auto b : h->bases) b->b.ham = nullptr;
+  delete h;
+  return INQX_OK;
+}
+
+int inqx_ham_set_potential(inqx_ham h, const double *v, int ncomp) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && v && ncomp >= 1 && ncomp <= 4, INQX_ERR_ARG, "bad potential arguments");
+  const long N = h->g.N;
+  invalidate_launch_cache(h);
+  h->ncomp = ncomp;
+  h->v.assign(v, v + (size_t)N * ncomp);
+  std::vector<double> v64((size_t)N * ncomp);
+  std::vector<float> v32((size_t)N * ncomp);
+  for (int c = 0; c < ncomp; ++c)
+    for (long ip = 0; ip < N; ++ip) {
+      const double x = v[(size_t)ip * ncomp + c] / (double)N;
+      v64[(size_t)c * N + ip] = x;
+      v32[(size_t)c * N + ip] = (float)x;
+    }
+  h->has_v = true;
+  if (ctx->c.host_only) return INQX_OK;
+  upload(h->v64, v64, ctx->c.stream);
+  upload(h->v32, v32, ctx->c.stream);
+  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END(ctx)
+}
+
+int inqx_ham_set_vector_potential(inqx_ham h, const double a_cov[3]) {
+  if (!h || !a_cov) return INQX_ERR_ARG;
+  for (int d = 0; d < 3; ++d) h->A[d] = a_cov[d];
+  return INQX_OK;
+}
+
+int inqx_ham_add_projector(inqx_ham h, const double center[3], double radius, int nlm, const double *matrix,
+                           const double *kb_coeff, double width) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  int idx = -1;
+  API_BEGIN
+  INQX_CHECK(h && center && kb_coeff, INQX_ERR_ARG, "null argument");
+  INQX_CHECK(nlm >= 1 && nlm <= kMaxNlm, INQX_ERR_UNSUPPORTED, "nlm must be 1..4");
+  INQX_CHECK(radius > 0.0, INQX_ERR_ARG, "radius must be positive");
+  Projector P;
+  for (int d = 0; d < 3; ++d) P.center[d] = center[d];
+  P.radius = radius;
+  P.nlm = nlm;
+  build_sphere(h->g, center, radius, P);
+  if (matrix) {
+    P.matrix.assign(matrix, matrix + (size_t)nlm * P.npts());
+  } else {
+    INQX_CHECK(width > 0.0, INQX_ERR_ARG, "Gaussian width must be positive");
+    gauss_matrix(h->g, P, width);
+  }
+  P.coeff.assign(kb_coeff, kb_coeff + nlm * nlm);
+  h->projs.push_back(std::move(P));
+  h->kb.built = false;
+  invalidate_launch_cache(h);
+  idx = (int)h->projs.size() - 1;
+  }
+  catch (const inqx::Error &e) {
+    if (ctx) ctx->c.err = e.msg;
+    return e.code;
+  }
+  return idx;
+}
+
+int inqx_ham_projector_size(inqx_ham h, int iproj, long *npts) {
+  if (!h || !npts || iproj < 0 || iproj >= (int)h->projs.size()) return INQX_ERR_ARG;
+  *npts = h->projs[iproj].npts();
+  return INQX_OK;
+}
+
+int inqx_ham_projector_points(inqx_ham h, int iproj, int *coords, float *dist, float *relpos) {
+  if (!h || iproj < 0 || iproj >= (int)h->projs.size()) return INQX_ERR_ARG;
+  const Projector &P = h->projs[iproj];
+  if (coords) std::memcpy(coords, P.coords.data(), sizeof(int) * P.coords.size());
+  if (dist) std::memcpy(dist, P.dist.data(), sizeof(float) * P.dist.size());
+  if (relpos) std::memcpy(relpos, P.relpos.data(), sizeof(float) * P.relpos.size());
+  return INQX_OK;
+}
+
+// ---------------------------------------------------------------- bases
+int inqx_basis_create(inqx_ham h, const double kcov[3], double ecut, int spin, inqx_basis *out) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && kcov && out, INQX_ERR_ARG, "null argument");
+  INQX_CHECK(spin >= 0 && spin < 4, INQX_ERR_ARG, "bad spin index");
+  inqx_basis bs = new inqx_basis_s();
+  Basis &b = bs->b;
+  b.ham = h;
+  for (int d = 0; d < 3; ++d) b.kcov[d] = kcov[d];
+  b.ecut = ecut;
+  b.spin = spin;
+  build_basis(h->g, b);
+  if (b.npw == 0) {
+    delete bs;
+    INQX_CHECK(false, INQX_ERR_ARG, "empty G sphere");
+  }
+  h->bases.push_back(bs);
+  *out = bs;
+  API_END(ctx)
+}
+
+int inqx_basis_destroy(inqx_basis bs) {
+  if (!bs) return INQX_OK;
+  if (bs->b.ham) {
+    invalidate_launch_cache(bs->b.ham);
+    auto &v = bs->b.ham->bases;
+    v.erase(std::remove(v.begin(), v.end(), bs), v.end());
+  }
+  delete bs;
+  return INQX_OK;
+}
+
+int inqx_basis_size(inqx_basis bs, long *npw) {
+  if (!bs || !npw) return INQX_ERR_ARG;
+  *npw = bs->b.npw;
+  return INQX_OK;
+}
+
+int inqx_basis_gindex(inqx_basis bs, int *gidx) {
+  if (!bs || !gidx) return INQX_ERR_ARG;
+  std::memcpy(gidx, bs->b.gidx.data(), sizeof(int) * bs->b.gidx.size());
+  return INQX_OK;
+}
+
+int inqx_basis_kinetic(inqx_basis bs, double *kin) {
+  if (!bs || !kin || !bs->b.ham) return INQX_ERR_ARG;
+  const Basis &b = bs->b;
+  const Grid &g = b.ham->g;
+  for (long i = 0; i < b.npw; ++i) {
+    const int f = b.gidx[i];
+    const int i2 = f % g.n[2], i1 = (f / g.n[2]) % g.n[1], i0 = f / (g.n[2] * g.n[1]);
+    double v[3] = {2.0 * M_PI * Grid::sym(i0, g.n[0]) + b.kcov[0] + b.ham->A[0],
+                   2.0 * M_PI * Grid::sym(i1, g.n[1]) + b.kcov[1] + b.ham->A[1],
+                   2.0 * M_PI * Grid::sym(i2, g.n[2]) + b.kcov[2] + b.ham->A[2]};
+    double c[3];
+    g.cov_to_cart(v, c);
+    kin[i] = 0.5 * dot3(c, c);
+  }
+  return INQX_OK;
+}
+
+int inqx_ham_prepare(inqx_ham h, inqx_basis bs, int nst, int precision) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && bs && nst > 0, INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
+  if (!h->kb.built) build_kb_tables(h);
+  ensure_basis_device(h, bs->b);
+  ensure_workspace(h, bs->b, nst, precision);
+  API_END(ctx)
+}
+
+int inqx_ham_apply_batch(inqx_ham h, int nblocks, const inqx_basis *bases, const int *nst, int precision,
+                         const void *const *psi, void *const *hpsi) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && bases && nst && psi && hpsi && nblocks > 0, INQX_ERR_ARG, "null argument");
+  INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  std::vector<Basis *> bb(nblocks);
+  for (int i = 0; i < nblocks; ++i) {
+    INQX_CHECK(bases[i] && bases[i]->b.ham == h, INQX_ERR_ARG, "basis belongs to another Hamiltonian");
+    INQX_CHECK(nst[i] > 0, INQX_ERR_ARG, "nst must be positive");
+    INQX_CHECK(psi[i] && hpsi[i], INQX_ERR_ARG, "null state pointer");
+    bb[i] = &bases[i]->b;
+  }
+  if (precision == INQX_FP64) launch_apply<double>(h, nblocks, bb.data(), nst, psi, hpsi);
+  else if (precision == INQX_FP32) launch_apply<float>(h, nblocks, bb.data(), nst, psi, hpsi);
+  else INQX_CHECK(false, INQX_ERR_ARG, "bad precision");
+  API_END(ctx)
+}
+
+int inqx_ham_apply(inqx_ham h, inqx_basis bs, int nst, int precision, const void *psi, void *hpsi) {
+  return inqx_ham_apply_batch(h, 1, &bs, &nst, precision, &psi, &hpsi);
+}
+
+int inqx_sphere_to_real(inqx_ham h, inqx_basis bs, int nst, int precision, const void *psi, void *psi_rs) {
+  (void)bs;
+  (void)nst;
+  (void)precision;
+  (void)psi;
+  (void)psi_rs;
+  if (h) h->ctx->c.err = "inqx_sphere_to_real: not built yet";
+  return INQX_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"

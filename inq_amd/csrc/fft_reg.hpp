@@ -1,0 +1,223 @@
+// fft_reg.hpp -- mixed-radix FFT of one line held in registers (gfx950).
+//
+// A line of N complex values lives in a register array x[N] of one lane.
+// The transform is a recursive decimation-in-time split N = R*M, fully
+// unrolled at compile time: every index and every twiddle is a constant, so
+// the array is promoted to VGPRs and the twiddles become instruction
+// immediates.  Twiddles come from a constexpr table evaluated by the
+// compiler (Taylor series after quadrant reduction, exact at multiples of
+// pi/2), so there is no table in memory and no sincos at run time.
+//
+// SIGN = -1: forward, unnormalised, exp(-2 pi i j k / N)
+// SIGN = +1: backward, unnormalised, exp(+2 pi i j k / N)
+// These are the conventions of the reference's to_fourier / to_real before
+// its separate 1/N pass (src/operations/transform.hpp:79,141,175-181).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace inqx {
+
+template <typename T> struct cplx_t;
+template <> struct cplx_t<float> { using type = float2; };
+template <> struct cplx_t<double> { using type = double2; };
+template <typename T> using cplx = typename cplx_t<T>::type;
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename C> __device__ __forceinline__ C cmulc(C a, C b) {  // conj(a) * b
+  return C{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+template <typename C, typename S> __device__ __forceinline__ C cscale(C a, S s) { return C{a.x * s, a.y * s}; }
+
+// ---------------------------------------------------------------- twiddles
+struct tw_cs {
+  double c, s;
+};
+
+// exp(+2 pi i k / n), evaluated at compile time
+constexpr tw_cs tw_value(long k, long n) {
+  k %= n;
+  if (k < 0) k += n;
+  long q = (4 * k) / n;
+  long r = 4 * k - q * n;  // angle = pi/2 * (q + r/n)
+  double phi = 1.5707963267948966 * (double)r / (double)n;
+  double p2 = phi * phi;
+  double c = 1.0, s = phi, tc = 1.0, ts = phi;
+  for (int m = 1; m < 24; ++m) {
+    tc *= -p2 / (double)((2 * m - 1) * (2 * m));
+    ts *= -p2 / (double)((2 * m) * (2 * m + 1));
+    c += tc;
+    s += ts;
+  }
+  if (r == 0) {
+    c = 1.0;
+    s = 0.0;
+  }
+  switch (q) {
+    case 0: return tw_cs{c, s};
+    case 1: return tw_cs{-s, c};
+    case 2: return tw_cs{-c, -s};
+    default: return tw_cs{s, -c};
+  }
+}
+
+template <int N> struct tw_table {
+  double c[N], s[N];
+  constexpr tw_table() : c(), s() {
+    for (int k = 0; k < N; ++k) {
+      tw_cs v = tw_value(k, N);
+      c[k] = v.c;
+      s[k] = v.s;
+    }
+  }
+};
+template <int N> inline constexpr tw_table<N> kTw{};
+
+// multiply by exp(SIGN * 2 pi i e / N); e is a constant after unrolling and
+// the exact 0 / +-1 cases fold to moves and negations.
+template <int N, int SIGN, typename C>
+__device__ __forceinline__ C twmul(C v, int e) {
+  using T = decltype(v.x);
+  const int k = ((e % N) + N) % N;
+  const double dc = kTw<N>.c[k];
+  const double ds = SIGN * kTw<N>.s[k];
+  if (ds == 0.0) {
+    if (dc == 1.0) return v;
+    if (dc == -1.0) return C{-v.x, -v.y};
+  }
+  if (dc == 0.0) {
+    if (ds == 1.0) return C{-v.y, v.x};
+    if (ds == -1.0) return C{v.y, -v.x};
+  }
+  const T c = (T)dc, s = (T)ds;
+  return C{v.x * c - v.y * s, v.x * s + v.y * c};
+}
+
+// ---------------------------------------------------------------- small DFTs
+template <int R, int SIGN, typename C> struct dft_small {
+  // generic R-point DFT, t[q] = sum_r W_R^{SIGN r q} t[r]
+  static __device__ __forceinline__ void run(C (&t)[R]) {
+    C o[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      C acc = t[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) acc = cadd(acc, twmul<R, SIGN>(t[r], r * q));
+      o[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) t[q] = o[q];
+  }
+};
+
+template <int SIGN, typename C> struct dft_small<2, SIGN, C> {
+  static __device__ __forceinline__ void run(C (&t)[2]) {
+    C a = t[0], b = t[1];
+    t[0] = cadd(a, b);
+    t[1] = csub(a, b);
+  }
+};
+
+template <int SIGN, typename C> struct dft_small<4, SIGN, C> {
+  static __device__ __forceinline__ void run(C (&t)[4]) {
+    C a0 = cadd(t[0], t[2]), a1 = csub(t[0], t[2]);
+    C b0 = cadd(t[1], t[3]), b1 = csub(t[1], t[3]);
+    // b1 * (SIGN i)
+    C jb1 = (SIGN > 0) ? C{-b1.y, b1.x} : C{b1.y, -b1.x};
+    t[0] = cadd(a0, b0);
+    t[2] = csub(a0, b0);
+    t[1] = cadd(a1, jb1);
+    t[3] = csub(a1, jb1);
+  }
+};
+
+template <int SIGN, typename C> struct dft_small<3, SIGN, C> {
+  static __device__ __forceinline__ void run(C (&t)[3]) {
+    using T = decltype(t[0].x);
+    const T c = (T)-0.5, s = (T)(SIGN * 0.86602540378443864676);
+    C a = t[0];
+    C u = cadd(t[1], t[2]);
+    C v = csub(t[1], t[2]);
+    C m = C{a.x + c * u.x, a.y + c * u.y};
+    C js = C{-s * v.y, s * v.x};
+    t[0] = cadd(a, u);
+    t[1] = cadd(m, js);
+    t[2] = csub(m, js);
+  }
+};
+
+template <int SIGN, typename C> struct dft_small<5, SIGN, C> {
+  static __device__ __forceinline__ void run(C (&t)[5]) {
+    using T = decltype(t[0].x);
+    const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+    const T s1 = (T)(SIGN * 0.95105651629515357212), s2 = (T)(SIGN * 0.58778525229247312917);
+    C a = t[0];
+    C u1 = cadd(t[1], t[4]), v1 = csub(t[1], t[4]);
+    C u2 = cadd(t[2], t[3]), v2 = csub(t[2], t[3]);
+    C m1 = C{a.x + c1 * u1.x + c2 * u2.x, a.y + c1 * u1.y + c2 * u2.y};
+    C m2 = C{a.x + c2 * u1.x + c1 * u2.x, a.y + c2 * u1.y + c1 * u2.y};
+    // i*(s1 v1 + s2 v2) and i*(s2 v1 - s1 v2)
+    C n1 = C{-(s1 * v1.y + s2 * v2.y), s1 * v1.x + s2 * v2.x};
+    C n2 = C{-(s2 * v1.y - s1 * v2.y), s2 * v1.x - s1 * v2.x};
+    t[0] = cadd(a, cadd(u1, u2));
+    t[1] = cadd(m1, n1);
+    t[4] = csub(m1, n1);
+    t[2] = cadd(m2, n2);
+    t[3] = csub(m2, n2);
+  }
+};
+
+// ---------------------------------------------------------------- radix choice
+constexpr int fft_radix(int n) {
+  if (n % 4 == 0 && n != 8) return 4;
+  if (n % 2 == 0) return 2;
+  if (n % 3 == 0) return 3;
+  if (n % 5 == 0) return 5;
+  if (n % 7 == 0) return 7;
+  return n;
+}
+
+constexpr bool fft_supported(int n) {
+  if (n < 1) return false;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
+  while (n % 7 == 0) n /= 7;
+  return n == 1;
+}
+
+// ---------------------------------------------------------------- recursive FFT
+template <int N, int SIGN, typename C> struct fft_line {
+  static_assert(fft_supported(N), "FFT length must be a product of 2, 3, 5, 7");
+  static __device__ __forceinline__ void run(C (&x)[N]) {
+    constexpr int R = fft_radix(N);
+    constexpr int M = N / R;
+    if constexpr (M == 1) {
+      dft_small<R, SIGN, C>::run(x);
+    } else {
+      C sub[R][M];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int m = 0; m < M; ++m) sub[r][m] = x[r + R * m];
+#pragma unroll
+      for (int r = 0; r < R; ++r) fft_line<M, SIGN, C>::run(sub[r]);
+#pragma unroll
+      for (int k1 = 0; k1 < M; ++k1) {
+        C t[R];
+        t[0] = sub[0][k1];
+#pragma unroll
+        for (int r = 1; r < R; ++r) t[r] = twmul<N, SIGN>(sub[r][k1], r * k1);
+        dft_small<R, SIGN, C>::run(t);
+#pragma unroll
+        for (int k2 = 0; k2 < R; ++k2) x[k1 + M * k2] = t[k2];
+      }
+    }
+  }
+};
+
+}  // namespace inqx

@@ -1,0 +1,231 @@
+// internal.hpp -- host-side objects behind the inqx C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/inqx.h"
+#include "fft_reg.hpp"
+
+namespace inqx {
+
+// ------------------------------------------------------------------ errors
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define INQX_HIP(call)                                                                        \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      throw ::inqx::Error{INQX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)};   \
+  } while (0)
+
+#define INQX_CHECK(cond, code, msg)                           \
+  do {                                                        \
+    if (!(cond)) throw ::inqx::Error{(code), std::string(msg)}; \
+  } while (0)
+
+// ------------------------------------------------------------------ device buffers
+struct DevMem {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevMem() = default;
+  DevMem(const DevMem &) = delete;
+  DevMem &operator=(const DevMem &) = delete;
+  ~DevMem() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t nb) {
+    if (nb <= bytes && p) return;
+    release();
+    if (nb == 0) nb = 16;
+    INQX_HIP(hipMalloc(&p, nb));
+    bytes = nb;
+  }
+  template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+template <typename T> void upload(DevMem &m, const std::vector<T> &v, hipStream_t s) {
+  m.ensure(sizeof(T) * (v.empty() ? 1 : v.size()));
+  if (!v.empty()) INQX_HIP(hipMemcpyAsync(m.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+}
+
+// ------------------------------------------------------------------ grid / cell metric
+// systems::cell and its cell_metric (src/systems/cell.hpp:36-52,146-224)
+struct Grid {
+  int n[3];
+  long N;
+  double a[3][3];   // lattice vectors, rows, cartesian
+  double b[3][3];   // reciprocal vectors 2 pi / V (a_j x a_k)
+  double rl[3][3];  // b / 2 pi
+  double vol;
+  void init(const double *lat, const int *nn);
+  void cov_to_cart(const double *v, double *c) const;
+  void contra_to_cart(const double *v, double *c) const;
+  void cart_to_contra(const double *c, double *v) const;
+  static int sym(int i, int n) { return (i >= (n + 1) / 2) ? i - n : i; }
+};
+
+// per-kernel timing with HIP events on the context's stream (the role of the
+// reference's CALI_CXX_MARK_* timers, src/utils/profiling.hpp)
+struct KStat {
+  long count = 0;
+  double total_ms = 0.0;
+};
+struct Ctx {
+  int device = 0;
+  bool host_only = false;  // device < 0: set-up queries only, no HIP calls
+  hipStream_t stream = nullptr;
+  std::string err;
+  int num_cu = 256;
+  bool prof = false;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, KStat> stats;
+  hipEvent_t get_event();
+  void flush_prof();
+};
+
+// brackets one kernel launch with events when profiling is on
+struct ProfScope {
+  Ctx &c;
+  std::string name;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ProfScope(Ctx &cc, const char *n) : c(cc) {
+    if (c.prof) {
+      name = n;
+      e0 = c.get_event();
+      e1 = c.get_event();
+      (void)hipEventRecord(e0, c.stream);
+    }
+  }
+  ~ProfScope() {
+    if (c.prof) {
+      (void)hipEventRecord(e1, c.stream);
+      c.pending.push_back({name, {e0, e1}});
+    }
+  }
+};
+
+// ------------------------------------------------------------------ projectors
+struct Projector {
+  double center[3];
+  double radius;
+  int nlm;
+  std::vector<int> coords;    // [npts][3]
+  std::vector<float> dist;    // [npts]
+  std::vector<float> relpos;  // [npts][3] contravariant
+  std::vector<double> matrix; // [nlm][npts]
+  std::vector<double> coeff;  // [nlm][nlm]
+  long npts() const { return (long)dist.size(); }
+};
+
+constexpr int kMaxNlm = 4;   // s + 3 p per atom, padded
+constexpr int kTile = 64;    // x-pencils (lines) per tile: one wavefront
+
+// tables of the real-space KB pass, shared by all blocks of a Hamiltonian
+struct KBTables {
+  bool built = false;
+  int natoms = 0, maxsph = 0, ntile = 0, nslot = 0, maxpts = 0;
+  long nent = 0, npt = 0, nmem = 0;
+  // per line L = i1*n2 + i2: bitmask of sphere points along i0 (3 words) and
+  // the line's first compact index inside its tile
+  DevMem line_mask;   // [n12][4] uint32: m0, m1, m2, base
+  DevMem tile_info;   // [ntile][4] int: npts, slot0, slot1, pt_base
+  DevMem slot_atom;   // [nslot]
+  DevMem slot_ent;    // [nslot+1]
+  DevMem ent_c;       // [nent] compact point index inside the tile
+  DevMem ent_ap;      // [nent] a*maxsph + p
+  DevMem atom_slots;  // [natoms+1] offsets followed by slot ids
+  DevMem atom_slot_list;
+  DevMem pt_mem;      // [npt+1] membership offsets
+  DevMem mem_ap;      // [nmem]
+  DevMem pos;         // [natoms][maxsph][3] float
+  DevMem beta64, beta32;  // [natoms][kMaxNlm][maxsph]
+  DevMem coeff;       // [natoms][kMaxNlm][kMaxNlm] double
+};
+
+// ------------------------------------------------------------------ device block descriptor
+template <typename T> struct SphereBlock {
+  using C = cplx<T>;
+  const C *psi;       // [nst][npw]
+  C *hpsi;            // [nst][npw]
+  C *t1;              // [nslot][nst][n1*n2] plane intermediates (in place for the x pass)
+  C *partial;         // [kb nslot][kMaxNlm][nst]
+  C *dp;              // [natoms][kMaxNlm][nst]
+  const int *slot_i0; // [nslot] plane FFT index of each slot
+  const int *slot_of; // [n0] slot of each plane index or -1
+  const int *seg_off; // [nslot+1] sphere segment of each plane
+  const int *row_off; // [nslot+1]
+  const int *rows;    // rows (i1) holding sphere points, per slot
+  const int *pw_yz;   // [npw] i1*n2 + i2
+  const T *kin;       // [npw] 1/2 |G+k|^2
+  const T *vloc;      // [N] V/N of the block's spin
+  int nst, npw, nslot;
+  int gamma;          // k+A == 0: no KB phases
+  double kcov[3];     // k + A, covariant
+};
+
+// one G-sphere block
+struct Basis {
+  struct inqx_ham_s *ham = nullptr;
+  double kcov[3];
+  double ecut;
+  int spin;
+  long npw = 0;
+  std::vector<int> gidx;      // flattened FFT index
+  std::vector<int> slot_i0, slot_of, seg_off, row_off, rows, pw_yz;
+  std::vector<double> kin;    // 1/2|G+k|^2 (without A)
+  DevMem d_slot_i0, d_slot_of, d_seg_off, d_row_off, d_rows, d_pw_yz, d_kin64, d_kin32;
+  double kin_A[3] = {1e300, 0, 0};  // A for which d_kin* were built
+  // workspaces per precision
+  DevMem t1[2], partial[2], dp[2];
+  int ws_nst[2] = {0, 0};
+  int ws_kb_nslot[2] = {-1, -1}, ws_natoms[2] = {-1, -1};
+};
+
+}  // namespace inqx
+
+struct inqx_ctx_s {
+  inqx::Ctx c;
+};
+
+struct inqx_ham_s {
+  inqx_ctx ctx;
+  inqx::Grid g;
+  int ncomp = 1;
+  bool has_v = false;
+  std::vector<double> v;  // [N][ncomp]
+  inqx::DevMem v64, v32;  // [ncomp][N] of V/N
+  double A[3] = {0, 0, 0};
+  std::vector<inqx::Projector> projs;
+  inqx::KBTables kb;
+  std::vector<inqx_basis> bases;
+};
+
+struct inqx_basis_s {
+  inqx::Basis b;
+};
+
+namespace inqx {
+// ham_kernels.hip
+void build_kb_tables(inqx_ham_s *h);
+void invalidate_launch_cache(inqx_ham_s *h);
+template <typename T>
+void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *nst, const void *const *psi,
+                  void *const *hpsi);
+void ensure_basis_device(inqx_ham_s *h, Basis &b);
+void ensure_workspace(inqx_ham_s *h, Basis &b, int nst, int prec);
+bool grid_supported(const int *n);
+}  // namespace inqx

@@ -1,0 +1,233 @@
+"""Host-side mirror of the reference's Hamiltonian call surface over the inqx
+C ABI.
+
+Names follow the reference: ``KSHamiltonian`` is hamiltonian::ks_hamiltonian
+(src/hamiltonian/ks_hamiltonian.hpp:35-275), ``Basis`` is the G sphere of one
+orbital_set block (src/states/orbital_set.hpp:18-192), and calling the
+Hamiltonian on a block of states applies H (ks_hamiltonian.hpp:160-180).
+States live in device memory (``DeviceArray``); numpy arrays are copied in and
+out explicitly.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import INQX_FP32, INQX_FP64, check, lib
+
+_DT = {INQX_FP64: np.complex128, INQX_FP32: np.complex64}
+
+
+def precision_of(dtype):
+    dtype = np.dtype(dtype)
+    if dtype == np.complex128:
+        return INQX_FP64
+    if dtype == np.complex64:
+        return INQX_FP32
+    raise ValueError(f"unsupported state dtype {dtype}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """inqx_ctx: one device and one stream (the default stream unless set)."""
+
+    def __init__(self, device=0, stream=None):
+        self.h = C.c_void_p()
+        check(lib().inqx_ctx_create(int(device), C.c_void_p(stream or 0), C.byref(self.h)), None, "inqx_ctx_create")
+
+    def sync(self):
+        check(lib().inqx_sync(self.h), self.h, "inqx_sync")
+
+    def profile(self, on=True):
+        check(lib().inqx_prof_enable(self.h, int(on)), self.h, "inqx_prof_enable")
+
+    def profile_reset(self):
+        check(lib().inqx_prof_reset(self.h), self.h, "inqx_prof_reset")
+
+    def kernel_times(self):
+        """{kernel name: (launches, total ms)} accumulated with HIP events."""
+        n = C.c_int()
+        check(lib().inqx_prof_count(self.h, C.byref(n)), self.h, "inqx_prof_count")
+        out = {}
+        buf = C.create_string_buffer(128)
+        for i in range(n.value):
+            cnt = C.c_long()
+            ms = C.c_double()
+            check(lib().inqx_prof_get(self.h, i, buf, 128, C.byref(cnt), C.byref(ms)), self.h, "inqx_prof_get")
+            out[buf.value.decode()] = (cnt.value, ms.value)
+        return out
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().inqx_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device allocation with a numpy shape/dtype."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = C.c_void_p()
+        check(lib().inqx_malloc(ctx.h, self.nbytes, C.byref(self.ptr)), ctx.h, "inqx_malloc")
+
+    @classmethod
+    def from_numpy(cls, ctx, a):
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib().inqx_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes), self.ctx.h,
+              "inqx_memcpy_h2d")
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib().inqx_memcpy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes), self.ctx.h,
+              "inqx_memcpy_d2h")
+        return out
+
+    def copy_from(self, other):
+        assert other.nbytes == self.nbytes
+        check(lib().inqx_memcpy_d2d(self.ctx.h, self.ptr, other.ptr, self.nbytes), self.ctx.h, "inqx_memcpy_d2d")
+
+    def free(self):
+        if self.ptr.value:
+            lib().inqx_free(self.ctx.h, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Basis:
+    """G sphere 1/2|G+k|^2 < ecut of one (k-point, spin) block."""
+
+    def __init__(self, ham, kpoint, ecut, spin=0):
+        self.ham = ham
+        self.kpoint = np.asarray(kpoint, dtype=np.float64)
+        self.ecut = float(ecut)
+        self.spin = int(spin)
+        self.h = C.c_void_p()
+        check(lib().inqx_basis_create(ham.h, _dp(self.kpoint), self.ecut, self.spin, C.byref(self.h)), ham.ctx.h,
+              "inqx_basis_create")
+        n = C.c_long()
+        check(lib().inqx_basis_size(self.h, C.byref(n)), ham.ctx.h, "inqx_basis_size")
+        self.npw = n.value
+
+    def gindex(self):
+        g = np.zeros(self.npw, dtype=np.int32)
+        check(lib().inqx_basis_gindex(self.h, g.ctypes.data_as(C.POINTER(C.c_int))), self.ham.ctx.h, "gindex")
+        return g
+
+    def kinetic(self):
+        k = np.zeros(self.npw)
+        check(lib().inqx_basis_kinetic(self.h, _dp(k)), self.ham.ctx.h, "kinetic")
+        return k
+
+    def __del__(self):
+        try:
+            if self.h.value:
+                lib().inqx_basis_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class KSHamiltonian:
+    """hamiltonian::ks_hamiltonian on a grid with a local potential and KB projectors."""
+
+    def __init__(self, ctx, lattice, n):
+        self.ctx = ctx
+        self.lattice = np.ascontiguousarray(lattice, dtype=np.float64).reshape(3, 3)
+        self.n = tuple(int(x) for x in n)
+        self.h = C.c_void_p()
+        nn = (C.c_int * 3)(*self.n)
+        check(lib().inqx_ham_create(ctx.h, _dp(self.lattice.ravel()), nn, C.byref(self.h)), ctx.h, "inqx_ham_create")
+        self._A = np.zeros(3)
+
+    def set_scalar_potential(self, v, ncomp=1):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        check(lib().inqx_ham_set_potential(self.h, _dp(v), int(ncomp)), self.ctx.h, "set_potential")
+
+    @property
+    def uniform_vector_potential(self):
+        return self._A.copy()
+
+    @uniform_vector_potential.setter
+    def uniform_vector_potential(self, a):
+        self._A = np.asarray(a, dtype=np.float64).copy()
+        check(lib().inqx_ham_set_vector_potential(self.h, _dp(self._A)), self.ctx.h, "set_vector_potential")
+
+    def add_projector(self, center, radius, nlm, kb_coeff, width=None, matrix=None):
+        c = np.asarray(center, dtype=np.float64)
+        d = np.ascontiguousarray(kb_coeff, dtype=np.float64)
+        m = None if matrix is None else np.ascontiguousarray(matrix, dtype=np.float64)
+        st = lib().inqx_ham_add_projector(self.h, _dp(c), float(radius), int(nlm),
+                                          None if m is None else _dp(m), _dp(d), float(width or 0.0))
+        return check(st, self.ctx.h, "add_projector")
+
+    def projector_points(self, i):
+        n = C.c_long()
+        check(lib().inqx_ham_projector_size(self.h, int(i), C.byref(n)), self.ctx.h, "projector_size")
+        coords = np.zeros((n.value, 3), dtype=np.int32)
+        dist = np.zeros(n.value, dtype=np.float32)
+        rel = np.zeros((n.value, 3), dtype=np.float32)
+        check(lib().inqx_ham_projector_points(self.h, int(i), coords.ctypes.data_as(C.POINTER(C.c_int)),
+                                              dist.ctypes.data_as(C.POINTER(C.c_float)),
+                                              rel.ctypes.data_as(C.POINTER(C.c_float))), self.ctx.h,
+              "projector_points")
+        return coords, dist, rel
+
+    def basis(self, kpoint, ecut, spin=0):
+        return Basis(self, kpoint, ecut, spin)
+
+    def prepare(self, basis, nst, dtype):
+        check(lib().inqx_ham_prepare(self.h, basis.h, int(nst), precision_of(dtype)), self.ctx.h, "prepare")
+
+    def apply(self, basis, psi, hpsi):
+        """hpsi = H psi for one block (device arrays [nst][npw])."""
+        nst = psi.shape[0]
+        assert psi.shape == (nst, basis.npw) and hpsi.shape == psi.shape and hpsi.dtype == psi.dtype
+        check(lib().inqx_ham_apply(self.h, basis.h, nst, precision_of(psi.dtype), psi.ptr, hpsi.ptr), self.ctx.h,
+              "inqx_ham_apply")
+
+    def apply_batch(self, bases, psis, hpsis):
+        nb = len(bases)
+        prec = precision_of(psis[0].dtype)
+        bb = (C.c_void_p * nb)(*[b.h.value for b in bases])
+        nst = (C.c_int * nb)(*[p.shape[0] for p in psis])
+        pp = (C.c_void_p * nb)(*[p.ptr.value for p in psis])
+        hp = (C.c_void_p * nb)(*[p.ptr.value for p in hpsis])
+        check(lib().inqx_ham_apply_batch(self.h, nb, bb, nst, prec, pp, hp), self.ctx.h, "inqx_ham_apply_batch")
+
+    def __call__(self, basis, psi):
+        """ks_hamiltonian::operator() on a block of sphere states: returns a new DeviceArray."""
+        out = DeviceArray(self.ctx, psi.shape, psi.dtype)
+        self.apply(basis, psi, out)
+        return out
+
+    def __del__(self):
+        try:
+            if self.h.value:
+                lib().inqx_ham_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
