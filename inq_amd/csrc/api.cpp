@@ -181,29 +181,45 @@ static void build_basis(const Grid &g, Basis &b) {
         if (0.5 * dot3(c, c) < b.ecut) b.gidx.push_back((i0 * g.n[1] + i1) * g.n[2] + i2);
       }
   b.npw = (long)b.gidx.size();
-  b.slot_i0.clear();
-  b.seg_off.clear();
-  b.row_off.clear();
-  b.rows.clear();
-  b.pw_yz.resize(b.npw);
-  b.slot_of.assign(g.n[0], -1);
+  // planes: the symmetric indices m of the sphere's i0 values form one range
   const int n12 = g.n[1] * g.n[2];
+  int mlo = 1 << 30, mhi = -(1 << 30);
+  b.pw_yz.resize(b.npw);
   for (long i = 0; i < b.npw; ++i) {
     const int f = b.gidx[i];
-    const int i0 = f / n12, yz = f - i0 * n12, i1 = yz / g.n[2];
-    b.pw_yz[i] = yz;
-    if (b.slot_i0.empty() || b.slot_i0.back() != i0) {
-      b.slot_of[i0] = (int)b.slot_i0.size();
-      b.slot_i0.push_back(i0);
-      b.seg_off.push_back((int)i);
-      b.row_off.push_back((int)b.rows.size());
-      b.rows.push_back(i1);
-    } else if (b.rows.back() != i1) {
-      b.rows.push_back(i1);
-    }
+    const int i0 = f / n12;
+    b.pw_yz[i] = f - i0 * n12;
+    const int m = Grid::sym(i0, g.n[0]);
+    mlo = std::min(mlo, m);
+    mhi = std::max(mhi, m);
   }
-  b.seg_off.push_back((int)b.npw);
-  b.row_off.push_back((int)b.rows.size());
+  b.m_lo = b.npw ? mlo : 0;
+  b.nslot = b.npw ? mhi - mlo + 1 : 0;
+  b.slot_seg.assign((size_t)b.nslot * 4, 0);
+  b.rows.clear();
+  // segments are contiguous in the FFT-index order of the sphere
+  long i = 0;
+  std::vector<long> seg_start(g.n[0], -1), seg_len(g.n[0], 0);
+  while (i < b.npw) {
+    const int i0 = b.gidx[i] / n12;
+    long j = i;
+    while (j < b.npw && b.gidx[j] / n12 == i0) ++j;
+    seg_start[i0] = i;
+    seg_len[i0] = j - i;
+    i = j;
+  }
+  for (int sl = 0; sl < b.nslot; ++sl) {
+    const int m = b.m_lo + sl;
+    const int i0 = (m % g.n[0] + g.n[0]) % g.n[0];
+    b.slot_seg[4 * sl + 0] = seg_start[i0] < 0 ? 0 : (int)seg_start[i0];
+    b.slot_seg[4 * sl + 1] = (int)seg_len[i0];
+    b.slot_seg[4 * sl + 2] = (int)b.rows.size();
+    for (long q = seg_start[i0]; q >= 0 && q < seg_start[i0] + seg_len[i0]; ++q) {
+      const int i1 = b.pw_yz[q] / g.n[2];
+      if (b.rows.size() == (size_t)b.slot_seg[4 * sl + 2] || b.rows.back() != i1) b.rows.push_back(i1);
+    }
+    b.slot_seg[4 * sl + 3] = (int)b.rows.size() - b.slot_seg[4 * sl + 2];
+  }
 }
 
 }  // namespace inqx

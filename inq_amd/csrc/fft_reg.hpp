@@ -18,10 +18,12 @@
 
 namespace inqx {
 
-template <typename T> struct cplx_t;
-template <> struct cplx_t<float> { using type = float2; };
-template <> struct cplx_t<double> { using type = double2; };
-template <typename T> using cplx = typename cplx_t<T>::type;
+// plain aggregate complex (trivially copyable, so it can be loaded and
+// stored through address-space-qualified pointers)
+template <typename T> struct alignas(2 * sizeof(T)) cplx_pod {
+  T x, y;
+};
+template <typename T> using cplx = cplx_pod<T>;
 
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }

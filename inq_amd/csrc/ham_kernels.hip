@@ -5,10 +5,10 @@
 // synchronisation:
 //
 //   plane_inverse  sphere -> zero-padded (i1,i2) planes of every plane index
-//                  i0 the sphere touches, inverse FFT along i2 (rows that hold
+//                  i0 of the sphere, inverse FFT along i2 (rows that hold
 //                  sphere points only) then along i1; writes t1[slot][ist][i1][i2]
 //   xpass_project  inverse FFT along i0 of each line (i1,i2) in registers,
-//                  gathers the line's KB sphere points into LDS and reduces
+//                  keeps the line's KB sphere points in LDS and reduces
 //                  beta * exp(+ik.r) * psi per (tile, atom) slot
 //   kb_reduce      sums the slots of each atom, scales by dV, multiplies by
 //                  the KB coefficient matrix D
@@ -26,36 +26,69 @@
 // (projector_all.hpp:215-288), to_fourier (transform.hpp:52-109) and
 // laplacian_add (laplacian.hpp:24-37) with the 1/2|k|^2 shift of
 // ks_hamiltonian.hpp:149 folded into the kinetic factor.
+//
+// Block descriptors are copied into scalar registers at the top of each work
+// item and every array access goes through an explicit global-address-space
+// pointer, so the compiler emits scalar loads for uniform data and
+// global_load/global_store (not flat) for the streams.
 #include "internal.hpp"
 
 #include <algorithm>
 #include <cstring>
 #include <numeric>
 
+// global address space in the device pass; the host pass of the same
+// templates sees plain pointers
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GAS __attribute__((address_space(1)))
+#else
+#define GAS
+#endif
+
 namespace inqx {
+
+template <typename T> __device__ __forceinline__ const GAS T *gp(const T *p) { return (const GAS T *)p; }
+template <typename T> __device__ __forceinline__ GAS T *gw(T *p) { return (GAS T *)p; }
+
+// 4- and 2-wide loads of plain records from global memory
+template <typename S> struct V4 {
+  S x, y, z, w;
+};
+template <typename S> struct V2 {
+  S x, y;
+};
+template <typename S> __device__ __forceinline__ V4<S> ld4(const void *p, size_t i) {
+  const GAS S *q = (const GAS S *)__builtin_assume_aligned(p, 4 * sizeof(S)) + 4 * i;
+  return V4<S>{q[0], q[1], q[2], q[3]};
+}
+template <typename S> __device__ __forceinline__ V2<S> ld2(const void *p, size_t i) {
+  const GAS S *q = (const GAS S *)__builtin_assume_aligned(p, 2 * sizeof(S)) + 2 * i;
+  return V2<S>{q[0], q[1]};
+}
 
 void invalidate_launch_cache(inqx_ham_s *h);
 
 struct HamDev {
   int n0, n1, n2, n12;
   long N;
-  int natoms, maxsph, has_kb, ntile;
+  int natoms, has_kb, ntile, maxpts;
   const uint32_t *line_mask;
   const int *tile_info;
-  const int *slot_atom, *slot_ent, *ent_c, *ent_ap;
+  const int *slot_atom;
+  const int *ent_cs;
+  const void *ent_b;
+  const float *ent_pos;
   const int *atom_slots, *atom_slot_list;
-  const int *pt_mem, *mem_ap;
-  const float *pos;
-  const void *beta;
   const double *coeff;
   double kb_scale;  // dV / N
 };
 
 __device__ __forceinline__ int find_block(const int *prefix, int nblocks, int wg) {
+  const GAS int *pf = gp(prefix);
   int lo = 0, hi = nblocks;  // prefix[lo] <= wg < prefix[hi]
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
-    if (prefix[mid] <= wg) lo = mid;
+    if (pf[mid] <= wg) lo = mid;
     else hi = mid;
   }
   return lo;
@@ -90,26 +123,29 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
   const int tid = threadIdx.x, nth = blockDim.x;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
-    const SphereBlock<T> &B = blocks[b];
-    const int local = wg - prefix[b];
+    const SphereBlock<T> B = blocks[b];
+    const int local = wg - gp(prefix)[b];
     const int ngrp = (B.nst + P - 1) / P;
     const int slot = local / ngrp;
     const int s0 = (local - slot * ngrp) * P;
     const int np = min(P, B.nst - s0);
+    const V4<int> sg = ld4<int>(B.slot_seg, slot);
+    const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
     for (int i = tid; i < np * Cfg::PLANE; i += nth) lds[i] = C{0, 0};
     __syncthreads();
-    const int seg0 = B.seg_off[slot], nseg = B.seg_off[slot + 1] - seg0;
+    const GAS C *psi = gp(B.psi);
+    const GAS int *pwyz = gp(B.pw_yz);
     for (int i = tid; i < np * nseg; i += nth) {
       const int p = i / nseg, j = i - p * nseg;
-      const int yz = B.pw_yz[seg0 + j];
+      const int yz = pwyz[seg0 + j];
       const int y = yz / N2, z = yz - y * N2;
-      lds[p * Cfg::PLANE + y * (N2 + 1) + z] = B.psi[(size_t)(s0 + p) * B.npw + seg0 + j];
+      lds[p * Cfg::PLANE + y * (N2 + 1) + z] = psi[(size_t)(s0 + p) * B.npw + seg0 + j];
     }
     __syncthreads();
-    const int r0 = B.row_off[slot], nrow = B.row_off[slot + 1] - r0;
+    const GAS int *rows = gp(B.rows);
     for (int i = tid; i < np * nrow; i += nth) {
       const int p = i / nrow;
-      const int y = B.rows[r0 + i - p * nrow];
+      const int y = rows[r0 + i - p * nrow];
       C *row = lds + p * Cfg::PLANE + y * (N2 + 1);
       C x[N2];
 #pragma unroll
@@ -119,6 +155,7 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
       for (int z = 0; z < N2; ++z) row[z] = x[z];
     }
     __syncthreads();
+    GAS C *t1 = gw(B.t1);
     for (int i = tid; i < np * N2; i += nth) {
       const int p = i / N2, z = i - p * N2;
       const C *col = lds + p * Cfg::PLANE + z;
@@ -126,7 +163,7 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
 #pragma unroll
       for (int y = 0; y < N1; ++y) x[y] = col[y * (N2 + 1)];
       fft_line<N1, +1, C>::run(x);
-      C *out = B.t1 + ((size_t)slot * B.nst + s0 + p) * (N1 * N2) + z;
+      GAS C *out = t1 + ((size_t)slot * B.nst + s0 + p) * (N1 * N2) + z;
 #pragma unroll
       for (int y = 0; y < N1; ++y) out[y * N2] = x[y];
     }
@@ -145,15 +182,19 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
   const int tid = threadIdx.x, nth = blockDim.x;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
-    const SphereBlock<T> &B = blocks[b];
-    const int local = wg - prefix[b];
+    const SphereBlock<T> B = blocks[b];
+    const int local = wg - gp(prefix)[b];
     const int ngrp = (B.nst + P - 1) / P;
     const int slot = local / ngrp;
     const int s0 = (local - slot * ngrp) * P;
     const int np = min(P, B.nst - s0);
+    const V4<int> sg = ld4<int>(B.slot_seg, slot);
+    const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
+    if (nseg == 0) continue;  // uniform: nothing of this plane is on the sphere
+    const GAS C *t1 = gp(B.t1);
     for (int i = tid; i < np * N2; i += nth) {
       const int p = i / N2, z = i - p * N2;
-      const C *in = B.t1 + ((size_t)slot * B.nst + s0 + p) * (N1 * N2) + z;
+      const GAS C *in = t1 + ((size_t)slot * B.nst + s0 + p) * (N1 * N2) + z;
       C x[N1];
 #pragma unroll
       for (int y = 0; y < N1; ++y) x[y] = in[y * N2];
@@ -163,10 +204,10 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
       for (int y = 0; y < N1; ++y) col[y * (N2 + 1)] = x[y];
     }
     __syncthreads();
-    const int r0 = B.row_off[slot], nrow = B.row_off[slot + 1] - r0;
+    const GAS int *rows = gp(B.rows);
     for (int i = tid; i < np * nrow; i += nth) {
       const int p = i / nrow;
-      const int y = B.rows[r0 + i - p * nrow];
+      const int y = rows[r0 + i - p * nrow];
       C *row = lds + p * Cfg::PLANE + y * (N2 + 1);
       C x[N2];
 #pragma unroll
@@ -176,16 +217,19 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
       for (int z = 0; z < N2; ++z) row[z] = x[z];
     }
     __syncthreads();
-    const int seg0 = B.seg_off[slot], nseg = B.seg_off[slot + 1] - seg0;
+    const GAS C *psi = gp(B.psi);
+    GAS C *hpsi = gw(B.hpsi);
+    const GAS int *pwyz = gp(B.pw_yz);
+    const GAS T *kin = gp(B.kin);
     for (int i = tid; i < np * nseg; i += nth) {
       const int p = i / nseg, j = i - p * nseg;
-      const int yz = B.pw_yz[seg0 + j];
+      const int yz = pwyz[seg0 + j];
       const int y = yz / N2, z = yz - y * N2;
       const size_t o = (size_t)(s0 + p) * B.npw + seg0 + j;
       const C v = lds[p * Cfg::PLANE + y * (N2 + 1) + z];
-      const C c = B.psi[o];
-      const T kf = B.kin[seg0 + j];
-      B.hpsi[o] = C{v.x + kf * c.x, v.y + kf * c.y};
+      const C c = psi[o];
+      const T kf = kin[seg0 + j];
+      hpsi[o] = C{v.x + kf * c.x, v.y + kf * c.y};
     }
     __syncthreads();
   }
@@ -194,97 +238,121 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
 // -------------------------------------------------------------------------------------------
 // x passes: one lane per line (i1,i2), 64 consecutive lines per wave, one state per wave
 // -------------------------------------------------------------------------------------------
+
+// symmetric plane index of FFT index i (compile time)
+template <int N> constexpr int sym_of(int i) { return i >= (N + 1) / 2 ? i - N : i; }
+
+// loads the plane slots of one line (zero outside the slot range) and
+// transforms along i0; Lc is the lane's line, clamped into the grid
 template <typename T, int N0>
-__device__ __forceinline__ void load_line_inverse(const SphereBlock<T> &B, int n12, int s, int L, bool valid,
+__device__ __forceinline__ void load_line_inverse(const SphereBlock<T> &B, int n12, int s, int Lc,
                                                   cplx<T> (&x)[N0]) {
   using C = cplx<T>;
+  const GAS C *base = gp(B.t1) + (size_t)s * n12 + Lc;
+  const size_t sstride = (size_t)B.nst * n12;
 #pragma unroll
   for (int i0 = 0; i0 < N0; ++i0) {
-    const int j = B.slot_of[i0];  // uniform
-    x[i0] = (j >= 0 && valid) ? B.t1[((size_t)j * B.nst + s) * n12 + L] : C{0, 0};
+    const int j = sym_of<N0>(i0) - B.m_lo;  // uniform
+    C v{0, 0};
+    if ((unsigned)j < (unsigned)B.nslot) v = base[(size_t)j * sstride];
+    x[i0] = v;
   }
   fft_line<N0, +1, C>::run(x);
 }
 
+
+__device__ __forceinline__ void lds_add(float *p, float v) { atomicAdd(p, v); }
+__device__ __forceinline__ void lds_add(double *p, double v) { atomicAdd(p, v); }
+
+// phase exp(sgn i k.pos) with the float relative position the reference
+// keeps (projector_all.hpp:123,284)
+template <typename T>
+__device__ __forceinline__ cplx<T> kb_phase(const double *kcov, V4<float> pos, T sgn) {
+  const T arg = sgn * (T)(kcov[0] * (double)pos.x + kcov[1] * (double)pos.y + kcov[2] * (double)pos.z);
+  T sn, cs;
+  sincos_t<T>(arg, &sn, &cs);
+  return cplx<T>{cs, sn};
+}
+
+// Inverse FFT along i0 of the wave's 64 lines, then the KB projections
+// P[slot][l] = sum_entries beta_l exp(+ik.pos) psi.  Each lane walks a
+// contiguous chunk of the tile's entries (sorted by slot), accumulates in
+// registers and flushes to the LDS slot accumulators only at slot changes.
 template <typename T, int N0>
 __global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                        int total, HamDev hd, int lds_per_wave) {
   using C = cplx<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  C *lds = reinterpret_cast<C *>(smem_raw) + wave * lds_per_wave;
-  const T *beta = reinterpret_cast<const T *>(hd.beta);
+  C *pts = reinterpret_cast<C *>(smem_raw) + wave * lds_per_wave;
+  C *acc = pts + hd.maxpts;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
-    const SphereBlock<T> &B = blocks[b];
-    const int local = wg - prefix[b];
+    const SphereBlock<T> B = blocks[b];
+    const int local = wg - gp(prefix)[b];
     const int ngrp = (B.nst + nwave - 1) / nwave;
     const int tile = local / ngrp;
     const int s = (local - tile * ngrp) * nwave + wave;
     const int L = tile * kTile + lane;
-    const int npts = hd.tile_info[4 * tile + 0];
+    const V4<int> ti = ld4<int>(hd.tile_info, 2 * tile);
+    const V4<int> tj = ld4<int>(hd.tile_info, 2 * tile + 1);
+    const int npts = ti.x, nsl = ti.y, slot_base = ti.z, ent_base = ti.w, nent = tj.x;
     const bool active = (s < B.nst) && npts > 0;
     if (active) {
       const bool valid = L < hd.n12;
       C x[N0];
-      load_line_inverse<T, N0>(B, hd.n12, s, L, valid, x);
-      uint32_t m[3] = {0, 0, 0};
-      int cnt = 0;
+      load_line_inverse<T, N0>(B, hd.n12, s, valid ? L : hd.n12 - 1, x);
       if (valid) {
-        const uint4 w = reinterpret_cast<const uint4 *>(hd.line_mask)[L];
-        m[0] = w.x;
-        m[1] = w.y;
-        m[2] = w.z;
-        cnt = (int)w.w;
-      }
+        const V4<uint32_t> w = ld4<uint32_t>(hd.line_mask, L);
+        const uint32_t m[3] = {w.x, w.y, w.z};
+        int cnt = (int)w.w;
 #pragma unroll
-      for (int i0 = 0; i0 < N0; ++i0) {
-        if ((m[i0 >> 5] >> (i0 & 31)) & 1u) {
-          lds[cnt] = x[i0];
-          ++cnt;
+        for (int i0 = 0; i0 < N0; ++i0) {
+          if ((m[i0 >> 5] >> (i0 & 31)) & 1u) {
+            pts[cnt] = x[i0];
+            ++cnt;
+          }
         }
+      }
+      for (int i = lane; i < nsl * kMaxNlm; i += 64) acc[i] = C{0, 0};
+    }
+    __syncthreads();
+    if (active) {
+      const int chunk = (nent + 63) >> 6;
+      int e = ent_base + lane * chunk;
+      const int eend = min(e + chunk, ent_base + nent);
+      int cur = -1;
+      C a0{0, 0}, a1{0, 0}, a2{0, 0}, a3{0, 0};
+      for (; e < eend; ++e) {
+        const V2<int> cs = ld2<int>(hd.ent_cs, e);
+        const V4<T> bt = ld4<T>(hd.ent_b, e);
+        if (cs.y != cur) {
+          if (cur >= 0) {
+            T *q = reinterpret_cast<T *>(acc + cur * kMaxNlm);
+            lds_add(q + 0, a0.x); lds_add(q + 1, a0.y); lds_add(q + 2, a1.x); lds_add(q + 3, a1.y);
+            lds_add(q + 4, a2.x); lds_add(q + 5, a2.y); lds_add(q + 6, a3.x); lds_add(q + 7, a3.y);
+          }
+          cur = cs.y;
+          a0 = a1 = a2 = a3 = C{0, 0};
+        }
+        C v = pts[cs.x];
+        if (!B.gamma) v = cmul(v, kb_phase<T>(B.kcov, ld4<float>(hd.ent_pos, e), (T)1));
+        a0.x += bt.x * v.x; a0.y += bt.x * v.y;
+        a1.x += bt.y * v.x; a1.y += bt.y * v.y;
+        a2.x += bt.z * v.x; a2.y += bt.z * v.y;
+        a3.x += bt.w * v.x; a3.y += bt.w * v.y;
+      }
+      if (cur >= 0) {
+        T *q = reinterpret_cast<T *>(acc + cur * kMaxNlm);
+        lds_add(q + 0, a0.x); lds_add(q + 1, a0.y); lds_add(q + 2, a1.x); lds_add(q + 3, a1.y);
+        lds_add(q + 4, a2.x); lds_add(q + 5, a2.y); lds_add(q + 6, a3.x); lds_add(q + 7, a3.y);
       }
     }
     __syncthreads();
     if (active) {
-      const int sl0 = hd.tile_info[4 * tile + 1], sl1 = hd.tile_info[4 * tile + 2];
-      for (int sl = sl0; sl < sl1; ++sl) {
-        const int a = hd.slot_atom[sl];
-        const int e0 = hd.slot_ent[sl], e1 = hd.slot_ent[sl + 1];
-        C acc[kMaxNlm];
-#pragma unroll
-        for (int l = 0; l < kMaxNlm; ++l) acc[l] = C{0, 0};
-        for (int e = e0 + lane; e < e1; e += 64) {
-          const int ap = hd.ent_ap[e];
-          C v = lds[hd.ent_c[e]];
-          if (!B.gamma) {
-            const float *ps = hd.pos + 3 * (size_t)ap;
-            const T arg = (T)(B.kcov[0] * (double)ps[0] + B.kcov[1] * (double)ps[1] + B.kcov[2] * (double)ps[2]);
-            T sn, cs;
-            sincos_t<T>(arg, &sn, &cs);
-            v = cmul(v, C{cs, sn});
-          }
-          const int p = ap - a * hd.maxsph;
-#pragma unroll
-          for (int l = 0; l < kMaxNlm; ++l) {
-            const T bt = beta[((size_t)a * kMaxNlm + l) * hd.maxsph + p];
-            acc[l].x += bt * v.x;
-            acc[l].y += bt * v.y;
-          }
-        }
-#pragma unroll
-        for (int l = 0; l < kMaxNlm; ++l) {
-#pragma unroll
-          for (int off = 32; off > 0; off >>= 1) {
-            acc[l].x += __shfl_xor(acc[l].x, off, 64);
-            acc[l].y += __shfl_xor(acc[l].y, off, 64);
-          }
-        }
-        if (lane == 0) {
-#pragma unroll
-          for (int l = 0; l < kMaxNlm; ++l) B.partial[((size_t)sl * kMaxNlm + l) * B.nst + s] = acc[l];
-        }
-      }
+      GAS C *part = gw(B.partial);
+      for (int i = lane; i < nsl * kMaxNlm; i += 64)
+        part[((size_t)(slot_base + i / kMaxNlm) * kMaxNlm + (i % kMaxNlm)) * B.nst + s] = acc[i];
     }
     __syncthreads();
   }
@@ -296,24 +364,26 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
   using C = cplx<T>;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
-    const SphereBlock<T> &B = blocks[b];
-    const int idx = (wg - prefix[b]) * blockDim.x + threadIdx.x;
+    const SphereBlock<T> B = blocks[b];
+    const int idx = (wg - gp(prefix)[b]) * blockDim.x + threadIdx.x;
     if (idx >= hd.natoms * B.nst) continue;
     const int a = idx / B.nst, s = idx - a * B.nst;
     double pr[kMaxNlm], pi[kMaxNlm];
 #pragma unroll
     for (int l = 0; l < kMaxNlm; ++l) pr[l] = pi[l] = 0.0;
-    const int q0 = hd.atom_slots[a], q1 = hd.atom_slots[a + 1];
+    const int q0 = gp(hd.atom_slots)[a], q1 = gp(hd.atom_slots)[a + 1];
+    const GAS C *part = gp(B.partial);
     for (int q = q0; q < q1; ++q) {
-      const int sl = hd.atom_slot_list[q];
+      const int sl = gp(hd.atom_slot_list)[q];
 #pragma unroll
       for (int l = 0; l < kMaxNlm; ++l) {
-        const C v = B.partial[((size_t)sl * kMaxNlm + l) * B.nst + s];
+        const C v = part[((size_t)sl * kMaxNlm + l) * B.nst + s];
         pr[l] += (double)v.x;
         pi[l] += (double)v.y;
       }
     }
-    const double *D = hd.coeff + (size_t)a * kMaxNlm * kMaxNlm;
+    const GAS double *D = gp(hd.coeff) + (size_t)a * kMaxNlm * kMaxNlm;
+    GAS C *dp = gw(B.dp);
 #pragma unroll
     for (int l = 0; l < kMaxNlm; ++l) {
       double dr = 0.0, di = 0.0;
@@ -322,90 +392,96 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
         dr += D[l * kMaxNlm + m] * pr[m];
         di += D[l * kMaxNlm + m] * pi[m];
       }
-      B.dp[((size_t)a * kMaxNlm + l) * B.nst + s] = C{(T)(dr * hd.kb_scale), (T)(di * hd.kb_scale)};
+      dp[((size_t)a * kMaxNlm + l) * B.nst + s] = C{(T)(dr * hd.kb_scale), (T)(di * hd.kb_scale)};
     }
   }
 }
 
+// Inverse FFT along i0, V(r)/N, KB term exp(-ik.pos) sum_l beta_l (D P)_l
+// added at the tile's sphere points through LDS, forward FFT along i0.
 template <typename T, int N0>
 __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                      int total, HamDev hd, int lds_per_wave) {
   using C = cplx<T>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  C *lds = reinterpret_cast<C *>(smem_raw) + wave * lds_per_wave;
-  const T *beta = reinterpret_cast<const T *>(hd.beta);
+  C *pts = reinterpret_cast<C *>(smem_raw) + wave * lds_per_wave;
+  C *dpl = pts + hd.maxpts;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
-    const SphereBlock<T> &B = blocks[b];
-    const int local = wg - prefix[b];
+    const SphereBlock<T> B = blocks[b];
+    const int local = wg - gp(prefix)[b];
     const int ngrp = (B.nst + nwave - 1) / nwave;
     const int tile = local / ngrp;
     const int s = (local - tile * ngrp) * nwave + wave;
     const int L = tile * kTile + lane;
-    const bool valid = (L < hd.n12) && (s < B.nst);
-    const int npts = hd.has_kb ? hd.tile_info[4 * tile + 0] : 0;
+    const bool valid = L < hd.n12;
+    const int Lc = valid ? L : hd.n12 - 1;
+    int npts = 0, nsl = 0, slot_base = 0, ent_base = 0, nent = 0;
+    if (hd.has_kb) {
+      const V4<int> ti = ld4<int>(hd.tile_info, 2 * tile);
+      const V4<int> tj = ld4<int>(hd.tile_info, 2 * tile + 1);
+      npts = ti.x;
+      nsl = ti.y;
+      slot_base = ti.z;
+      ent_base = ti.w;
+      nent = tj.x;
+    }
+    const bool act = s < B.nst;
     C x[N0];
-    if (s < B.nst) {
-      load_line_inverse<T, N0>(B, hd.n12, s, L, valid, x);
-      if (valid) {
-#pragma unroll
-        for (int i0 = 0; i0 < N0; ++i0) {
-          const T v = B.vloc[(size_t)i0 * hd.n12 + L];
-          x[i0].x *= v;
-          x[i0].y *= v;
-        }
-      }
+    if (act) {
       if (npts > 0) {
-        // KB values of the tile's sphere points, one point per lane
-        const int ptb = hd.tile_info[4 * tile + 3];
-        for (int c = lane; c < npts; c += 64) {
-          const int q0 = hd.pt_mem[ptb + c], q1 = hd.pt_mem[ptb + c + 1];
-          C val = C{0, 0};
-          for (int q = q0; q < q1; ++q) {
-            const int ap = hd.mem_ap[q];
-            const int a = ap / hd.maxsph, p = ap - a * hd.maxsph;
-            C t = C{0, 0};
+        const GAS C *dp = gp(B.dp);
+        for (int i = lane; i < nsl * kMaxNlm; i += 64)
+          dpl[i] = dp[((size_t)gp(hd.slot_atom)[slot_base + i / kMaxNlm] * kMaxNlm + (i % kMaxNlm)) * B.nst + s];
+        for (int i = lane; i < npts; i += 64) pts[i] = C{0, 0};
+      }
+      load_line_inverse<T, N0>(B, hd.n12, s, Lc, x);
+      const GAS T *vl = gp(B.vloc) + Lc;
 #pragma unroll
-            for (int l = 0; l < kMaxNlm; ++l) {
-              const T bt = beta[((size_t)a * kMaxNlm + l) * hd.maxsph + p];
-              const C d = B.dp[((size_t)a * kMaxNlm + l) * B.nst + s];
-              t.x += bt * d.x;
-              t.y += bt * d.y;
-            }
-            if (!B.gamma) {
-              const float *ps = hd.pos + 3 * (size_t)ap;
-              const T arg = (T)(-(B.kcov[0] * (double)ps[0] + B.kcov[1] * (double)ps[1] + B.kcov[2] * (double)ps[2]));
-              T sn, cs;
-              sincos_t<T>(arg, &sn, &cs);
-              t = cmul(t, C{cs, sn});
-            }
-            val = cadd(val, t);
-          }
-          lds[c] = val;
-        }
+      for (int i0 = 0; i0 < N0; ++i0) {
+        const T v = vl[(size_t)i0 * hd.n12];
+        x[i0].x *= v;
+        x[i0].y *= v;
       }
     }
     __syncthreads();
-    if (s < B.nst) {
+    if (act && npts > 0) {
+      for (int e = ent_base + lane; e < ent_base + nent; e += 64) {
+        const V2<int> cs = ld2<int>(hd.ent_cs, e);
+        const V4<T> bt = ld4<T>(hd.ent_b, e);
+        const C *d = dpl + cs.y * kMaxNlm;
+        C t;
+        t.x = bt.x * d[0].x + bt.y * d[1].x + bt.z * d[2].x + bt.w * d[3].x;
+        t.y = bt.x * d[0].y + bt.y * d[1].y + bt.z * d[2].y + bt.w * d[3].y;
+        if (!B.gamma) t = cmul(t, kb_phase<T>(B.kcov, ld4<float>(hd.ent_pos, e), (T)-1));
+        T *q = reinterpret_cast<T *>(pts + cs.x);
+        lds_add(q + 0, t.x);
+        lds_add(q + 1, t.y);
+      }
+    }
+    __syncthreads();
+    if (act) {
       if (npts > 0 && valid) {
-        const uint4 w = reinterpret_cast<const uint4 *>(hd.line_mask)[L];
+        const V4<uint32_t> w = ld4<uint32_t>(hd.line_mask, L);
         const uint32_t m[3] = {w.x, w.y, w.z};
         int cnt = (int)w.w;
 #pragma unroll
         for (int i0 = 0; i0 < N0; ++i0) {
           if ((m[i0 >> 5] >> (i0 & 31)) & 1u) {
-            x[i0] = cadd(x[i0], lds[cnt]);
+            x[i0] = cadd(x[i0], pts[cnt]);
             ++cnt;
           }
         }
       }
       fft_line<N0, -1, C>::run(x);
       if (valid) {
+        GAS C *base = gw(B.t1) + (size_t)s * hd.n12 + L;
+        const size_t sstride = (size_t)B.nst * hd.n12;
 #pragma unroll
         for (int i0 = 0; i0 < N0; ++i0) {
-          const int j = B.slot_of[i0];
-          if (j >= 0) B.t1[((size_t)j * B.nst + s) * hd.n12 + L] = x[i0];
+          const int j = sym_of<N0>(i0) - B.m_lo;
+          if ((unsigned)j < (unsigned)B.nslot) base[(size_t)j * sstride] = x[i0];
         }
       }
     }
@@ -443,7 +519,6 @@ void build_kb_tables(inqx_ham_s *h) {
   kb.maxsph = 1;
   for (auto &p : h->projs) kb.maxsph = std::max<int>(kb.maxsph, (int)p.npts());
   kb.ntile = (n12 + kTile - 1) / kTile;
-  // memberships (a, p) per grid point, grouped by tile / line / i0
   struct Mem {
     int tile, lane, i0, a, p;
   };
@@ -456,23 +531,17 @@ void build_kb_tables(inqx_ham_s *h) {
       mem.push_back(Mem{L / kTile, L % kTile, c[0], a, (int)ip});
     }
   }
-  // unique points in each tile sorted by (lane, i0): compact index
+  // unique points of each tile sorted by (lane, i0) give the compact index
   std::vector<uint32_t> mask((size_t)n12 * 4, 0u);
-  std::vector<std::vector<std::pair<int, int>>> tile_pts(kb.ntile);  // (lane, i0)
+  std::vector<std::vector<std::pair<int, int>>> tile_pts(kb.ntile);
   for (auto &m : mem) tile_pts[m.tile].push_back({m.lane, m.i0});
-  std::vector<int> tile_info((size_t)kb.ntile * 4, 0);
-  std::vector<int> pt_mem, mem_ap;
-  std::vector<std::vector<int>> cidx_of(kb.ntile);
+  std::vector<int> tile_info((size_t)kb.ntile * 8, 0);
   kb.maxpts = 1;
-  long ptbase = 0;
-  std::vector<int> ptcount(kb.ntile);
   for (int t = 0; t < kb.ntile; ++t) {
     auto &v = tile_pts[t];
     std::sort(v.begin(), v.end());
     v.erase(std::unique(v.begin(), v.end()), v.end());
-    ptcount[t] = (int)v.size();
     kb.maxpts = std::max(kb.maxpts, (int)v.size());
-    // line bitmasks and bases
     int c = 0;
     for (size_t k = 0; k < v.size(); ++k) {
       const int L = t * kTile + v[k].first;
@@ -480,71 +549,58 @@ void build_kb_tables(inqx_ham_s *h) {
       mask[(size_t)L * 4 + (v[k].second >> 5)] |= 1u << (v[k].second & 31);
       ++c;
     }
-    tile_info[4 * t + 0] = (int)v.size();
-    tile_info[4 * t + 3] = (int)ptbase;
-    ptbase += (long)v.size();
+    tile_info[8 * t + 0] = (int)v.size();
   }
   auto compact = [&](int t, int lane, int i0) {
     auto &v = tile_pts[t];
     return (int)(std::lower_bound(v.begin(), v.end(), std::make_pair(lane, i0)) - v.begin());
   };
-  // K4 memberships per unique point, in projector order
-  std::vector<std::vector<int>> per_pt(ptbase);
-  for (auto &m : mem) {
-    long gi = tile_info[4 * m.tile + 3] + compact(m.tile, m.lane, m.i0);
-    per_pt[gi].push_back(m.a * kb.maxsph + m.p);
-  }
-  pt_mem.push_back(0);
-  for (auto &v : per_pt) {
-    for (int x : v) mem_ap.push_back(x);
-    pt_mem.push_back((int)mem_ap.size());
-  }
-  // K2 slots: (tile, atom) pairs with their entries
-  std::vector<Mem> sorted = mem;
-  std::stable_sort(sorted.begin(), sorted.end(), [](const Mem &x, const Mem &y) {
+  // entries sorted by (tile, atom, point) -> slots = (tile, atom) runs
+  std::stable_sort(mem.begin(), mem.end(), [](const Mem &x, const Mem &y) {
     if (x.tile != y.tile) return x.tile < y.tile;
     return x.a < y.a;
   });
-  std::vector<int> slot_atom, slot_ent, ent_c, ent_ap;
+  std::vector<int> slot_atom, ent_cs;
+  std::vector<float> eb32, epos;
+  std::vector<double> eb64;
   std::vector<std::vector<int>> atom_slot(kb.natoms);
-  for (size_t k = 0; k < sorted.size(); ++k) {
-    const Mem &m = sorted[k];
-    if (k == 0 || m.tile != sorted[k - 1].tile || m.a != sorted[k - 1].a) {
-      if (k == 0 || m.tile != sorted[k - 1].tile) tile_info[4 * m.tile + 1] = (int)slot_atom.size();
+  kb.maxslot = 1;
+  for (size_t k = 0; k < mem.size(); ++k) {
+    const Mem &m = mem[k];
+    const bool new_tile = (k == 0 || m.tile != mem[k - 1].tile);
+    if (new_tile) {
+      tile_info[8 * m.tile + 2] = (int)slot_atom.size();
+      tile_info[8 * m.tile + 3] = (int)k;
+    }
+    if (new_tile || m.a != mem[k - 1].a) {
       atom_slot[m.a].push_back((int)slot_atom.size());
       slot_atom.push_back(m.a);
-      slot_ent.push_back((int)ent_c.size());
     }
-    ent_c.push_back(compact(m.tile, m.lane, m.i0));
-    ent_ap.push_back(m.a * kb.maxsph + m.p);
-    tile_info[4 * m.tile + 2] = (int)slot_atom.size();
+    tile_info[8 * m.tile + 1] = (int)slot_atom.size() - tile_info[8 * m.tile + 2];
+    tile_info[8 * m.tile + 4] = (int)k + 1 - tile_info[8 * m.tile + 3];
+    kb.maxslot = std::max(kb.maxslot, tile_info[8 * m.tile + 1]);
+    ent_cs.push_back(compact(m.tile, m.lane, m.i0));
+    ent_cs.push_back((int)slot_atom.size() - 1 - tile_info[8 * m.tile + 2]);
+    const Projector &P = h->projs[m.a];
+    for (int l = 0; l < kMaxNlm; ++l) {
+      const double v = l < P.nlm ? P.matrix[(size_t)l * P.npts() + m.p] : 0.0;
+      eb64.push_back(v);
+      eb32.push_back((float)v);
+    }
+    for (int d = 0; d < 3; ++d) epos.push_back(P.relpos[3 * (size_t)m.p + d]);
+    epos.push_back(0.0f);
   }
-  slot_ent.push_back((int)ent_c.size());
   kb.nslot = (int)slot_atom.size();
-  kb.nent = (long)ent_c.size();
-  kb.npt = ptbase;
-  kb.nmem = (long)mem_ap.size();
+  kb.nent = (long)mem.size();
   std::vector<int> atom_slots(kb.natoms + 1, 0), atom_slot_list;
   for (int a = 0; a < kb.natoms; ++a) {
     atom_slots[a] = (int)atom_slot_list.size();
     for (int x : atom_slot[a]) atom_slot_list.push_back(x);
   }
   atom_slots[kb.natoms] = (int)atom_slot_list.size();
-  // per-atom tables
-  std::vector<float> pos((size_t)kb.natoms * kb.maxsph * 3, 0.0f);
-  std::vector<double> b64((size_t)kb.natoms * kMaxNlm * kb.maxsph, 0.0);
-  std::vector<float> b32(b64.size(), 0.0f);
-  std::vector<double> coeff((size_t)kb.natoms * kMaxNlm * kMaxNlm, 0.0);
+  std::vector<double> coeff((size_t)std::max(1, kb.natoms) * kMaxNlm * kMaxNlm, 0.0);
   for (int a = 0; a < kb.natoms; ++a) {
     const Projector &P = h->projs[a];
-    for (long ip = 0; ip < P.npts(); ++ip)
-      for (int d = 0; d < 3; ++d) pos[((size_t)a * kb.maxsph + ip) * 3 + d] = P.relpos[3 * ip + d];
-    for (int l = 0; l < P.nlm; ++l)
-      for (long ip = 0; ip < P.npts(); ++ip) {
-        double v = P.matrix[(size_t)l * P.npts() + ip];
-        b64[((size_t)a * kMaxNlm + l) * kb.maxsph + ip] = v;
-        b32[((size_t)a * kMaxNlm + l) * kb.maxsph + ip] = (float)v;
-      }
     for (int l = 0; l < P.nlm; ++l)
       for (int m = 0; m < P.nlm; ++m) coeff[((size_t)a * kMaxNlm + l) * kMaxNlm + m] = P.coeff[l * P.nlm + m];
   }
@@ -555,16 +611,12 @@ void build_kb_tables(inqx_ham_s *h) {
   upload(kb.line_mask, mask, st);
   upload(kb.tile_info, tile_info, st);
   upload(kb.slot_atom, slot_atom, st);
-  upload(kb.slot_ent, slot_ent, st);
-  upload(kb.ent_c, ent_c, st);
-  upload(kb.ent_ap, ent_ap, st);
+  upload(kb.ent_cs, ent_cs, st);
+  upload(kb.ent_b32, eb32, st);
+  upload(kb.ent_b64, eb64, st);
+  upload(kb.ent_pos, epos, st);
   upload(kb.atom_slots, atom_slots, st);
   upload(kb.atom_slot_list, atom_slot_list, st);
-  upload(kb.pt_mem, pt_mem, st);
-  upload(kb.mem_ap, mem_ap, st);
-  upload(kb.pos, pos, st);
-  upload(kb.beta64, b64, st);
-  upload(kb.beta32, b32, st);
   upload(kb.coeff, coeff, st);
   INQX_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
   kb.built = true;
@@ -573,10 +625,7 @@ void build_kb_tables(inqx_ham_s *h) {
 void ensure_basis_device(inqx_ham_s *h, Basis &b) {
   hipStream_t st = h->ctx->c.stream;
   if (!b.d_pw_yz.p) {
-    upload(b.d_slot_i0, b.slot_i0, st);
-    upload(b.d_slot_of, b.slot_of, st);
-    upload(b.d_seg_off, b.seg_off, st);
-    upload(b.d_row_off, b.row_off, st);
+    upload(b.d_slot_seg, b.slot_seg, st);
     upload(b.d_rows, b.rows, st);
     upload(b.d_pw_yz, b.pw_yz, st);
     INQX_HIP(hipStreamSynchronize(st));
@@ -610,7 +659,7 @@ void ensure_workspace(inqx_ham_s *h, Basis &b, int nst, int prec) {
     return;
   const int n12 = h->g.n[1] * h->g.n[2];
   const int ns = std::max(nst, b.ws_nst[prec]);
-  b.t1[prec].ensure(cs * (size_t)b.slot_i0.size() * ns * n12);
+  b.t1[prec].ensure(cs * (size_t)std::max(1, b.nslot) * ns * n12);
   b.partial[prec].ensure(cs * (size_t)std::max(1, h->kb.nslot) * kMaxNlm * ns);
   b.dp[prec].ensure(cs * (size_t)std::max(1, h->kb.natoms) * kMaxNlm * ns);
   b.ws_nst[prec] = ns;
@@ -761,7 +810,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     }
     // waves (states) per x-pass workgroup within the LDS budget
     const int cs = (int)sizeof(cplx<T>);
-    ld->lds_per_wave = h->kb.maxpts;
+    ld->lds_per_wave = h->kb.maxpts + kMaxNlm * h->kb.maxslot;
     int nw = 4;
     while (nw > 1 && nw * ld->lds_per_wave * cs > 160 * 1024) --nw;
     INQX_CHECK(ld->lds_per_wave * cs <= 160 * 1024, INQX_ERR_UNSUPPORTED, "KB tile too dense for LDS");
@@ -775,10 +824,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
       d.t1 = B.t1[prec].as<cplx<T>>();
       d.partial = B.partial[prec].as<cplx<T>>();
       d.dp = B.dp[prec].as<cplx<T>>();
-      d.slot_i0 = B.d_slot_i0.as<int>();
-      d.slot_of = B.d_slot_of.as<int>();
-      d.seg_off = B.d_seg_off.as<int>();
-      d.row_off = B.d_row_off.as<int>();
+      d.slot_seg = B.d_slot_seg.as<int>();
       d.rows = B.d_rows.as<int>();
       d.pw_yz = B.d_pw_yz.as<int>();
       d.kin = prec == INQX_FP64 ? (const T *)B.d_kin64.p : (const T *)B.d_kin32.p;
@@ -786,7 +832,9 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
       d.vloc = reinterpret_cast<const T *>(vm.as<char>() + (size_t)B.spin * g.N * sizeof(T));
       d.nst = nst[i];
       d.npw = (int)B.npw;
-      d.nslot = (int)B.slot_i0.size();
+      d.nslot = B.nslot;
+      d.m_lo = B.m_lo;
+      d.pad_ = 0;
       for (int k = 0; k < 3; ++k) d.kcov[k] = B.kcov[k] + Afull[k];
       d.gamma = (d.kcov[0] == 0.0 && d.kcov[1] == 0.0 && d.kcov[2] == 0.0) ? 1 : 0;
       pp[i + 1] = pp[i] + d.nslot * ((nst[i] + plane_p - 1) / plane_p);
@@ -820,21 +868,17 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   hd.n12 = g.n[1] * g.n[2];
   hd.N = g.N;
   hd.natoms = h->kb.natoms;
-  hd.maxsph = h->kb.maxsph;
   hd.has_kb = has_kb ? 1 : 0;
   hd.ntile = h->kb.ntile;
+  hd.maxpts = h->kb.maxpts;
   hd.line_mask = h->kb.line_mask.as<uint32_t>();
   hd.tile_info = h->kb.tile_info.as<int>();
   hd.slot_atom = h->kb.slot_atom.as<int>();
-  hd.slot_ent = h->kb.slot_ent.as<int>();
-  hd.ent_c = h->kb.ent_c.as<int>();
-  hd.ent_ap = h->kb.ent_ap.as<int>();
+  hd.ent_cs = h->kb.ent_cs.as<int>();
+  hd.ent_b = prec == INQX_FP64 ? h->kb.ent_b64.p : h->kb.ent_b32.p;
+  hd.ent_pos = h->kb.ent_pos.as<float>();
   hd.atom_slots = h->kb.atom_slots.as<int>();
   hd.atom_slot_list = h->kb.atom_slot_list.as<int>();
-  hd.pt_mem = h->kb.pt_mem.as<int>();
-  hd.mem_ap = h->kb.mem_ap.as<int>();
-  hd.pos = h->kb.pos.as<float>();
-  hd.beta = prec == INQX_FP64 ? h->kb.beta64.p : h->kb.beta32.p;
   hd.coeff = h->kb.coeff.as<double>();
   hd.kb_scale = (g.vol / (double)g.N) / (double)g.N;
   switch (g.n[0] * 1000 + g.n[1]) {

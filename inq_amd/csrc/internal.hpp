@@ -135,46 +135,47 @@ constexpr int kMaxNlm = 4;   // s + 3 p per atom, padded
 constexpr int kTile = 64;    // x-pencils (lines) per tile: one wavefront
 
 // tables of the real-space KB pass, shared by all blocks of a Hamiltonian
+// A tile is 64 consecutive lines L = i1*n2 + i2.  Every (atom, sphere point)
+// pair becomes one entry of the tile that holds the point; a slot is one
+// (tile, atom) pair.  Entries carry everything a lane needs (compact point
+// index, tile-local slot, projector values, relative position), so the
+// x-pass kernels read them with independent, coalesced loads.
 struct KBTables {
   bool built = false;
-  int natoms = 0, maxsph = 0, ntile = 0, nslot = 0, maxpts = 0;
-  long nent = 0, npt = 0, nmem = 0;
-  // per line L = i1*n2 + i2: bitmask of sphere points along i0 (3 words) and
-  // the line's first compact index inside its tile
-  DevMem line_mask;   // [n12][4] uint32: m0, m1, m2, base
-  DevMem tile_info;   // [ntile][4] int: npts, slot0, slot1, pt_base
-  DevMem slot_atom;   // [nslot]
-  DevMem slot_ent;    // [nslot+1]
-  DevMem ent_c;       // [nent] compact point index inside the tile
-  DevMem ent_ap;      // [nent] a*maxsph + p
-  DevMem atom_slots;  // [natoms+1] offsets followed by slot ids
+  int natoms = 0, maxsph = 0, ntile = 0, nslot = 0, maxpts = 0, maxslot = 0;
+  long nent = 0;
+  // per line: bitmask of sphere points along i0 (3 words) and the line's
+  // first compact point index inside its tile
+  DevMem line_mask;   // [n12] uint4: m0, m1, m2, base
+  DevMem tile_info;   // [ntile] int4: npts, nslot_t, slot_base, ent_base ; [ntile] ent count in a 2nd int4
+  DevMem slot_atom;   // [nslot] atom of each global slot
+  DevMem ent_cs;      // [nent] int2: compact point index, tile-local slot
+  DevMem ent_b32;     // [nent] float4: projector values l = 0..3
+  DevMem ent_b64;     // [nent] double4
+  DevMem ent_pos;     // [nent] float4: contravariant relative position (float, as the reference stores it)
+  DevMem atom_slots;  // [natoms+1] offsets into atom_slot_list
   DevMem atom_slot_list;
-  DevMem pt_mem;      // [npt+1] membership offsets
-  DevMem mem_ap;      // [nmem]
-  DevMem pos;         // [natoms][maxsph][3] float
-  DevMem beta64, beta32;  // [natoms][kMaxNlm][maxsph]
   DevMem coeff;       // [natoms][kMaxNlm][kMaxNlm] double
 };
 
 // ------------------------------------------------------------------ device block descriptor
 template <typename T> struct SphereBlock {
   using C = cplx<T>;
-  const C *psi;       // [nst][npw]
-  C *hpsi;            // [nst][npw]
-  C *t1;              // [nslot][nst][n1*n2] plane intermediates (in place for the x pass)
-  C *partial;         // [kb nslot][kMaxNlm][nst]
-  C *dp;              // [natoms][kMaxNlm][nst]
-  const int *slot_i0; // [nslot] plane FFT index of each slot
-  const int *slot_of; // [n0] slot of each plane index or -1
-  const int *seg_off; // [nslot+1] sphere segment of each plane
-  const int *row_off; // [nslot+1]
-  const int *rows;    // rows (i1) holding sphere points, per slot
-  const int *pw_yz;   // [npw] i1*n2 + i2
-  const T *kin;       // [npw] 1/2 |G+k|^2
-  const T *vloc;      // [N] V/N of the block's spin
-  int nst, npw, nslot;
-  int gamma;          // k+A == 0: no KB phases
-  double kcov[3];     // k + A, covariant
+  const C *psi;        // [nst][npw]
+  C *hpsi;             // [nst][npw]
+  C *t1;               // [nslot][nst][n1*n2] plane intermediates (in place for the x pass)
+  C *partial;          // [kb nslot][kMaxNlm][nst]
+  C *dp;               // [natoms][kMaxNlm][nst]
+  const int *slot_seg; // [nslot] int4: sphere segment start, length, row list start, row count
+  const int *rows;     // rows (i1) holding sphere points, per slot
+  const int *pw_yz;    // [npw] i1*n2 + i2
+  const T *kin;        // [npw] 1/2 |G+k|^2
+  const T *vloc;       // [N] V/N of the block's spin
+  int nst, npw;
+  int nslot, m_lo;     // slot j holds plane index m = m_lo + j (symmetric range)
+  int gamma;           // k+A == 0: no KB phases
+  int pad_;
+  double kcov[3];      // k + A, covariant
 };
 
 // one G-sphere block
@@ -185,9 +186,11 @@ struct Basis {
   int spin;
   long npw = 0;
   std::vector<int> gidx;      // flattened FFT index
-  std::vector<int> slot_i0, slot_of, seg_off, row_off, rows, pw_yz;
-  std::vector<double> kin;    // 1/2|G+k|^2 (without A)
-  DevMem d_slot_i0, d_slot_of, d_seg_off, d_row_off, d_rows, d_pw_yz, d_kin64, d_kin32;
+  // plane slots: slot j <-> symmetric plane index m_lo + j, contiguous range
+  int m_lo = 0, nslot = 0;
+  std::vector<int> slot_seg;  // [nslot][4] segment start, length, row start, row count
+  std::vector<int> rows, pw_yz;
+  DevMem d_slot_seg, d_rows, d_pw_yz, d_kin64, d_kin32;
   double kin_A[3] = {1e300, 0, 0};  // A for which d_kin* were built
   // workspaces per precision
   DevMem t1[2], partial[2], dp[2];
