@@ -129,6 +129,33 @@ int inqx_ham_apply(inqx_ham ham, inqx_basis basis, int nst, int precision, const
 int inqx_ham_apply_batch(inqx_ham ham, int nblocks, const inqx_basis *bases, const int *nst, int precision,
                          const void *const *psi, void *const *hpsi);
 
+/* ------------------------------------------------------------ overlap / orthogonalisation */
+/* All matrices are device arrays of complex128, row-major.  States are
+ * band-major device arrays [n][npw] of the given precision.  vol_el is the
+ * volume element of the basis (V/N^2 on the G sphere,
+ * src/basis/fourier_space.hpp:48-50). */
+/* operations::overlap (src/operations/overlap.hpp:26-95): S = vol_el phi1^H phi2 */
+int inqx_overlap(inqx_ctx ctx, int precision, int n1, int n2, long npw, double vol_el, const void *phi1,
+                 const void *phi2, void *S);
+/* matrix::cholesky (src/matrix/cholesky.hpp:28-100): in place S = L L^H, L lower,
+ * upper part zeroed; INQX_ERR_NOT_PD when a leading minor is not positive
+ * (the reference prints a warning or aborts, cholesky.hpp:77-82) */
+int inqx_cholesky(inqx_ctx ctx, int n, void *S);
+/* matrix::invert_triangular (src/matrix/invert_triangular.hpp:20-50): Linv = L^{-1} */
+int inqx_invert_triangular(inqx_ctx ctx, int n, const void *L, void *Linv);
+/* operations::rotate, 2-argument form (src/operations/rotate.hpp:26-80): phi <- phi rot^H */
+int inqx_rotate(inqx_ctx ctx, int precision, int n, long npw, const void *rot, void *phi);
+/* operations::rotate, 5-argument form (rotate.hpp:84-140): rotphi <- alpha phi rot + beta rotphi */
+int inqx_rotate_add(inqx_ctx ctx, int precision, int n, long npw, const void *rot, const void *phi, void *rotphi,
+                    double alpha_re, double alpha_im, double beta_re, double beta_im);
+/* operations::orthogonalize (src/operations/orthogonalize.hpp:25-32): overlap ->
+ * Cholesky -> rotate_trs, in place */
+int inqx_orthogonalize(inqx_ctx ctx, int precision, int n, long npw, double vol_el, void *phi);
+/* ground_state::subspace_diagonalization (src/ground_state/subspace_diagonalization.hpp:21-29):
+ * H_sub = phi^H H phi, FP64 eigendecomposition (eigenvalues ascending, host
+ * array), phi <- phi U */
+int inqx_subspace_diag(inqx_ham ham, inqx_basis basis, int nst, int precision, void *phi, double *eigenvalues);
+
 /* ------------------------------------------------------------ sphere <-> grids */
 /* operations::transform::to_real (transform.hpp:220-249) of sphere states into
  * the reference real-space layout [N][nst] (normalised by 1/N) */

@@ -53,6 +53,14 @@ SIGNATURES = {
     "inqx_ham_apply": (C.c_int, [P, P, C.c_int, C.c_int, P, P]),
     "inqx_ham_apply_batch": (C.c_int, [P, C.c_int, PP, IP, C.c_int, PP, PP]),
     "inqx_sphere_to_real": (C.c_int, [P, P, C.c_int, C.c_int, P, P]),
+    "inqx_overlap": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_long, C.c_double, P, P, P]),
+    "inqx_cholesky": (C.c_int, [P, C.c_int, P]),
+    "inqx_invert_triangular": (C.c_int, [P, C.c_int, P, P]),
+    "inqx_rotate": (C.c_int, [P, C.c_int, C.c_int, C.c_long, P, P]),
+    "inqx_rotate_add": (C.c_int, [P, C.c_int, C.c_int, C.c_long, P, P, P, C.c_double, C.c_double, C.c_double,
+                                  C.c_double]),
+    "inqx_orthogonalize": (C.c_int, [P, C.c_int, C.c_int, C.c_long, C.c_double, P]),
+    "inqx_subspace_diag": (C.c_int, [P, P, C.c_int, C.c_int, P, DP]),
 }
 
 

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <array>
+#include <complex>
 #include <cmath>
 #include <cstring>
 
@@ -219,6 +220,68 @@ static void build_basis(const Grid &g, Basis &b) {
       if (b.rows.size() == (size_t)b.slot_seg[4 * sl + 2] || b.rows.back() != i1) b.rows.push_back(i1);
     }
     b.slot_seg[4 * sl + 3] = (int)b.rows.size() - b.slot_seg[4 * sl + 2];
+  }
+}
+
+// Hermitian eigensolver for the subspace step, FP64 on the host: cyclic
+// Jacobi sweeps with a threshold (skip rotations whose off-diagonal entry is
+// negligible against the diagonal pair), eigenvalues sorted ascending.  On
+// exit u[k*n + j] holds component j of the k-th eigenvector.
+static void hermitian_eig(int n, std::vector<std::complex<double>> a, std::vector<double> &w,
+                          std::vector<std::complex<double>> &u) {
+  using cd = std::complex<double>;
+  std::vector<cd> v((size_t)n * n, cd(0.0, 0.0));
+  for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
+  auto A = [&](int i, int j) -> cd & { return a[(size_t)i * n + j]; };
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const double m = std::norm(A(i, j));
+        tot += m;
+        if (i != j) off += m;
+      }
+    if (off <= 1e-30 * tot) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const cd apq = A(p, q);
+        const double mag = std::abs(apq);
+        const double app = A(p, p).real(), aqq = A(q, q).real();
+        if (mag <= 1e-300 || mag < 1e-18 * (std::fabs(app) + std::fabs(aqq))) continue;
+        // real rotation on the phase-aligned pair: tan(2t) = 2|apq| / (aqq - app)
+        const double tau = (aqq - app) / (2.0 * mag);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+        const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c;
+        const cd ph = apq / mag;
+        // columns p, q of J: p' = c e_p - sn conj(ph) e_q, q' = sn ph e_p + c e_q
+        const cd jqp = -sn * std::conj(ph), jpq = sn * ph;
+        for (int k = 0; k < n; ++k) {
+          const cd akp = A(k, p), akq = A(k, q);
+          A(k, p) = akp * c + akq * jqp;
+          A(k, q) = akp * jpq + akq * c;
+        }
+        for (int k = 0; k < n; ++k) {
+          const cd apk = A(p, k), aqk = A(q, k);
+          A(p, k) = c * apk + std::conj(jqp) * aqk;
+          A(q, k) = std::conj(jpq) * apk + c * aqk;
+        }
+        A(p, q) = 0.0;
+        A(q, p) = 0.0;
+        for (int k = 0; k < n; ++k) {
+          const cd vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+          v[(size_t)k * n + p] = vkp * c + vkq * jqp;
+          v[(size_t)k * n + q] = vkp * jpq + vkq * c;
+        }
+      }
+  }
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return A(x, x).real() < A(y, y).real(); });
+  w.resize(n);
+  u.assign((size_t)n * n, cd(0.0, 0.0));
+  for (int k = 0; k < n; ++k) {
+    w[k] = A(ord[k], ord[k]).real();
+    for (int j = 0; j < n; ++j) u[(size_t)k * n + j] = v[(size_t)j * n + ord[k]];
   }
 }
 
@@ -571,6 +634,122 @@ int inqx_sphere_to_real(inqx_ham h, inqx_basis bs, int nst, int precision, const
   (void)psi_rs;
   if (h) h->ctx->c.err = "inqx_sphere_to_real: not built yet";
   return INQX_ERR_UNSUPPORTED;
+}
+
+
+// ---------------------------------------------------------------- overlap / orthogonalisation
+static void check_states(inqx_ctx ctx, int precision, int n, long npw) {
+  INQX_CHECK(ctx && !ctx->c.host_only, INQX_ERR_STATE, "needs a device context");
+  INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
+  INQX_CHECK(n > 0 && npw > 0, INQX_ERR_ARG, "bad sizes");
+}
+
+int inqx_overlap(inqx_ctx ctx, int precision, int n1, int n2, long npw, double vol_el, const void *phi1,
+                 const void *phi2, void *S) {
+  API_BEGIN
+  check_states(ctx, precision, n1, npw);
+  INQX_CHECK(n2 > 0 && phi1 && phi2 && S, INQX_ERR_ARG, "null argument");
+  launch_overlap(ctx->c, precision, n1, n2, npw, phi1, phi2, vol_el, (double *)S, ctx->c.ws_partial);
+  API_END(ctx)
+}
+
+static int cholesky_impl(inqx_ctx ctx, int n, double *S) {
+  ctx->c.ws_info.ensure(sizeof(int));
+  launch_cholesky(ctx->c, n, S, ctx->c.ws_info.as<int>());
+  int info = 0;
+  INQX_HIP(hipMemcpyAsync(&info, ctx->c.ws_info.p, sizeof(int), hipMemcpyDeviceToHost, ctx->c.stream));
+  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+  return info;
+}
+
+int inqx_cholesky(inqx_ctx ctx, int n, void *S) {
+  API_BEGIN
+  INQX_CHECK(ctx && !ctx->c.host_only && S && n > 0, INQX_ERR_ARG, "bad arguments");
+  const int info = cholesky_impl(ctx, n, (double *)S);
+  INQX_CHECK(info == 0, INQX_ERR_NOT_PD, "Cholesky: leading minor " + std::to_string(info) + " is not positive definite");
+  API_END(ctx)
+}
+
+int inqx_invert_triangular(inqx_ctx ctx, int n, const void *L, void *Linv) {
+  API_BEGIN
+  INQX_CHECK(ctx && !ctx->c.host_only && L && Linv && n > 0 && L != Linv, INQX_ERR_ARG, "bad arguments");
+  launch_trinv(ctx->c, n, (const double *)L, (double *)Linv);
+  API_END(ctx)
+}
+
+// phi <- M phi through a temporary (rows of the output depend on all rows of phi)
+static void rotate_inplace(inqx_ctx ctx, int precision, int n, long npw, const double *R, int mode, void *phi) {
+  const size_t cs = precision == INQX_FP64 ? 16 : 8;
+  ctx->c.ws_tmp.ensure(cs * (size_t)n * npw);
+  launch_rotate(ctx->c, precision, n, npw, R, mode, 1.0, 0.0, phi, ctx->c.ws_tmp.p, 0.0, 0.0, ctx->c.ws_ops);
+  INQX_HIP(hipMemcpyAsync(phi, ctx->c.ws_tmp.p, cs * (size_t)n * npw, hipMemcpyDeviceToDevice, ctx->c.stream));
+}
+
+int inqx_rotate(inqx_ctx ctx, int precision, int n, long npw, const void *rot, void *phi) {
+  API_BEGIN
+  check_states(ctx, precision, n, npw);
+  INQX_CHECK(rot && phi, INQX_ERR_ARG, "null argument");
+  rotate_inplace(ctx, precision, n, npw, (const double *)rot, 0, phi);
+  API_END(ctx)
+}
+
+int inqx_rotate_add(inqx_ctx ctx, int precision, int n, long npw, const void *rot, const void *phi, void *rotphi,
+                    double alpha_re, double alpha_im, double beta_re, double beta_im) {
+  API_BEGIN
+  check_states(ctx, precision, n, npw);
+  INQX_CHECK(rot && phi && rotphi && phi != rotphi, INQX_ERR_ARG, "bad arguments");
+  launch_rotate(ctx->c, precision, n, npw, (const double *)rot, 1, alpha_re, alpha_im, phi, rotphi, beta_re, beta_im,
+                ctx->c.ws_ops);
+  API_END(ctx)
+}
+
+int inqx_orthogonalize(inqx_ctx ctx, int precision, int n, long npw, double vol_el, void *phi) {
+  API_BEGIN
+  check_states(ctx, precision, n, npw);
+  INQX_CHECK(phi, INQX_ERR_ARG, "null argument");
+  ctx->c.ws_S.ensure(16 * (size_t)n * n);
+  ctx->c.ws_X.ensure(16 * (size_t)n * n);
+  double *S = ctx->c.ws_S.as<double>(), *X = ctx->c.ws_X.as<double>();
+  launch_overlap(ctx->c, precision, n, n, npw, phi, phi, vol_el, S, ctx->c.ws_partial);
+  const int info = cholesky_impl(ctx, n, S);
+  INQX_CHECK(info == 0, INQX_ERR_NOT_PD, "orthogonalize: overlap not positive definite (minor " + std::to_string(info) + ")");
+  launch_trinv(ctx->c, n, S, X);
+  rotate_inplace(ctx, precision, n, npw, X, 0, phi);  // phi <- phi Linv^H
+  API_END(ctx)
+}
+
+int inqx_subspace_diag(inqx_ham h, inqx_basis bs, int nst, int precision, void *phi, double *eigenvalues) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && bs && phi && eigenvalues && nst > 0, INQX_ERR_ARG, "null argument");
+  check_states(ctx, precision, nst, bs->b.npw);
+  const long npw = bs->b.npw;
+  const size_t cs = precision == INQX_FP64 ? 16 : 8;
+  ctx->c.ws_tmp2.ensure(cs * (size_t)nst * npw);
+  ctx->c.ws_S.ensure(16 * (size_t)nst * nst);
+  void *hphi = ctx->c.ws_tmp2.p;
+  Basis *bb = &bs->b;
+  const void *pin = phi;
+  void *pout = hphi;
+  if (precision == INQX_FP64) launch_apply<double>(h, 1, &bb, &nst, &pin, &pout);
+  else launch_apply<float>(h, 1, &bb, &nst, &pin, &pout);
+  const double vol = h->g.vol / ((double)h->g.N * (double)h->g.N);
+  launch_overlap(ctx->c, precision, nst, nst, npw, phi, hphi, vol, ctx->c.ws_S.as<double>(), ctx->c.ws_partial);
+  std::vector<std::complex<double>> m((size_t)nst * nst);
+  INQX_HIP(hipMemcpyAsync(m.data(), ctx->c.ws_S.p, 16 * (size_t)nst * nst, hipMemcpyDeviceToHost, ctx->c.stream));
+  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+  std::vector<double> w;
+  std::vector<std::complex<double>> u;
+  hermitian_eig(nst, m, w, u);
+  // rotation rows: phi_k <- sum_j u_k[j] phi_j  (mode 2: M = R)
+  INQX_HIP(hipMemcpyAsync(ctx->c.ws_S.p, u.data(), 16 * (size_t)nst * nst, hipMemcpyHostToDevice, ctx->c.stream));
+  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+  ctx->c.ws_tmp.ensure(cs * (size_t)nst * npw);
+  launch_rotate(ctx->c, precision, nst, npw, ctx->c.ws_S.as<double>(), 2, 1.0, 0.0, phi, ctx->c.ws_tmp.p, 0.0, 0.0,
+                ctx->c.ws_ops);
+  INQX_HIP(hipMemcpyAsync(phi, ctx->c.ws_tmp.p, cs * (size_t)nst * npw, hipMemcpyDeviceToDevice, ctx->c.stream));
+  for (int k = 0; k < nst; ++k) eigenvalues[k] = w[k];
+  API_END(ctx)
 }
 
 }  // extern "C"

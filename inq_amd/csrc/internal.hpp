@@ -93,6 +93,8 @@ struct Ctx {
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, KStat> stats;
+  // workspaces of the linear-algebra entry points
+  DevMem ws_partial, ws_ops, ws_tmp, ws_tmp2, ws_S, ws_X, ws_info;
   hipEvent_t get_event();
   void flush_prof();
 };
@@ -231,4 +233,11 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
 void ensure_basis_device(inqx_ham_s *h, Basis &b);
 void ensure_workspace(inqx_ham_s *h, Basis &b, int nst, int prec);
 bool grid_supported(const int *n);
+// linalg.hip
+void launch_overlap(Ctx &cx, int prec, int n1, int n2, long K, const void *A, const void *B, double vol, double *S,
+                    DevMem &partial);
+void launch_rotate(Ctx &cx, int prec, int n, long K, const double *R, int mode, double ar, double ai, const void *Cm,
+                   void *Out, double br, double bi, DevMem &ops);
+void launch_cholesky(Ctx &cx, int n, double *S, int *info);
+void launch_trinv(Ctx &cx, int n, const double *L, double *X);
 }  // namespace inqx

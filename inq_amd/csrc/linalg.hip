@@ -1,0 +1,427 @@
+// linalg.hip -- contractions over band-major sphere states on MFMA, and the
+// FP64 small-matrix steps of orthogonalisation.
+//
+//   overlap   S = vol * A^H B   (operations::overlap, src/operations/overlap.hpp:26-41)
+//             A [n1][K], B [n2][K] complex; split-K over K with
+//             v_mfma_{f32,f64}_16x16x4 on the interleaved (re, im) storage read
+//             as a real K' = 2K contraction:
+//               Re S = sum_k' A'[i][k'] B'[j][k'],
+//               Im S = sum_k' A'[i][k'] B''[j][k'],  B''[j][k'] = (+/-) B'[j][k' ^ 1]
+//             Partial tiles are reduced in FP64 in a fixed order.
+//   rotate    Out = M C (+ beta Out): C [n][K] complex, M [n][n] complex
+//             (operations::rotate, src/operations/rotate.hpp:26-156), MFMA with
+//             the real operands A' = [Mr, -Mi], A'' = [Mi, Mr] interleaved along j.
+//   cholesky  FP64 lower Cholesky S = L L^H, upper zeroed (matrix::cholesky,
+//             src/matrix/cholesky.hpp:28-100), one work-group.
+//   trinv     FP64 inverse of a lower-triangular matrix (matrix::invert_triangular,
+//             src/matrix/invert_triangular.hpp:20-50), one lane per column.
+//
+// MFMA 16x16x4 lane layout (CDNA4): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] and receives column l%16 of D in its four accumulators; the
+// accumulator-to-row map differs between the f32 and f64 forms (mfma_t::row),
+// and tests/test_gpu_linalg.py checks both against numpy.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GAS __attribute__((address_space(1)))
+#else
+#define GAS
+#endif
+
+namespace inqx {
+
+template <typename T> __device__ __forceinline__ const GAS T *lgp(const T *p) { return (const GAS T *)p; }
+template <typename T> __device__ __forceinline__ GAS T *lgw(T *p) { return (GAS T *)p; }
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// row of the 16x16 output tile held in accumulator r of a lane (column = lane % 16)
+template <typename T> struct mfma_t;
+template <> struct mfma_t<float> {
+  using acc = floatx4;
+  static __device__ __forceinline__ acc run(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <> struct mfma_t<double> {
+  using acc = doublex4;
+  static __device__ __forceinline__ acc run(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+// ------------------------------------------------------------------------ overlap
+constexpr int kOvTile = 64;   // output tile edge per work-group
+constexpr int kOvKC = 32;     // real K per LDS stage
+constexpr int kOvChunk = 2048;  // real K per work-group (split K)
+
+// partial[chunk][tile_i][tile_j] holds a 64x64 complex tile in T
+template <typename T>
+__global__ void __launch_bounds__(256) k_overlap_partial(const T *A, const T *B, int n1, int n2, long K2,
+                                                         int tiles_i, int tiles_j, T *partial) {
+  using M = mfma_t<T>;
+  __shared__ T At[kOvTile][kOvKC + 1];
+  __shared__ T Bt[kOvTile][kOvKC + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tj = blockIdx.x % tiles_j, ti = (blockIdx.x / tiles_j) % tiles_i, chunk = blockIdx.x / (tiles_j * tiles_i);
+  const long k_begin = (long)chunk * kOvChunk;
+  const long k_end = min(k_begin + kOvChunk, K2);
+  const int i0 = ti * kOvTile, j0 = tj * kOvTile;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;  // wave sub-tile 32x32
+  typename M::acc re[2][2], im[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      re[a][b] = typename M::acc{0, 0, 0, 0};
+      im[a][b] = typename M::acc{0, 0, 0, 0};
+    }
+  const GAS T *Ag = lgp(A), *Bg = lgp(B);
+  for (long kb = k_begin; kb < k_end; kb += kOvKC) {
+    // stage 64 x 32 of A and B (real index k' along rows of the complex arrays)
+    for (int e = tid; e < kOvTile * kOvKC; e += 256) {
+      const int r = e / kOvKC, c = e - r * kOvKC;
+      const long k = kb + c;
+      const int ia = i0 + r, jb = j0 + r;
+      At[r][c] = (ia < n1 && k < k_end) ? Ag[(long)ia * K2 + k] : (T)0;
+      Bt[r][c] = (jb < n2 && k < k_end) ? Bg[(long)jb * K2 + k] : (T)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kOvKC; ks += 4) {
+      const int kk = ks + (lane >> 4);
+      const T sgn = (kk & 1) ? (T)-1 : (T)1;
+      T a[2], bp[2], bq[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        a[t] = At[wr + 16 * t + (lane & 15)][kk];
+        bp[t] = Bt[wc + 16 * t + (lane & 15)][kk];
+        bq[t] = sgn * Bt[wc + 16 * t + (lane & 15)][kk ^ 1];
+      }
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+          re[x][y] = M::run(a[x], bp[y], re[x][y]);
+          im[x][y] = M::run(a[x], bq[y], im[x][y]);
+        }
+    }
+    __syncthreads();
+  }
+  // write the 64x64 complex partial tile: row i = wr + 16x + 4(l/16) + r, col j = wc + 16y + l%16
+  GAS T *P = lgw(partial) + ((long)blockIdx.x) * kOvTile * kOvTile * 2;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = wr + 16 * x + M::row(lane, r), j = wc + 16 * y + (lane & 15);
+        P[(i * kOvTile + j) * 2 + 0] = re[x][y][r];
+        P[(i * kOvTile + j) * 2 + 1] = im[x][y][r];
+      }
+}
+
+// S[i][j] = vol * sum_chunks partial, FP64, fixed chunk order
+template <typename T>
+__global__ void k_overlap_reduce(const T *partial, int n1, int n2, int tiles_i, int tiles_j, int nchunk, double vol,
+                                 double *S) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n1 * n2) return;
+  const int i = idx / n2, j = idx - i * n2;
+  const int ti = i / kOvTile, tj = j / kOvTile, ii = i % kOvTile, jj = j % kOvTile;
+  double sr = 0.0, si = 0.0;
+  const GAS T *P = lgp(partial);
+  for (int c = 0; c < nchunk; ++c) {
+    const long blk = ((long)c * tiles_i + ti) * tiles_j + tj;
+    const long o = (blk * kOvTile * kOvTile + ii * kOvTile + jj) * 2;
+    sr += (double)P[o];
+    si += (double)P[o + 1];
+  }
+  GAS double *Sg = lgw(S);
+  Sg[2 * idx] = vol * sr;
+  Sg[2 * idx + 1] = vol * si;
+}
+
+// ------------------------------------------------------------------------ rotate
+// operand matrices Ap = [Mr, -Mi], Aq = [Mi, Mr] interleaved along j: [n][2n] in T
+// mode 0: M = conj(R) (Phi <- Phi R^H);  mode 1: M = alpha R^T (5-argument rotate);
+// mode 2: M = R
+__global__ void k_rot_prep32(const double *R, int n, int mode, double ar, double ai, float *Ap, float *Aq) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int i = idx / n, j = idx - i * n;
+  double mr, mi;
+  if (mode == 0) {
+    mr = R[2 * (i * n + j)];
+    mi = -R[2 * (i * n + j) + 1];
+  } else if (mode == 1) {
+    const double rr = R[2 * (j * n + i)], ri = R[2 * (j * n + i) + 1];
+    mr = ar * rr - ai * ri;
+    mi = ar * ri + ai * rr;
+  } else {
+    mr = R[2 * (i * n + j)];
+    mi = R[2 * (i * n + j) + 1];
+  }
+  Ap[(long)i * 2 * n + 2 * j] = (float)mr;
+  Ap[(long)i * 2 * n + 2 * j + 1] = (float)-mi;
+  Aq[(long)i * 2 * n + 2 * j] = (float)mi;
+  Aq[(long)i * 2 * n + 2 * j + 1] = (float)mr;
+}
+__global__ void k_rot_prep64(const double *R, int n, int mode, double ar, double ai, double *Ap, double *Aq) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int i = idx / n, j = idx - i * n;
+  double mr, mi;
+  if (mode == 0) {
+    mr = R[2 * (i * n + j)];
+    mi = -R[2 * (i * n + j) + 1];
+  } else if (mode == 1) {
+    const double rr = R[2 * (j * n + i)], ri = R[2 * (j * n + i) + 1];
+    mr = ar * rr - ai * ri;
+    mi = ar * ri + ai * rr;
+  } else {
+    mr = R[2 * (i * n + j)];
+    mi = R[2 * (i * n + j) + 1];
+  }
+  Ap[(long)i * 2 * n + 2 * j] = mr;
+  Ap[(long)i * 2 * n + 2 * j + 1] = -mi;
+  Aq[(long)i * 2 * n + 2 * j] = mi;
+  Aq[(long)i * 2 * n + 2 * j + 1] = mr;
+}
+
+constexpr int kRotJC = 16;  // complex j per LDS stage (32 real k')
+
+// Out[i][p] = sum_j M[i][j] C[j][p] (+ beta Out[i][p]); work-group = 64 rows x 64 columns p
+template <typename T>
+__global__ void __launch_bounds__(256) k_rotate(const T *Ap, const T *Aq, const cplx<T> *Cm, int n, long K,
+                                                cplx<T> *Out, double br, double bi, int tiles_p) {
+  using M = mfma_t<T>;
+  using C = cplx<T>;
+  __shared__ C Ct[kRotJC][64 + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tp = blockIdx.x % tiles_p, ti = blockIdx.x / tiles_p;
+  const long p0 = (long)tp * 64;
+  const int i0 = ti * 64 + wave * 16;  // this wave's 16 rows
+  typename M::acc re[4], im[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    re[c] = typename M::acc{0, 0, 0, 0};
+    im[c] = typename M::acc{0, 0, 0, 0};
+  }
+  const GAS T *Apg = lgp(Ap), *Aqg = lgp(Aq);
+  const GAS C *Cg = lgp(Cm);
+  const int arow = i0 + (lane & 15);
+  for (int jb = 0; jb < n; jb += kRotJC) {
+    for (int e = tid; e < kRotJC * 64; e += 256) {
+      const int r = e / 64, c = e - r * 64;
+      const int j = jb + r;
+      const long p = p0 + c;
+      C v{0, 0};
+      if (j < n && p < K) v = Cg[(long)j * K + p];
+      Ct[r][c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 2 * kRotJC; ks += 4) {
+      const int kk = ks + (lane >> 4);  // real k' within the stage
+      const int jglob = jb + (kk >> 1);
+      T ap = 0, aq = 0;
+      if (arow < n && jglob < n) {
+        ap = Apg[(long)arow * 2 * n + 2 * jb + kk];
+        aq = Aqg[(long)arow * 2 * n + 2 * jb + kk];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const C v = Ct[kk >> 1][16 * c + (lane & 15)];
+        const T b = (kk & 1) ? v.y : v.x;
+        re[c] = M::run(ap, b, re[c]);
+        im[c] = M::run(aq, b, im[c]);
+      }
+    }
+    __syncthreads();
+  }
+  GAS C *Og = lgw(Out);
+  const bool beta = (br != 0.0 || bi != 0.0);
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + M::row(lane, r);
+      const long p = p0 + 16 * c + (lane & 15);
+      if (i < n && p < K) {
+        C o{re[c][r], im[c][r]};
+        if (beta) {
+          const C q = Og[(long)i * K + p];
+          o.x += (T)(br * (double)q.x - bi * (double)q.y);
+          o.y += (T)(br * (double)q.y + bi * (double)q.x);
+        }
+        Og[(long)i * K + p] = o;
+      }
+    }
+}
+
+// ------------------------------------------------------------------------ Cholesky / inverse (FP64)
+// in place on S (complex double, row-major n x n); info[0] = 0 or the first
+// non-positive leading minor (LAPACK convention)
+__global__ void __launch_bounds__(1024) k_cholesky(double *S, int n, int *info) {
+  __shared__ double red[1024];
+  __shared__ int bad;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    double part = 0.0;
+    for (int k = tid; k < j; k += nth) {
+      const double lr = S[2 * (j * n + k)], li = S[2 * (j * n + k) + 1];
+      part += lr * lr + li * li;
+    }
+    red[tid] = part;
+    __syncthreads();
+    for (int w = nth >> 1; w > 0; w >>= 1) {
+      if (tid < w) red[tid] += red[tid + w];
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const double d = S[2 * (j * n + j)] - red[0];
+      if (!(d > 0.0)) {
+        bad = j + 1;
+      } else {
+        S[2 * (j * n + j)] = sqrt(d);
+        S[2 * (j * n + j) + 1] = 0.0;
+      }
+    }
+    __syncthreads();
+    if (bad) break;
+    const double ljj = S[2 * (j * n + j)];
+    for (int i = j + 1 + tid; i < n; i += nth) {
+      double sr = S[2 * (i * n + j)], si = S[2 * (i * n + j) + 1];
+      for (int k = 0; k < j; ++k) {
+        const double ar = S[2 * (i * n + k)], ai = S[2 * (i * n + k) + 1];
+        const double br = S[2 * (j * n + k)], bi = S[2 * (j * n + k) + 1];
+        // a * conj(b)
+        sr -= ar * br + ai * bi;
+        si -= ai * br - ar * bi;
+      }
+      S[2 * (i * n + j)] = sr / ljj;
+      S[2 * (i * n + j) + 1] = si / ljj;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) info[0] = bad;
+  if (!bad) {
+    for (int e = tid; e < n * n; e += nth) {
+      const int i = e / n, j = e - i * n;
+      if (j > i) {
+        S[2 * e] = 0.0;
+        S[2 * e + 1] = 0.0;
+      }
+    }
+  }
+}
+
+// X = L^{-1} for lower-triangular L (complex double), one lane per column
+__global__ void k_trinv(const double *L, int n, double *X) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  for (int i = 0; i < c; ++i) {
+    X[2 * (i * n + c)] = 0.0;
+    X[2 * (i * n + c) + 1] = 0.0;
+  }
+  {
+    const double lr = L[2 * (c * n + c)], li = L[2 * (c * n + c) + 1];
+    const double d = lr * lr + li * li;
+    X[2 * (c * n + c)] = lr / d;
+    X[2 * (c * n + c) + 1] = -li / d;
+  }
+  for (int i = c + 1; i < n; ++i) {
+    double sr = 0.0, si = 0.0;
+    for (int k = c; k < i; ++k) {
+      const double ar = L[2 * (i * n + k)], ai = L[2 * (i * n + k) + 1];
+      const double xr = X[2 * (k * n + c)], xi = X[2 * (k * n + c) + 1];
+      sr += ar * xr - ai * xi;
+      si += ar * xi + ai * xr;
+    }
+    const double lr = L[2 * (i * n + i)], li = L[2 * (i * n + i) + 1];
+    const double d = lr * lr + li * li;
+    // -(s / l)
+    X[2 * (i * n + c)] = -(sr * lr + si * li) / d;
+    X[2 * (i * n + c) + 1] = -(si * lr - sr * li) / d;
+  }
+}
+
+// ------------------------------------------------------------------------ host launchers
+void launch_overlap(Ctx &cx, int prec, int n1, int n2, long K, const void *A, const void *B, double vol, double *S,
+                    DevMem &partial) {
+  const long K2 = 2 * K;
+  const int ti = (n1 + kOvTile - 1) / kOvTile, tj = (n2 + kOvTile - 1) / kOvTile;
+  const int nchunk = (int)((K2 + kOvChunk - 1) / kOvChunk);
+  const size_t es = prec == INQX_FP64 ? 8 : 4;
+  partial.ensure(es * (size_t)nchunk * ti * tj * kOvTile * kOvTile * 2);
+  const int grid = nchunk * ti * tj;
+  if (prec == INQX_FP64) {
+    {
+      ProfScope ps(cx, "overlap_mfma_f64");
+      hipLaunchKernelGGL(k_overlap_partial<double>, dim3(grid), dim3(256), 0, cx.stream, (const double *)A,
+                         (const double *)B, n1, n2, K2, ti, tj, partial.as<double>());
+    }
+    INQX_HIP(hipGetLastError());
+    ProfScope ps(cx, "overlap_reduce_f64");
+    hipLaunchKernelGGL(k_overlap_reduce<double>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, cx.stream,
+                       partial.as<double>(), n1, n2, ti, tj, nchunk, vol, S);
+  } else {
+    {
+      ProfScope ps(cx, "overlap_mfma_f32");
+      hipLaunchKernelGGL(k_overlap_partial<float>, dim3(grid), dim3(256), 0, cx.stream, (const float *)A,
+                         (const float *)B, n1, n2, K2, ti, tj, partial.as<float>());
+    }
+    INQX_HIP(hipGetLastError());
+    ProfScope ps(cx, "overlap_reduce_f32");
+    hipLaunchKernelGGL(k_overlap_reduce<float>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, cx.stream,
+                       partial.as<float>(), n1, n2, ti, tj, nchunk, vol, S);
+  }
+  INQX_HIP(hipGetLastError());
+}
+
+void launch_rotate(Ctx &cx, int prec, int n, long K, const double *R, int mode, double ar, double ai, const void *Cm,
+                   void *Out, double br, double bi, DevMem &ops) {
+  const size_t es = prec == INQX_FP64 ? 8 : 4;
+  ops.ensure(es * (size_t)n * 2 * n * 2);
+  const int tiles_p = (int)((K + 63) / 64), tiles_i = (n + 63) / 64;
+  if (prec == INQX_FP64) {
+    double *Ap = ops.as<double>(), *Aq = Ap + (size_t)n * 2 * n;
+    hipLaunchKernelGGL(k_rot_prep64, dim3((n * n + 255) / 256), dim3(256), 0, cx.stream, R, n, mode, ar, ai, Ap, Aq);
+    INQX_HIP(hipGetLastError());
+    ProfScope ps(cx, "rotate_mfma_f64");
+    hipLaunchKernelGGL(k_rotate<double>, dim3(tiles_p * tiles_i), dim3(256), 0, cx.stream, Ap, Aq,
+                       (const cplx<double> *)Cm, n, K, (cplx<double> *)Out, br, bi, tiles_p);
+  } else {
+    float *Ap = ops.as<float>(), *Aq = Ap + (size_t)n * 2 * n;
+    hipLaunchKernelGGL(k_rot_prep32, dim3((n * n + 255) / 256), dim3(256), 0, cx.stream, R, n, mode, ar, ai, Ap, Aq);
+    INQX_HIP(hipGetLastError());
+    ProfScope ps(cx, "rotate_mfma_f32");
+    hipLaunchKernelGGL(k_rotate<float>, dim3(tiles_p * tiles_i), dim3(256), 0, cx.stream, Ap, Aq,
+                       (const cplx<float> *)Cm, n, K, (cplx<float> *)Out, br, bi, tiles_p);
+  }
+  INQX_HIP(hipGetLastError());
+}
+
+void launch_cholesky(Ctx &cx, int n, double *S, int *info) {
+  ProfScope ps(cx, "cholesky_f64");
+  hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(n >= 1024 ? 1024 : (n >= 256 ? 256 : 64)), 0, cx.stream, S, n, info);
+  INQX_HIP(hipGetLastError());
+}
+
+void launch_trinv(Ctx &cx, int n, const double *L, double *X) {
+  ProfScope ps(cx, "trinv_f64");
+  hipLaunchKernelGGL(k_trinv, dim3((n + 63) / 64), dim3(64), 0, cx.stream, L, n, X);
+  INQX_HIP(hipGetLastError());
+}
+
+}  // namespace inqx
