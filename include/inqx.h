@@ -41,6 +41,7 @@ extern "C" {
 typedef struct inqx_ctx_s *inqx_ctx;
 typedef struct inqx_ham_s *inqx_ham;
 typedef struct inqx_basis_s *inqx_basis;
+typedef struct inqx_prop_s *inqx_prop;
 
 enum {
   INQX_OK = 0,
@@ -155,6 +156,36 @@ int inqx_orthogonalize(inqx_ctx ctx, int precision, int n, long npw, double vol_
  * H_sub = phi^H H phi, FP64 eigendecomposition (eigenvalues ascending, host
  * array), phi <- phi U */
 int inqx_subspace_diag(inqx_ham ham, inqx_basis basis, int nst, int precision, void *phi, double *eigenvalues);
+
+/* ------------------------------------------------------------ real-time propagation */
+/* operations::exponential_in_place (src/operations/exponential.hpp:21-34):
+ * phi <- sum_{n<=order} (f H)^n / n! phi, f = f_re + i f_im, in place */
+int inqx_exponential(inqx_ham ham, inqx_basis basis, int nst, int precision, double f_re, double f_im, int order,
+                     void *phi);
+/* operations::exponential_2_for_1 (exponential.hpp:49-69): expphi <- exp_T(f1 H) phi and
+ * phi <- exp_T(f2 H) phi sharing the H^n phi */
+int inqx_exponential_2_for_1(inqx_ham ham, inqx_basis basis, int nst, int precision, double f1_re, double f1_im,
+                             double f2_re, double f2_im, int order, void *phi, void *expphi);
+/* real_time::propagate's step loop (src/real_time/propagate.hpp:87-119) for a
+ * Hamiltonian independent of the density: each step is the ETRS step
+ * (src/real_time/etrs.hpp:22-79, Taylor order 4) followed by
+ * energy::calculate (src/hamiltonian/energy.hpp:61-113) and the electronic
+ * dipole (src/observables/dipole.hpp:24-59).  Every step appends one record
+ * {kinetic, sum occ eigenvalue, non-local, sum of norms, dipole x, y, z,
+ * step} (FP64) to a device ring of ring_capacity records.  phi are the
+ * caller's band-major device states (updated in place), occ host arrays.
+ * With use_graph the step is captured once in a hipGraph and replayed. */
+int inqx_prop_create(inqx_ham ham, int nblocks, const inqx_basis *bases, const int *nst, int precision,
+                     void *const *phi, const double *const *occ, double dt, int ring_capacity, inqx_prop *out);
+int inqx_prop_step(inqx_prop prop, int nsteps, int use_graph);
+/* observables of the current states (no propagation), appended to the ring */
+int inqx_prop_energy(inqx_prop prop);
+/* number of records written so far and the most recent min(count, cap) records [k][8] */
+int inqx_prop_records(inqx_prop prop, int *count, double *records, int cap);
+/* a propagator that captured a graph pins its Hamiltonian and bases: they
+ * cannot be destroyed (INQX_ERR_STATE) or have their workspaces grown until
+ * the propagator is destroyed */
+int inqx_prop_destroy(inqx_prop prop);
 
 /* ------------------------------------------------------------ sphere <-> grids */
 /* operations::transform::to_real (transform.hpp:220-249) of sphere states into

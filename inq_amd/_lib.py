@@ -61,6 +61,14 @@ SIGNATURES = {
                                   C.c_double]),
     "inqx_orthogonalize": (C.c_int, [P, C.c_int, C.c_int, C.c_long, C.c_double, P]),
     "inqx_subspace_diag": (C.c_int, [P, P, C.c_int, C.c_int, P, DP]),
+    "inqx_exponential": (C.c_int, [P, P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, P]),
+    "inqx_exponential_2_for_1": (C.c_int, [P, P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                           C.c_int, P, P]),
+    "inqx_prop_create": (C.c_int, [P, C.c_int, PP, IP, C.c_int, PP, PP, C.c_double, C.c_int, PP]),
+    "inqx_prop_step": (C.c_int, [P, C.c_int, C.c_int]),
+    "inqx_prop_energy": (C.c_int, [P]),
+    "inqx_prop_records": (C.c_int, [P, IP, DP, C.c_int]),
+    "inqx_prop_destroy": (C.c_int, [P]),
 }
 
 

@@ -446,6 +446,10 @@ int inqx_ham_create(inqx_ctx ctx, const double lattice[9], const int n[3], inqx_
 
 int inqx_ham_destroy(inqx_ham h) {
   if (!h) return INQX_OK;
+  if (h->pinned > 0) {
+    h->ctx->c.err = "inqx_ham_destroy: a propagator with a captured graph still uses this Hamiltonian";
+    return INQX_ERR_STATE;
+  }
   invalidate_launch_cache(h);
   for (auto b : h->bases) b->b.ham = nullptr;
   delete h;
@@ -553,6 +557,10 @@ int inqx_basis_create(inqx_ham h, const double kcov[3], double ecut, int spin, i
 
 int inqx_basis_destroy(inqx_basis bs) {
   if (!bs) return INQX_OK;
+  if (bs->b.pinned > 0) {
+    if (bs->b.ham) bs->b.ham->ctx->c.err = "inqx_basis_destroy: a propagator with a captured graph still uses this basis";
+    return INQX_ERR_STATE;
+  }
   if (bs->b.ham) {
     invalidate_launch_cache(bs->b.ham);
     auto &v = bs->b.ham->bases;

@@ -81,6 +81,7 @@ struct HamDev {
   const int *atom_slots, *atom_slot_list;
   const double *coeff;
   double kb_scale;  // dV / N
+  double lat[9];    // lattice vectors (rows), for positions r of grid points
 };
 
 __device__ __forceinline__ int find_block(const int *prefix, int nblocks, int wg) {
@@ -173,7 +174,7 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
 
 template <typename T, int N1, int N2>
 __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                       int total) {
+                                                       int total, T c1r, T c1i, T c2r, T c2i) {
   using C = cplx<T>;
   using Cfg = PlaneCfg<T, N1, N2>;
   constexpr int P = Cfg::P;
@@ -229,7 +230,22 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
       const C v = lds[p * Cfg::PLANE + y * (N2 + 1) + z];
       const C c = psi[o];
       const T kf = kin[seg0 + j];
-      hpsi[o] = C{v.x + kf * c.x, v.y + kf * c.y};
+      const C out{v.x + kf * c.x, v.y + kf * c.y};
+      if (B.write_h) hpsi[o] = out;
+      if (B.acc1) {
+        GAS C *a = gw(B.acc1) + o;
+        C q = *a;
+        q.x += c1r * out.x - c1i * out.y;
+        q.y += c1r * out.y + c1i * out.x;
+        *a = q;
+      }
+      if (B.acc2) {
+        GAS C *a = gw(B.acc2) + o;
+        C q = *a;
+        q.x += c2r * out.x - c2i * out.y;
+        q.y += c2r * out.y + c2i * out.x;
+        *a = q;
+      }
     }
     __syncthreads();
   }
@@ -384,6 +400,12 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
     }
     const GAS double *D = gp(hd.coeff) + (size_t)a * kMaxNlm * kMaxNlm;
     GAS C *dp = gw(B.dp);
+    if (B.proj) {
+      GAS C *pj = gw(B.proj);
+#pragma unroll
+      for (int l = 0; l < kMaxNlm; ++l)
+        pj[((size_t)a * kMaxNlm + l) * B.nst + s] = C{(T)(pr[l] * hd.kb_scale), (T)(pi[l] * hd.kb_scale)};
+    }
 #pragma unroll
     for (int l = 0; l < kMaxNlm; ++l) {
       double dr = 0.0, di = 0.0;
@@ -437,6 +459,34 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
         for (int i = lane; i < npts; i += 64) pts[i] = C{0, 0};
       }
       load_line_inverse<T, N0>(B, hd.n12, s, Lc, x);
+      if (B.dip) {
+        // sum |u|^2 and sum f0 |u|^2 along the line, f0 = sym(i0)/n0
+        double s0 = 0.0, s1 = 0.0;
+        if (valid) {
+#pragma unroll
+          for (int i0 = 0; i0 < N0; ++i0) {
+            const double w = (double)x[i0].x * (double)x[i0].x + (double)x[i0].y * (double)x[i0].y;
+            s0 += w;
+            s1 += w * ((double)sym_of<N0>(i0) / (double)N0);
+          }
+        }
+        const int i1 = Lc / hd.n2, i2 = Lc - i1 * hd.n2;
+        const double f1 = (double)(i1 >= (hd.n1 + 1) / 2 ? i1 - hd.n1 : i1) / hd.n1;
+        const double f2 = (double)(i2 >= (hd.n2 + 1) / 2 ? i2 - hd.n2 : i2) / hd.n2;
+        double d[4];
+        d[0] = s0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) d[c + 1] = s1 * hd.lat[c] + s0 * (f1 * hd.lat[3 + c] + f2 * hd.lat[6 + c]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) d[c] += __shfl_xor(d[c], off, 64);
+        if (lane == 0) {
+          GAS double *dp = gw(B.dip) + ((size_t)tile * B.nst + s) * 4;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) dp[c] = d[c];
+        }
+      }
       const GAS T *vl = gp(B.vloc) + Lc;
 #pragma unroll
       for (int i0 = 0; i0 < N0; ++i0) {
@@ -489,6 +539,117 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
   }
 }
 
+
+// -------------------------------------------------------------------------------------------
+// observables of a block of sphere states (FP64 accumulation, fixed reduction order)
+// -------------------------------------------------------------------------------------------
+constexpr int kDotChunk = 4096;
+
+// per (state, chunk): sum kin |c|^2, sum |c|^2, sum conj(c) h (re, im)
+template <typename T>
+__global__ void __launch_bounds__(256) k_sphere_dots(const cplx<T> *psi, const cplx<T> *hpsi, const T *kin, int nst,
+                                                     int npw, int nch, double *partial) {
+  using C = cplx<T>;
+  __shared__ double red[4][256];
+  const int s = blockIdx.x / nch, ch = blockIdx.x - s * nch;
+  const int tid = threadIdx.x;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  const GAS C *ps = gp(psi) + (size_t)s * npw;
+  const GAS C *hs = gp(hpsi) + (size_t)s * npw;
+  const GAS T *kn = gp(kin);
+  const int e0 = ch * kDotChunk, e1 = min(e0 + kDotChunk, npw);
+  for (int i = e0 + tid; i < e1; i += 256) {
+    const C c = ps[i], h = hs[i];
+    const double n2 = (double)c.x * c.x + (double)c.y * c.y;
+    a[0] += (double)kn[i] * n2;
+    a[1] += n2;
+    a[2] += (double)c.x * h.x + (double)c.y * h.y;
+    a[3] += (double)c.x * h.y - (double)c.y * h.x;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) red[c][tid] = a[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) red[c][tid] += red[c][tid + w];
+    __syncthreads();
+  }
+  if (tid < 4) partial[((size_t)s * nch + ch) * 4 + tid] = red[tid][0];
+}
+
+struct ObsBlock {
+  const double *dots;   // [nst][nch][4]
+  const void *proj;     // [natoms][kMaxNlm][nst] complex T
+  const void *dp;       // [natoms][kMaxNlm][nst] complex T
+  const double *dip;    // [ntile][nst][4]
+  const double *occ;    // [nst]
+  int nst, nch;
+};
+
+// one work-group: combines every block into record[slot] =
+// {kinetic, eigenvalue sum, non-local, norm sum, dipole x, y, z, step}
+// (energy::calculate, src/hamiltonian/energy.hpp:61-113; dipole, src/observables/dipole.hpp:24-59)
+template <typename T>
+__global__ void __launch_bounds__(256) k_obs_combine(const ObsBlock *ob, int nblocks, int natoms, int ntile,
+                                                     double vol_g, double dip_scale, double *ring, int ring_cap,
+                                                     int *counter) {
+  using C = cplx<T>;
+  __shared__ double red[7][256];
+  const int tid = threadIdx.x;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nblocks; ++b) {
+    const ObsBlock B = ob[b];
+    for (int s = tid; s < B.nst; s += 256) {
+      double d[4] = {0, 0, 0, 0};
+      for (int ch = 0; ch < B.nch; ++ch)
+        for (int c = 0; c < 4; ++c) d[c] += B.dots[((size_t)s * B.nch + ch) * 4 + c];
+      const double occ = B.occ[s];
+      acc[0] += occ * d[0] / d[1];
+      acc[1] += occ * d[2] / d[1];
+      acc[3] += d[1] * vol_g;
+      double enl = 0.0;
+      if (B.proj) {
+        const C *pj = reinterpret_cast<const C *>(B.proj), *dq = reinterpret_cast<const C *>(B.dp);
+        for (int a = 0; a < natoms; ++a)
+          for (int l = 0; l < kMaxNlm; ++l) {
+            const C p = pj[((size_t)a * kMaxNlm + l) * B.nst + s], q = dq[((size_t)a * kMaxNlm + l) * B.nst + s];
+            enl += (double)p.x * q.x + (double)p.y * q.y;
+          }
+      }
+      acc[2] += occ * enl;
+      if (B.dip) {
+        double dx = 0, dy = 0, dz = 0;
+        for (int t = 0; t < ntile; ++t) {
+          const double *q = B.dip + ((size_t)t * B.nst + s) * 4;
+          dx += q[1];
+          dy += q[2];
+          dz += q[3];
+        }
+        acc[4] += occ * dx * dip_scale;
+        acc[5] += occ * dy * dip_scale;
+        acc[6] += occ * dz * dip_scale;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 7; ++c) red[c][tid] = acc[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w)
+#pragma unroll
+      for (int c = 0; c < 7; ++c) red[c][tid] += red[c][tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const int step = counter[0];
+    double *rec = ring + (size_t)(step % ring_cap) * 8;
+    for (int c = 0; c < 7; ++c) rec[c] = red[c][0];
+    rec[7] = (double)step;
+    counter[0] = step + 1;
+  }
+}
+
 // -------------------------------------------------------------------------------------------
 // host side: tables and launches
 // -------------------------------------------------------------------------------------------
@@ -509,6 +670,7 @@ bool grid_supported(const int *n) {
 }
 
 void build_kb_tables(inqx_ham_s *h) {
+  INQX_CHECK(h->pinned == 0, INQX_ERR_STATE, "projector tables are pinned by a captured propagation graph");
   invalidate_launch_cache(h);
   KBTables &kb = h->kb;
   const Grid &g = h->g;
@@ -657,6 +819,7 @@ void ensure_workspace(inqx_ham_s *h, Basis &b, int nst, int prec) {
   if (b.ws_nst[prec] >= nst && b.t1[prec].p && b.ws_kb_nslot[prec] == h->kb.nslot &&
       b.ws_natoms[prec] == h->kb.natoms)
     return;
+  INQX_CHECK(b.pinned == 0, INQX_ERR_STATE, "basis workspaces are pinned by a captured propagation graph");
   const int n12 = h->g.n[1] * h->g.n[2];
   const int ns = std::max(nst, b.ws_nst[prec]);
   b.t1[prec].ensure(cs * (size_t)std::max(1, b.nslot) * ns * n12);
@@ -680,7 +843,12 @@ struct LaunchData {
   int nwave = 4;
   int lds_per_wave = 0;
 };
-static std::map<std::pair<inqx_ham_s *, LaunchKey>, std::unique_ptr<LaunchData>> g_launch_cache;
+static std::map<std::pair<inqx_ham_s *, LaunchKey>, std::shared_ptr<LaunchData>> g_launch_cache;
+
+// while a hipGraph is being captured, every descriptor it uses is kept alive
+// by the capturing object, whatever happens to the cache later
+static thread_local std::vector<std::shared_ptr<void>> *g_capture_keep = nullptr;
+void set_capture_keep(std::vector<std::shared_ptr<void>> *keep) { g_capture_keep = keep; }
 
 // descriptors hold raw pointers to workspaces, tables and bases: drop every
 // cached descriptor of a Hamiltonian whenever one of those may change
@@ -713,7 +881,7 @@ template <typename T, int N0> static void set_x_attr(int lds) {
 }
 
 template <typename T, int N0, int N12>
-static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev &hd) {
+static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev &hd, const double *cf) {
   using Cfg = PlaneCfg<T, N12, N12>;
   hipStream_t st = h->ctx->c.stream;
   const SphereBlock<T> *blk = ld.desc.as<SphereBlock<T>>();
@@ -758,14 +926,16 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   {
     ProfScope ps(cx, f64 ? "plane_forward_f64" : "plane_forward_f32");
     hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), Cfg::LDS, st, blk,
-                       pref_plane, nblocks, ld.total_plane);
+                       pref_plane, nblocks, ld.total_plane, (T)cf[0], (T)cf[1], (T)cf[2], (T)cf[3]);
   }
   INQX_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *nst, const void *const *psi,
-                  void *const *hpsi) {
+                  void *const *hpsi, const ApplyOpts *opts) {
+  static const ApplyOpts kNone;
+  if (!opts) opts = &kNone;
   const int prec = std::is_same<T, double>::value ? INQX_FP64 : INQX_FP32;
   const Grid &g = h->g;
   INQX_CHECK(h->has_v, INQX_ERR_STATE, "potential not set");
@@ -783,6 +953,24 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     key.k.push_back((uintptr_t)psi[i]);
     key.k.push_back((uintptr_t)hpsi[i]);
     key.k.push_back((uintptr_t)bases[i]->t1[prec].p);
+    key.k.push_back((uintptr_t)(opts->acc1 ? opts->acc1[i] : nullptr));
+    key.k.push_back((uintptr_t)(opts->acc2 ? opts->acc2[i] : nullptr));
+  }
+  key.k.push_back((uintptr_t)(opts->write_h ? 1 : 0));
+  key.k.push_back((uintptr_t)(opts->observables ? 1 : 0));
+  if (opts->observables) {
+    for (int i = 0; i < nblocks; ++i) {
+      Basis &B = *bases[i];
+      const size_t cs = prec == INQX_FP64 ? 16 : 8;
+      const size_t need_p = cs * (size_t)std::max(1, h->kb.natoms) * kMaxNlm * nst[i];
+      const size_t need_d = 32 * (size_t)h->kb.ntile * nst[i];
+      if (B.proj[prec].bytes < need_p || B.dip.bytes < need_d) {
+        INQX_CHECK(B.pinned == 0, INQX_ERR_STATE, "basis workspaces are pinned by a captured propagation graph");
+        B.proj[prec].ensure(need_p);
+        B.dip.ensure(need_d);
+        invalidate_launch_cache(h);
+      }
+    }
   }
   for (int d = 0; d < 3; ++d) {
     uint64_t bits;
@@ -793,7 +981,8 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   auto it = g_launch_cache.find(ck);
   if (it == g_launch_cache.end()) {
     if (g_launch_cache.size() > 256) g_launch_cache.clear();
-    auto ld = std::make_unique<LaunchData>();
+    INQX_CHECK(!g_capture_keep, INQX_ERR_STATE, "launch configuration first seen during graph capture");
+    auto ld = std::make_shared<LaunchData>();
     std::vector<SphereBlock<T>> blk(nblocks);
     const int P = PlaneCfg<T, 16, 16>::P;  // placeholder, replaced below by the real plane config
     (void)P;
@@ -828,13 +1017,17 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
       d.rows = B.d_rows.as<int>();
       d.pw_yz = B.d_pw_yz.as<int>();
       d.kin = prec == INQX_FP64 ? (const T *)B.d_kin64.p : (const T *)B.d_kin32.p;
+      d.acc1 = opts->acc1 ? reinterpret_cast<cplx<T> *>(opts->acc1[i]) : nullptr;
+      d.acc2 = opts->acc2 ? reinterpret_cast<cplx<T> *>(opts->acc2[i]) : nullptr;
+      d.proj = opts->observables ? B.proj[prec].as<cplx<T>>() : nullptr;
+      d.dip = opts->observables ? B.dip.as<double>() : nullptr;
+      d.write_h = opts->write_h ? 1 : 0;
       const DevMem &vm = prec == INQX_FP64 ? h->v64 : h->v32;
       d.vloc = reinterpret_cast<const T *>(vm.as<char>() + (size_t)B.spin * g.N * sizeof(T));
       d.nst = nst[i];
       d.npw = (int)B.npw;
       d.nslot = B.nslot;
       d.m_lo = B.m_lo;
-      d.pad_ = 0;
       for (int k = 0; k < 3; ++k) d.kcov[k] = B.kcov[k] + Afull[k];
       d.gamma = (d.kcov[0] == 0.0 && d.kcov[1] == 0.0 && d.kcov[2] == 0.0) ? 1 : 0;
       pp[i + 1] = pp[i] + d.nslot * ((nst[i] + plane_p - 1) / plane_p);
@@ -861,6 +1054,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     it = g_launch_cache.emplace(ck, std::move(ld)).first;
   }
   LaunchData &ld = *it->second;
+  if (g_capture_keep) g_capture_keep->push_back(it->second);
   HamDev hd;
   hd.n0 = g.n[0];
   hd.n1 = g.n[1];
@@ -881,10 +1075,13 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   hd.atom_slot_list = h->kb.atom_slot_list.as<int>();
   hd.coeff = h->kb.coeff.as<double>();
   hd.kb_scale = (g.vol / (double)g.N) / (double)g.N;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) hd.lat[3 * r + c] = g.a[r][c];
+  const double cf[4] = {opts->c1r, opts->c1i, opts->c2r, opts->c2i};
   switch (g.n[0] * 1000 + g.n[1]) {
 #define CASE(S)                                            \
   case S * 1000 + S:                                       \
-    run_passes<T, S, S>(h, ld, nblocks, hd);               \
+    run_passes<T, S, S>(h, ld, nblocks, hd, cf);           \
     break;
     INQX_SIZES(CASE)
 #undef CASE
@@ -893,7 +1090,36 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   }
 }
 
-template void launch_apply<double>(inqx_ham_s *, int, Basis *const *, const int *, const void *const *, void *const *);
-template void launch_apply<float>(inqx_ham_s *, int, Basis *const *, const int *, const void *const *, void *const *);
+int dots_chunks(int npw) { return (npw + kDotChunk - 1) / kDotChunk; }
+
+void launch_sphere_dots(Ctx &cx, int prec, const void *psi, const void *hpsi, const void *kin, int nst, int npw,
+                        double *partial) {
+  const int nch = dots_chunks(npw);
+  ProfScope ps(cx, "sphere_dots");
+  if (prec == INQX_FP64)
+    hipLaunchKernelGGL(k_sphere_dots<double>, dim3(nst * nch), dim3(256), 0, cx.stream, (const cplx<double> *)psi,
+                       (const cplx<double> *)hpsi, (const double *)kin, nst, npw, nch, partial);
+  else
+    hipLaunchKernelGGL(k_sphere_dots<float>, dim3(nst * nch), dim3(256), 0, cx.stream, (const cplx<float> *)psi,
+                       (const cplx<float> *)hpsi, (const float *)kin, nst, npw, nch, partial);
+  INQX_HIP(hipGetLastError());
+}
+
+void launch_obs_combine(Ctx &cx, int prec, const void *obs_blocks, int nblocks, int natoms, int ntile, double vol_g,
+                        double dip_scale, double *ring, int ring_cap, int *counter) {
+  ProfScope ps(cx, "obs_combine");
+  if (prec == INQX_FP64)
+    hipLaunchKernelGGL(k_obs_combine<double>, dim3(1), dim3(256), 0, cx.stream, (const ObsBlock *)obs_blocks, nblocks,
+                       natoms, ntile, vol_g, dip_scale, ring, ring_cap, counter);
+  else
+    hipLaunchKernelGGL(k_obs_combine<float>, dim3(1), dim3(256), 0, cx.stream, (const ObsBlock *)obs_blocks, nblocks,
+                       natoms, ntile, vol_g, dip_scale, ring, ring_cap, counter);
+  INQX_HIP(hipGetLastError());
+}
+
+template void launch_apply<double>(inqx_ham_s *, int, Basis *const *, const int *, const void *const *, void *const *,
+                                   const ApplyOpts *);
+template void launch_apply<float>(inqx_ham_s *, int, Basis *const *, const int *, const void *const *, void *const *,
+                                  const ApplyOpts *);
 
 }  // namespace inqx

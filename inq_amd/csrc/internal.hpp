@@ -173,11 +173,23 @@ template <typename T> struct SphereBlock {
   const int *pw_yz;    // [npw] i1*n2 + i2
   const T *kin;        // [npw] 1/2 |G+k|^2
   const T *vloc;       // [N] V/N of the block's spin
+  C *acc1, *acc2;      // Taylor accumulators acc += c (H psi), or null
+  C *proj;             // KB projections P [natoms][kMaxNlm][nst], or null
+  double *dip;         // dipole partials [ntile][nst][4], or null
   int nst, npw;
   int nslot, m_lo;     // slot j holds plane index m = m_lo + j (symmetric range)
   int gamma;           // k+A == 0: no KB phases
-  int pad_;
+  int write_h;         // store H psi into hpsi
   double kcov[3];      // k + A, covariant
+};
+
+// optional work fused into one H application
+struct ApplyOpts {
+  void *const *acc1 = nullptr;  // per block, acc1 += c1 (H psi)
+  void *const *acc2 = nullptr;  // per block, acc2 += c2 (H psi)
+  double c1r = 0.0, c1i = 0.0, c2r = 0.0, c2i = 0.0;
+  bool write_h = true;
+  bool observables = false;     // projections and dipole partials into the basis workspaces
 };
 
 // one G-sphere block
@@ -195,9 +207,10 @@ struct Basis {
   DevMem d_slot_seg, d_rows, d_pw_yz, d_kin64, d_kin32;
   double kin_A[3] = {1e300, 0, 0};  // A for which d_kin* were built
   // workspaces per precision
-  DevMem t1[2], partial[2], dp[2];
+  DevMem t1[2], partial[2], dp[2], proj[2], dip, dots, hn[2][2], save[2];
   int ws_nst[2] = {0, 0};
   int ws_kb_nslot[2] = {-1, -1}, ws_natoms[2] = {-1, -1};
+  int pinned = 0;  // > 0 while a captured graph uses the workspaces
 };
 
 }  // namespace inqx
@@ -217,6 +230,7 @@ struct inqx_ham_s {
   std::vector<inqx::Projector> projs;
   inqx::KBTables kb;
   std::vector<inqx_basis> bases;
+  int pinned = 0;  // > 0 while a captured graph uses the projector tables
 };
 
 struct inqx_basis_s {
@@ -227,9 +241,10 @@ namespace inqx {
 // ham_kernels.hip
 void build_kb_tables(inqx_ham_s *h);
 void invalidate_launch_cache(inqx_ham_s *h);
+void set_capture_keep(std::vector<std::shared_ptr<void>> *keep);
 template <typename T>
 void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *nst, const void *const *psi,
-                  void *const *hpsi);
+                  void *const *hpsi, const ApplyOpts *opts = nullptr);
 void ensure_basis_device(inqx_ham_s *h, Basis &b);
 void ensure_workspace(inqx_ham_s *h, Basis &b, int nst, int prec);
 bool grid_supported(const int *n);
