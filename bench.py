@@ -5,12 +5,20 @@ Si supercell (L = 20.52 bohr, 72^3 grid, E_cut = 30 Ha, Gamma, 64 Gaussian
 s+p KB projectors), states resident in HBM on their G sphere.  The headline
 value is the mixed-precision (complex64) path; the FP64 path is timed on the
 same input and reported beside it, with the FP32-vs-FP64 error on H|psi> and
-on the band-energy sum per atom.  The dominant kernel's achieved HBM
-bandwidth comes from HIP events recorded around each launch on the
-library's stream.  The CPU baseline is the FP64 oracle (a C/OpenMP port of
-the reference CPU data flow) on a bounded subset of the same states.
+on the band energy per atom.  The dominant kernel's achieved HBM bandwidth
+comes from HIP events recorded around each launch on the library's stream.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Secondary lines in the same JSON object (same seeded inputs):
+  hbm_copy      plain streaming copy kernel, the measured HBM reference
+  c3_kpoints    Al4 fcc, 64 k-points x 2 spins (256 uneven blocks) in one batched launch
+  c4_tddft      ETRS steps on Si64 with a velocity-gauge kick, hipGraph replay,
+                FP32 vs FP64 dipole trace and energy drift
+  c1_diamond    C8 diamond apply + overlap + Cholesky, GPU and CPU oracle
+The CPU baseline is the FP64 oracle (a C/OpenMP port of the reference CPU data
+flow) on a bounded subset of the Si64 states.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32|fp64|both]
+                       [--no-cpu] [--no-extras]
 """
 import argparse
 import json
@@ -31,8 +39,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name, nst, npw, nslot, n12, N, prec_bytes, nsph):
-    """algorithmic HBM bytes of one launch of a pipeline kernel on nst states"""
+def kernel_bytes(name, nst, npw, nslot, n12, N, prec_bytes):
+    """algorithmic HBM bytes of one launch of a pipeline kernel on nst states:
+    compulsory reads + writes of the streams it touches (the plane
+    intermediates hold nslot planes of n12 points per state)."""
     cs = 2 * prec_bytes
     plane = nslot * n12 * cs
     if name.startswith("plane_inverse"):
@@ -43,15 +53,167 @@ def kernel_bytes(name, nst, npw, nslot, n12, N, prec_bytes, nsph):
         return nst * 2 * plane + N * prec_bytes
     if name.startswith("plane_forward"):
         return nst * (plane + 2 * npw * cs)
-    if name.startswith("kb_reduce"):
-        return nst * nsph * 0  # tiny
     return 0
 
 
 def b_alg(npw, N, nsph, nst, s_c, s_r, nlm=4):
-    """algorithmic bytes per state-apply of the 2-pass model
+    """algorithmic bytes per state-apply of the 2-pass FFT model:
     s_c (2 N_pw + 4 N + 3 N_sph) + [s_r N + N_sph (nlm s_r + 24)] / nst"""
     return s_c * (2 * npw + 4 * N + 3 * nsph) + (s_r * N + nsph * (nlm * s_r + 24)) / nst
+
+
+def build(ctx, sysm, v=None):
+    from inq_amd import KSHamiltonian
+    from inq_amd import systems as S
+    gh = KSHamiltonian(ctx, sysm.lattice, sysm.n)
+    gh.set_scalar_potential(S.local_potential(sysm) if v is None else v)
+    for p, D in zip(sysm.positions, sysm.kb_coeffs()):
+        gh.add_projector(p, sysm.kb_radius, sysm.nlm, D, width=sysm.kb_width)
+    return gh
+
+
+def time_applies(ctx, gh, b, d_in, d_out, steps, warmup):
+    for _ in range(warmup):
+        gh.apply(b, d_in, d_out)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gh.apply(b, d_in, d_out)
+    ctx.sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def hbm_copy(ctx, nbytes=1 << 30, reps=10):
+    from inq_amd import DeviceArray, lib
+    a = DeviceArray(ctx, (nbytes // 8,), np.complex64)
+    b = DeviceArray(ctx, (nbytes // 8,), np.complex64)
+    for _ in range(2):
+        lib().inqx_stream_copy(ctx.h, a.ptr, b.ptr, nbytes)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        lib().inqx_stream_copy(ctx.h, a.ptr, b.ptr, nbytes)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / reps
+    a.free()
+    b.free()
+    return 2 * nbytes / dt / 1e9
+
+
+def c3_kpoints(ctx, steps):
+    from inq_amd import DeviceArray
+    from inq_amd import systems as S
+    sysm = S.config(3)
+    gh = build(ctx, sysm)
+    vol = S.sphere_volume_element(sysm)
+    bases, ins, outs = [], [], []
+    for i, (k, spin, ec) in enumerate(sysm.blocks()):
+        b = gh.basis(k, ec, 0)
+        c = S.random_states(b.npw, sysm.nst, sysm.seed + i, vol).astype(np.complex64)
+        bases.append(b)
+        ins.append(DeviceArray.from_numpy(ctx, c))
+        outs.append(DeviceArray(ctx, c.shape, np.complex64))
+    npws = [b.npw for b in bases]
+    gh.apply_batch(bases, ins, outs)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gh.apply_batch(bases, ins, outs)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / steps
+    nsa = sysm.nst * len(bases)
+    return {"blocks": len(bases), "nst": sysm.nst, "npw_min": int(min(npws)), "npw_max": int(max(npws)),
+            "ms_per_batch": dt * 1e3, "state_applies_per_s": nsa / dt, "dtype": "c64"}
+
+
+def c4_tddft(ctx, nsteps):
+    from inq_amd import DeviceArray
+    from inq_amd import real_time as rt
+    from inq_amd import systems as S
+    sysm = S.config(4)
+    v = S.local_potential(sysm)
+    out = {}
+    recs = {}
+    for dtype, name in ((np.complex64, "fp32"), (np.complex128, "fp64")):
+        gh = build(ctx, sysm, v)
+        # velocity-gauge kick 0.01 along x: A = -0.01 x (src/perturbations/kick.hpp:36-48), covariant
+        gh.uniform_vector_potential = sysm.lattice @ np.array([-0.01, 0.0, 0.0])
+        b = gh.basis((0, 0, 0), sysm.ecut)
+        c = S.random_states(b.npw, sysm.nst, sysm.seed, S.sphere_volume_element(sysm))
+        d = DeviceArray.from_numpy(ctx, c.astype(dtype))
+        prop = rt.Propagator(gh, [b], [d], [np.full(sysm.nst, 2.0)], dt=0.05)
+        prop.energy()
+        prop.step(2, graph=True)  # eager first step + capture
+        ctx.sync()
+        t0 = time.perf_counter()
+        prop.step(nsteps, graph=True)
+        ctx.sync()
+        dt = (time.perf_counter() - t0) / nsteps
+        recs[name] = prop.records()
+        out[name] = {"ms_per_step": dt * 1e3, "steps_per_s": 1.0 / dt,
+                     "state_applies_per_s": 9 * sysm.nst / dt}
+        prop.close()
+        del prop
+    r32, r64 = recs["fp32"], recs["fp64"]
+    m = min(len(r32), len(r64))
+    dip_err = float(np.max(np.abs(r32[:m, 4:7] - r64[:m, 4:7])))
+    e32 = r32[:m, 1]
+    e64 = r64[:m, 1]
+    out["steps"] = int(m - 1)
+    out["dipole_trace_max_abs_err_fp32_vs_fp64"] = dip_err
+    out["energy_drift_fp64_Ha"] = float(abs(e64[-1] - e64[0]))
+    out["energy_drift_fp32_Ha"] = float(abs(e32[-1] - e32[0]))
+    out["band_energy_err_per_atom_Ha"] = float(np.max(np.abs(e32 - e64)) / sysm.natoms)
+    out["norm_err_fp32"] = float(np.max(np.abs(r32[:m, 3] - sysm.nst)))
+    return out
+
+
+def c1_diamond(ctx, cpu):
+    from inq_amd import DeviceArray
+    from inq_amd import operations as ops
+    from inq_amd import systems as S
+    sysm = S.config(1)
+    v = S.local_potential(sysm)
+    gh = build(ctx, sysm, v)
+    b = gh.basis((0, 0, 0), sysm.ecut)
+    vol = S.sphere_volume_element(sysm)
+    c = S.random_states(b.npw, sysm.nst, sysm.seed, vol)
+    d = DeviceArray.from_numpy(ctx, c)
+    h = DeviceArray(ctx, c.shape, np.complex128)
+
+    def gpu_once():
+        gh.apply(b, d, h)
+        S_ = ops.overlap(ctx, d, d, vol)
+        ops.cholesky(ctx, S_)
+
+    gpu_once()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        gpu_once()
+    ctx.sync()
+    res = {"gpu_fp64_ms": (time.perf_counter() - t0) / 20 * 1e3}
+    if cpu:
+        from oracle import oracle as orc
+        oh = orc.Ham(sysm.lattice, sysm.n)
+        oh.set_potential(v)
+        for p, D in zip(sysm.positions, sysm.kb_coeffs()):
+            oh.add_projector_gauss(p, sysm.kb_radius, sysm.nlm, sysm.kb_width, D)
+        g = b.gindex()
+
+        def cpu_once():
+            oh.apply_sphere(c, g)
+            s = orc.overlap(c, c, vol, True)
+            orc.cholesky(s)
+
+        cpu_once()
+        t0 = time.perf_counter()
+        reps = 0
+        while reps < 3 or time.perf_counter() - t0 < 3.0:
+            cpu_once()
+            reps += 1
+        res["cpu_fp64_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    return res
 
 
 def main():
@@ -59,25 +221,25 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", default="both", choices=("fp32", "fp64", "both"))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--copy", action="store_true", help="run the HBM copy reference even with --no-extras")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding of the path is described in DESIGN.md and not measured.", file=sys.stderr)
         sys.exit(2)
 
-    from inq_amd import Context, DeviceArray, KSHamiltonian
+    from inq_amd import Context, DeviceArray
     from inq_amd import systems as S
 
     sysm = S.config(2)
     log(f"bench: {sysm.name}, {sysm.natoms} atoms, nst={sysm.nst}")
     v = S.local_potential(sysm)
     ctx = Context(0)
-    gh = KSHamiltonian(ctx, sysm.lattice, sysm.n)
-    gh.set_scalar_potential(v)
-    for p, D in zip(sysm.positions, sysm.kb_coeffs()):
-        gh.add_projector(p, sysm.kb_radius, sysm.nlm, D, width=sysm.kb_width)
+    gh = build(ctx, sysm, v)
     b = gh.basis((0, 0, 0), sysm.ecut)
     vol = S.sphere_volume_element(sysm)
     c = S.random_states(b.npw, sysm.nst, sysm.seed, vol)
@@ -86,22 +248,14 @@ def main():
     n12 = sysm.n[1] * sysm.n[2]
     nsph = sum(len(gh.projector_points(a)[0]) for a in range(sysm.natoms))
 
-    res = {}
-    outs = {}
-    for dtype in (np.complex64, np.complex128):
+    res, outs = {}, {}
+    precs = {"fp32": [np.complex64], "fp64": [np.complex128], "both": [np.complex64, np.complex128]}[args.precision]
+    for dtype in precs:
         name = "fp32" if dtype == np.complex64 else "fp64"
         d_in = DeviceArray.from_numpy(ctx, c.astype(dtype))
         d_out = DeviceArray(ctx, c.shape, dtype)
         gh.prepare(b, sysm.nst, dtype)
-        for _ in range(args.warmup):
-            gh.apply(b, d_in, d_out)
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            gh.apply(b, d_in, d_out)
-        ctx.sync()
-        dt = (time.perf_counter() - t0) / args.steps
-        # per-kernel times with HIP events on the same stream
+        dt = time_applies(ctx, gh, b, d_in, d_out, args.steps, args.warmup)
         ctx.profile(True)
         ctx.profile_reset()
         for _ in range(args.steps):
@@ -110,29 +264,68 @@ def main():
         ctx.profile(False)
         outs[name] = d_out.numpy().astype(np.complex128)
         res[name] = {"ms": dt * 1e3, "rate": sysm.nst / dt, "kernels": ktimes}
-        log(f"bench: {name} {dt*1e3:.3f} ms per {sysm.nst}-state apply, {sysm.nst/dt:.4g} state-applies/s")
+        log(f"bench: {name} {dt * 1e3:.3f} ms per {sysm.nst}-state apply, {sysm.nst / dt:.4g} state-applies/s")
         for kname, (cnt, ms) in sorted(ktimes.items()):
-            log(f"   {kname:24s} {ms/cnt*1e3:9.1f} us/launch")
+            log(f"   {kname:24s} {ms / cnt * 1e3:9.1f} us/launch")
         d_in.free()
         d_out.free()
 
-    # FP32 vs FP64 accuracy on the same input
-    ci = c.astype(np.complex64).astype(np.complex128)
-    h32, h64 = outs["fp32"], outs["fp64"]
-    err_h = float(np.max(np.abs(h32 - h64)) / np.max(np.abs(h64)))
-    e32 = np.real(np.sum(ci.conj() * h32, 1)) * vol
-    e64 = np.real(np.sum(ci.conj() * h64, 1)) * vol
-    err_e_atom = float(abs(np.sum(e32) - np.sum(e64)) / sysm.natoms)
-
-    # roofline of the dominant FP32 kernel
-    k32 = res["fp32"]["kernels"]
-    dom = max(k32, key=lambda k: k32[k][1])
-    cnt, ms = k32[dom]
+    head = "fp32" if "fp32" in res else "fp64"
+    prec_bytes = 4 if head == "fp32" else 8
+    kt = res[head]["kernels"]
+    dom = max(kt, key=lambda k: kt[k][1])
+    cnt, ms = kt[dom]
     avg_s = ms / cnt * 1e-3
-    kb = kernel_bytes(dom, sysm.nst, b.npw, nslot, n12, sysm.N, 4, nsph)
+    kb = kernel_bytes(dom, sysm.nst, b.npw, nslot, n12, sysm.N, prec_bytes)
     achieved = kb / avg_s / 1e9
-    rate32 = res["fp32"]["rate"]
-    balg = b_alg(b.npw, sysm.N, nsph, sysm.nst, 8, 4)
+    rate = res[head]["rate"]
+    balg = b_alg(b.npw, sysm.N, nsph, sysm.nst, 2 * prec_bytes, prec_bytes)
+
+    out = {
+        "metric": METRIC,
+        "value": rate,
+        "unit": "state-applies/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res[head]["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 (f32)" if head == "fp32" else "c128 (f64)",
+        "data": "synthetic (seeded N(0,1) states on the G sphere, Gaussian V_loc and KB projectors)",
+        "config": {"workload": "H|psi> on Si64 2x2x2 diamond, L=20.52 bohr, 72^3 grid, Ecut 30 Ha, Gamma, "
+                               "128 states, 64 atoms x 4 KB projectors (r_c 2.0)",
+                   "npw": int(b.npw), "n_sph": int(nsph), "nst": sysm.nst, "grid": list(sysm.n)},
+        "hbm_gbs_effective_b_alg": balg * rate / 1e9,
+        "b_alg_bytes_per_state_apply": balg,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": kb, "avg_launch_us": avg_s * 1e6},
+        "kernels_us": {k: vv[1] / vv[0] * 1e3 for k, vv in kt.items()},
+    }
+    if "fp32" in res and "fp64" in res:
+        ci = c.astype(np.complex64).astype(np.complex128)
+        h32, h64 = outs["fp32"], outs["fp64"]
+        e32 = np.real(np.sum(ci.conj() * h32, 1)) * vol
+        e64 = np.real(np.sum(ci.conj() * h64, 1)) * vol
+        out["fp64"] = {"value": res["fp64"]["rate"], "ms_per_step": res["fp64"]["ms"],
+                       "kernels_us": {k: vv[1] / vv[0] * 1e3 for k, vv in res["fp64"]["kernels"].items()}}
+        out["speedup_fp32_vs_fp64"] = res["fp32"]["rate"] / res["fp64"]["rate"]
+        out["err_fp32_vs_fp64"] = {"hpsi_rel_max": float(np.max(np.abs(h32 - h64)) / np.max(np.abs(h64))),
+                                   "band_energy_per_atom_Ha": float(abs(np.sum(e32) - np.sum(e64)) / sysm.natoms)}
+
+    if args.copy and args.no_extras:
+        out["hbm_copy_gbs"] = hbm_copy(ctx)
+    if not args.no_extras:
+        out["hbm_copy_gbs"] = hbm_copy(ctx)
+        log(f"bench: hbm copy {out['hbm_copy_gbs']:.0f} GB/s")
+        out["c3_kpoints"] = c3_kpoints(ctx, max(3, args.steps // 4))
+        log(f"bench: c3 {out['c3_kpoints']}")
+        out["c4_tddft"] = c4_tddft(ctx, max(4, args.steps // 2))
+        log(f"bench: c4 {out['c4_tddft']}")
+        out["c1_diamond"] = c1_diamond(ctx, not args.no_cpu)
+        log(f"bench: c1 {out['c1_diamond']}")
 
     cpu = None
     if not args.no_cpu:
@@ -157,34 +350,8 @@ def main():
         cpu = {"value": nsub / tcpu, "unit": "state-applies/s", "cores": cores, "kind": "port",
                "sample": f"{reps} FP64 oracle applies of {nsub} of the {sysm.nst} Si64 states "
                          f"(full 72^3 FFTs, OpenMP over lines, own Stockham FFT)"}
-        log(f"bench: cpu oracle {nsub/tcpu:.4g} state-applies/s on {cores} threads")
-
-    out = {
-        "metric": METRIC,
-        "value": rate32,
-        "unit": "state-applies/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": res["fp32"]["ms"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "c64 (f32)",
-        "data": "synthetic (seeded N(0,1) states on the G sphere, Gaussian V_loc and KB projectors)",
-        "config": {"workload": "H|psi> on Si64 2x2x2 diamond, L=20.52 bohr, 72^3 grid, Ecut 30 Ha, Gamma, "
-                               "128 states, 64 atoms x 4 KB projectors (r_c 2.0)",
-                   "npw": int(b.npw), "n_sph": int(nsph), "nst": sysm.nst, "grid": list(sysm.n)},
-        "fp64": {"value": res["fp64"]["rate"], "ms_per_step": res["fp64"]["ms"]},
-        "speedup_fp32_vs_fp64": rate32 / res["fp64"]["rate"],
-        "err_fp32_vs_fp64": {"hpsi_rel_max": err_h, "band_energy_per_atom_Ha": err_e_atom},
-        "hbm_gbs_effective_b_alg": balg * rate32 / 1e9,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": kb, "avg_launch_us": avg_s * 1e6},
-        "kernels_us": {k: v[1] / v[0] * 1e3 for k, v in k32.items()},
-        "cpu_baseline": cpu,
-    }
+        log(f"bench: cpu oracle {nsub / tcpu:.4g} state-applies/s on {cores} threads")
+    out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
 

@@ -82,6 +82,11 @@ int inqx_memcpy_h2d(inqx_ctx ctx, void *dst, const void *src, size_t bytes);
 int inqx_memcpy_d2h(inqx_ctx ctx, void *dst, const void *src, size_t bytes);
 int inqx_memcpy_d2d(inqx_ctx ctx, void *dst, const void *src, size_t bytes);
 
+/* device-to-device copy with a plain streaming kernel (bytes a multiple of
+ * 16): the HBM bandwidth reference the roofline figures are checked against
+ * (the role of speed_tests/copies.cpp in the reference) */
+int inqx_stream_copy(inqx_ctx ctx, const void *src, void *dst, size_t bytes);
+
 /* ------------------------------------------------------------ Hamiltonian */
 /* hamiltonian::ks_hamiltonian constructor (src/hamiltonian/ks_hamiltonian.hpp:92-102):
  * lattice rows are the cell vectors in bohr (src/systems/cell.hpp:36-52), n the

@@ -37,6 +37,7 @@ SIGNATURES = {
     "inqx_memcpy_h2d": (C.c_int, [P, P, P, C.c_size_t]),
     "inqx_memcpy_d2h": (C.c_int, [P, P, P, C.c_size_t]),
     "inqx_memcpy_d2d": (C.c_int, [P, P, P, C.c_size_t]),
+    "inqx_stream_copy": (C.c_int, [P, P, P, C.c_size_t]),
     "inqx_ham_create": (C.c_int, [P, DP, IP, PP]),
     "inqx_ham_destroy": (C.c_int, [P]),
     "inqx_ham_set_potential": (C.c_int, [P, DP, C.c_int]),

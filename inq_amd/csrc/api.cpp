@@ -760,4 +760,11 @@ int inqx_subspace_diag(inqx_ham h, inqx_basis bs, int nst, int precision, void *
   API_END(ctx)
 }
 
+int inqx_stream_copy(inqx_ctx ctx, const void *src, void *dst, size_t bytes) {
+  API_BEGIN
+  INQX_CHECK(ctx && !ctx->c.host_only && src && dst && bytes % 16 == 0, INQX_ERR_ARG, "bad arguments");
+  launch_stream_copy(ctx->c, src, dst, bytes);
+  API_END(ctx)
+}
+
 }  // extern "C"

@@ -255,4 +255,5 @@ void launch_rotate(Ctx &cx, int prec, int n, long K, const double *R, int mode, 
                    void *Out, double br, double bi, DevMem &ops);
 void launch_cholesky(Ctx &cx, int n, double *S, int *info);
 void launch_trinv(Ctx &cx, int n, const double *L, double *X);
+void launch_stream_copy(Ctx &cx, const void *src, void *dst, size_t bytes);
 }  // namespace inqx

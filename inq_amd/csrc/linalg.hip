@@ -37,6 +37,9 @@ template <typename T> __device__ __forceinline__ const GAS T *lgp(const T *p) { 
 template <typename T> __device__ __forceinline__ GAS T *lgw(T *p) { return (GAS T *)p; }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+struct alignas(16) V4f {
+  float x, y, z, w;
+};
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 // row of the 16x16 output tile held in accumulator r of a lane (column = lane % 16)
@@ -356,6 +359,14 @@ __global__ void k_trinv(const double *L, int n, double *X) {
   }
 }
 
+// ------------------------------------------------------------------------ HBM copy (calibration)
+// plain 16-byte-per-lane device copy: the STREAM-style bandwidth reference and
+// the known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters
+__global__ void __launch_bounds__(256) k_stream_copy(const V4f *src, V4f *dst, long n) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // ------------------------------------------------------------------------ host launchers
 void launch_overlap(Ctx &cx, int prec, int n1, int n2, long K, const void *A, const void *B, double vol, double *S,
                     DevMem &partial) {
@@ -415,6 +426,14 @@ void launch_rotate(Ctx &cx, int prec, int n, long K, const double *R, int mode, 
 void launch_cholesky(Ctx &cx, int n, double *S, int *info) {
   ProfScope ps(cx, "cholesky_f64");
   hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(n >= 1024 ? 1024 : (n >= 256 ? 256 : 64)), 0, cx.stream, S, n, info);
+  INQX_HIP(hipGetLastError());
+}
+
+void launch_stream_copy(Ctx &cx, const void *src, void *dst, size_t bytes) {
+  const long n = (long)(bytes / 16);
+  const int grid = (int)std::min<long>((n + 255) / 256, (long)cx.num_cu * 32);
+  ProfScope ps(cx, "stream_copy");
+  hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, cx.stream, (const V4f *)src, (V4f *)dst, n);
   INQX_HIP(hipGetLastError());
 }
 
