@@ -56,6 +56,25 @@ def kernel_bytes(name, nst, npw, nslot, n12, N, prec_bytes):
     return 0
 
 
+COUNTER_PROFILE = os.path.join(ROOT, "profiles", "hpsi_si64_fp32_hbm_counters.txt")
+
+
+def counter_traffic(kernel):
+    """HBM bytes per launch of a complex64 Si64 pipeline kernel, from the committed
+    FETCH_SIZE/WRITE_SIZE counter passes (already scaled by the copy-kernel
+    calibration: 2 x FETCH + 1 x WRITE).  None when the kernel has no row."""
+    stem = kernel.rsplit("_f", 1)[0]
+    try:
+        with open(COUNTER_PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 5 and parts[0].startswith("k_" + stem + "<"):
+                    return float(parts[4]) * 2**20
+    except OSError:
+        return None
+    return None
+
+
 def b_alg(npw, N, nsph, nst, s_c, s_r, nlm=4):
     """algorithmic bytes per state-apply of the 2-pass FFT model:
     s_c (2 N_pw + 4 N + 3 N_sph) + [s_r N + N_sph (nlm s_r + 24)] / nst"""
@@ -300,7 +319,9 @@ def main():
         "hbm_gbs_effective_b_alg": balg * rate / 1e9,
         "b_alg_bytes_per_state_apply": balg,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": counter_traffic(dom) if head == "fp32" else None,
+                     "traffic_source": "profiles/hpsi_si64_fp32_hbm_counters.txt (2 x FETCH_SIZE + WRITE_SIZE)",
                      "bytes_per_launch": kb, "avg_launch_us": avg_s * 1e6},
         "kernels_us": {k: vv[1] / vv[0] * 1e3 for k, vv in kt.items()},
     }
