@@ -192,10 +192,47 @@ int inqx_prop_records(inqx_prop prop, int *count, double *records, int cap);
  * the propagator is destroyed */
 int inqx_prop_destroy(inqx_prop prop);
 
-/* ------------------------------------------------------------ sphere <-> grids */
-/* operations::transform::to_real (transform.hpp:220-249) of sphere states into
- * the reference real-space layout [N][nst] (normalised by 1/N) */
-int inqx_sphere_to_real(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *psi, void *psi_rs);
+/* ------------------------------------------------------------ sphere <-> real space */
+/* real-space layouts: state-major [nst][N] (the layout of the exchange
+ * entry points), or point-major [N][nst], the reference's field_set layout
+ * (src/basis/field_set.hpp:52,62); N = n0 n1 n2, point index (i0 n1 + i1) n2 + i2 */
+enum { INQX_STATE_MAJOR = 0, INQX_POINT_MAJOR = 1 };
+/* operations::transform::to_real (src/operations/transform.hpp:113-182) of
+ * sphere states: psi_rs = (1/N) IFFT(scatter(psi)) */
+int inqx_sphere_to_real(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *psi, int layout,
+                        void *psi_rs);
+/* operations::transform::to_fourier (transform.hpp:52-109) restricted to the
+ * sphere: psi = mask(FFT(psi_rs)) (unnormalised); accumulate != 0 adds into psi */
+int inqx_real_to_sphere(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *psi_rs, int layout,
+                        int accumulate, void *psi);
+
+/* ------------------------------------------------------------ exact exchange */
+/* solvers::poisson::in_place (src/solvers/poisson.hpp:71-110,199-213) on n
+ * real-space densities, state-major [n][N], in place: rho_G <- -4 pi/N *
+ * (-1/|G+q|^2) rho_G, zeroterm at |G+q|^2 < 1e-6, unnormalised inverse FFT.
+ * gshift_cart = q (cartesian). */
+int inqx_poisson(inqx_ham ham, int n, int precision, void *rho, const double gshift_cart[3], double zeroterm);
+/* exchange_operator::direct (src/hamiltonian/exchange_operator.hpp:124-186)
+ * at the Gamma point, in sphere form: exxphi += mask(FFT(sum_j f occ_j hf_j
+ * Poisson(conj(hf_j) phi_i))), f = -exx_coeff scale / 2, phi_i = to_real of
+ * the sphere states (j with |occ_j| < 1e-10 skipped).  phi [nst][npw], hf
+ * [nhf][npw] and exxphi [nst][npw] on the basis' sphere; occ host [nhf]. */
+int inqx_exchange_direct(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *phi, int nhf,
+                         const void *hf, const double *occ, double exx_coeff, double scale, double zeroterm,
+                         void *exxphi);
+/* exchange_operator::update's ACE build (exchange_operator.hpp:93-117):
+ * W = direct(phi, -1), M = vol_el W^H phi (FP64), E_x = -1/2 sum occ_i
+ * Re M_ii, M = L L^H, xi = W L^{-H}.  xi [nst][npw] on the sphere (the
+ * sphere projection of the reference's real-space xi); *energy = E_x. */
+int inqx_ace_build(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *phi, const double *occ,
+                   double exx_coeff, double zeroterm, void *xi, double *energy);
+/* exchange_operator::ace (exchange_operator.hpp:217-231):
+ * exxphi <- exxphi - xi (vol_el xi^H psi), xi [nxi][npw], psi/exxphi [nst][npw] */
+int inqx_ace_apply(inqx_ctx ctx, int precision, int nxi, int nst, long npw, double vol_el, const void *xi,
+                   const void *psi, void *exxphi);
+/* upper bound on the pair workspace of the exchange entry points (bytes,
+ * default 4 GiB; at least one j times all states is always used) */
+int inqx_ham_set_exchange_workspace(inqx_ham ham, size_t bytes);
 
 #ifdef __cplusplus
 }

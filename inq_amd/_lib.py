@@ -53,7 +53,14 @@ SIGNATURES = {
     "inqx_ham_prepare": (C.c_int, [P, P, C.c_int, C.c_int]),
     "inqx_ham_apply": (C.c_int, [P, P, C.c_int, C.c_int, P, P]),
     "inqx_ham_apply_batch": (C.c_int, [P, C.c_int, PP, IP, C.c_int, PP, PP]),
-    "inqx_sphere_to_real": (C.c_int, [P, P, C.c_int, C.c_int, P, P]),
+    "inqx_sphere_to_real": (C.c_int, [P, P, C.c_int, C.c_int, P, C.c_int, P]),
+    "inqx_real_to_sphere": (C.c_int, [P, P, C.c_int, C.c_int, P, C.c_int, C.c_int, P]),
+    "inqx_poisson": (C.c_int, [P, C.c_int, C.c_int, P, DP, C.c_double]),
+    "inqx_exchange_direct": (C.c_int, [P, P, C.c_int, C.c_int, P, C.c_int, P, DP, C.c_double, C.c_double,
+                                       C.c_double, P]),
+    "inqx_ace_build": (C.c_int, [P, P, C.c_int, C.c_int, P, DP, C.c_double, C.c_double, P, DP]),
+    "inqx_ace_apply": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_long, C.c_double, P, P, P]),
+    "inqx_ham_set_exchange_workspace": (C.c_int, [P, C.c_size_t]),
     "inqx_overlap": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_long, C.c_double, P, P, P]),
     "inqx_cholesky": (C.c_int, [P, C.c_int, P]),
     "inqx_invert_triangular": (C.c_int, [P, C.c_int, P, P]),

@@ -634,14 +634,47 @@ int inqx_ham_apply(inqx_ham h, inqx_basis bs, int nst, int precision, const void
   return inqx_ham_apply_batch(h, 1, &bs, &nst, precision, &psi, &hpsi);
 }
 
-int inqx_sphere_to_real(inqx_ham h, inqx_basis bs, int nst, int precision, const void *psi, void *psi_rs) {
-  (void)bs;
-  (void)nst;
-  (void)precision;
-  (void)psi;
-  (void)psi_rs;
-  if (h) h->ctx->c.err = "inqx_sphere_to_real: not built yet";
-  return INQX_ERR_UNSUPPORTED;
+// sphere <-> real space through the apply pipeline's transform modes
+static void transform_impl(inqx_ham h, Basis &B, int nst, int precision, int mode, const void *sph_in, void *rs,
+                           int layout, bool accumulate, void *sph_out) {
+  ApplyOpts o;
+  o.mode = mode;
+  o.rs = &rs;
+  o.rs_layout = layout;
+  o.accumulate = accumulate;
+  Basis *bb = &B;
+  const void *pin = mode == 1 ? sph_in : sph_out;
+  void *pout = mode == 1 ? rs : sph_out;
+  if (precision == INQX_FP64) launch_apply<double>(h, 1, &bb, &nst, &pin, &pout, &o);
+  else launch_apply<float>(h, 1, &bb, &nst, &pin, &pout, &o);
+}
+
+static void check_transform(inqx_ham h, inqx_basis bs, int nst, int precision, int layout) {
+  INQX_CHECK(h && bs && nst > 0, INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!h->ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(bs->b.ham == h, INQX_ERR_ARG, "basis belongs to another Hamiltonian");
+  INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
+  INQX_CHECK(layout == INQX_STATE_MAJOR || layout == INQX_POINT_MAJOR, INQX_ERR_ARG, "bad layout");
+}
+
+int inqx_sphere_to_real(inqx_ham h, inqx_basis bs, int nst, int precision, const void *psi, int layout,
+                        void *psi_rs) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_transform(h, bs, nst, precision, layout);
+  INQX_CHECK(psi && psi_rs && psi != psi_rs, INQX_ERR_ARG, "bad state pointers");
+  transform_impl(h, bs->b, nst, precision, 1, psi, psi_rs, layout, false, nullptr);
+  API_END(ctx)
+}
+
+int inqx_real_to_sphere(inqx_ham h, inqx_basis bs, int nst, int precision, const void *psi_rs, int layout,
+                        int accumulate, void *psi) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_transform(h, bs, nst, precision, layout);
+  INQX_CHECK(psi && psi_rs && psi != psi_rs, INQX_ERR_ARG, "bad state pointers");
+  transform_impl(h, bs->b, nst, precision, 2, nullptr, const_cast<void *>(psi_rs), layout, accumulate != 0, psi);
+  API_END(ctx)
 }
 
 
@@ -689,7 +722,7 @@ int inqx_invert_triangular(inqx_ctx ctx, int n, const void *L, void *Linv) {
 static void rotate_inplace(inqx_ctx ctx, int precision, int n, long npw, const double *R, int mode, void *phi) {
   const size_t cs = precision == INQX_FP64 ? 16 : 8;
   ctx->c.ws_tmp.ensure(cs * (size_t)n * npw);
-  launch_rotate(ctx->c, precision, n, npw, R, mode, 1.0, 0.0, phi, ctx->c.ws_tmp.p, 0.0, 0.0, ctx->c.ws_ops);
+  launch_rotate(ctx->c, precision, n, n, npw, R, mode, 1.0, 0.0, phi, ctx->c.ws_tmp.p, 0.0, 0.0, ctx->c.ws_ops);
   INQX_HIP(hipMemcpyAsync(phi, ctx->c.ws_tmp.p, cs * (size_t)n * npw, hipMemcpyDeviceToDevice, ctx->c.stream));
 }
 
@@ -706,7 +739,7 @@ int inqx_rotate_add(inqx_ctx ctx, int precision, int n, long npw, const void *ro
   API_BEGIN
   check_states(ctx, precision, n, npw);
   INQX_CHECK(rot && phi && rotphi && phi != rotphi, INQX_ERR_ARG, "bad arguments");
-  launch_rotate(ctx->c, precision, n, npw, (const double *)rot, 1, alpha_re, alpha_im, phi, rotphi, beta_re, beta_im,
+  launch_rotate(ctx->c, precision, n, n, npw, (const double *)rot, 1, alpha_re, alpha_im, phi, rotphi, beta_re, beta_im,
                 ctx->c.ws_ops);
   API_END(ctx)
 }
@@ -753,10 +786,148 @@ int inqx_subspace_diag(inqx_ham h, inqx_basis bs, int nst, int precision, void *
   INQX_HIP(hipMemcpyAsync(ctx->c.ws_S.p, u.data(), 16 * (size_t)nst * nst, hipMemcpyHostToDevice, ctx->c.stream));
   INQX_HIP(hipStreamSynchronize(ctx->c.stream));
   ctx->c.ws_tmp.ensure(cs * (size_t)nst * npw);
-  launch_rotate(ctx->c, precision, nst, npw, ctx->c.ws_S.as<double>(), 2, 1.0, 0.0, phi, ctx->c.ws_tmp.p, 0.0, 0.0,
+  launch_rotate(ctx->c, precision, nst, nst, npw, ctx->c.ws_S.as<double>(), 2, 1.0, 0.0, phi, ctx->c.ws_tmp.p, 0.0, 0.0,
                 ctx->c.ws_ops);
   INQX_HIP(hipMemcpyAsync(phi, ctx->c.ws_tmp.p, cs * (size_t)nst * npw, hipMemcpyDeviceToDevice, ctx->c.stream));
   for (int k = 0; k < nst; ++k) eigenvalues[k] = w[k];
+  API_END(ctx)
+}
+
+// ---------------------------------------------------------------- exact exchange
+int inqx_poisson(inqx_ham h, int n, int precision, void *rho, const double gshift_cart[3], double zeroterm) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && n > 0 && rho, INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
+  PairArgs a;
+  a.phi = rho;
+  a.t = rho;
+  a.w = rho;
+  a.nst = n;
+  a.nj = 1;
+  if (gshift_cart)
+    for (int d = 0; d < 3; ++d) a.gshift[d] = gshift_cart[d];
+  a.zeroterm = zeroterm;
+  launch_pairs(h, precision, a);
+  API_END(ctx)
+}
+
+// exchange_operator::direct in sphere form: out (+)= mask(FFT(sum_j f occ_j hf_j Poisson(conj(hf_j) phi_i)))
+static void exchange_direct_impl(inqx_ham h, Basis &B, int nst, int prec, const void *phi, int nhf, const void *hf,
+                                 const double *occ, double coeff, double scale, double zeroterm, void *out,
+                                 bool accumulate) {
+  Ctx &cx = h->ctx->c;
+  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "exchange needs a cubic grid of a built size");
+  const size_t per = (prec == INQX_FP64 ? 16 : 8) * (size_t)h->g.N;
+  const double factor = -0.5 * scale * coeff;
+  std::vector<int> jl;
+  std::vector<double> cf(nhf);
+  for (int j = 0; j < nhf; ++j) {
+    cf[j] = factor * occ[j];
+    if (std::fabs(occ[j]) >= 1e-10) jl.push_back(j);  // exchange_operator.hpp:144
+  }
+  h->exx_phi.ensure(per * nst);
+  transform_impl(h, B, nst, prec, 1, phi, h->exx_phi.p, INQX_STATE_MAJOR, false, nullptr);
+  const void *hf_rs = h->exx_phi.p;
+  if (!(hf == phi && nhf == nst)) {
+    h->exx_hf.ensure(per * nhf);
+    transform_impl(h, B, nhf, prec, 1, hf, h->exx_hf.p, INQX_STATE_MAJOR, false, nullptr);
+    hf_rs = h->exx_hf.p;
+  }
+  h->exx_w.ensure(per * nst);
+  launch_fill_zero(cx, h->exx_w.p, per * nst);
+  if (!jl.empty()) {
+    upload(h->exx_jlist, jl, cx.stream);
+    upload(h->exx_coef, cf, cx.stream);
+    INQX_HIP(hipStreamSynchronize(cx.stream));
+    const size_t njmax = std::max<size_t>(1, std::min<size_t>(jl.size(), h->exx_pair_bytes / (per * nst)));
+    h->exx_t.ensure(per * nst * njmax);
+    for (size_t c0 = 0; c0 < jl.size(); c0 += njmax) {
+      PairArgs a;
+      a.phi = h->exx_phi.p;
+      a.hf = hf_rs;
+      a.t = h->exx_t.p;
+      a.w = h->exx_w.p;
+      a.jlist = h->exx_jlist.as<int>() + c0;
+      a.coef = h->exx_coef.as<double>();
+      a.nst = nst;
+      a.nj = (int)std::min(njmax, jl.size() - c0);
+      a.zeroterm = zeroterm;
+      launch_pairs(h, prec, a);
+    }
+  }
+  transform_impl(h, B, nst, prec, 2, nullptr, h->exx_w.p, INQX_STATE_MAJOR, accumulate, out);
+}
+
+static void check_exchange(inqx_ham h, inqx_basis bs, int nst, int precision) {
+  INQX_CHECK(h && bs && nst > 0, INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!h->ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(bs->b.ham == h, INQX_ERR_ARG, "basis belongs to another Hamiltonian");
+  INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
+  const double *k = bs->b.kcov;
+  INQX_CHECK(k[0] == 0.0 && k[1] == 0.0 && k[2] == 0.0 && h->A[0] == 0.0 && h->A[1] == 0.0 && h->A[2] == 0.0,
+             INQX_ERR_UNSUPPORTED, "exchange is built for the Gamma point only");
+}
+
+int inqx_exchange_direct(inqx_ham h, inqx_basis bs, int nst, int precision, const void *phi, int nhf, const void *hf,
+                         const double *occ, double exx_coeff, double scale, double zeroterm, void *exxphi) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_exchange(h, bs, nst, precision);
+  INQX_CHECK(phi && hf && occ && exxphi && nhf > 0, INQX_ERR_ARG, "null argument");
+  INQX_CHECK(exxphi != phi && exxphi != hf, INQX_ERR_ARG, "exxphi must not alias the inputs");
+  exchange_direct_impl(h, bs->b, nst, precision, phi, nhf, hf, occ, exx_coeff, scale, zeroterm, exxphi, true);
+  API_END(ctx)
+}
+
+int inqx_ace_build(inqx_ham h, inqx_basis bs, int nst, int precision, const void *phi, const double *occ,
+                   double exx_coeff, double zeroterm, void *xi, double *energy) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_exchange(h, bs, nst, precision);
+  INQX_CHECK(phi && occ && xi && energy && xi != phi, INQX_ERR_ARG, "bad arguments");
+  Basis &B = bs->b;
+  const long npw = B.npw;
+  // W = direct(phi, -1) on the sphere (exchange_operator.hpp:105)
+  exchange_direct_impl(h, B, nst, precision, phi, nst, phi, occ, exx_coeff, -1.0, zeroterm, xi, false);
+  // M = vol_el W^H phi; phi is band limited, so the sphere sum equals the real-space one
+  const double vol = h->g.vol / ((double)h->g.N * (double)h->g.N);
+  ctx->c.ws_S.ensure(16 * (size_t)nst * nst);
+  ctx->c.ws_X.ensure(16 * (size_t)nst * nst);
+  double *S = ctx->c.ws_S.as<double>(), *X = ctx->c.ws_X.as<double>();
+  launch_overlap(ctx->c, precision, nst, nst, npw, xi, phi, vol, S, ctx->c.ws_partial);
+  std::vector<std::complex<double>> m((size_t)nst * nst);
+  INQX_HIP(hipMemcpyAsync(m.data(), S, 16 * (size_t)nst * nst, hipMemcpyDeviceToHost, ctx->c.stream));
+  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+  double en = 0.0;  // -1/2 sum occ_i M_ii (exchange_operator.hpp:108)
+  for (int i = 0; i < nst; ++i) en += -0.5 * occ[i] * m[(size_t)i * nst + i].real();
+  const int info = cholesky_impl(ctx, nst, S);
+  INQX_CHECK(info == 0, INQX_ERR_NOT_PD, "ACE: exchange matrix not positive definite (minor " + std::to_string(info) + ")");
+  launch_trinv(ctx->c, nst, S, X);
+  rotate_inplace(ctx, precision, nst, npw, X, 0, xi);  // xi <- W L^{-H} (rotate_trs)
+  *energy = en;
+  API_END(ctx)
+}
+
+int inqx_ace_apply(inqx_ctx ctx, int precision, int nxi, int nst, long npw, double vol_el, const void *xi,
+                   const void *psi, void *exxphi) {
+  API_BEGIN
+  check_states(ctx, precision, nst, npw);
+  INQX_CHECK(nxi > 0 && xi && psi && exxphi && exxphi != xi && exxphi != psi, INQX_ERR_ARG, "bad arguments");
+  ctx->c.ws_S.ensure(16 * (size_t)nxi * nst);
+  double *S = ctx->c.ws_S.as<double>();
+  launch_overlap(ctx->c, precision, nxi, nst, npw, xi, psi, vol_el, S, ctx->c.ws_partial);  // [nxi][nst]
+  // exxphi_i <- exxphi_i - sum_a S[a][i] xi_a  (rotate mode 1: M = -S^T)
+  launch_rotate(ctx->c, precision, nst, nxi, npw, S, 1, -1.0, 0.0, xi, exxphi, 1.0, 0.0, ctx->c.ws_ops);
+  API_END(ctx)
+}
+
+int inqx_ham_set_exchange_workspace(inqx_ham h, size_t bytes) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h, INQX_ERR_ARG, "null Hamiltonian");
+  h->exx_pair_bytes = bytes;
   API_END(ctx)
 }
 

@@ -1,0 +1,303 @@
+// exx_kernels.hip -- batched pair-density Poisson solves for exact exchange, gfx950.
+//
+// The reference's exchange_operator::block_exchange
+// (src/hamiltonian/exchange_operator.hpp:124-161) loops over the occupied
+// states j and, for all states i at once, forms rho_ij = conj(hf_j) phi_i,
+// solves Poisson (solvers::poisson::in_place, src/solvers/poisson.hpp:71-110,
+// 199-213: forward FFT, v_G = -4 pi/N * (-1/|G+q|^2) rho_G, zeroterm at
+// |G+q|^2 < 1e-6, unnormalised inverse FFT) and accumulates
+// exxphi_i += factor occ_j hf_j v_ij.
+//
+// Here a launch covers a chunk of j's times all i, as three kernels over
+// real-space arrays in state-major layout [state][i0][i1][i2]:
+//
+//   pair_plane_fwd   per (j, plane i0, i): forms rho on the fly from the two
+//                    planes, 2-D forward FFT along i1 (registers, one column
+//                    per lane) then i2 (LDS rows), stores the plane of t
+//   pair_xline       per (j, i, 64 lines (i1,i2)): forward FFT along i0 in
+//                    registers, Poisson kernel, inverse FFT along i0, in place
+//   pair_plane_inv   per (plane i0, i), looping over the chunk's j in order:
+//                    2-D inverse FFT, multiplies by coef_j hf_j and adds into
+//                    w_i (deterministic order; the w plane stays in cache
+//                    between consecutive j)
+//
+// Without hf the same kernels solve Poisson in place on a batch of densities.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GAS __attribute__((address_space(1)))
+#else
+#define GAS
+#endif
+
+namespace inqx {
+
+namespace {
+
+template <typename T> __device__ __forceinline__ const GAS T *xg(const T *p) { return (const GAS T *)p; }
+template <typename T> __device__ __forceinline__ GAS T *xw(T *p) { return (GAS T *)p; }
+
+template <typename T> struct PairDev {
+  const cplx<T> *phi;  // [nst][N]
+  const cplx<T> *hf;   // [nhf][N] or null
+  cplx<T> *t;          // [nj][nst][N]
+  cplx<T> *w;          // [nst][N]
+  const int *jlist;    // [nj]
+  const double *coef;  // [nhf]
+  int nst, nj;
+  T b[3][3];           // reciprocal lattice vectors (rows, cartesian, 2 pi included)
+  T q[3];              // cartesian shift added to G
+  T scal;              // 4 pi / N
+  T zero;              // value of the kernel at |G + q|^2 < 1e-6: zeroterm / N
+};
+
+template <typename T, int N1, int N2> struct PairPlaneCfg {
+  static constexpr int STRIDE = N2 + 1;  // padded LDS row
+  static constexpr int THREADS = ((N1 > N2 ? N1 : N2) + 63) / 64 * 64;  // <= 128 (launch bound)
+  static_assert(THREADS <= 128, "plane pass sized for at most 128 lines");
+  static constexpr int LDS = N1 * STRIDE * (int)sizeof(cplx<T>);
+};
+
+template <int N> __host__ __device__ constexpr int symi(int i) { return i >= (N + 1) / 2 ? i - N : i; }
+
+template <typename T, int N0, int N1, int N2>
+__global__ void __launch_bounds__(128) k_pair_plane_fwd(PairDev<T> d, int total) {
+  using C = cplx<T>;
+  using Cfg = PairPlaneCfg<T, N1, N2>;
+  constexpr int NP = N1 * N2;
+  constexpr long N = (long)N0 * NP;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C *lds = reinterpret_cast<C *>(smem_raw);
+  const int tid = threadIdx.x;
+  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+    const int i = wg % d.nst;
+    const int r = wg / d.nst;
+    const int i0 = r % N0;
+    const int jj = r / N0;
+    const size_t plane = (size_t)i0 * NP;
+    const GAS C *ph = xg(d.phi) + (size_t)i * N + plane;
+    const GAS C *hf = nullptr;
+    if (d.hf) hf = xg(d.hf) + (size_t)xg(d.jlist)[jj] * N + plane;
+    if (tid < N2) {
+      const int z = tid;
+      C x[N1];
+      if (hf) {
+#pragma unroll
+        for (int y = 0; y < N1; ++y) {
+          const C a = ph[y * N2 + z], h = hf[y * N2 + z];
+          x[y] = C{h.x * a.x + h.y * a.y, h.x * a.y - h.y * a.x};  // conj(h) a
+        }
+      } else {
+#pragma unroll
+        for (int y = 0; y < N1; ++y) x[y] = ph[y * N2 + z];
+      }
+      fft_line<N1, -1, C>::run(x);
+#pragma unroll
+      for (int y = 0; y < N1; ++y) lds[y * Cfg::STRIDE + z] = x[y];
+    }
+    __syncthreads();
+    if (tid < N1) {
+      C *row = lds + tid * Cfg::STRIDE;
+      C x[N2];
+#pragma unroll
+      for (int z = 0; z < N2; ++z) x[z] = row[z];
+      fft_line<N2, -1, C>::run(x);
+#pragma unroll
+      for (int z = 0; z < N2; ++z) row[z] = x[z];
+    }
+    __syncthreads();
+    GAS C *tp = xw(d.t) + ((size_t)jj * d.nst + i) * N + plane;
+    for (int e = tid; e < NP; e += Cfg::THREADS) {
+      const int y = e / N2, z = e - y * N2;
+      tp[e] = lds[y * Cfg::STRIDE + z];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int N0, int N1, int N2>
+__global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
+  using C = cplx<T>;
+  constexpr int NP = N1 * N2;
+  constexpr int NTILE = (NP + 63) / 64;
+  constexpr int NGRP = (NTILE + 3) / 4;
+  constexpr long N = (long)N0 * NP;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+    const int pair = wg / NGRP;
+    const int tile = (wg - pair * NGRP) * 4 + wave;
+    if (tile >= NTILE) continue;  // wave-uniform
+    const int L = tile * 64 + lane;
+    const int Lc = L < NP ? L : NP - 1;
+    GAS C *tp = xw(d.t) + (size_t)pair * N + Lc;
+    C x[N0];
+#pragma unroll
+    for (int i0 = 0; i0 < N0; ++i0) x[i0] = tp[(size_t)i0 * NP];
+    fft_line<N0, -1, C>::run(x);
+    // G + q = m0 b0 + (m1 b1 + m2 b2 + q)
+    const int i1 = Lc / N2, i2 = Lc - (Lc / N2) * N2;
+    const T m1 = (T)(i1 >= (N1 + 1) / 2 ? i1 - N1 : i1), m2 = (T)(i2 >= (N2 + 1) / 2 ? i2 - N2 : i2);
+    T u[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[c] = m1 * d.b[1][c] + m2 * d.b[2][c] + d.q[c];
+#pragma unroll
+    for (int i0 = 0; i0 < N0; ++i0) {
+      const T m0 = (T)symi<N0>(i0);
+      const T g0 = m0 * d.b[0][0] + u[0], g1 = m0 * d.b[0][1] + u[1], g2 = m0 * d.b[0][2] + u[2];
+      const T gg = g0 * g0 + g1 * g1 + g2 * g2;
+      const T f = gg < (T)1e-6 ? d.zero : d.scal / gg;
+      x[i0].x *= f;
+      x[i0].y *= f;
+    }
+    fft_line<N0, +1, C>::run(x);
+    if (L < NP) {
+#pragma unroll
+      for (int i0 = 0; i0 < N0; ++i0) tp[(size_t)i0 * NP] = x[i0];
+    }
+  }
+}
+
+template <typename T, int N0, int N1, int N2>
+__global__ void __launch_bounds__(128) k_pair_plane_inv(PairDev<T> d, int total) {
+  using C = cplx<T>;
+  using Cfg = PairPlaneCfg<T, N1, N2>;
+  constexpr int NP = N1 * N2;
+  constexpr long N = (long)N0 * NP;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C *lds = reinterpret_cast<C *>(smem_raw);
+  const int tid = threadIdx.x;
+  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+    const int i = wg % d.nst;
+    const int i0 = wg / d.nst;
+    const size_t plane = (size_t)i0 * NP;
+    GAS C *wp = xw(d.w) + (size_t)i * N + plane;
+    for (int jj = 0; jj < d.nj; ++jj) {
+      const GAS C *tp = xg(d.t) + ((size_t)jj * d.nst + i) * N + plane;
+      for (int e = tid; e < NP; e += Cfg::THREADS) {
+        const int y = e / N2, z = e - y * N2;
+        lds[y * Cfg::STRIDE + z] = tp[e];
+      }
+      __syncthreads();
+      if (tid < N1) {
+        C *row = lds + tid * Cfg::STRIDE;
+        C x[N2];
+#pragma unroll
+        for (int z = 0; z < N2; ++z) x[z] = row[z];
+        fft_line<N2, +1, C>::run(x);
+#pragma unroll
+        for (int z = 0; z < N2; ++z) row[z] = x[z];
+      }
+      __syncthreads();
+      if (tid < N2) {
+        const int z = tid;
+        C x[N1];
+#pragma unroll
+        for (int y = 0; y < N1; ++y) x[y] = lds[y * Cfg::STRIDE + z];
+        fft_line<N1, +1, C>::run(x);
+        if (d.hf) {
+          const int j = xg(d.jlist)[jj];
+          const T c = (T)xg(d.coef)[j];
+          const GAS C *hf = xg(d.hf) + (size_t)j * N + plane;
+#pragma unroll
+          for (int y = 0; y < N1; ++y) {
+            const C h = hf[y * N2 + z];
+            C o = wp[y * N2 + z];
+            o.x += c * (h.x * x[y].x - h.y * x[y].y);
+            o.y += c * (h.x * x[y].y + h.y * x[y].x);
+            wp[y * N2 + z] = o;
+          }
+        } else {
+#pragma unroll
+          for (int y = 0; y < N1; ++y) wp[y * N2 + z] = x[y];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <typename T, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
+  using Cfg = PairPlaneCfg<T, S, S>;
+  Ctx &cx = h->ctx->c;
+  const Grid &g = h->g;
+  PairDev<T> d;
+  d.phi = reinterpret_cast<const cplx<T> *>(a.phi);
+  d.hf = reinterpret_cast<const cplx<T> *>(a.hf);
+  d.t = reinterpret_cast<cplx<T> *>(a.t);
+  d.w = reinterpret_cast<cplx<T> *>(a.w);
+  d.jlist = a.jlist;
+  d.coef = a.coef;
+  d.nst = a.nst;
+  d.nj = a.hf ? a.nj : 1;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) d.b[r][c] = (T)g.b[r][c];
+  for (int c = 0; c < 3; ++c) d.q[c] = (T)a.gshift[c];
+  d.scal = (T)(4.0 * M_PI / (double)g.N);
+  d.zero = (T)(a.zeroterm / (double)g.N);
+  static bool attr = false;
+  if (!attr && Cfg::LDS > 65536) {
+    INQX_HIP(hipFuncSetAttribute((const void *)k_pair_plane_fwd<T, S, S, S>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
+    INQX_HIP(hipFuncSetAttribute((const void *)k_pair_plane_inv<T, S, S, S>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
+  }
+  attr = true;
+  const bool f64 = std::is_same<T, double>::value;
+  const int cap = cx.num_cu * 16;
+  const int tot1 = d.nj * S * d.nst;
+  {
+    ProfScope ps(cx, f64 ? "pair_plane_fwd_f64" : "pair_plane_fwd_f32");
+    hipLaunchKernelGGL((k_pair_plane_fwd<T, S, S, S>), dim3(std::min(tot1, cap)), dim3(Cfg::THREADS), Cfg::LDS,
+                       cx.stream, d, tot1);
+  }
+  INQX_HIP(hipGetLastError());
+  constexpr int ngrp = ((S * S + 63) / 64 + 3) / 4;
+  const int tot2 = d.nj * d.nst * ngrp;
+  {
+    ProfScope ps(cx, f64 ? "pair_xline_f64" : "pair_xline_f32");
+    hipLaunchKernelGGL((k_pair_xline<T, S, S, S>), dim3(std::min(tot2, cap)), dim3(256), 0, cx.stream, d, tot2);
+  }
+  INQX_HIP(hipGetLastError());
+  const int tot3 = S * d.nst;
+  {
+    ProfScope ps(cx, f64 ? "pair_plane_inv_f64" : "pair_plane_inv_f32");
+    hipLaunchKernelGGL((k_pair_plane_inv<T, S, S, S>), dim3(std::min(tot3, cap)), dim3(Cfg::THREADS), Cfg::LDS,
+                       cx.stream, d, tot3);
+  }
+  INQX_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+bool pair_grid_supported(const int *n) {
+  if (n[0] != n[1] || n[1] != n[2]) return false;
+  switch (n[0]) {
+#define CASE(S) case S:
+    INQX_SIZES(CASE)
+#undef CASE
+    return true;
+    default: return false;
+  }
+}
+
+void launch_pairs(inqx_ham_s *h, int prec, const PairArgs &a) {
+  const int *n = h->g.n;
+  INQX_CHECK(pair_grid_supported(n), INQX_ERR_UNSUPPORTED, "pair Poisson solves need a cubic grid of a built size");
+  INQX_CHECK(a.nst > 0 && a.nj > 0 && a.phi && a.t && a.w, INQX_ERR_ARG, "bad pair arguments");
+  INQX_CHECK(!a.hf || (a.jlist && a.coef), INQX_ERR_ARG, "pair chunk without j list");
+  switch (n[0]) {
+#define CASE(S)                                    \
+  case S:                                          \
+    if (prec == INQX_FP64) run_pairs<double, S>(h, a); \
+    else run_pairs<float, S>(h, a);                \
+    break;
+    INQX_SIZES(CASE)
+#undef CASE
+    default: break;
+  }
+}
+
+}  // namespace inqx

@@ -227,11 +227,23 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
       const int yz = pwyz[seg0 + j];
       const int y = yz / N2, z = yz - y * N2;
       const size_t o = (size_t)(s0 + p) * B.npw + seg0 + j;
-      const C v = lds[p * Cfg::PLANE + y * (N2 + 1) + z];
-      const C c = psi[o];
-      const T kf = kin[seg0 + j];
-      const C out{v.x + kf * c.x, v.y + kf * c.y};
-      if (B.write_h) hpsi[o] = out;
+      C out = lds[p * Cfg::PLANE + y * (N2 + 1) + z];
+      if (B.kin) {  // H application: + 1/2|G+k|^2 psi; without it a plain masked forward FFT
+        const C c = psi[o];
+        const T kf = kin[seg0 + j];
+        out.x += kf * c.x;
+        out.y += kf * c.y;
+      }
+      if (B.write_h) {
+        if (B.accumulate) {
+          C q = hpsi[o];
+          q.x += out.x;
+          q.y += out.y;
+          hpsi[o] = q;
+        } else {
+          hpsi[o] = out;
+        }
+      }
       if (B.acc1) {
         GAS C *a = gw(B.acc1) + o;
         C q = *a;
@@ -653,7 +665,6 @@ __global__ void __launch_bounds__(256) k_obs_combine(const ObsBlock *ob, int nbl
 // -------------------------------------------------------------------------------------------
 // host side: tables and launches
 // -------------------------------------------------------------------------------------------
-#define INQX_SIZES(X) X(12) X(16) X(18) X(20) X(24) X(28) X(30) X(32) X(36) X(40) X(48) X(54) X(60) X(64) X(72) X(80) X(96)
 
 bool grid_supported(const int *n) {
   if (n[1] != n[2]) return false;
@@ -842,6 +853,8 @@ struct LaunchData {
   size_t off_pref[3] = {0, 0, 0};
   int nwave = 4;
   int lds_per_wave = 0;
+  int mode = 0;                      // ApplyOpts::mode
+  int total_line = 0, ntile_line = 0;  // line passes of the transform modes
 };
 static std::map<std::pair<inqx_ham_s *, LaunchKey>, std::shared_ptr<LaunchData>> g_launch_cache;
 
@@ -856,6 +869,88 @@ void invalidate_launch_cache(inqx_ham_s *h) {
   for (auto it = g_launch_cache.begin(); it != g_launch_cache.end();) {
     if (it->first.first == h) it = g_launch_cache.erase(it);
     else ++it;
+  }
+}
+
+// -------------------------------------------------------------------------------------------
+// sphere <-> real-space line passes: one lane per line, 64 lines x 4 states per work-group
+// -------------------------------------------------------------------------------------------
+
+// inverse FFT along i0 of the plane slots written by k_plane_inverse, times
+// 1/N, into the full real-space grid (operations::transform::to_real,
+// src/operations/transform.hpp:113-182)
+template <typename T, int N0>
+__global__ void __launch_bounds__(256) k_line_to_real(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
+                                                      int total, int n12, int ntile, T scale) {
+  using C = cplx<T>;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+    const int b = find_block(prefix, nblocks, wg);
+    const SphereBlock<T> B = blocks[b];
+    const int local = wg - gp(prefix)[b];
+    const int ngrp = (B.nst + 3) >> 2;
+    const int tile = local / ngrp;
+    const int s = (local - tile * ngrp) * 4 + wave;
+    if (s >= B.nst) continue;  // wave-uniform
+    const int L = tile * kTile + lane;
+    const int Lc = L < n12 ? L : n12 - 1;
+    C x[N0];
+    load_line_inverse<T, N0>(B, n12, s, Lc, x);
+    if (L < n12) {
+      GAS C *out = gw(B.rs);
+      if (B.rs_layout == 0) {
+        out += (size_t)s * N0 * n12 + L;
+#pragma unroll
+        for (int i0 = 0; i0 < N0; ++i0) out[(size_t)i0 * n12] = C{x[i0].x * scale, x[i0].y * scale};
+      } else {
+        out += (size_t)L * B.nst + s;
+#pragma unroll
+        for (int i0 = 0; i0 < N0; ++i0)
+          out[(size_t)i0 * n12 * B.nst] = C{x[i0].x * scale, x[i0].y * scale};
+      }
+    }
+  }
+}
+
+// forward FFT along i0 of full real-space lines, keeping the plane indices of
+// the sphere for k_plane_forward (operations::transform::to_fourier,
+// src/operations/transform.hpp:52-109, restricted to the sphere)
+template <typename T, int N0>
+__global__ void __launch_bounds__(256) k_line_from_real(const SphereBlock<T> *blocks, const int *prefix,
+                                                        int nblocks, int total, int n12, int ntile) {
+  using C = cplx<T>;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+    const int b = find_block(prefix, nblocks, wg);
+    const SphereBlock<T> B = blocks[b];
+    const int local = wg - gp(prefix)[b];
+    const int ngrp = (B.nst + 3) >> 2;
+    const int tile = local / ngrp;
+    const int s = (local - tile * ngrp) * 4 + wave;
+    if (s >= B.nst) continue;
+    const int L = tile * kTile + lane;
+    const int Lc = L < n12 ? L : n12 - 1;
+    C x[N0];
+    const GAS C *in = gp(B.rs);
+    if (B.rs_layout == 0) {
+      in += (size_t)s * N0 * n12 + Lc;
+#pragma unroll
+      for (int i0 = 0; i0 < N0; ++i0) x[i0] = in[(size_t)i0 * n12];
+    } else {
+      in += (size_t)Lc * B.nst + s;
+#pragma unroll
+      for (int i0 = 0; i0 < N0; ++i0) x[i0] = in[(size_t)i0 * n12 * B.nst];
+    }
+    fft_line<N0, -1, C>::run(x);
+    if (L < n12) {
+      GAS C *t1 = gw(B.t1) + (size_t)s * n12 + L;
+      const size_t sstride = (size_t)B.nst * n12;
+#pragma unroll
+      for (int i0 = 0; i0 < N0; ++i0) {
+        const int j = sym_of<N0>(i0) - B.m_lo;
+        if ((unsigned)j < (unsigned)B.nslot) t1[(size_t)j * sstride] = x[i0];
+      }
+    }
   }
 }
 
@@ -893,12 +988,36 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   const int grid_plane = std::min(ld.total_plane, cu * 16);
   Ctx &cx = h->ctx->c;
   const bool f64 = std::is_same<T, double>::value;
+  if (ld.mode == 2) {  // real space -> sphere
+    const int grid_l = std::min(ld.total_line, cu * 16);
+    {
+      ProfScope ps(cx, f64 ? "line_from_real_f64" : "line_from_real_f32");
+      hipLaunchKernelGGL((k_line_from_real<T, N0>), dim3(grid_l), dim3(256), 0, st, blk, pref_x, nblocks,
+                         ld.total_line, hd.n12, ld.ntile_line);
+    }
+    INQX_HIP(hipGetLastError());
+    {
+      ProfScope ps(cx, f64 ? "plane_forward_f64" : "plane_forward_f32");
+      hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), Cfg::LDS, st, blk,
+                         pref_plane, nblocks, ld.total_plane, (T)0, (T)0, (T)0, (T)0);
+    }
+    INQX_HIP(hipGetLastError());
+    return;
+  }
   {
   ProfScope ps(cx, f64 ? "plane_inverse_f64" : "plane_inverse_f32");
   hipLaunchKernelGGL((k_plane_inverse<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), Cfg::LDS, st, blk,
                      pref_plane, nblocks, ld.total_plane);
   }
   INQX_HIP(hipGetLastError());
+  if (ld.mode == 1) {  // sphere -> real space
+    const int grid_l = std::min(ld.total_line, cu * 16);
+    ProfScope ps(cx, f64 ? "line_to_real_f64" : "line_to_real_f32");
+    hipLaunchKernelGGL((k_line_to_real<T, N0>), dim3(grid_l), dim3(256), 0, st, blk, pref_x, nblocks, ld.total_line,
+                       hd.n12, ld.ntile_line, (T)(1.0 / (double)hd.N));
+    INQX_HIP(hipGetLastError());
+    return;
+  }
   const int xthreads = 64 * ld.nwave;
   const int xlds = ld.nwave * ld.lds_per_wave * (int)sizeof(cplx<T>);
   set_x_attr<T, N0>(xlds);
@@ -938,13 +1057,23 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   if (!opts) opts = &kNone;
   const int prec = std::is_same<T, double>::value ? INQX_FP64 : INQX_FP32;
   const Grid &g = h->g;
-  INQX_CHECK(h->has_v, INQX_ERR_STATE, "potential not set");
+  const int mode = opts->mode;
   INQX_CHECK(grid_supported(g.n), INQX_ERR_UNSUPPORTED, "grid size not built for the sphere path");
-  if (!h->kb.built) build_kb_tables(h);
-  const bool has_kb = h->kb.natoms > 0;
-  if (prec == INQX_FP32 && !h->v32.p) INQX_CHECK(false, INQX_ERR_STATE, "potential not uploaded");
+  if (mode == 0) {
+    INQX_CHECK(h->has_v, INQX_ERR_STATE, "potential not set");
+    if (!h->kb.built) build_kb_tables(h);
+    if (prec == INQX_FP32 && !h->v32.p) INQX_CHECK(false, INQX_ERR_STATE, "potential not uploaded");
+  } else {
+    INQX_CHECK(opts->rs != nullptr, INQX_ERR_ARG, "transform without real-space arrays");
+  }
+  const bool has_kb = mode == 0 && h->kb.natoms > 0;
   LaunchKey key;
   key.k.push_back((uintptr_t)prec);
+  key.k.push_back((uintptr_t)mode);
+  key.k.push_back((uintptr_t)opts->rs_layout);
+  key.k.push_back((uintptr_t)(opts->accumulate ? 1 : 0));
+  if (mode != 0)
+    for (int i = 0; i < nblocks; ++i) key.k.push_back((uintptr_t)opts->rs[i]);
   for (int i = 0; i < nblocks; ++i) {
     ensure_basis_device(h, *bases[i]);
     ensure_workspace(h, *bases[i], nst[i], prec);
@@ -999,10 +1128,14 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     }
     // waves (states) per x-pass workgroup within the LDS budget
     const int cs = (int)sizeof(cplx<T>);
-    ld->lds_per_wave = h->kb.maxpts + kMaxNlm * h->kb.maxslot;
     int nw = 4;
-    while (nw > 1 && nw * ld->lds_per_wave * cs > 160 * 1024) --nw;
-    INQX_CHECK(ld->lds_per_wave * cs <= 160 * 1024, INQX_ERR_UNSUPPORTED, "KB tile too dense for LDS");
+    ld->mode = mode;
+    ld->ntile_line = (g.n[1] * g.n[2] + kTile - 1) / kTile;
+    if (mode == 0) {
+      ld->lds_per_wave = h->kb.maxpts + kMaxNlm * h->kb.maxslot;
+      while (nw > 1 && nw * ld->lds_per_wave * cs > 160 * 1024) --nw;
+      INQX_CHECK(ld->lds_per_wave * cs <= 160 * 1024, INQX_ERR_UNSUPPORTED, "KB tile too dense for LDS");
+    }
     ld->nwave = nw;
     const double *Afull = h->A;
     for (int i = 0; i < nblocks; ++i) {
@@ -1016,14 +1149,17 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
       d.slot_seg = B.d_slot_seg.as<int>();
       d.rows = B.d_rows.as<int>();
       d.pw_yz = B.d_pw_yz.as<int>();
-      d.kin = prec == INQX_FP64 ? (const T *)B.d_kin64.p : (const T *)B.d_kin32.p;
+      d.kin = mode == 2 ? nullptr : prec == INQX_FP64 ? (const T *)B.d_kin64.p : (const T *)B.d_kin32.p;
+      d.rs = mode != 0 ? reinterpret_cast<cplx<T> *>(opts->rs[i]) : nullptr;
+      d.rs_layout = opts->rs_layout;
+      d.accumulate = opts->accumulate ? 1 : 0;
       d.acc1 = opts->acc1 ? reinterpret_cast<cplx<T> *>(opts->acc1[i]) : nullptr;
       d.acc2 = opts->acc2 ? reinterpret_cast<cplx<T> *>(opts->acc2[i]) : nullptr;
       d.proj = opts->observables ? B.proj[prec].as<cplx<T>>() : nullptr;
       d.dip = opts->observables ? B.dip.as<double>() : nullptr;
-      d.write_h = opts->write_h ? 1 : 0;
+      d.write_h = (opts->write_h || mode == 2) ? 1 : 0;
       const DevMem &vm = prec == INQX_FP64 ? h->v64 : h->v32;
-      d.vloc = reinterpret_cast<const T *>(vm.as<char>() + (size_t)B.spin * g.N * sizeof(T));
+      d.vloc = vm.p ? reinterpret_cast<const T *>(vm.as<char>() + (size_t)B.spin * g.N * sizeof(T)) : nullptr;
       d.nst = nst[i];
       d.npw = (int)B.npw;
       d.nslot = B.nslot;
@@ -1031,11 +1167,15 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
       for (int k = 0; k < 3; ++k) d.kcov[k] = B.kcov[k] + Afull[k];
       d.gamma = (d.kcov[0] == 0.0 && d.kcov[1] == 0.0 && d.kcov[2] == 0.0) ? 1 : 0;
       pp[i + 1] = pp[i] + d.nslot * ((nst[i] + plane_p - 1) / plane_p);
-      px[i + 1] = px[i] + h->kb.ntile * ((nst[i] + nw - 1) / nw);
+      if (mode == 0)
+        px[i + 1] = px[i] + h->kb.ntile * ((nst[i] + nw - 1) / nw);
+      else
+        px[i + 1] = px[i] + ld->ntile_line * ((nst[i] + 3) / 4);
       pr[i + 1] = pr[i] + (std::max(1, h->kb.natoms) * nst[i] + 255) / 256;
     }
     ld->total_plane = pp[nblocks];
     ld->total_x = px[nblocks];
+    ld->total_line = px[nblocks];
     ld->total_red = pr[nblocks];
     size_t bytes = sizeof(SphereBlock<T>) * nblocks;
     bytes = (bytes + 255) & ~size_t(255);

@@ -133,6 +133,9 @@ struct Projector {
   long npts() const { return (long)dist.size(); }
 };
 
+// line lengths with compiled register FFTs (radix-4/2/3/5/7 stages)
+#define INQX_SIZES(X) X(12) X(16) X(18) X(20) X(24) X(28) X(30) X(32) X(36) X(40) X(48) X(54) X(60) X(64) X(72) X(80) X(96)
+
 constexpr int kMaxNlm = 4;   // s + 3 p per atom, padded
 constexpr int kTile = 64;    // x-pencils (lines) per tile: one wavefront
 
@@ -181,6 +184,10 @@ template <typename T> struct SphereBlock {
   int gamma;           // k+A == 0: no KB phases
   int write_h;         // store H psi into hpsi
   double kcov[3];      // k + A, covariant
+  // sphere <-> real-space transforms (kin == nullptr in plane_forward: plain masked FFT)
+  C *rs;               // real-space states, [nst][N] (rs_layout 0) or [N][nst] (1)
+  int rs_layout;
+  int accumulate;      // plane_forward adds into hpsi instead of storing
 };
 
 // optional work fused into one H application
@@ -190,6 +197,12 @@ struct ApplyOpts {
   double c1r = 0.0, c1i = 0.0, c2r = 0.0, c2i = 0.0;
   bool write_h = true;
   bool observables = false;     // projections and dipole partials into the basis workspaces
+  // mode 0: H application; 1: sphere -> real space (psi -> rs, x 1/N);
+  // 2: real space -> sphere (rs -> hpsi, masked forward FFT)
+  int mode = 0;
+  void *const *rs = nullptr;    // per block real-space arrays for modes 1 and 2
+  int rs_layout = 0;            // 0: [nst][N], 1: [N][nst]
+  bool accumulate = false;      // mode 2: hpsi += result
 };
 
 // one G-sphere block
@@ -231,6 +244,9 @@ struct inqx_ham_s {
   inqx::KBTables kb;
   std::vector<inqx_basis> bases;
   int pinned = 0;  // > 0 while a captured graph uses the projector tables
+  // exact-exchange workspaces: real-space states, hf states, accumulator, pair planes
+  inqx::DevMem exx_phi, exx_hf, exx_w, exx_t, exx_jlist, exx_coef;
+  size_t exx_pair_bytes = size_t(4) << 30;  // budget of the pair workspace (one j-chunk)
 };
 
 struct inqx_basis_s {
@@ -251,9 +267,29 @@ bool grid_supported(const int *n);
 // linalg.hip
 void launch_overlap(Ctx &cx, int prec, int n1, int n2, long K, const void *A, const void *B, double vol, double *S,
                     DevMem &partial);
-void launch_rotate(Ctx &cx, int prec, int n, long K, const double *R, int mode, double ar, double ai, const void *Cm,
-                   void *Out, double br, double bi, DevMem &ops);
+// Out[nr][K] = M C (+ beta Out), C [nc][K]; M from R by mode (0: conj(R), 2: R, both square;
+// 1: alpha R^T with R [nc][nr])
+void launch_rotate(Ctx &cx, int prec, int nr, int nc, long K, const double *R, int mode, double ar, double ai,
+                   const void *Cm, void *Out, double br, double bi, DevMem &ops);
 void launch_cholesky(Ctx &cx, int n, double *S, int *info);
 void launch_trinv(Ctx &cx, int n, const double *L, double *X);
 void launch_stream_copy(Ctx &cx, const void *src, void *dst, size_t bytes);
+void launch_fill_zero(Ctx &cx, void *p, size_t bytes);
+// exx_kernels.hip: batched pair-density Poisson solves on real-space arrays in
+// state-major layout [n][N].  With hf: for every chunk entry jj (hf index
+// j = jlist[jj]) and state i, rho = conj(hf_j) phi_i -> Poisson -> w_i +=
+// coef[j] hf_j v.  Without hf (nj = 1): w_i = Poisson(phi_i).
+struct PairArgs {
+  const void *phi = nullptr;   // [nst][N]
+  const void *hf = nullptr;    // [nhf][N] or null
+  void *t = nullptr;           // [nj][nst][N] pair workspace (may be phi when hf is null)
+  void *w = nullptr;           // [nst][N]
+  const int *jlist = nullptr;  // device [nj]: hf index of each chunk entry
+  const double *coef = nullptr;  // device [nhf]: coefficient per hf index
+  int nst = 0, nj = 1;
+  double gshift[3] = {0, 0, 0};  // cartesian, added to G
+  double zeroterm = 0.0;         // kernel value at |G + q|^2 < 1e-6 (times -4 pi)
+};
+void launch_pairs(inqx_ham_s *h, int prec, const PairArgs &a);
+bool pair_grid_supported(const int *n);
 }  // namespace inqx

@@ -153,45 +153,47 @@ __global__ void k_overlap_reduce(const T *partial, int n1, int n2, int tiles_i, 
 }
 
 // ------------------------------------------------------------------------ rotate
-// operand matrices Ap = [Mr, -Mi], Aq = [Mi, Mr] interleaved along j: [n][2n] in T
-// mode 0: M = conj(R) (Phi <- Phi R^H);  mode 1: M = alpha R^T (5-argument rotate);
-// mode 2: M = R
-__global__ void k_rot_prep32(const double *R, int n, int mode, double ar, double ai, float *Ap, float *Aq) {
+// operand matrices Ap = [Mr, -Mi], Aq = [Mi, Mr] interleaved along j: [nr][2 nc] in T
+// mode 0: M = conj(R) (Phi <- Phi R^H);  mode 1: M = alpha R^T (5-argument rotate,
+// R is [nc][nr]); mode 2: M = R
+__global__ void k_rot_prep32(const double *R, int nr, int nc, int mode, double ar, double ai, float *Ap, float *Aq) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n * n) return;
-  const int i = idx / n, j = idx - i * n;
+  if (idx >= nr * nc) return;
+  const int i = idx / nc, j = idx - i * nc;
+  const int n = nc;
   double mr, mi;
   if (mode == 0) {
-    mr = R[2 * (i * n + j)];
-    mi = -R[2 * (i * n + j) + 1];
+    mr = R[2 * ((long)i * nc + j)];
+    mi = -R[2 * ((long)i * nc + j) + 1];
   } else if (mode == 1) {
-    const double rr = R[2 * (j * n + i)], ri = R[2 * (j * n + i) + 1];
+    const double rr = R[2 * ((long)j * nr + i)], ri = R[2 * ((long)j * nr + i) + 1];
     mr = ar * rr - ai * ri;
     mi = ar * ri + ai * rr;
   } else {
-    mr = R[2 * (i * n + j)];
-    mi = R[2 * (i * n + j) + 1];
+    mr = R[2 * ((long)i * nc + j)];
+    mi = R[2 * ((long)i * nc + j) + 1];
   }
   Ap[(long)i * 2 * n + 2 * j] = (float)mr;
   Ap[(long)i * 2 * n + 2 * j + 1] = (float)-mi;
   Aq[(long)i * 2 * n + 2 * j] = (float)mi;
   Aq[(long)i * 2 * n + 2 * j + 1] = (float)mr;
 }
-__global__ void k_rot_prep64(const double *R, int n, int mode, double ar, double ai, double *Ap, double *Aq) {
+__global__ void k_rot_prep64(const double *R, int nr, int nc, int mode, double ar, double ai, double *Ap, double *Aq) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n * n) return;
-  const int i = idx / n, j = idx - i * n;
+  if (idx >= nr * nc) return;
+  const int i = idx / nc, j = idx - i * nc;
+  const int n = nc;
   double mr, mi;
   if (mode == 0) {
-    mr = R[2 * (i * n + j)];
-    mi = -R[2 * (i * n + j) + 1];
+    mr = R[2 * ((long)i * nc + j)];
+    mi = -R[2 * ((long)i * nc + j) + 1];
   } else if (mode == 1) {
-    const double rr = R[2 * (j * n + i)], ri = R[2 * (j * n + i) + 1];
+    const double rr = R[2 * ((long)j * nr + i)], ri = R[2 * ((long)j * nr + i) + 1];
     mr = ar * rr - ai * ri;
     mi = ar * ri + ai * rr;
   } else {
-    mr = R[2 * (i * n + j)];
-    mi = R[2 * (i * n + j) + 1];
+    mr = R[2 * ((long)i * nc + j)];
+    mi = R[2 * ((long)i * nc + j) + 1];
   }
   Ap[(long)i * 2 * n + 2 * j] = mr;
   Ap[(long)i * 2 * n + 2 * j + 1] = -mi;
@@ -201,9 +203,9 @@ __global__ void k_rot_prep64(const double *R, int n, int mode, double ar, double
 
 constexpr int kRotJC = 16;  // complex j per LDS stage (32 real k')
 
-// Out[i][p] = sum_j M[i][j] C[j][p] (+ beta Out[i][p]); work-group = 64 rows x 64 columns p
+// Out[i][p] = sum_j M[i][j] C[j][p] (+ beta Out[i][p]), i < nr, j < n; work-group = 64 rows x 64 columns p
 template <typename T>
-__global__ void __launch_bounds__(256) k_rotate(const T *Ap, const T *Aq, const cplx<T> *Cm, int n, long K,
+__global__ void __launch_bounds__(256) k_rotate(const T *Ap, const T *Aq, const cplx<T> *Cm, int nr, int n, long K,
                                                 cplx<T> *Out, double br, double bi, int tiles_p) {
   using M = mfma_t<T>;
   using C = cplx<T>;
@@ -236,7 +238,7 @@ __global__ void __launch_bounds__(256) k_rotate(const T *Ap, const T *Aq, const 
       const int kk = ks + (lane >> 4);  // real k' within the stage
       const int jglob = jb + (kk >> 1);
       T ap = 0, aq = 0;
-      if (arow < n && jglob < n) {
+      if (arow < nr && jglob < n) {
         ap = Apg[(long)arow * 2 * n + 2 * jb + kk];
         aq = Aqg[(long)arow * 2 * n + 2 * jb + kk];
       }
@@ -258,7 +260,7 @@ __global__ void __launch_bounds__(256) k_rotate(const T *Ap, const T *Aq, const 
     for (int r = 0; r < 4; ++r) {
       const int i = i0 + M::row(lane, r);
       const long p = p0 + 16 * c + (lane & 15);
-      if (i < n && p < K) {
+      if (i < nr && p < K) {
         C o{re[c][r], im[c][r]};
         if (beta) {
           const C q = Og[(long)i * K + p];
@@ -400,26 +402,46 @@ void launch_overlap(Ctx &cx, int prec, int n1, int n2, long K, const void *A, co
   INQX_HIP(hipGetLastError());
 }
 
-void launch_rotate(Ctx &cx, int prec, int n, long K, const double *R, int mode, double ar, double ai, const void *Cm,
-                   void *Out, double br, double bi, DevMem &ops) {
+void launch_rotate(Ctx &cx, int prec, int nr, int nc, long K, const double *R, int mode, double ar, double ai,
+                   const void *Cm, void *Out, double br, double bi, DevMem &ops) {
+  INQX_CHECK(mode == 1 || nr == nc, INQX_ERR_ARG, "rotate: modes 0 and 2 need a square matrix");
   const size_t es = prec == INQX_FP64 ? 8 : 4;
-  ops.ensure(es * (size_t)n * 2 * n * 2);
-  const int tiles_p = (int)((K + 63) / 64), tiles_i = (n + 63) / 64;
+  const size_t mat = (size_t)nr * 2 * nc;
+  ops.ensure(es * mat * 2);
+  const int tiles_p = (int)((K + 63) / 64), tiles_i = (nr + 63) / 64;
+  const int gprep = (int)(((size_t)nr * nc + 255) / 256);
   if (prec == INQX_FP64) {
-    double *Ap = ops.as<double>(), *Aq = Ap + (size_t)n * 2 * n;
-    hipLaunchKernelGGL(k_rot_prep64, dim3((n * n + 255) / 256), dim3(256), 0, cx.stream, R, n, mode, ar, ai, Ap, Aq);
+    double *Ap = ops.as<double>(), *Aq = Ap + mat;
+    hipLaunchKernelGGL(k_rot_prep64, dim3(gprep), dim3(256), 0, cx.stream, R, nr, nc, mode, ar, ai, Ap, Aq);
     INQX_HIP(hipGetLastError());
     ProfScope ps(cx, "rotate_mfma_f64");
     hipLaunchKernelGGL(k_rotate<double>, dim3(tiles_p * tiles_i), dim3(256), 0, cx.stream, Ap, Aq,
-                       (const cplx<double> *)Cm, n, K, (cplx<double> *)Out, br, bi, tiles_p);
+                       (const cplx<double> *)Cm, nr, nc, K, (cplx<double> *)Out, br, bi, tiles_p);
   } else {
-    float *Ap = ops.as<float>(), *Aq = Ap + (size_t)n * 2 * n;
-    hipLaunchKernelGGL(k_rot_prep32, dim3((n * n + 255) / 256), dim3(256), 0, cx.stream, R, n, mode, ar, ai, Ap, Aq);
+    float *Ap = ops.as<float>(), *Aq = Ap + mat;
+    hipLaunchKernelGGL(k_rot_prep32, dim3(gprep), dim3(256), 0, cx.stream, R, nr, nc, mode, ar, ai, Ap, Aq);
     INQX_HIP(hipGetLastError());
     ProfScope ps(cx, "rotate_mfma_f32");
     hipLaunchKernelGGL(k_rotate<float>, dim3(tiles_p * tiles_i), dim3(256), 0, cx.stream, Ap, Aq,
-                       (const cplx<float> *)Cm, n, K, (cplx<float> *)Out, br, bi, tiles_p);
+                       (const cplx<float> *)Cm, nr, nc, K, (cplx<float> *)Out, br, bi, tiles_p);
   }
+  INQX_HIP(hipGetLastError());
+}
+
+// zero fill with vector stores (no memset node when a caller captures the stream)
+__global__ void __launch_bounds__(256) k_fill_zero(V4f *p, long n16, unsigned char *tail, int ntail) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) p[i] = V4f{0.f, 0.f, 0.f, 0.f};
+  if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+void launch_fill_zero(Ctx &cx, void *p, size_t bytes) {
+  if (bytes == 0) return;
+  const long n16 = (long)(bytes / 16);
+  const int ntail = (int)(bytes - 16 * (size_t)n16);
+  const int grid = (int)std::max<long>(1, std::min<long>((n16 + 255) / 256, (long)cx.num_cu * 16));
+  hipLaunchKernelGGL(k_fill_zero, dim3(grid), dim3(256), 0, cx.stream, (V4f *)p, n16,
+                     (unsigned char *)p + 16 * (size_t)n16, ntail);
   INQX_HIP(hipGetLastError());
 }
 

@@ -1,0 +1,66 @@
+"""Exact exchange over the inqx C ABI: hamiltonian::exchange_operator
+(src/hamiltonian/exchange_operator.hpp) at the Gamma point, with states on
+the G sphere.  ``update`` builds the ACE projectors xi (exchange_operator.hpp:
+52-120), ``ace`` applies K psi = -xi (xi^H psi) (:217-231) and ``direct``
+runs the pair-density Poisson solves (:124-186)."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+from .hamiltonian import DeviceArray, precision_of
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class ExchangeOperator:
+    def __init__(self, ham, basis, exchange_coefficient=1.0, zeroterm=0.0):
+        self.ham = ham
+        self.basis = basis
+        self.coefficient = float(exchange_coefficient)
+        self.zeroterm = float(zeroterm)
+        self.xi = None
+        self.energy = None
+
+    @property
+    def vol_el(self):
+        N = float(np.prod(self.ham.n))
+        return abs(np.linalg.det(self.ham.lattice)) / (N * N)
+
+    def direct(self, phi, hf, occ, scale=1.0, exxphi=None):
+        """exxphi += mask(FFT(sum_j -scale alpha/2 occ_j hf_j Poisson(conj(hf_j) phi)))."""
+        occ = np.ascontiguousarray(occ, dtype=np.float64)
+        if exxphi is None:
+            exxphi = DeviceArray(self.ham.ctx, phi.shape, phi.dtype)
+            exxphi.upload(np.zeros(phi.shape, phi.dtype))
+        check(lib().inqx_exchange_direct(self.ham.h, self.basis.h, phi.shape[0], precision_of(phi.dtype), phi.ptr,
+                                          hf.shape[0], hf.ptr, _dp(occ), self.coefficient, float(scale),
+                                          self.zeroterm, exxphi.ptr), self.ham.ctx.h, "inqx_exchange_direct")
+        return exxphi
+
+    def update(self, phi, occ):
+        """ACE build from the states phi; returns the exchange energy."""
+        occ = np.ascontiguousarray(occ, dtype=np.float64)
+        if self.xi is None or self.xi.shape != phi.shape or self.xi.dtype != phi.dtype:
+            self.xi = DeviceArray(self.ham.ctx, phi.shape, phi.dtype)
+        en = C.c_double()
+        check(lib().inqx_ace_build(self.ham.h, self.basis.h, phi.shape[0], precision_of(phi.dtype), phi.ptr, _dp(occ),
+                                   self.coefficient, self.zeroterm, self.xi.ptr, C.byref(en)), self.ham.ctx.h,
+              "inqx_ace_build")
+        self.energy = en.value
+        return self.energy
+
+    def ace(self, psi, exxphi):
+        """exxphi <- exxphi - xi (vol_el xi^H psi)."""
+        assert self.xi is not None, "update() first"
+        check(lib().inqx_ace_apply(self.ham.ctx.h, precision_of(psi.dtype), self.xi.shape[0], psi.shape[0],
+                                   psi.shape[1], self.vol_el, self.xi.ptr, psi.ptr, exxphi.ptr), self.ham.ctx.h,
+              "inqx_ace_apply")
+        return exxphi
+
+    def __call__(self, psi):
+        out = DeviceArray(self.ham.ctx, psi.shape, psi.dtype)
+        out.upload(np.zeros(psi.shape, psi.dtype))
+        return self.ace(psi, out)

@@ -218,6 +218,34 @@ class KSHamiltonian:
         hp = (C.c_void_p * nb)(*[p.ptr.value for p in hpsis])
         check(lib().inqx_ham_apply_batch(self.h, nb, bb, nst, prec, pp, hp), self.ctx.h, "inqx_ham_apply_batch")
 
+    def to_real(self, basis, psi, layout=0, out=None):
+        """transform::to_real of sphere states (transform.hpp:113-182): [nst][N]
+        (layout 0) or the reference's [N][nst] (layout 1), normalised by 1/N."""
+        nst = psi.shape[0]
+        N = int(np.prod(self.n))
+        shape = (nst, N) if layout == 0 else (N, nst)
+        out = out if out is not None else DeviceArray(self.ctx, shape, psi.dtype)
+        check(lib().inqx_sphere_to_real(self.h, basis.h, nst, precision_of(psi.dtype), psi.ptr, int(layout), out.ptr),
+              self.ctx.h, "inqx_sphere_to_real")
+        return out
+
+    def to_sphere(self, basis, psi_rs, nst, layout=0, out=None, accumulate=False):
+        """transform::to_fourier restricted to the sphere (transform.hpp:52-109), unnormalised."""
+        out = out if out is not None else DeviceArray(self.ctx, (nst, basis.npw), psi_rs.dtype)
+        check(lib().inqx_real_to_sphere(self.h, basis.h, int(nst), precision_of(psi_rs.dtype), psi_rs.ptr, int(layout),
+                                        int(bool(accumulate)), out.ptr), self.ctx.h, "inqx_real_to_sphere")
+        return out
+
+    def poisson(self, rho, gshift=(0.0, 0.0, 0.0), zeroterm=0.0):
+        """solvers::poisson::in_place on densities [n][N] (poisson.hpp:71-110,199-213), in place."""
+        g = np.ascontiguousarray(gshift, dtype=np.float64)
+        check(lib().inqx_poisson(self.h, rho.shape[0], precision_of(rho.dtype), rho.ptr, _dp(g), float(zeroterm)),
+              self.ctx.h, "inqx_poisson")
+        return rho
+
+    def set_exchange_workspace(self, nbytes):
+        check(lib().inqx_ham_set_exchange_workspace(self.h, int(nbytes)), self.ctx.h, "set_exchange_workspace")
+
     def __call__(self, basis, psi):
         """ks_hamiltonian::operator() on a block of sphere states: returns a new DeviceArray."""
         out = DeviceArray(self.ctx, psi.shape, psi.dtype)

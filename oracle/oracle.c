@@ -1222,12 +1222,19 @@ double orc_ace_build(const orc_ham *h, int nst, const cplx *phi, const double *o
   return en;
 }
 
-/* ACE apply (exchange_operator.hpp:217-231): exxphi -= xi (dV xi^H phi) */
-void orc_ace_apply(const orc_ham *h, int nst, const cplx *xi, const cplx *phi, cplx *exxphi) {
+/* ACE apply (exchange_operator.hpp:217-231): exxphi -= xi (dV xi^H phi);
+ * xi [N][nxi], phi and exxphi [N][nst] */
+void orc_ace_apply(const orc_ham *h, int nxi, const cplx *xi, int nst, const cplx *phi, cplx *exxphi) {
   const grid_t *g = &h->g;
   long N = g->N;
-  cplx *m = (cplx *)malloc(sizeof(cplx) * (size_t)nst * nst);
-  orc_overlap(N, nst, nst, g->vol / (double)N, 0, xi, phi, m);
-  orc_rotate5(N, nst, 0, m, xi, exxphi, -1.0, 1.0);
+  cplx *m = (cplx *)malloc(sizeof(cplx) * (size_t)nxi * nst);
+  orc_overlap(N, nxi, nst, g->vol / (double)N, 0, xi, phi, m);
+#pragma omp parallel for schedule(static)
+  for (long ip = 0; ip < N; ++ip)
+    for (int b = 0; b < nst; ++b) {
+      cplx acc = 0.0;
+      for (int a = 0; a < nxi; ++a) acc += xi[ip * nxi + a] * m[(long)a * nst + b];
+      exxphi[ip * nst + b] -= acc;
+    }
   free(m);
 }

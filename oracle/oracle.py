@@ -72,7 +72,7 @@ def lib():
             "orc_sing_correction": (None, [dp, C.c_int, dp, dp, dp, dp]),
             "orc_ace_build": (C.c_double, [P, C.c_int, cp, dp, C.c_double, cp]),
             "orc_exchange_direct": (None, [P, C.c_int, cp, C.c_int, cp, dp, C.c_double, C.c_double, cp]),
-            "orc_ace_apply": (None, [P, C.c_int, cp, cp, cp]),
+            "orc_ace_apply": (None, [P, C.c_int, cp, C.c_int, cp, cp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -320,7 +320,7 @@ class Ham:
         xi = _c128(xi)
         phi_rs = _c128(phi_rs)
         out = np.zeros_like(phi_rs)
-        lib().orc_ace_apply(self.h, phi_rs.size // self.N, xi, phi_rs, out)
+        lib().orc_ace_apply(self.h, xi.size // self.N, xi, phi_rs.size // self.N, phi_rs, out)
         return out
 
 
