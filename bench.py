@@ -235,6 +235,87 @@ def c1_diamond(ctx, cpu):
     return res
 
 
+def c5_ace(ctx, cpu, cpu_seconds=8.0):
+    """ACE build on Si216 (96^3, 432 complex64 states, Gamma): nst^2 pair-density
+    Poisson solves + W^H phi + FP64 Cholesky + xi = W L^-H, and one ACE apply."""
+    from inq_amd import DeviceArray, KSHamiltonian
+    from inq_amd import operations as ops
+    from inq_amd import systems as S
+    from inq_amd.exchange import ExchangeOperator
+    sysm = S.config(5)
+    gh = KSHamiltonian(ctx, sysm.lattice, sysm.n)
+    b = gh.basis((0, 0, 0), sysm.ecut)
+    nst, N = sysm.nst, sysm.N
+    vol = S.sphere_volume_element(sysm)
+    rng = np.random.default_rng(sysm.seed)
+    c = rng.standard_normal((nst, b.npw)) + 1j * rng.standard_normal((nst, b.npw))
+    d64 = DeviceArray.from_numpy(ctx, c)
+    ops.orthogonalize(ctx, d64, vol)  # FP64 Cholesky orthonormalisation on the device
+    c = d64.numpy()
+    d = DeviceArray.from_numpy(ctx, c.astype(np.complex64))
+    occ = np.full(nst, 2.0)
+    ex = ExchangeOperator(gh, b, exchange_coefficient=1.0)
+    ex.update(d, occ)  # warm-up: workspaces and descriptors
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    energy = ex.update(d, occ)
+    ctx.sync()
+    t_build = time.perf_counter() - t0
+    kt = ctx.kernel_times()
+    ctx.profile(False)
+    out = DeviceArray(ctx, c.shape, np.complex64)
+    out.upload(np.zeros(c.shape, np.complex64))
+    ex.ace(d, out)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        ex.ace(d, out)
+    ctx.sync()
+    t_apply = (time.perf_counter() - t0) / 5
+    pairs = nst * nst
+    sc = 8
+    # per pair (one j per chunk): phi_i read, t written, t read+written, t read, w read+written
+    pair_bytes = 7 * N * sc
+    kern = {k: v[1] / v[0] * 1e3 for k, v in kt.items()}
+    pk = {k: kern[k] for k in kern if k.startswith("pair_")}
+    res = {"states": nst, "grid": list(sysm.n), "npw": int(b.npw), "pairs": pairs,
+           "build_s": t_build, "pair_solves_per_s": pairs / t_build,
+           "exchange_energy_Ha": energy, "ace_apply_ms": t_apply * 1e3,
+           "pair_bytes": pair_bytes,
+           "pair_pipeline_gbs": pair_bytes * pairs / (sum(kt[k][1] for k in kt if k.startswith("pair_")) * 1e-3) / 1e9,
+           "kernels_us": pk}
+    # FP32 vs FP64 on 4 hf states x all states (direct exchange, same inputs)
+    hf_idx = 4
+    d32h = DeviceArray.from_numpy(ctx, c[:hf_idx].astype(np.complex64))
+    d64a = DeviceArray.from_numpy(ctx, c.astype(np.complex64).astype(np.complex128))
+    d32h_in = DeviceArray.from_numpy(ctx, c[:hf_idx].astype(np.complex64).astype(np.complex128))
+    k32 = ex.direct(d, d32h, occ[:hf_idx]).numpy().astype(np.complex128)
+    k64 = ex.direct(d64a, d32h_in, occ[:hf_idx]).numpy()
+    res["direct_fp32_vs_fp64_rel"] = float(np.max(np.abs(k32 - k64)) / np.max(np.abs(k64)))
+    for a in (d32h, d64a, d32h_in, out, d, d64):
+        a.free()
+    if cpu:
+        from oracle import oracle as orc
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        orc.set_threads(cores)
+        oh = orc.Ham(sysm.lattice, sysm.n)
+        ns = 8
+        sub = DeviceArray.from_numpy(ctx, c[:ns].copy())
+        rs = gh.to_real(b, sub, layout=1).numpy()  # [N][ns], the oracle's layout
+        hf = np.ascontiguousarray(rs[:, :1])
+        t0 = time.perf_counter()
+        reps = 0
+        while reps < 1 or time.perf_counter() - t0 < cpu_seconds:
+            oh.exchange_direct(rs, hf, occ[:1], exx_coeff=1.0, scale=-1.0)
+            reps += 1
+        dt = (time.perf_counter() - t0) / reps
+        res["cpu_oracle"] = {"pair_solves_per_s": ns / dt, "cores": cores,
+                             "sample": f"{reps} x ({ns} states x 1 hf state) FP64 pair solves at 96^3"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,6 +326,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--copy", action="store_true", help="run the HBM copy reference even with --no-extras")
+    ap.add_argument("--only-c5", action="store_true", help="run only the Si216 ACE line (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -253,6 +335,11 @@ def main():
 
     from inq_amd import Context, DeviceArray
     from inq_amd import systems as S
+
+    if args.only_c5:
+        res = c5_ace(Context(0), not args.no_cpu)
+        print(json.dumps({"c5_ace": res}), flush=True)
+        return
 
     sysm = S.config(2)
     log(f"bench: {sysm.name}, {sysm.natoms} atoms, nst={sysm.nst}")
@@ -347,6 +434,8 @@ def main():
         log(f"bench: c4 {out['c4_tddft']}")
         out["c1_diamond"] = c1_diamond(ctx, not args.no_cpu)
         log(f"bench: c1 {out['c1_diamond']}")
+        out["c5_ace"] = c5_ace(ctx, not args.no_cpu)
+        log(f"bench: c5 {out['c5_ace']}")
 
     cpu = None
     if not args.no_cpu:

@@ -230,8 +230,10 @@ int inqx_ace_build(inqx_ham ham, inqx_basis basis, int nst, int precision, const
  * exxphi <- exxphi - xi (vol_el xi^H psi), xi [nxi][npw], psi/exxphi [nst][npw] */
 int inqx_ace_apply(inqx_ctx ctx, int precision, int nxi, int nst, long npw, double vol_el, const void *xi,
                    const void *psi, void *exxphi);
-/* upper bound on the pair workspace of the exchange entry points (bytes,
- * default 4 GiB; at least one j times all states is always used) */
+/* upper bound on the pair workspace of the exchange entry points (bytes;
+ * 0, the default, takes a quarter of the free device memory, at most 64 GiB;
+ * at least one j times all states is always used).  Larger chunks amortise
+ * the accumulation into the exchange result over more j. */
 int inqx_ham_set_exchange_workspace(inqx_ham ham, size_t bytes);
 
 #ifdef __cplusplus

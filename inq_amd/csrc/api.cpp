@@ -841,7 +841,13 @@ static void exchange_direct_impl(inqx_ham h, Basis &B, int nst, int prec, const 
     upload(h->exx_jlist, jl, cx.stream);
     upload(h->exx_coef, cf, cx.stream);
     INQX_HIP(hipStreamSynchronize(cx.stream));
-    const size_t njmax = std::max<size_t>(1, std::min<size_t>(jl.size(), h->exx_pair_bytes / (per * nst)));
+    size_t budget = h->exx_pair_bytes;
+    if (budget == 0) {  // a quarter of what is free, at most 64 GiB
+      size_t fr = 0, tot = 0;
+      INQX_HIP(hipMemGetInfo(&fr, &tot));
+      budget = std::min<size_t>(fr / 4 + h->exx_t.bytes, size_t(64) << 30);
+    }
+    const size_t njmax = std::max<size_t>(1, std::min<size_t>(jl.size(), budget / (per * nst)));
     h->exx_t.ensure(per * nst * njmax);
     for (size_t c0 = 0; c0 < jl.size(); c0 += njmax) {
       PairArgs a;

@@ -160,20 +160,34 @@ __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
   }
 }
 
+// the column of w_i that a lane owns stays in registers over the whole chunk
+// of j when it fits beside the FFT line (complex64 always, complex128 up to
+// 48 points); otherwise w is read, updated and written back for every j
+template <typename T, int N1> struct PairAccInRegs {
+  static constexpr bool value = sizeof(T) == 4 || N1 <= 48;
+};
+
 template <typename T, int N0, int N1, int N2>
 __global__ void __launch_bounds__(128) k_pair_plane_inv(PairDev<T> d, int total) {
   using C = cplx<T>;
   using Cfg = PairPlaneCfg<T, N1, N2>;
   constexpr int NP = N1 * N2;
   constexpr long N = (long)N0 * NP;
+  constexpr bool ACC = PairAccInRegs<T, N1>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x;
+  const bool acc_regs = ACC && d.hf != nullptr;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int i = wg % d.nst;
     const int i0 = wg / d.nst;
     const size_t plane = (size_t)i0 * NP;
     GAS C *wp = xw(d.w) + (size_t)i * N + plane;
+    C acc[N1];
+    if (acc_regs && tid < N2) {
+#pragma unroll
+      for (int y = 0; y < N1; ++y) acc[y] = wp[y * N2 + tid];
+    }
     for (int jj = 0; jj < d.nj; ++jj) {
       const GAS C *tp = xg(d.t) + ((size_t)jj * d.nst + i) * N + plane;
       for (int e = tid; e < NP; e += Cfg::THREADS) {
@@ -201,13 +215,22 @@ __global__ void __launch_bounds__(128) k_pair_plane_inv(PairDev<T> d, int total)
           const int j = xg(d.jlist)[jj];
           const T c = (T)xg(d.coef)[j];
           const GAS C *hf = xg(d.hf) + (size_t)j * N + plane;
+          if (ACC) {
 #pragma unroll
-          for (int y = 0; y < N1; ++y) {
-            const C h = hf[y * N2 + z];
-            C o = wp[y * N2 + z];
-            o.x += c * (h.x * x[y].x - h.y * x[y].y);
-            o.y += c * (h.x * x[y].y + h.y * x[y].x);
-            wp[y * N2 + z] = o;
+            for (int y = 0; y < N1; ++y) {
+              const C h = hf[y * N2 + z];
+              acc[y].x += c * (h.x * x[y].x - h.y * x[y].y);
+              acc[y].y += c * (h.x * x[y].y + h.y * x[y].x);
+            }
+          } else {
+#pragma unroll
+            for (int y = 0; y < N1; ++y) {
+              const C h = hf[y * N2 + z];
+              C o = wp[y * N2 + z];
+              o.x += c * (h.x * x[y].x - h.y * x[y].y);
+              o.y += c * (h.x * x[y].y + h.y * x[y].x);
+              wp[y * N2 + z] = o;
+            }
           }
         } else {
 #pragma unroll
@@ -215,6 +238,10 @@ __global__ void __launch_bounds__(128) k_pair_plane_inv(PairDev<T> d, int total)
         }
       }
       __syncthreads();
+    }
+    if (acc_regs && tid < N2) {
+#pragma unroll
+      for (int y = 0; y < N1; ++y) wp[y * N2 + tid] = acc[y];
     }
   }
 }

@@ -246,7 +246,7 @@ struct inqx_ham_s {
   int pinned = 0;  // > 0 while a captured graph uses the projector tables
   // exact-exchange workspaces: real-space states, hf states, accumulator, pair planes
   inqx::DevMem exx_phi, exx_hf, exx_w, exx_t, exx_jlist, exx_coef;
-  size_t exx_pair_bytes = size_t(4) << 30;  // budget of the pair workspace (one j-chunk)
+  size_t exx_pair_bytes = 0;  // budget of the pair workspace (one j-chunk); 0: a quarter of free memory, <= 64 GiB
 };
 
 struct inqx_basis_s {
