@@ -53,6 +53,8 @@ enum {
 };
 
 enum { INQX_FP64 = 0, INQX_FP32 = 1 };
+/* value type of inqx_randomize beyond the two state precisions: real double */
+enum { INQX_REAL64 = 2 };
 
 /* ------------------------------------------------------------ context */
 /* Environment/device selection the reference leaves to input::environment
@@ -127,6 +129,15 @@ int inqx_basis_kinetic(inqx_basis basis, double *kin);
  * later calls allocate nothing (required before hipGraph capture) */
 int inqx_ham_prepare(inqx_ham ham, inqx_basis basis, int nst, int precision);
 
+/* launch-shape autotuner: times the H application of nst states on this basis
+ * for each candidate x-pass shape (1, 2 or 4 states per work-group) and keeps
+ * the fastest for this Hamiltonian and precision.  Allocates scratch states for
+ * the timing; do not call during graph capture.  best_xwaves / best_ms may be
+ * NULL. */
+int inqx_ham_autotune(inqx_ham ham, inqx_basis basis, int nst, int precision, int *best_xwaves, double *best_ms);
+/* sets the x-pass shape directly (1, 2 or 4 states per work-group) */
+int inqx_ham_set_launch_shape(inqx_ham ham, int precision, int xwaves);
+
 /* ------------------------------------------------------------ H |psi> */
 /* ks_hamiltonian::operator()(orbital_set<fourier_space>) (ks_hamiltonian.hpp:160-180)
  * restricted to the G sphere: hpsi = P H P psi, psi/hpsi band-major device arrays. */
@@ -143,6 +154,12 @@ int inqx_ham_apply_batch(inqx_ham ham, int nblocks, const inqx_basis *bases, con
 /* operations::overlap (src/operations/overlap.hpp:26-95): S = vol_el phi1^H phi2 */
 int inqx_overlap(inqx_ctx ctx, int precision, int n1, int n2, long npw, double vol_el, const void *phi1,
                  const void *phi2, void *S);
+/* operations::overlap_diagonal (src/operations/overlap_diagonal.hpp:25-60):
+ * out[i] = vol_el sum conj(phi1_i) phi2_i; with normalized != 0,
+ * overlap_diagonal_normalized (:100-153): out[i] = <phi1_i|phi2_i> / <phi2_i|phi2_i>.
+ * out is a device complex128 array [n]; sums in FP64. */
+int inqx_overlap_diagonal(inqx_ctx ctx, int precision, int n, long npw, double vol_el, const void *phi1,
+                          const void *phi2, int normalized, void *out);
 /* matrix::cholesky (src/matrix/cholesky.hpp:28-100): in place S = L L^H, L lower,
  * upper part zeroed; INQX_ERR_NOT_PD when a leading minor is not positive
  * (the reference prints a warning or aborts, cholesky.hpp:77-82) */
@@ -192,6 +209,14 @@ int inqx_prop_records(inqx_prop prop, int *count, double *records, int cap);
  * the propagator is destroyed */
 int inqx_prop_destroy(inqx_prop prop);
 
+/* operations::randomize (src/operations/randomize.hpp:48-70): fills a field of
+ * npoints x nst values with the reference's pcg32 sequence (seed npoints*nst +
+ * index, U[-1,1), complex values real part first).  type: INQX_FP64
+ * (complex128), INQX_FP32 (complex64, the double values rounded) or
+ * INQX_REAL64; layout INQX_POINT_MAJOR ([npoints][nst], the reference's) or
+ * INQX_STATE_MAJOR ([nst][npoints]); out is a device array. */
+int inqx_randomize(inqx_ctx ctx, int type, long npoints, int nst, int index, int layout, void *out);
+
 /* ------------------------------------------------------------ sphere <-> real space */
 /* real-space layouts: state-major [nst][N] (the layout of the exchange
  * entry points), or point-major [N][nst], the reference's field_set layout
@@ -205,6 +230,19 @@ int inqx_sphere_to_real(inqx_ham ham, inqx_basis basis, int nst, int precision, 
  * sphere: psi = mask(FFT(psi_rs)) (unnormalised); accumulate != 0 adds into psi */
 int inqx_real_to_sphere(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *psi_rs, int layout,
                         int accumulate, void *psi);
+
+/* ks_hamiltonian::operator()(orbital_set<real_space>) (ks_hamiltonian.hpp:137-156)
+ * on arbitrary real-space states: hpsi_rs = to_real(H to_fourier(psi_rs)).
+ * The basis must hold the whole cube (inqx_basis_create with an ecut above
+ * every |G+k|^2/2, e.g. 1e30), so that no component is masked. */
+int inqx_ham_apply_real(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *psi_rs, int layout,
+                        void *hpsi_rs);
+/* observables::density::calculate_add (src/observables/density.hpp:24-38),
+ * non-spinor: density[p][ncomp] (component ispin) += sum_i occ_i |phi_i(p)|^2
+ * with phi_i = to_real of the sphere states; density is a device double
+ * array [N][ncomp], occ a host array [nst]; FP64 sums. */
+int inqx_density_add(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *phi, const double *occ,
+                     int ncomp, int ispin, void *density);
 
 /* ------------------------------------------------------------ exact exchange */
 /* solvers::poisson::in_place (src/solvers/poisson.hpp:71-110,199-213) on n

@@ -61,6 +61,12 @@ SIGNATURES = {
     "inqx_ace_build": (C.c_int, [P, P, C.c_int, C.c_int, P, DP, C.c_double, C.c_double, P, DP]),
     "inqx_ace_apply": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_long, C.c_double, P, P, P]),
     "inqx_ham_set_exchange_workspace": (C.c_int, [P, C.c_size_t]),
+    "inqx_ham_autotune": (C.c_int, [P, P, C.c_int, C.c_int, IP, DP]),
+    "inqx_randomize": (C.c_int, [P, C.c_int, C.c_long, C.c_int, C.c_int, C.c_int, P]),
+    "inqx_overlap_diagonal": (C.c_int, [P, C.c_int, C.c_int, C.c_long, C.c_double, P, P, C.c_int, P]),
+    "inqx_ham_apply_real": (C.c_int, [P, P, C.c_int, C.c_int, P, C.c_int, P]),
+    "inqx_density_add": (C.c_int, [P, P, C.c_int, C.c_int, P, DP, C.c_int, C.c_int, P]),
+    "inqx_ham_set_launch_shape": (C.c_int, [P, C.c_int, C.c_int]),
     "inqx_overlap": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_long, C.c_double, P, P, P]),
     "inqx_cholesky": (C.c_int, [P, C.c_int, P]),
     "inqx_invert_triangular": (C.c_int, [P, C.c_int, P, P]),

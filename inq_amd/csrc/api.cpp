@@ -630,6 +630,85 @@ int inqx_ham_apply_batch(inqx_ham h, int nblocks, const inqx_basis *bases, const
   API_END(ctx)
 }
 
+int inqx_ham_set_launch_shape(inqx_ham h, int precision, int xwaves) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h, INQX_ERR_ARG, "null Hamiltonian");
+  INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
+  INQX_CHECK(xwaves == 1 || xwaves == 2 || xwaves == 4, INQX_ERR_ARG, "x-pass shape must be 1, 2 or 4");
+  INQX_CHECK(h->pinned == 0, INQX_ERR_STATE, "Hamiltonian pinned by a captured propagation graph");
+  if (h->xwaves[precision] != xwaves) {
+    h->xwaves[precision] = xwaves;
+    invalidate_launch_cache(h);
+  }
+  API_END(ctx)
+}
+
+int inqx_ham_autotune(inqx_ham h, inqx_basis bs, int nst, int precision, int *best_xwaves, double *best_ms) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && bs && nst > 0, INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(bs->b.ham == h, INQX_ERR_ARG, "basis belongs to another Hamiltonian");
+  INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
+  INQX_CHECK(h->pinned == 0, INQX_ERR_STATE, "Hamiltonian pinned by a captured propagation graph");
+  Ctx &cx = ctx->c;
+  const size_t bytes = (precision == INQX_FP64 ? 16 : 8) * (size_t)nst * bs->b.npw;
+  DevMem a, o;
+  a.ensure(bytes);
+  o.ensure(bytes);
+  launch_fill_zero(cx, a.p, bytes);
+  Basis *bb = &bs->b;
+  const void *pin = a.p;
+  void *pout = o.p;
+  hipEvent_t e0, e1;
+  INQX_HIP(hipEventCreate(&e0));
+  INQX_HIP(hipEventCreate(&e1));
+  const bool prof = cx.prof;
+  cx.prof = false;
+  int best = h->xwaves[precision];
+  double best_t = 1e300;
+  try {
+    for (int w : {4, 2, 1}) {
+      if (h->xwaves[precision] != w) {
+        h->xwaves[precision] = w;
+        invalidate_launch_cache(h);
+      }
+      auto run = [&]() {
+        if (precision == INQX_FP64) launch_apply<double>(h, 1, &bb, &nst, &pin, &pout);
+        else launch_apply<float>(h, 1, &bb, &nst, &pin, &pout);
+      };
+      run();
+      INQX_HIP(hipEventRecord(e0, cx.stream));
+      for (int r = 0; r < 3; ++r) run();
+      INQX_HIP(hipEventRecord(e1, cx.stream));
+      INQX_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      INQX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms / 3.0 < best_t) {
+        best_t = ms / 3.0;
+        best = w;
+      }
+    }
+  } catch (...) {
+    cx.prof = prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  cx.prof = prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (h->xwaves[precision] != best) {
+    h->xwaves[precision] = best;
+    invalidate_launch_cache(h);
+  }
+  INQX_HIP(hipStreamSynchronize(cx.stream));
+  if (best_xwaves) *best_xwaves = best;
+  if (best_ms) *best_ms = best_t;
+  API_END(ctx)
+}
+
 int inqx_ham_apply(inqx_ham h, inqx_basis bs, int nst, int precision, const void *psi, void *hpsi) {
   return inqx_ham_apply_batch(h, 1, &bs, &nst, precision, &psi, &hpsi);
 }
@@ -691,6 +770,15 @@ int inqx_overlap(inqx_ctx ctx, int precision, int n1, int n2, long npw, double v
   check_states(ctx, precision, n1, npw);
   INQX_CHECK(n2 > 0 && phi1 && phi2 && S, INQX_ERR_ARG, "null argument");
   launch_overlap(ctx->c, precision, n1, n2, npw, phi1, phi2, vol_el, (double *)S, ctx->c.ws_partial);
+  API_END(ctx)
+}
+
+int inqx_overlap_diagonal(inqx_ctx ctx, int precision, int n, long npw, double vol_el, const void *phi1,
+                          const void *phi2, int normalized, void *out) {
+  API_BEGIN
+  check_states(ctx, precision, n, npw);
+  INQX_CHECK(phi1 && phi2 && out, INQX_ERR_ARG, "null argument");
+  launch_overlap_diagonal(ctx->c, precision, n, npw, vol_el, phi1, phi2, normalized, (double *)out);
   API_END(ctx)
 }
 
@@ -790,6 +878,54 @@ int inqx_subspace_diag(inqx_ham h, inqx_basis bs, int nst, int precision, void *
                 ctx->c.ws_ops);
   INQX_HIP(hipMemcpyAsync(phi, ctx->c.ws_tmp.p, cs * (size_t)nst * npw, hipMemcpyDeviceToDevice, ctx->c.stream));
   for (int k = 0; k < nst; ++k) eigenvalues[k] = w[k];
+  API_END(ctx)
+}
+
+int inqx_randomize(inqx_ctx ctx, int type, long npoints, int nst, int index, int layout, void *out) {
+  API_BEGIN
+  INQX_CHECK(ctx && !ctx->c.host_only, INQX_ERR_STATE, "needs a device context");
+  INQX_CHECK(type == INQX_FP64 || type == INQX_FP32 || type == INQX_REAL64, INQX_ERR_ARG, "bad value type");
+  INQX_CHECK(layout == INQX_STATE_MAJOR || layout == INQX_POINT_MAJOR, INQX_ERR_ARG, "bad layout");
+  INQX_CHECK(npoints > 0 && nst > 0 && out, INQX_ERR_ARG, "bad arguments");
+  launch_randomize(ctx->c, type, npoints, nst, index, layout, out);
+  API_END(ctx)
+}
+
+int inqx_ham_apply_real(inqx_ham h, inqx_basis bs, int nst, int precision, const void *psi_rs, int layout,
+                        void *hpsi_rs) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_transform(h, bs, nst, precision, layout);
+  INQX_CHECK(psi_rs && hpsi_rs && psi_rs != hpsi_rs, INQX_ERR_ARG, "bad state pointers");
+  Basis &B = bs->b;
+  INQX_CHECK(B.npw == h->g.N, INQX_ERR_ARG, "real-space apply needs a basis holding the whole cube");
+  const size_t bytes = (precision == INQX_FP64 ? 16 : 8) * (size_t)nst * B.npw;
+  h->rs_a.ensure(bytes);
+  h->rs_b.ensure(bytes);
+  transform_impl(h, B, nst, precision, 2, nullptr, const_cast<void *>(psi_rs), layout, false, h->rs_a.p);
+  Basis *bb = &B;
+  const void *pin = h->rs_a.p;
+  void *pout = h->rs_b.p;
+  if (precision == INQX_FP64) launch_apply<double>(h, 1, &bb, &nst, &pin, &pout);
+  else launch_apply<float>(h, 1, &bb, &nst, &pin, &pout);
+  transform_impl(h, B, nst, precision, 1, h->rs_b.p, hpsi_rs, layout, false, nullptr);
+  API_END(ctx)
+}
+
+int inqx_density_add(inqx_ham h, inqx_basis bs, int nst, int precision, const void *phi, const double *occ,
+                     int ncomp, int ispin, void *density) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_transform(h, bs, nst, precision, INQX_STATE_MAJOR);
+  INQX_CHECK(phi && occ && density && ncomp > 0 && ispin >= 0 && ispin < ncomp, INQX_ERR_ARG, "bad arguments");
+  const size_t per = (precision == INQX_FP64 ? 16 : 8) * (size_t)h->g.N;
+  h->exx_phi.ensure(per * nst);
+  transform_impl(h, bs->b, nst, precision, 1, phi, h->exx_phi.p, INQX_STATE_MAJOR, false, nullptr);
+  std::vector<double> oc(occ, occ + nst);
+  upload(h->dens_occ, oc, ctx->c.stream);
+  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+  launch_density_add(ctx->c, precision, h->exx_phi.p, h->dens_occ.as<double>(), nst, h->g.N, ncomp, ispin,
+                     (double *)density);
   API_END(ctx)
 }
 

@@ -63,6 +63,11 @@ template <typename T, int N1, int N2> struct PairPlaneCfg {
 
 template <int N> __host__ __device__ constexpr int symi(int i) { return i >= (N + 1) / 2 ? i - N : i; }
 
+// reciprocal: the hardware approximation (1 ulp) for the complex64 path, an
+// exact division for complex128
+__device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
+
 template <typename T, int N0, int N1, int N2>
 __global__ void __launch_bounds__(128) k_pair_plane_fwd(PairDev<T> d, int total) {
   using C = cplx<T>;
@@ -148,7 +153,7 @@ __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
       const T m0 = (T)symi<N0>(i0);
       const T g0 = m0 * d.b[0][0] + u[0], g1 = m0 * d.b[0][1] + u[1], g2 = m0 * d.b[0][2] + u[2];
       const T gg = g0 * g0 + g1 * g1 + g2 * g2;
-      const T f = gg < (T)1e-6 ? d.zero : d.scal / gg;
+      const T f = gg < (T)1e-6 ? d.zero : d.scal * rcp_t(gg);
       x[i0].x *= f;
       x[i0].y *= f;
     }

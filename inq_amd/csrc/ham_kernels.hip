@@ -1072,6 +1072,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   key.k.push_back((uintptr_t)mode);
   key.k.push_back((uintptr_t)opts->rs_layout);
   key.k.push_back((uintptr_t)(opts->accumulate ? 1 : 0));
+  key.k.push_back((uintptr_t)h->xwaves[prec]);
   if (mode != 0)
     for (int i = 0; i < nblocks; ++i) key.k.push_back((uintptr_t)opts->rs[i]);
   for (int i = 0; i < nblocks; ++i) {
@@ -1128,7 +1129,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     }
     // waves (states) per x-pass workgroup within the LDS budget
     const int cs = (int)sizeof(cplx<T>);
-    int nw = 4;
+    int nw = std::max(1, std::min(4, h->xwaves[prec]));
     ld->mode = mode;
     ld->ntile_line = (g.n[1] * g.n[2] + kTile - 1) / kTile;
     if (mode == 0) {

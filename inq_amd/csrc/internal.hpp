@@ -244,8 +244,12 @@ struct inqx_ham_s {
   inqx::KBTables kb;
   std::vector<inqx_basis> bases;
   int pinned = 0;  // > 0 while a captured graph uses the projector tables
+  // launch shape of the x-pass kernels per precision: states (waves) per work-group
+  int xwaves[2] = {4, 4};
   // exact-exchange workspaces: real-space states, hf states, accumulator, pair planes
   inqx::DevMem exx_phi, exx_hf, exx_w, exx_t, exx_jlist, exx_coef;
+  // real-space apply and density workspaces (sphere arrays of a full-cube basis, occupations)
+  inqx::DevMem rs_a, rs_b, dens_occ;
   size_t exx_pair_bytes = 0;  // budget of the pair workspace (one j-chunk); 0: a quarter of free memory, <= 64 GiB
 };
 
@@ -275,6 +279,11 @@ void launch_cholesky(Ctx &cx, int n, double *S, int *info);
 void launch_trinv(Ctx &cx, int n, const double *L, double *X);
 void launch_stream_copy(Ctx &cx, const void *src, void *dst, size_t bytes);
 void launch_fill_zero(Ctx &cx, void *p, size_t bytes);
+void launch_randomize(Ctx &cx, int type, long npoints, int nst, int index, int layout, void *out);
+void launch_overlap_diagonal(Ctx &cx, int prec, int n, long K, double vol, const void *A, const void *B,
+                             int normalized, double *out);
+void launch_density_add(Ctx &cx, int prec, const void *phi, const double *occ, int nst, long N, int ncomp,
+                        int ispin, double *den);
 // exx_kernels.hip: batched pair-density Poisson solves on real-space arrays in
 // state-major layout [n][N].  With hf: for every chunk entry jj (hf index
 // j = jlist[jj]) and state i, rho = conj(hf_j) phi_i -> Poisson -> w_i +=

@@ -435,6 +435,148 @@ __global__ void __launch_bounds__(256) k_fill_zero(V4f *p, long n16, unsigned ch
   if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
 }
 
+// ------------------------------------------------------------------------ overlap_diagonal
+// per state: sum conj(phi1) phi2 and sum |phi2|^2 in FP64 (one work-group per
+// state, fixed reduction order); out = vol * value, or value / norm
+template <typename T>
+__global__ void __launch_bounds__(256) k_overlap_diag(const cplx<T> *A, const cplx<T> *B, long K, double vol,
+                                                     int normalized, double *out) {
+  __shared__ double red[3][256];
+  const int tid = threadIdx.x, s = blockIdx.x;
+  const GAS cplx<T> *a = lgp(A) + (size_t)s * K;
+  const GAS cplx<T> *b = lgp(B) + (size_t)s * K;
+  double vr = 0.0, vi = 0.0, nn = 0.0;
+  for (long p = tid; p < K; p += 256) {
+    const cplx<T> x = a[p], y = b[p];
+    vr += (double)x.x * y.x + (double)x.y * y.y;
+    vi += (double)x.x * y.y - (double)x.y * y.x;
+    nn += (double)y.x * y.x + (double)y.y * y.y;
+  }
+  red[0][tid] = vr;
+  red[1][tid] = vi;
+  red[2][tid] = nn;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w)
+      for (int c = 0; c < 3; ++c) red[c][tid] += red[c][tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    GAS double *o = lgw(out) + 2 * (size_t)s;
+    if (normalized) {
+      o[0] = red[0][0] / red[2][0];
+      o[1] = red[1][0] / red[2][0];
+    } else {
+      o[0] = vol * red[0][0];
+      o[1] = vol * red[1][0];
+    }
+  }
+}
+
+void launch_overlap_diagonal(Ctx &cx, int prec, int n, long K, double vol, const void *A, const void *B,
+                             int normalized, double *out) {
+  ProfScope ps(cx, "overlap_diagonal");
+  if (prec == INQX_FP64)
+    hipLaunchKernelGGL(k_overlap_diag<double>, dim3(n), dim3(256), 0, cx.stream, (const cplx<double> *)A,
+                       (const cplx<double> *)B, K, vol, normalized, out);
+  else
+    hipLaunchKernelGGL(k_overlap_diag<float>, dim3(n), dim3(256), 0, cx.stream, (const cplx<float> *)A,
+                       (const cplx<float> *)B, K, vol, normalized, out);
+  INQX_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------ density
+// density[p][ncomp] (component ispin) += sum_i occ_i |phi_i(p)|^2, states
+// state-major [nst][N]; FP64, states summed in order
+template <typename T>
+__global__ void __launch_bounds__(256) k_density_add(const cplx<T> *phi, const double *occ, int nst, long N,
+                                                    int ncomp, int ispin, double *den) {
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += (long)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int i = 0; i < nst; ++i) {
+      const cplx<T> v = lgp(phi)[(size_t)i * N + p];
+      acc += lgp(occ)[i] * ((double)v.x * v.x + (double)v.y * v.y);
+    }
+    lgw(den)[p * ncomp + ispin] += acc;
+  }
+}
+
+void launch_density_add(Ctx &cx, int prec, const void *phi, const double *occ, int nst, long N, int ncomp,
+                        int ispin, double *den) {
+  const int grid = (int)std::max<long>(1, std::min<long>((N + 255) / 256, (long)cx.num_cu * 16));
+  ProfScope ps(cx, "density_add");
+  if (prec == INQX_FP64)
+    hipLaunchKernelGGL(k_density_add<double>, dim3(grid), dim3(256), 0, cx.stream, (const cplx<double> *)phi, occ,
+                       nst, N, ncomp, ispin, den);
+  else
+    hipLaunchKernelGGL(k_density_add<float>, dim3(grid), dim3(256), 0, cx.stream, (const cplx<float> *)phi, occ,
+                       nst, N, ncomp, ispin, den);
+  INQX_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------ randomize
+// operations::randomize (src/operations/randomize.hpp:48-70): value (ip, ist)
+// comes from pcg32 (64-bit LCG, XSH-RR output of the pre-step state) seeded
+// with bump(seed + inc) and advanced by (ip nst + ist) samples (jump-ahead,
+// Brown 1994); U = 2 r / 2^32 - 1, real part first
+constexpr uint64_t kPcgMult = 6364136223846793005ULL;
+constexpr uint64_t kPcgInc = 1442695040888963407ULL;
+
+__device__ __forceinline__ uint64_t pcg_jump(uint64_t state, uint64_t delta) {
+  uint64_t cm = kPcgMult, cp = kPcgInc, am = 1, ap = 0;
+  while (delta > 0) {
+    if (delta & 1) {
+      am *= cm;
+      ap = ap * cm + cp;
+    }
+    cp = (cm + 1) * cp;
+    cm *= cm;
+    delta >>= 1;
+  }
+  return am * state + ap;
+}
+
+__device__ __forceinline__ double pcg_uniform(uint64_t s) {
+  const uint32_t rot = (uint32_t)(s >> 59);
+  const uint32_t x = (uint32_t)(((s >> 18) ^ s) >> 27);
+  const uint32_t r = (x >> rot) | (x << ((32u - rot) & 31u));
+  return 2.0 * (double)r / 4294967296.0 - 1.0;
+}
+
+__global__ void __launch_bounds__(256) k_randomize(long npoints, int nst, uint64_t start, int type, int layout,
+                                                   void *out) {
+  const long total = npoints * nst;
+  const int per = type == 2 ? 1 : 2;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const long ip = idx / nst;
+    const int ist = (int)(idx - ip * nst);
+    uint64_t st = pcg_jump(start, (uint64_t)idx * per);
+    const double re = pcg_uniform(st);
+    double im = 0.0;
+    if (per == 2) im = pcg_uniform(st * kPcgMult + kPcgInc);
+    const long o = layout == 0 ? (long)ist * npoints + ip : idx;
+    if (type == INQX_FP64) {
+      lgw((double *)out)[2 * o] = re;
+      lgw((double *)out)[2 * o + 1] = im;
+    } else if (type == INQX_FP32) {
+      lgw((float *)out)[2 * o] = (float)re;
+      lgw((float *)out)[2 * o + 1] = (float)im;
+    } else {
+      lgw((double *)out)[o] = re;
+    }
+  }
+}
+
+void launch_randomize(Ctx &cx, int type, long npoints, int nst, int index, int layout, void *out) {
+  const uint64_t seed = (uint64_t)npoints * (uint64_t)nst + (uint64_t)index;
+  const uint64_t start = (seed + kPcgInc) * kPcgMult + kPcgInc;
+  const long total = npoints * nst;
+  const int grid = (int)std::max<long>(1, std::min<long>((total + 255) / 256, (long)cx.num_cu * 32));
+  ProfScope ps(cx, "randomize");
+  hipLaunchKernelGGL(k_randomize, dim3(grid), dim3(256), 0, cx.stream, npoints, nst, start, type, layout, out);
+  INQX_HIP(hipGetLastError());
+}
+
 void launch_fill_zero(Ctx &cx, void *p, size_t bytes) {
   if (bytes == 0) return;
   const long n16 = (long)(bytes / 16);

@@ -236,12 +236,44 @@ class KSHamiltonian:
                                         int(bool(accumulate)), out.ptr), self.ctx.h, "inqx_real_to_sphere")
         return out
 
+    def apply_real(self, basis, psi_rs, nst, layout=0, out=None):
+        """ks_hamiltonian::operator() on real-space states (ks_hamiltonian.hpp:137-156);
+        basis must hold the whole cube."""
+        out = out if out is not None else DeviceArray(self.ctx, psi_rs.shape, psi_rs.dtype)
+        check(lib().inqx_ham_apply_real(self.h, basis.h, int(nst), precision_of(psi_rs.dtype), psi_rs.ptr, int(layout),
+                                        out.ptr), self.ctx.h, "inqx_ham_apply_real")
+        return out
+
+    def density_add(self, basis, phi, occ, density=None, ncomp=1, ispin=0):
+        """density::calculate_add (density.hpp:24-38): density [N][ncomp] += sum occ |phi(r)|^2."""
+        N = int(np.prod(self.n))
+        if density is None:
+            density = DeviceArray(self.ctx, (N, ncomp), np.float64)
+            density.upload(np.zeros((N, ncomp)))
+        occ = np.ascontiguousarray(occ, dtype=np.float64)
+        check(lib().inqx_density_add(self.h, basis.h, phi.shape[0], precision_of(phi.dtype), phi.ptr, _dp(occ),
+                                     int(ncomp), int(ispin), density.ptr), self.ctx.h, "inqx_density_add")
+        return density
+
     def poisson(self, rho, gshift=(0.0, 0.0, 0.0), zeroterm=0.0):
         """solvers::poisson::in_place on densities [n][N] (poisson.hpp:71-110,199-213), in place."""
         g = np.ascontiguousarray(gshift, dtype=np.float64)
         check(lib().inqx_poisson(self.h, rho.shape[0], precision_of(rho.dtype), rho.ptr, _dp(g), float(zeroterm)),
               self.ctx.h, "inqx_poisson")
         return rho
+
+    def autotune(self, basis, nst, dtype):
+        """times the x-pass launch shapes on this block size and keeps the fastest;
+        returns (states per work-group, ms per apply)."""
+        w = C.c_int()
+        ms = C.c_double()
+        check(lib().inqx_ham_autotune(self.h, basis.h, int(nst), precision_of(dtype), C.byref(w), C.byref(ms)),
+              self.ctx.h, "inqx_ham_autotune")
+        return w.value, ms.value
+
+    def set_launch_shape(self, dtype, xwaves):
+        check(lib().inqx_ham_set_launch_shape(self.h, precision_of(dtype), int(xwaves)), self.ctx.h,
+              "set_launch_shape")
 
     def set_exchange_workspace(self, nbytes):
         check(lib().inqx_ham_set_exchange_workspace(self.h, int(nbytes)), self.ctx.h, "set_exchange_workspace")

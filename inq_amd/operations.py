@@ -24,6 +24,29 @@ def overlap(ctx, phi1, phi2, vol_el):
     return S
 
 
+def overlap_diagonal(ctx, phi1, phi2, vol_el, normalized=False):
+    """per-state vol_el <phi1_i|phi2_i> (overlap_diagonal.hpp:25-60), or with
+    normalized <phi1_i|phi2_i>/<phi2_i|phi2_i> (:100-153); device complex128 [n]."""
+    n, npw = phi1.shape
+    assert phi2.shape == phi1.shape and phi1.dtype == phi2.dtype
+    out = DeviceArray(ctx, (n,), np.complex128)
+    check(lib().inqx_overlap_diagonal(ctx.h, precision_of(phi1.dtype), n, npw, float(vol_el), phi1.ptr, phi2.ptr,
+                                      int(bool(normalized)), out.ptr), ctx.h, "inqx_overlap_diagonal")
+    return out
+
+
+def randomize(ctx, npoints, nst, index=0, dtype=np.complex128, layout=1):
+    """operations::randomize (randomize.hpp:48-70): pcg32 U[-1,1) values, [npoints][nst] (layout 1)
+    or [nst][npoints] (layout 0); dtype complex128, complex64 or float64."""
+    dt = np.dtype(dtype)
+    kind = {np.dtype(np.complex128): 0, np.dtype(np.complex64): 1, np.dtype(np.float64): 2}[dt]
+    shape = (npoints, nst) if layout == 1 else (nst, npoints)
+    out = DeviceArray(ctx, shape, dt)
+    check(lib().inqx_randomize(ctx.h, kind, int(npoints), int(nst), int(index), int(layout), out.ptr), ctx.h,
+          "inqx_randomize")
+    return out
+
+
 def cholesky(ctx, S):
     """in place S = L L^H, L lower (cholesky.hpp:28-100)."""
     n = S.shape[0]

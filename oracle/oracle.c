@@ -37,6 +37,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
+#include <stdint.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -1237,4 +1238,52 @@ void orc_ace_apply(const orc_ham *h, int nxi, const cplx *xi, int nst, const cpl
       exxphi[ip * nst + b] -= acc;
     }
   free(m);
+}
+
+/* ===================================================================== */
+/* operations::randomize (src/operations/randomize.hpp:48-70)             */
+/* pcg32 of the vendored pcg-cpp (external_libs/pcg-cpp/pcg_random.hpp):  */
+/* 64-bit LCG (default multiplier/increment), XSH-RR output of the state  */
+/* before each step, seeded as bump(seed + increment), discard by the     */
+/* jump-ahead of Brown (1994).                                            */
+/* ===================================================================== */
+static const uint64_t PCG_MULT = 6364136223846793005ULL;
+static const uint64_t PCG_INC = 1442695040888963407ULL;
+
+static uint64_t pcg_advance(uint64_t state, uint64_t delta) {
+  uint64_t cur_mult = PCG_MULT, cur_plus = PCG_INC, acc_mult = 1, acc_plus = 0;
+  while (delta > 0) {
+    if (delta & 1) {
+      acc_mult *= cur_mult;
+      acc_plus = acc_plus * cur_mult + cur_plus;
+    }
+    cur_plus = (cur_mult + 1) * cur_plus;
+    cur_mult *= cur_mult;
+    delta >>= 1;
+  }
+  return acc_mult * state + acc_plus;
+}
+
+static uint32_t pcg_output(uint64_t s) {
+  uint32_t rot = (uint32_t)(s >> 59);
+  uint32_t x = (uint32_t)(((s >> 18) ^ s) >> 27);
+  return (x >> rot) | (x << ((32u - rot) & 31u));
+}
+
+/* out [npoints][nst] real (is_complex = 0) or interleaved complex values:
+ * value (ip, ist) uses the generator seeded with npoints*nst + index and
+ * advanced by (ip*nst + ist) * samples; U = 2 r / 2^32 - 1, real part first */
+void orc_randomize(long npoints, int nst, int index, int is_complex, double *out) {
+  const uint64_t seed = (uint64_t)npoints * (uint64_t)nst + (uint64_t)index;
+  const uint64_t start = (seed + PCG_INC) * PCG_MULT + PCG_INC;
+  const int per = is_complex ? 2 : 1;
+#pragma omp parallel for schedule(static)
+  for (long ip = 0; ip < npoints; ++ip)
+    for (int ist = 0; ist < nst; ++ist) {
+      uint64_t st = pcg_advance(start, ((uint64_t)ip * nst + ist) * per);
+      for (int c = 0; c < per; ++c) {
+        out[((size_t)ip * nst + ist) * per + c] = 2.0 * (double)pcg_output(st) / 4294967296.0 - 1.0;
+        st = st * PCG_MULT + PCG_INC;
+      }
+    }
 }

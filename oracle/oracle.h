@@ -128,6 +128,9 @@ void   orc_exchange_direct(const orc_ham *h, int nst, const orc_cplx *phi_rs, in
 void   orc_ace_apply(const orc_ham *h, int nxi, const orc_cplx *xi, int nst, const orc_cplx *phi_rs,
                      orc_cplx *exxphi);
 
+/* ---- operations::randomize (pcg32), values [npoints][nst] ---- */
+void orc_randomize(long npoints, int nst, int index, int is_complex, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,7 @@ def lib():
             "orc_density_dipole_sphere": (None, [P, C.c_int, C.c_long, ip, cp, dp, P, dp]),
             "orc_poisson_in_place": (None, [P, C.c_int, cp, dp, C.c_double]),
             "orc_sing_auxiliary": (C.c_double, [dp, dp]),
+            "orc_randomize": (None, [C.c_long, C.c_int, C.c_int, C.c_int, dp]),
             "orc_sing_correction": (None, [dp, C.c_int, dp, dp, dp, dp]),
             "orc_ace_build": (C.c_double, [P, C.c_int, cp, dp, C.c_double, cp]),
             "orc_exchange_direct": (None, [P, C.c_int, cp, C.c_int, cp, dp, C.c_double, C.c_double, cp]),
@@ -382,6 +383,13 @@ def rotate5(rot, phi, rotphi, alpha, beta, band_major):
     npnt = phi.size // nst
     lib().orc_rotate5(npnt, nst, int(band_major), _c128(rot), _c128(phi), rotphi, _c(alpha), _c(beta))
     return rotphi
+
+
+def randomize(npoints, nst, index=0, complex_values=True):
+    """operations::randomize values [npoints][nst] (complex128 or float64)."""
+    out = np.zeros((npoints, nst, 2 if complex_values else 1))
+    lib().orc_randomize(int(npoints), int(nst), int(index), int(bool(complex_values)), out)
+    return out[..., 0] + 1j * out[..., 1] if complex_values else out[..., 0]
 
 
 def sing_auxiliary(lattice, q):

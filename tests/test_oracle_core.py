@@ -499,3 +499,18 @@ def test_etrs_step_unitary_and_phase():
     t4 = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
     out = h.etrs_step(c, g, dt)
     assert abs(out[0, 7] - t4 ** 2) < 1e-14
+
+
+def test_randomize_kats():
+    """randomize.hpp:96-144: norms of pcg32 U[-1,1) fields on the 20^3 grid of a
+    10 bohr cube (spacing 0.49672941), 12 states, volume element 1000/8000."""
+    npts, nst = 8000, 12
+    r = orc.randomize(npts, nst, 0, complex_values=False)
+    c = orc.randomize(npts, nst, 0, complex_values=True)
+    kat_r = [330.1381395023, 330.5444105287, 331.5435469092, 340.0201855451, 332.959828166, 333.1967703871,
+             328.9429576951, 336.8659386972, 331.6228884743, 334.5374311686, 334.6847273315, 334.1846706334]
+    kat_c = [669.1459385544, 664.8544595319, 669.6590362331, 668.4278672543, 668.0965017232, 665.1942561788,
+             665.6688584485, 672.3987040517, 671.7988968555, 672.6486737858, 664.3526206348, 674.5678452953]
+    assert np.allclose(0.125 * np.sum(r * r, 0), kat_r, rtol=1e-10, atol=0)
+    assert np.allclose(0.125 * np.sum(np.abs(c) ** 2, 0), kat_c, rtol=1e-10, atol=0)
+    assert np.all(np.abs(c.real) <= 1.0) and np.all(np.abs(c.imag) <= 1.0)
