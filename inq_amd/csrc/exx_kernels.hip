@@ -40,6 +40,34 @@ namespace {
 template <typename T> __device__ __forceinline__ const GAS T *xg(const T *p) { return (const GAS T *)p; }
 template <typename T> __device__ __forceinline__ GAS T *xw(T *p) { return (GAS T *)p; }
 
+// raw buffer access to one complex value: per-lane byte offset in a VGPR and a
+// wave-uniform byte offset (a constant after unrolling) in an SGPR, so that a
+// line of loads needs a single address register
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<float> &v) {
+  const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  v.x = __int_as_float(q[0]);
+  v.y = __int_as_float(q[1]);
+}
+__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<double> &v) {
+  const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  v.x = __longlong_as_double(((long long)(unsigned)q[1] << 32) | (unsigned)q[0]);
+  v.y = __longlong_as_double(((long long)(unsigned)q[3] << 32) | (unsigned)q[2]);
+}
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<float> v) {
+  typedef int i2 __attribute__((ext_vector_type(2)));
+  i2 q = {__float_as_int(v.x), __float_as_int(v.y)};
+  __builtin_amdgcn_raw_buffer_store_b64(q, r, voff, soff, 0);
+}
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<double> v) {
+  typedef int i4 __attribute__((ext_vector_type(4)));
+  const long long a = __double_as_longlong(v.x), b = __double_as_longlong(v.y);
+  i4 q = {(int)a, (int)(a >> 32), (int)b, (int)(b >> 32)};
+  __builtin_amdgcn_raw_buffer_store_b128(q, r, voff, soff, 0);
+}
+
 template <typename T> struct PairDev {
   const cplx<T> *phi;  // [nst][N]
   const cplx<T> *hf;   // [nhf][N] or null
@@ -123,44 +151,52 @@ __global__ void __launch_bounds__(128) k_pair_plane_fwd(PairDev<T> d, int total)
   }
 }
 
+// each line is split over a lane pair (lane, lane + 32): 32 lines per wave
 template <typename T, int N0, int N1, int N2>
 __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
   using C = cplx<T>;
   constexpr int NP = N1 * N2;
-  constexpr int NTILE = (NP + 63) / 64;
+  constexpr int H = N0 / 2;
+  constexpr int NTILE = (NP + 31) / 32;
   constexpr int NGRP = (NTILE + 3) / 4;
   constexpr long N = (long)N0 * NP;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool upper = lane >= 32;
+  const int half = upper ? 1 : 0;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int pair = wg / NGRP;
     const int tile = (wg - pair * NGRP) * 4 + wave;
     if (tile >= NTILE) continue;  // wave-uniform
-    const int L = tile * 64 + lane;
+    const int L = tile * 32 + (lane & 31);
     const int Lc = L < NP ? L : NP - 1;
-    GAS C *tp = xw(d.t) + (size_t)pair * N + Lc;
-    C x[N0];
+    const auto rs = make_rsrc(d.t + (size_t)pair * N, (unsigned)(N * sizeof(C)));
+    const unsigned voff = (unsigned)((Lc + half * NP) * sizeof(C));
+    C v[H];
 #pragma unroll
-    for (int i0 = 0; i0 < N0; ++i0) x[i0] = tp[(size_t)i0 * NP];
-    fft_line<N0, -1, C>::run(x);
-    // G + q = m0 b0 + (m1 b1 + m2 b2 + q)
+    for (int n = 0; n < H; ++n) bload(rs, voff, (unsigned)(2 * n * NP * sizeof(C)), v[n]);
+    fft_split<N0, -1, C>::dit(v, upper);
+    __builtin_amdgcn_sched_barrier(0);  // keep the kernel factors out of the FFT's live range
+    // frequency f = k + half H; G + q = m0 b0 + (m1 b1 + m2 b2 + q)
     const int i1 = Lc / N2, i2 = Lc - (Lc / N2) * N2;
     const T m1 = (T)(i1 >= (N1 + 1) / 2 ? i1 - N1 : i1), m2 = (T)(i2 >= (N2 + 1) / 2 ? i2 - N2 : i2);
     T u[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) u[c] = m1 * d.b[1][c] + m2 * d.b[2][c] + d.q[c];
 #pragma unroll
-    for (int i0 = 0; i0 < N0; ++i0) {
-      const T m0 = (T)symi<N0>(i0);
+    for (int k = 0; k < H; ++k) {
+      const int f = k + half * H;
+      const T m0 = (T)(f >= (N0 + 1) / 2 ? f - N0 : f);
       const T g0 = m0 * d.b[0][0] + u[0], g1 = m0 * d.b[0][1] + u[1], g2 = m0 * d.b[0][2] + u[2];
       const T gg = g0 * g0 + g1 * g1 + g2 * g2;
-      const T f = gg < (T)1e-6 ? d.zero : d.scal * rcp_t(gg);
-      x[i0].x *= f;
-      x[i0].y *= f;
+      const T fct = gg < (T)1e-6 ? d.zero : d.scal * rcp_t(gg);
+      v[k].x *= fct;
+      v[k].y *= fct;
     }
-    fft_line<N0, +1, C>::run(x);
+    __builtin_amdgcn_sched_barrier(0);
+    fft_split<N0, +1, C>::dif(v, upper);
     if (L < NP) {
 #pragma unroll
-      for (int i0 = 0; i0 < N0; ++i0) tp[(size_t)i0 * NP] = x[i0];
+      for (int n = 0; n < H; ++n) bstore(rs, voff, (unsigned)(2 * n * NP * sizeof(C)), v[n]);
     }
   }
 }
@@ -286,7 +322,7 @@ template <typename T, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
                        cx.stream, d, tot1);
   }
   INQX_HIP(hipGetLastError());
-  constexpr int ngrp = ((S * S + 63) / 64 + 3) / 4;
+  constexpr int ngrp = ((S * S + 31) / 32 + 3) / 4;
   const int tot2 = d.nj * d.nst * ngrp;
   {
     ProfScope ps(cx, f64 ? "pair_xline_f64" : "pair_xline_f32");

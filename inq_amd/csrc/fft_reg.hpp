@@ -222,4 +222,63 @@ template <int N, int SIGN, typename C> struct fft_line {
   }
 };
 
+// ---------------------------------------------------------------- lines split over lane pairs
+// A line of even length N is shared by lane l < 32 ("lower") and lane l + 32
+// ("upper") of a wave, N/2 values each, so a lane holds half the registers of
+// fft_line.  One radix-2 stage crosses the pair; v_permlane32_swap hands each
+// lane both values of its pair (no LDS traffic).
+
+// the values held by the lower and the upper lane of the pair, on both lanes
+__device__ __forceinline__ void pair_vals(float v, float &lo, float &hi) {
+  const unsigned u = __float_as_uint(v);
+  const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+  lo = __uint_as_float(r[0]);
+  hi = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void pair_vals(double v, double &lo, double &hi) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned a = (unsigned)u, b = (unsigned)(u >> 32);
+  const auto ra = __builtin_amdgcn_permlane32_swap(a, a, false, false);
+  const auto rb = __builtin_amdgcn_permlane32_swap(b, b, false, false);
+  lo = __longlong_as_double((long long)(((unsigned long long)rb[0] << 32) | ra[0]));
+  hi = __longlong_as_double((long long)(((unsigned long long)rb[1] << 32) | ra[1]));
+}
+template <typename C> __device__ __forceinline__ void pair_vals(C v, C &lo, C &hi) {
+  pair_vals(v.x, lo.x, hi.x);
+  pair_vals(v.y, lo.y, hi.y);
+}
+
+template <int N, int SIGN, typename C> struct fft_split {
+  static_assert(N % 2 == 0, "split lines need an even length");
+  static constexpr int H = N / 2;
+  using T = decltype(C{}.x);
+  // decimation in time: lower lanes hold x[2n], upper lanes x[2n+1] (n < H);
+  // afterwards lower lanes hold X[k], upper lanes X[k + H] (k < H)
+  static __device__ __forceinline__ void dit(C (&v)[H], bool upper) {
+    fft_line<H, SIGN, C>::run(v);
+    const T sg = upper ? (T)-1 : (T)1;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      C a, b;
+      pair_vals(v[k], a, b);
+      const C t = twmul<N, SIGN>(b, k);
+      v[k] = C{a.x + sg * t.x, a.y + sg * t.y};
+    }
+  }
+  // decimation in frequency: lower lanes hold x[n], upper lanes x[n + H];
+  // afterwards lower lanes hold X[2k], upper lanes X[2k + 1]
+  static __device__ __forceinline__ void dif(C (&v)[H], bool upper) {
+    const T sg = upper ? (T)-1 : (T)1;
+#pragma unroll
+    for (int n = 0; n < H; ++n) {
+      C a, b;
+      pair_vals(v[n], a, b);
+      const C d{a.x + sg * b.x, a.y + sg * b.y};
+      const C t = twmul<N, SIGN>(d, n);
+      v[n] = upper ? t : d;
+    }
+    fft_line<H, SIGN, C>::run(v);
+  }
+};
+
 }  // namespace inqx
