@@ -71,7 +71,7 @@ void invalidate_launch_cache(inqx_ham_s *h);
 struct HamDev {
   int n0, n1, n2, n12;
   long N;
-  int natoms, has_kb, ntile, maxpts;
+  int natoms, has_kb, ntile, maxpts, maxslot;
   const uint32_t *line_mask;
   const int *tile_info;
   const int *slot_atom;
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
 #pragma unroll
       for (int y = 0; y < N1; ++y) x[y] = col[y * (N2 + 1)];
       fft_line<N1, +1, C>::run(x);
-      GAS C *out = t1 + ((size_t)slot * B.nst + s0 + p) * (N1 * N2) + z;
+      GAS C *out = t1 + ((size_t)(s0 + p) * B.nslot + slot) * (N1 * N2) + z;
 #pragma unroll
       for (int y = 0; y < N1; ++y) out[y * N2] = x[y];
     }
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
     const GAS C *t1 = gp(B.t1);
     for (int i = tid; i < np * N2; i += nth) {
       const int p = i / N2, z = i - p * N2;
-      const GAS C *in = t1 + ((size_t)slot * B.nst + s0 + p) * (N1 * N2) + z;
+      const GAS C *in = t1 + ((size_t)(s0 + p) * B.nslot + slot) * (N1 * N2) + z;
       C x[N1];
 #pragma unroll
       for (int y = 0; y < N1; ++y) x[y] = in[y * N2];
@@ -270,24 +270,113 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
 // symmetric plane index of FFT index i (compile time)
 template <int N> constexpr int sym_of(int i) { return i >= (N + 1) / 2 ? i - N : i; }
 
-// loads the plane slots of one line (zero outside the slot range) and
-// transforms along i0; Lc is the lane's line, clamped into the grid
-template <typename T, int N0>
-__device__ __forceinline__ void load_line_inverse(const SphereBlock<T> &B, int n12, int s, int Lc,
-                                                  cplx<T> (&x)[N0]) {
-  using C = cplx<T>;
-  const GAS C *base = gp(B.t1) + (size_t)s * n12 + Lc;
-  const size_t sstride = (size_t)B.nst * n12;
-#pragma unroll
-  for (int i0 = 0; i0 < N0; ++i0) {
-    const int j = sym_of<N0>(i0) - B.m_lo;  // uniform
-    C v{0, 0};
-    if ((unsigned)j < (unsigned)B.nslot) v = base[(size_t)j * sstride];
-    x[i0] = v;
-  }
-  fft_line<N0, +1, C>::run(x);
+// ---- raw buffer access: per-lane byte offset in a VGPR, wave-uniform part in an SGPR
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<float> &v) {
+  const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  v.x = __int_as_float(q[0]);
+  v.y = __int_as_float(q[1]);
+}
+__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<double> &v) {
+  const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  v.x = __longlong_as_double(((long long)(unsigned)q[1] << 32) | (unsigned)q[0]);
+  v.y = __longlong_as_double(((long long)(unsigned)q[3] << 32) | (unsigned)q[2]);
+}
+__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float &v) {
+  v = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double &v) {
+  const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  v = __longlong_as_double(((long long)(unsigned)q[1] << 32) | (unsigned)q[0]);
+}
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<float> v) {
+  typedef int i2 __attribute__((ext_vector_type(2)));
+  i2 q = {__float_as_int(v.x), __float_as_int(v.y)};
+  __builtin_amdgcn_raw_buffer_store_b64(q, r, voff, soff, 0);
+}
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<double> v) {
+  typedef int i4 __attribute__((ext_vector_type(4)));
+  const long long a = __double_as_longlong(v.x), b = __double_as_longlong(v.y);
+  i4 q = {(int)a, (int)(a >> 32), (int)b, (int)(b >> 32)};
+  __builtin_amdgcn_raw_buffer_store_b128(q, r, voff, soff, 0);
 }
 
+// A line (i1,i2) along i0 is shared by a lane pair (lane, lane + 32): the
+// lower lane holds i0 in [0, N0/2), the upper lane [N0/2, N0) after an
+// inverse transform, and the even / odd i0 before it.  t1 is state-major
+// [state][slot][i1][i2]; slot j holds the symmetric plane index m_lo + j.
+//
+// loads the plane slots i0 = 2n + half of the pair's line (zero outside the
+// slot range) and transforms along i0: afterwards v[k] = x[k + half N0/2]
+template <typename T, int N0>
+__device__ __forceinline__ void load_line_inverse_split(const SphereBlock<T> &B, int n12, int s, int Lc, bool upper,
+                                                        cplx<T> (&v)[N0 / 2]) {
+  using C = cplx<T>;
+  constexpr int H = N0 / 2;
+  const unsigned pl = (unsigned)(n12 * sizeof(C));
+  const auto rs = make_rsrc(B.t1 + (size_t)s * B.nslot * n12, (unsigned)B.nslot * pl);
+  const unsigned vl = (unsigned)(Lc * sizeof(C));
+#pragma unroll
+  for (int n = 0; n < H; ++n) {
+    const int jl = sym_of<N0>(2 * n) - B.m_lo, jh = sym_of<N0>(2 * n + 1) - B.m_lo;  // uniform
+    const bool vlo = (unsigned)jl < (unsigned)B.nslot, vhi = (unsigned)jh < (unsigned)B.nslot;
+    C x{0, 0};
+    if (vlo && vhi && jh == jl + 1) {
+      bload(rs, vl + (upper ? pl : 0u), (unsigned)jl * pl, x);
+    } else if (vlo || vhi) {
+      const int j = upper ? jh : jl;
+      if ((unsigned)j < (unsigned)B.nslot) bload(rs, vl + (unsigned)j * pl, 0u, x);
+    }
+    v[n] = x;
+  }
+  fft_split<N0, +1, C>::dit(v, upper);
+}
+
+// stores v[k] = X[2k + half] (after the forward split transform) into the
+// plane slots of the line
+template <typename T, int N0>
+__device__ __forceinline__ void store_line_slots_split(const SphereBlock<T> &B, int n12, int s, int L, bool upper,
+                                                       const cplx<T> (&v)[N0 / 2]) {
+  using C = cplx<T>;
+  constexpr int H = N0 / 2;
+  const unsigned pl = (unsigned)(n12 * sizeof(C));
+  const auto rs = make_rsrc(B.t1 + (size_t)s * B.nslot * n12, (unsigned)B.nslot * pl);
+  const unsigned vl = (unsigned)(L * sizeof(C));
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const int jl = sym_of<N0>(2 * k) - B.m_lo, jh = sym_of<N0>(2 * k + 1) - B.m_lo;
+    const bool vlo = (unsigned)jl < (unsigned)B.nslot, vhi = (unsigned)jh < (unsigned)B.nslot;
+    if (vlo && vhi && jh == jl + 1) {
+      bstore(rs, vl + (upper ? pl : 0u), (unsigned)jl * pl, v[k]);
+    } else if (vlo || vhi) {
+      const int j = upper ? jh : jl;
+      if ((unsigned)j < (unsigned)B.nslot) bstore(rs, vl + (unsigned)j * pl, 0u, v[k]);
+    }
+  }
+}
+
+// bit i0 of a line's 96-bit point mask, i0 = k + half N0/2 (k a constant)
+template <int N0>
+__device__ __forceinline__ bool mask_bit(const uint32_t (&m)[3], int k, bool upper) {
+  const int il = k, ih = k + N0 / 2;
+  const bool bl = (m[il >> 5] >> (il & 31)) & 1u;
+  const bool bh = (m[ih >> 5] >> (ih & 31)) & 1u;
+  return upper ? bh : bl;
+}
+// number of mask bits below i0 = N0/2
+template <int N0> __device__ __forceinline__ int mask_count_low(const uint32_t (&m)[3]) {
+  constexpr int H = N0 / 2;
+  int c = 0;
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    const int lo = w * 32;
+    if (H >= lo + 32) c += __popc(m[w]);
+    else if (H > lo) c += __popc(m[w] & ((1u << (H - lo)) - 1u));
+  }
+  return c;
+}
 
 __device__ __forceinline__ void lds_add(float *p, float v) { atomicAdd(p, v); }
 __device__ __forceinline__ void lds_add(double *p, double v) { atomicAdd(p, v); }
@@ -302,52 +391,59 @@ __device__ __forceinline__ cplx<T> kb_phase(const double *kcov, V4<float> pos, T
   return cplx<T>{cs, sn};
 }
 
-// Inverse FFT along i0 of the wave's 64 lines, then the KB projections
-// P[slot][l] = sum_entries beta_l exp(+ik.pos) psi.  Each lane walks a
-// contiguous chunk of the tile's entries (sorted by slot), accumulates in
-// registers and flushes to the LDS slot accumulators only at slot changes.
+// Inverse FFT along i0 of the 64 lines of a tile (a wave pair per state, a
+// lane pair per line), then the KB projections P[slot][l] = sum_entries
+// beta_l exp(+ik.pos) psi.  Each of the pair's 128 lanes walks a contiguous
+// chunk of the tile's entries (sorted by slot), accumulates in registers and
+// flushes to the LDS slot accumulators only at slot changes.
 template <typename T, int N0>
 __global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                       int total, HamDev hd, int lds_per_wave) {
+                                                       int total, HamDev hd, int lds_per_state) {
   using C = cplx<T>;
+  constexpr int H = N0 / 2;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  C *pts = reinterpret_cast<C *>(smem_raw) + wave * lds_per_wave;
-  C *acc = pts + hd.maxpts;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ns = blockDim.x >> 7, sw = wave >> 1, part = wave & 1, ptid = threadIdx.x & 127;
+  const bool upper = lane >= 32;
+  C *pts = reinterpret_cast<C *>(smem_raw) + sw * lds_per_state;
+  // one accumulator set per wave of the pair, summed in a fixed order: the
+  // projections do not depend on the timing of the two waves
+  C *acc0 = pts + hd.maxpts;
+  C *acc = acc0 + part * kMaxNlm * hd.maxslot;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
     const int local = wg - gp(prefix)[b];
-    const int ngrp = (B.nst + nwave - 1) / nwave;
+    const int ngrp = (B.nst + ns - 1) / ns;
     const int tile = local / ngrp;
-    const int s = (local - tile * ngrp) * nwave + wave;
-    const int L = tile * kTile + lane;
+    const int s = (local - tile * ngrp) * ns + sw;
+    const int L = tile * kTile + part * 32 + (lane & 31);
     const V4<int> ti = ld4<int>(hd.tile_info, 2 * tile);
     const V4<int> tj = ld4<int>(hd.tile_info, 2 * tile + 1);
     const int npts = ti.x, nsl = ti.y, slot_base = ti.z, ent_base = ti.w, nent = tj.x;
     const bool active = (s < B.nst) && npts > 0;
     if (active) {
       const bool valid = L < hd.n12;
-      C x[N0];
-      load_line_inverse<T, N0>(B, hd.n12, s, valid ? L : hd.n12 - 1, x);
+      C v[H];
+      load_line_inverse_split<T, N0>(B, hd.n12, s, valid ? L : hd.n12 - 1, upper, v);
       if (valid) {
         const V4<uint32_t> w = ld4<uint32_t>(hd.line_mask, L);
         const uint32_t m[3] = {w.x, w.y, w.z};
-        int cnt = (int)w.w;
+        int cnt = (int)w.w + (upper ? mask_count_low<N0>(m) : 0);
 #pragma unroll
-        for (int i0 = 0; i0 < N0; ++i0) {
-          if ((m[i0 >> 5] >> (i0 & 31)) & 1u) {
-            pts[cnt] = x[i0];
+        for (int k = 0; k < H; ++k) {
+          if (mask_bit<N0>(m, k, upper)) {
+            pts[cnt] = v[k];
             ++cnt;
           }
         }
       }
-      for (int i = lane; i < nsl * kMaxNlm; i += 64) acc[i] = C{0, 0};
+      for (int i = ptid; i < 2 * kMaxNlm * hd.maxslot; i += 128) acc0[i] = C{0, 0};
     }
     __syncthreads();
     if (active) {
-      const int chunk = (nent + 63) >> 6;
-      int e = ent_base + lane * chunk;
+      const int chunk = (nent + 127) >> 7;
+      int e = ent_base + ptid * chunk;
       const int eend = min(e + chunk, ent_base + nent);
       int cur = -1;
       C a0{0, 0}, a1{0, 0}, a2{0, 0}, a3{0, 0};
@@ -378,9 +474,10 @@ __global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blo
     }
     __syncthreads();
     if (active) {
-      GAS C *part = gw(B.partial);
-      for (int i = lane; i < nsl * kMaxNlm; i += 64)
-        part[((size_t)(slot_base + i / kMaxNlm) * kMaxNlm + (i % kMaxNlm)) * B.nst + s] = acc[i];
+      GAS C *pt = gw(B.partial);
+      const C *acc1 = acc0 + kMaxNlm * hd.maxslot;
+      for (int i = ptid; i < nsl * kMaxNlm; i += 128)
+        pt[((size_t)(slot_base + i / kMaxNlm) * kMaxNlm + (i % kMaxNlm)) * B.nst + s] = cadd(acc0[i], acc1[i]);
     }
     __syncthreads();
   }
@@ -430,25 +527,32 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
     }
   }
 }
-
-// Inverse FFT along i0, V(r)/N, KB term exp(-ik.pos) sum_l beta_l (D P)_l
-// added at the tile's sphere points through LDS, forward FFT along i0.
+// Inverse FFT along i0, times V/N, plus the KB term exp(-ik.r) beta D P at
+// the sphere points, forward FFT along i0, stored back into the sphere's
+// plane slots; optionally the dipole partial sums of the line.  A wave pair
+// per (tile, state), a lane pair per line.
 template <typename T, int N0>
 __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                     int total, HamDev hd, int lds_per_wave) {
+                                                     int total, HamDev hd, int lds_per_state) {
   using C = cplx<T>;
+  constexpr int H = N0 / 2;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  C *pts = reinterpret_cast<C *>(smem_raw) + wave * lds_per_wave;
-  C *dpl = pts + hd.maxpts;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ns = blockDim.x >> 7, sw = wave >> 1, part = wave & 1, ptid = threadIdx.x & 127;
+  const bool upper = lane >= 32;
+  // one point accumulator per wave of the pair (added in a fixed order)
+  C *pts0 = reinterpret_cast<C *>(smem_raw) + sw * lds_per_state;
+  C *pts1 = pts0 + hd.maxpts;
+  C *dpl = pts1 + hd.maxpts;
+  __shared__ double dipw[2][2][4];  // [state slot][wave of the pair][4]
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
     const int local = wg - gp(prefix)[b];
-    const int ngrp = (B.nst + nwave - 1) / nwave;
+    const int ngrp = (B.nst + ns - 1) / ns;
     const int tile = local / ngrp;
-    const int s = (local - tile * ngrp) * nwave + wave;
-    const int L = tile * kTile + lane;
+    const int s = (local - tile * ngrp) * ns + sw;
+    const int L = tile * kTile + part * 32 + (lane & 31);
     const bool valid = L < hd.n12;
     const int Lc = valid ? L : hd.n12 - 1;
     int npts = 0, nsl = 0, slot_base = 0, ent_base = 0, nent = 0;
@@ -462,24 +566,25 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
       nent = tj.x;
     }
     const bool act = s < B.nst;
-    C x[N0];
+    C v[H];
     if (act) {
       if (npts > 0) {
         const GAS C *dp = gp(B.dp);
-        for (int i = lane; i < nsl * kMaxNlm; i += 64)
+        for (int i = ptid; i < nsl * kMaxNlm; i += 128)
           dpl[i] = dp[((size_t)gp(hd.slot_atom)[slot_base + i / kMaxNlm] * kMaxNlm + (i % kMaxNlm)) * B.nst + s];
-        for (int i = lane; i < npts; i += 64) pts[i] = C{0, 0};
+        for (int i = ptid; i < 2 * hd.maxpts; i += 128) pts0[i] = C{0, 0};
       }
-      load_line_inverse<T, N0>(B, hd.n12, s, Lc, x);
+      load_line_inverse_split<T, N0>(B, hd.n12, s, Lc, upper, v);
       if (B.dip) {
-        // sum |u|^2 and sum f0 |u|^2 along the line, f0 = sym(i0)/n0
+        // sum |u|^2 and sum f0 |u|^2 along the lane's half line, f0 = sym(i0)/n0
         double s0 = 0.0, s1 = 0.0;
         if (valid) {
 #pragma unroll
-          for (int i0 = 0; i0 < N0; ++i0) {
-            const double w = (double)x[i0].x * (double)x[i0].x + (double)x[i0].y * (double)x[i0].y;
+          for (int k = 0; k < H; ++k) {
+            const double w = (double)v[k].x * (double)v[k].x + (double)v[k].y * (double)v[k].y;
+            const double f0 = upper ? (double)sym_of<N0>(k + H) / (double)N0 : (double)sym_of<N0>(k) / (double)N0;
             s0 += w;
-            s1 += w * ((double)sym_of<N0>(i0) / (double)N0);
+            s1 += w * f0;
           }
         }
         const int i1 = Lc / hd.n2, i2 = Lc - i1 * hd.n2;
@@ -493,31 +598,38 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
         for (int c = 0; c < 4; ++c)
 #pragma unroll
           for (int off = 32; off > 0; off >>= 1) d[c] += __shfl_xor(d[c], off, 64);
-        if (lane == 0) {
-          GAS double *dp = gw(B.dip) + ((size_t)tile * B.nst + s) * 4;
+        if (lane == 0)
 #pragma unroll
-          for (int c = 0; c < 4; ++c) dp[c] = d[c];
-        }
+          for (int c = 0; c < 4; ++c) dipw[sw][part][c] = d[c];
       }
-      const GAS T *vl = gp(B.vloc) + Lc;
+      // V/N at i0 = k + half N0/2
+      const unsigned ps = (unsigned)(hd.n12 * sizeof(T));
+      const auto rv = make_rsrc(B.vloc, (unsigned)N0 * ps);
+      const unsigned vo = (unsigned)(Lc * sizeof(T)) + (upper ? (unsigned)H * ps : 0u);
 #pragma unroll
-      for (int i0 = 0; i0 < N0; ++i0) {
-        const T v = vl[(size_t)i0 * hd.n12];
-        x[i0].x *= v;
-        x[i0].y *= v;
+      for (int k = 0; k < H; ++k) {
+        T vv;
+        bload(rv, vo, (unsigned)k * ps, vv);
+        v[k].x *= vv;
+        v[k].y *= vv;
       }
     }
     __syncthreads();
+    if (act && B.dip && part == 0 && lane == 0) {
+      GAS double *dq = gw(B.dip) + ((size_t)tile * B.nst + s) * 4;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) dq[c] = dipw[sw][0][c] + dipw[sw][1][c];
+    }
     if (act && npts > 0) {
-      for (int e = ent_base + lane; e < ent_base + nent; e += 64) {
+      for (int e = ent_base + ptid; e < ent_base + nent; e += 128) {
         const V2<int> cs = ld2<int>(hd.ent_cs, e);
         const V4<T> bt = ld4<T>(hd.ent_b, e);
-        const C *d = dpl + cs.y * kMaxNlm;
+        const C *dd = dpl + cs.y * kMaxNlm;
         C t;
-        t.x = bt.x * d[0].x + bt.y * d[1].x + bt.z * d[2].x + bt.w * d[3].x;
-        t.y = bt.x * d[0].y + bt.y * d[1].y + bt.z * d[2].y + bt.w * d[3].y;
+        t.x = bt.x * dd[0].x + bt.y * dd[1].x + bt.z * dd[2].x + bt.w * dd[3].x;
+        t.y = bt.x * dd[0].y + bt.y * dd[1].y + bt.z * dd[2].y + bt.w * dd[3].y;
         if (!B.gamma) t = cmul(t, kb_phase<T>(B.kcov, ld4<float>(hd.ent_pos, e), (T)-1));
-        T *q = reinterpret_cast<T *>(pts + cs.x);
+        T *q = reinterpret_cast<T *>((part ? pts1 : pts0) + cs.x);
         lds_add(q + 0, t.x);
         lds_add(q + 1, t.y);
       }
@@ -527,25 +639,17 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
       if (npts > 0 && valid) {
         const V4<uint32_t> w = ld4<uint32_t>(hd.line_mask, L);
         const uint32_t m[3] = {w.x, w.y, w.z};
-        int cnt = (int)w.w;
+        int cnt = (int)w.w + (upper ? mask_count_low<N0>(m) : 0);
 #pragma unroll
-        for (int i0 = 0; i0 < N0; ++i0) {
-          if ((m[i0 >> 5] >> (i0 & 31)) & 1u) {
-            x[i0] = cadd(x[i0], pts[cnt]);
+        for (int k = 0; k < H; ++k) {
+          if (mask_bit<N0>(m, k, upper)) {
+            v[k] = cadd(cadd(v[k], pts0[cnt]), pts1[cnt]);
             ++cnt;
           }
         }
       }
-      fft_line<N0, -1, C>::run(x);
-      if (valid) {
-        GAS C *base = gw(B.t1) + (size_t)s * hd.n12 + L;
-        const size_t sstride = (size_t)B.nst * hd.n12;
-#pragma unroll
-        for (int i0 = 0; i0 < N0; ++i0) {
-          const int j = sym_of<N0>(i0) - B.m_lo;
-          if ((unsigned)j < (unsigned)B.nslot) base[(size_t)j * sstride] = x[i0];
-        }
-      }
+      fft_split<N0, -1, C>::dif(v, upper);
+      if (valid) store_line_slots_split<T, N0>(B, hd.n12, s, L, upper, v);
     }
     __syncthreads();
   }
@@ -878,12 +982,15 @@ void invalidate_launch_cache(inqx_ham_s *h) {
 
 // inverse FFT along i0 of the plane slots written by k_plane_inverse, times
 // 1/N, into the full real-space grid (operations::transform::to_real,
-// src/operations/transform.hpp:113-182)
+// src/operations/transform.hpp:113-182); 32 lines (lane pairs) per wave,
+// one state per wave
 template <typename T, int N0>
 __global__ void __launch_bounds__(256) k_line_to_real(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                       int total, int n12, int ntile, T scale) {
   using C = cplx<T>;
+  constexpr int H = N0 / 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool upper = lane >= 32;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
@@ -892,21 +999,21 @@ __global__ void __launch_bounds__(256) k_line_to_real(const SphereBlock<T> *bloc
     const int tile = local / ngrp;
     const int s = (local - tile * ngrp) * 4 + wave;
     if (s >= B.nst) continue;  // wave-uniform
-    const int L = tile * kTile + lane;
+    const int L = tile * 32 + (lane & 31);
     const int Lc = L < n12 ? L : n12 - 1;
-    C x[N0];
-    load_line_inverse<T, N0>(B, n12, s, Lc, x);
+    C v[H];
+    load_line_inverse_split<T, N0>(B, n12, s, Lc, upper, v);
     if (L < n12) {
-      GAS C *out = gw(B.rs);
       if (B.rs_layout == 0) {
-        out += (size_t)s * N0 * n12 + L;
+        const unsigned pl = (unsigned)(n12 * sizeof(C));
+        const auto rs = make_rsrc(B.rs + (size_t)s * N0 * n12, (unsigned)N0 * pl);
+        const unsigned vo = (unsigned)(L * sizeof(C)) + (upper ? (unsigned)H * pl : 0u);
 #pragma unroll
-        for (int i0 = 0; i0 < N0; ++i0) out[(size_t)i0 * n12] = C{x[i0].x * scale, x[i0].y * scale};
+        for (int k = 0; k < H; ++k) bstore(rs, vo, (unsigned)k * pl, C{v[k].x * scale, v[k].y * scale});
       } else {
-        out += (size_t)L * B.nst + s;
+        GAS C *out = gw(B.rs) + ((size_t)L + (upper ? (size_t)H * n12 : 0)) * B.nst + s;
 #pragma unroll
-        for (int i0 = 0; i0 < N0; ++i0)
-          out[(size_t)i0 * n12 * B.nst] = C{x[i0].x * scale, x[i0].y * scale};
+        for (int k = 0; k < H; ++k) out[(size_t)k * n12 * B.nst] = C{v[k].x * scale, v[k].y * scale};
       }
     }
   }
@@ -919,7 +1026,9 @@ template <typename T, int N0>
 __global__ void __launch_bounds__(256) k_line_from_real(const SphereBlock<T> *blocks, const int *prefix,
                                                         int nblocks, int total, int n12, int ntile) {
   using C = cplx<T>;
+  constexpr int H = N0 / 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool upper = lane >= 32;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
@@ -928,29 +1037,22 @@ __global__ void __launch_bounds__(256) k_line_from_real(const SphereBlock<T> *bl
     const int tile = local / ngrp;
     const int s = (local - tile * ngrp) * 4 + wave;
     if (s >= B.nst) continue;
-    const int L = tile * kTile + lane;
+    const int L = tile * 32 + (lane & 31);
     const int Lc = L < n12 ? L : n12 - 1;
-    C x[N0];
-    const GAS C *in = gp(B.rs);
+    C v[H];
     if (B.rs_layout == 0) {
-      in += (size_t)s * N0 * n12 + Lc;
+      const unsigned pl = (unsigned)(n12 * sizeof(C));
+      const auto rs = make_rsrc(B.rs + (size_t)s * N0 * n12, (unsigned)N0 * pl);
+      const unsigned vo = (unsigned)(Lc * sizeof(C)) + (upper ? (unsigned)H * pl : 0u);
 #pragma unroll
-      for (int i0 = 0; i0 < N0; ++i0) x[i0] = in[(size_t)i0 * n12];
+      for (int k = 0; k < H; ++k) bload(rs, vo, (unsigned)k * pl, v[k]);
     } else {
-      in += (size_t)Lc * B.nst + s;
+      const GAS C *in = gp(B.rs) + ((size_t)Lc + (upper ? (size_t)H * n12 : 0)) * B.nst + s;
 #pragma unroll
-      for (int i0 = 0; i0 < N0; ++i0) x[i0] = in[(size_t)i0 * n12 * B.nst];
+      for (int k = 0; k < H; ++k) v[k] = in[(size_t)k * n12 * B.nst];
     }
-    fft_line<N0, -1, C>::run(x);
-    if (L < n12) {
-      GAS C *t1 = gw(B.t1) + (size_t)s * n12 + L;
-      const size_t sstride = (size_t)B.nst * n12;
-#pragma unroll
-      for (int i0 = 0; i0 < N0; ++i0) {
-        const int j = sym_of<N0>(i0) - B.m_lo;
-        if ((unsigned)j < (unsigned)B.nslot) t1[(size_t)j * sstride] = x[i0];
-      }
-    }
+    fft_split<N0, -1, C>::dif(v, upper);
+    if (L < n12) store_line_slots_split<T, N0>(B, n12, s, L, upper, v);
   }
 }
 
@@ -1018,7 +1120,7 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
     INQX_HIP(hipGetLastError());
     return;
   }
-  const int xthreads = 64 * ld.nwave;
+  const int xthreads = 128 * ld.nwave;  // a wave pair per state
   const int xlds = ld.nwave * ld.lds_per_wave * (int)sizeof(cplx<T>);
   set_x_attr<T, N0>(xlds);
   const int grid_x = std::min(ld.total_x, cu * 16);
@@ -1129,11 +1231,12 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     }
     // waves (states) per x-pass workgroup within the LDS budget
     const int cs = (int)sizeof(cplx<T>);
-    int nw = std::max(1, std::min(4, h->xwaves[prec]));
+    int nw = std::max(1, std::min(2, h->xwaves[prec]));  // states per x-pass work-group
     ld->mode = mode;
-    ld->ntile_line = (g.n[1] * g.n[2] + kTile - 1) / kTile;
+    ld->ntile_line = (g.n[1] * g.n[2] + 31) / 32;
     if (mode == 0) {
-      ld->lds_per_wave = h->kb.maxpts + kMaxNlm * h->kb.maxslot;
+      // per state: two point sets + D P (apply), or points + two accumulator sets (project)
+      ld->lds_per_wave = 2 * h->kb.maxpts + 2 * kMaxNlm * h->kb.maxslot;
       while (nw > 1 && nw * ld->lds_per_wave * cs > 160 * 1024) --nw;
       INQX_CHECK(ld->lds_per_wave * cs <= 160 * 1024, INQX_ERR_UNSUPPORTED, "KB tile too dense for LDS");
     }
@@ -1206,6 +1309,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   hd.has_kb = has_kb ? 1 : 0;
   hd.ntile = h->kb.ntile;
   hd.maxpts = h->kb.maxpts;
+  hd.maxslot = h->kb.maxslot;
   hd.line_mask = h->kb.line_mask.as<uint32_t>();
   hd.tile_info = h->kb.tile_info.as<int>();
   hd.slot_atom = h->kb.slot_atom.as<int>();

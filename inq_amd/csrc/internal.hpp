@@ -244,8 +244,9 @@ struct inqx_ham_s {
   inqx::KBTables kb;
   std::vector<inqx_basis> bases;
   int pinned = 0;  // > 0 while a captured graph uses the projector tables
-  // launch shape of the x-pass kernels per precision: states (waves) per work-group
-  int xwaves[2] = {4, 4};
+  // launch shape of the x-pass kernels per precision: states per work-group
+  // (a wave pair each)
+  int xwaves[2] = {2, 2};
   // exact-exchange workspaces: real-space states, hf states, accumulator, pair planes
   inqx::DevMem exx_phi, exx_hf, exx_w, exx_t, exx_jlist, exx_coef;
   // real-space apply and density workspaces (sphere arrays of a full-cube basis, occupations)
