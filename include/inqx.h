@@ -130,12 +130,12 @@ int inqx_basis_kinetic(inqx_basis basis, double *kin);
 int inqx_ham_prepare(inqx_ham ham, inqx_basis basis, int nst, int precision);
 
 /* launch-shape autotuner: times the H application of nst states on this basis
- * for each candidate x-pass shape (1 or 2 states per work-group) and keeps
+ * for each candidate x-pass shape (1, 2 or 4 states per work-group) and keeps
  * the fastest for this Hamiltonian and precision.  Allocates scratch states for
  * the timing; do not call during graph capture.  best_xwaves / best_ms may be
  * NULL. */
 int inqx_ham_autotune(inqx_ham ham, inqx_basis basis, int nst, int precision, int *best_xwaves, double *best_ms);
-/* sets the x-pass shape directly (1 or 2 states per work-group) */
+/* sets the x-pass shape directly (1, 2 or 4 states per work-group) */
 int inqx_ham_set_launch_shape(inqx_ham ham, int precision, int xwaves);
 
 /* ------------------------------------------------------------ H |psi> */

@@ -635,7 +635,7 @@ int inqx_ham_set_launch_shape(inqx_ham h, int precision, int xwaves) {
   API_BEGIN
   INQX_CHECK(h, INQX_ERR_ARG, "null Hamiltonian");
   INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
-  INQX_CHECK(xwaves == 1 || xwaves == 2, INQX_ERR_ARG, "x-pass shape must be 1 or 2 states per work-group");
+  INQX_CHECK(xwaves == 1 || xwaves == 2 || xwaves == 4, INQX_ERR_ARG, "x-pass shape must be 1, 2 or 4");
   INQX_CHECK(h->pinned == 0, INQX_ERR_STATE, "Hamiltonian pinned by a captured propagation graph");
   if (h->xwaves[precision] != xwaves) {
     h->xwaves[precision] = xwaves;
@@ -669,7 +669,7 @@ int inqx_ham_autotune(inqx_ham h, inqx_basis bs, int nst, int precision, int *be
   int best = h->xwaves[precision];
   double best_t = 1e300;
   try {
-    for (int w : {2, 1}) {
+    for (int w : {4, 2, 1}) {
       if (h->xwaves[precision] != w) {
         h->xwaves[precision] = w;
         invalidate_launch_cache(h);

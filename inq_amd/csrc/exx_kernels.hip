@@ -43,8 +43,14 @@ template <typename T> __device__ __forceinline__ GAS T *xw(T *p) { return (GAS T
 // raw buffer access to one complex value: per-lane byte offset in a VGPR and a
 // wave-uniform byte offset (a constant after unrolling) in an SGPR, so that a
 // line of loads needs a single address register
+// the base and size of a buffer resource are wave-uniform by construction;
+// readfirstlane makes that explicit so the resource stays in SGPRs
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, unsigned bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+  const unsigned long long a = (unsigned long long)base;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  void *p = (void *)(((unsigned long long)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)__builtin_amdgcn_readfirstlane(bytes), 0x00020000);
 }
 __device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<float> &v) {
   const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
@@ -160,7 +166,7 @@ __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
   constexpr int NTILE = (NP + 31) / 32;
   constexpr int NGRP = (NTILE + 3) / 4;
   constexpr long N = (long)N0 * NP;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
   const bool upper = lane >= 32;
   const int half = upper ? 1 : 0;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {

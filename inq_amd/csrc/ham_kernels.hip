@@ -136,11 +136,23 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
     __syncthreads();
     const GAS C *psi = gp(B.psi);
     const GAS int *pwyz = gp(B.pw_yz);
-    for (int i = tid; i < np * nseg; i += nth) {
-      const int p = i / nseg, j = i - p * nseg;
-      const int yz = pwyz[seg0 + j];
-      const int y = yz / N2, z = yz - y * N2;
-      lds[p * Cfg::PLANE + y * (N2 + 1) + z] = psi[(size_t)(s0 + p) * B.npw + seg0 + j];
+    // four sphere points per thread and step: their loads are in flight together
+    const int tot = np * nseg;
+    for (int i0 = tid; i0 < tot; i0 += 4 * nth) {
+      int ldx[4];
+      C val[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = min(i0 + u * nth, tot - 1);
+        const int p = i / nseg, j = i - p * nseg;
+        const int yz = pwyz[seg0 + j];
+        const int y = yz / N2, z = yz - y * N2;
+        ldx[u] = p * Cfg::PLANE + y * (N2 + 1) + z;
+        val[u] = psi[(size_t)(s0 + p) * B.npw + seg0 + j];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (i0 + u * nth < tot) lds[ldx[u]] = val[u];
     }
     __syncthreads();
     const GAS int *rows = gp(B.rows);
@@ -222,15 +234,33 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
     GAS C *hpsi = gw(B.hpsi);
     const GAS int *pwyz = gp(B.pw_yz);
     const GAS T *kin = gp(B.kin);
-    for (int i = tid; i < np * nseg; i += nth) {
-      const int p = i / nseg, j = i - p * nseg;
-      const int yz = pwyz[seg0 + j];
-      const int y = yz / N2, z = yz - y * N2;
-      const size_t o = (size_t)(s0 + p) * B.npw + seg0 + j;
-      C out = lds[p * Cfg::PLANE + y * (N2 + 1) + z];
+    const int tot = np * nseg;
+    for (int i0 = tid; i0 < tot; i0 += 4 * nth) {
+      // four sphere points per thread and step: loads first, then the updates
+      size_t oo[4];
+      C outv[4], cv[4];
+      T kv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = min(i0 + u * nth, tot - 1);
+        const int p = i / nseg, j = i - p * nseg;
+        const int yz = pwyz[seg0 + j];
+        const int y = yz / N2, z = yz - y * N2;
+        oo[u] = (size_t)(s0 + p) * B.npw + seg0 + j;
+        outv[u] = lds[p * Cfg::PLANE + y * (N2 + 1) + z];
+        if (B.kin) {
+          cv[u] = psi[oo[u]];
+          kv[u] = kin[seg0 + j];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+      if (i0 + u * nth >= tot) continue;
+      const size_t o = oo[u];
+      C out = outv[u];
       if (B.kin) {  // H application: + 1/2|G+k|^2 psi; without it a plain masked forward FFT
-        const C c = psi[o];
-        const T kf = kin[seg0 + j];
+        const C c = cv[u];
+        const T kf = kv[u];
         out.x += kf * c.x;
         out.y += kf * c.y;
       }
@@ -258,6 +288,7 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
         q.y += c2r * out.y + c2i * out.x;
         *a = q;
       }
+      }
     }
     __syncthreads();
   }
@@ -271,8 +302,14 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
 template <int N> constexpr int sym_of(int i) { return i >= (N + 1) / 2 ? i - N : i; }
 
 // ---- raw buffer access: per-lane byte offset in a VGPR, wave-uniform part in an SGPR
+// the base and size of a buffer resource are wave-uniform by construction;
+// readfirstlane makes that explicit so the resource stays in SGPRs
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, unsigned bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+  const unsigned long long a = (unsigned long long)base;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  void *p = (void *)(((unsigned long long)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)__builtin_amdgcn_readfirstlane(bytes), 0x00020000);
 }
 __device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<float> &v) {
   const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
@@ -303,6 +340,9 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, 
   __builtin_amdgcn_raw_buffer_store_b128(q, r, voff, soff, 0);
 }
 
+// a per-lane buffer offset beyond every record: loads return 0, stores are dropped
+constexpr unsigned kOOB = 0x80000000u;
+
 // A line (i1,i2) along i0 is shared by a lane pair (lane, lane + 32): the
 // lower lane holds i0 in [0, N0/2), the upper lane [N0/2, N0) after an
 // inverse transform, and the even / odd i0 before it.  t1 is state-major
@@ -311,8 +351,8 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, 
 // loads the plane slots i0 = 2n + half of the pair's line (zero outside the
 // slot range) and transforms along i0: afterwards v[k] = x[k + half N0/2]
 template <typename T, int N0>
-__device__ __forceinline__ void load_line_inverse_split(const SphereBlock<T> &B, int n12, int s, int Lc, bool upper,
-                                                        cplx<T> (&v)[N0 / 2]) {
+__device__ __forceinline__ void load_line_slots_split(const SphereBlock<T> &B, int n12, int s, int Lc, bool upper,
+                                                      cplx<T> (&v)[N0 / 2]) {
   using C = cplx<T>;
   constexpr int H = N0 / 2;
   const unsigned pl = (unsigned)(n12 * sizeof(C));
@@ -325,25 +365,30 @@ __device__ __forceinline__ void load_line_inverse_split(const SphereBlock<T> &B,
     C x{0, 0};
     if (vlo && vhi && jh == jl + 1) {
       bload(rs, vl + (upper ? pl : 0u), (unsigned)jl * pl, x);
-    } else if (vlo || vhi) {
+    } else if (vlo || vhi) {  // no divergent branch: the invalid half reads out of range (zero)
       const int j = upper ? jh : jl;
-      if ((unsigned)j < (unsigned)B.nslot) bload(rs, vl + (unsigned)j * pl, 0u, x);
+      bload(rs, (unsigned)j < (unsigned)B.nslot ? vl + (unsigned)j * pl : kOOB, 0u, x);
     }
     v[n] = x;
   }
-  fft_split<N0, +1, C>::dit(v, upper);
+}
+template <typename T, int N0>
+__device__ __forceinline__ void load_line_inverse_split(const SphereBlock<T> &B, int n12, int s, int Lc, bool upper,
+                                                        cplx<T> (&v)[N0 / 2]) {
+  load_line_slots_split<T, N0>(B, n12, s, Lc, upper, v);
+  fft_split<N0, +1, cplx<T>>::dit(v, upper);
 }
 
 // stores v[k] = X[2k + half] (after the forward split transform) into the
-// plane slots of the line
+// plane slots of the line; lanes with ok == false store nothing
 template <typename T, int N0>
 __device__ __forceinline__ void store_line_slots_split(const SphereBlock<T> &B, int n12, int s, int L, bool upper,
-                                                       const cplx<T> (&v)[N0 / 2]) {
+                                                       bool ok, const cplx<T> (&v)[N0 / 2]) {
   using C = cplx<T>;
   constexpr int H = N0 / 2;
   const unsigned pl = (unsigned)(n12 * sizeof(C));
   const auto rs = make_rsrc(B.t1 + (size_t)s * B.nslot * n12, (unsigned)B.nslot * pl);
-  const unsigned vl = (unsigned)(L * sizeof(C));
+  const unsigned vl = ok ? (unsigned)(L * sizeof(C)) : kOOB;
 #pragma unroll
   for (int k = 0; k < H; ++k) {
     const int jl = sym_of<N0>(2 * k) - B.m_lo, jh = sym_of<N0>(2 * k + 1) - B.m_lo;
@@ -352,7 +397,7 @@ __device__ __forceinline__ void store_line_slots_split(const SphereBlock<T> &B, 
       bstore(rs, vl + (upper ? pl : 0u), (unsigned)jl * pl, v[k]);
     } else if (vlo || vhi) {
       const int j = upper ? jh : jl;
-      if ((unsigned)j < (unsigned)B.nslot) bstore(rs, vl + (unsigned)j * pl, 0u, v[k]);
+      bstore(rs, (unsigned)j < (unsigned)B.nslot ? vl + (unsigned)j * pl : kOOB, 0u, v[k]);
     }
   }
 }
@@ -391,33 +436,31 @@ __device__ __forceinline__ cplx<T> kb_phase(const double *kcov, V4<float> pos, T
   return cplx<T>{cs, sn};
 }
 
-// Inverse FFT along i0 of the 64 lines of a tile (a wave pair per state, a
-// lane pair per line), then the KB projections P[slot][l] = sum_entries
-// beta_l exp(+ik.pos) psi.  Each of the pair's 128 lanes walks a contiguous
-// chunk of the tile's entries (sorted by slot), accumulates in registers and
-// flushes to the LDS slot accumulators only at slot changes.
+// Inverse FFT along i0 of the 32 lines of a tile (one wave per (tile,
+// state), a lane pair per line), then the KB projections P[slot][l] =
+// sum_entries beta_l exp(+ik.pos) psi.  Each lane walks a contiguous chunk of
+// the tile's entries (sorted by slot), accumulates in registers and flushes to
+// the wave's LDS slot accumulators only at slot changes.  Every LDS buffer is
+// private to one wave.
 template <typename T, int N0>
 __global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                       int total, HamDev hd, int lds_per_state) {
+                                                       int total, HamDev hd, int lds_per_wave) {
   using C = cplx<T>;
   constexpr int H = N0 / 2;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ns = blockDim.x >> 7, sw = wave >> 1, part = wave & 1, ptid = threadIdx.x & 127;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
+  const int ns = blockDim.x >> 6, ptid = lane;
   const bool upper = lane >= 32;
-  C *pts = reinterpret_cast<C *>(smem_raw) + sw * lds_per_state;
-  // one accumulator set per wave of the pair, summed in a fixed order: the
-  // projections do not depend on the timing of the two waves
-  C *acc0 = pts + hd.maxpts;
-  C *acc = acc0 + part * kMaxNlm * hd.maxslot;
+  C *pts = reinterpret_cast<C *>(smem_raw) + wave * lds_per_wave;
+  C *acc = pts + hd.maxpts;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
     const int local = wg - gp(prefix)[b];
     const int ngrp = (B.nst + ns - 1) / ns;
     const int tile = local / ngrp;
-    const int s = (local - tile * ngrp) * ns + sw;
-    const int L = tile * kTile + part * 32 + (lane & 31);
+    const int s = (local - tile * ngrp) * ns + wave;
+    const int L = tile * kTile + (lane & 31);
     const V4<int> ti = ld4<int>(hd.tile_info, 2 * tile);
     const V4<int> tj = ld4<int>(hd.tile_info, 2 * tile + 1);
     const int npts = ti.x, nsl = ti.y, slot_base = ti.z, ent_base = ti.w, nent = tj.x;
@@ -438,33 +481,47 @@ __global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blo
           }
         }
       }
-      for (int i = ptid; i < 2 * kMaxNlm * hd.maxslot; i += 128) acc0[i] = C{0, 0};
+      for (int i = ptid; i < nsl * kMaxNlm; i += 64) acc[i] = C{0, 0};
     }
     __syncthreads();
     if (active) {
-      const int chunk = (nent + 127) >> 7;
+      const int chunk = (nent + 63) >> 6;
       int e = ent_base + ptid * chunk;
       const int eend = min(e + chunk, ent_base + nent);
       int cur = -1;
       C a0{0, 0}, a1{0, 0}, a2{0, 0}, a3{0, 0};
-      for (; e < eend; ++e) {
-        const V2<int> cs = ld2<int>(hd.ent_cs, e);
-        const V4<T> bt = ld4<T>(hd.ent_b, e);
-        if (cs.y != cur) {
-          if (cur >= 0) {
-            T *q = reinterpret_cast<T *>(acc + cur * kMaxNlm);
-            lds_add(q + 0, a0.x); lds_add(q + 1, a0.y); lds_add(q + 2, a1.x); lds_add(q + 3, a1.y);
-            lds_add(q + 4, a2.x); lds_add(q + 5, a2.y); lds_add(q + 6, a3.x); lds_add(q + 7, a3.y);
-          }
-          cur = cs.y;
-          a0 = a1 = a2 = a3 = C{0, 0};
+      for (; e < eend; e += 4) {
+        // four consecutive entries: loads first, then the in-order accumulation
+        V2<int> cs[4];
+        V4<T> bt[4];
+        V4<float> ps4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int eu = min(e + u, eend - 1);
+          cs[u] = ld2<int>(hd.ent_cs, eu);
+          bt[u] = ld4<T>(hd.ent_b, eu);
+          if (!B.gamma) ps4[u] = ld4<float>(hd.ent_pos, eu);
         }
-        C v = pts[cs.x];
-        if (!B.gamma) v = cmul(v, kb_phase<T>(B.kcov, ld4<float>(hd.ent_pos, e), (T)1));
-        a0.x += bt.x * v.x; a0.y += bt.x * v.y;
-        a1.x += bt.y * v.x; a1.y += bt.y * v.y;
-        a2.x += bt.z * v.x; a2.y += bt.z * v.y;
-        a3.x += bt.w * v.x; a3.y += bt.w * v.y;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (e + u < eend) {
+            if (cs[u].y != cur) {
+              if (cur >= 0) {
+                T *q = reinterpret_cast<T *>(acc + cur * kMaxNlm);
+                lds_add(q + 0, a0.x); lds_add(q + 1, a0.y); lds_add(q + 2, a1.x); lds_add(q + 3, a1.y);
+                lds_add(q + 4, a2.x); lds_add(q + 5, a2.y); lds_add(q + 6, a3.x); lds_add(q + 7, a3.y);
+              }
+              cur = cs[u].y;
+              a0 = a1 = a2 = a3 = C{0, 0};
+            }
+            C v = pts[cs[u].x];
+            if (!B.gamma) v = cmul(v, kb_phase<T>(B.kcov, ps4[u], (T)1));
+            a0.x += bt[u].x * v.x; a0.y += bt[u].x * v.y;
+            a1.x += bt[u].y * v.x; a1.y += bt[u].y * v.y;
+            a2.x += bt[u].z * v.x; a2.y += bt[u].z * v.y;
+            a3.x += bt[u].w * v.x; a3.y += bt[u].w * v.y;
+          }
+        }
       }
       if (cur >= 0) {
         T *q = reinterpret_cast<T *>(acc + cur * kMaxNlm);
@@ -475,9 +532,8 @@ __global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blo
     __syncthreads();
     if (active) {
       GAS C *pt = gw(B.partial);
-      const C *acc1 = acc0 + kMaxNlm * hd.maxslot;
-      for (int i = ptid; i < nsl * kMaxNlm; i += 128)
-        pt[((size_t)(slot_base + i / kMaxNlm) * kMaxNlm + (i % kMaxNlm)) * B.nst + s] = cadd(acc0[i], acc1[i]);
+      for (int i = ptid; i < nsl * kMaxNlm; i += 64)
+        pt[((size_t)(slot_base + i / kMaxNlm) * kMaxNlm + (i % kMaxNlm)) * B.nst + s] = acc[i];
     }
     __syncthreads();
   }
@@ -529,30 +585,27 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
 }
 // Inverse FFT along i0, times V/N, plus the KB term exp(-ik.r) beta D P at
 // the sphere points, forward FFT along i0, stored back into the sphere's
-// plane slots; optionally the dipole partial sums of the line.  A wave pair
-// per (tile, state), a lane pair per line.
+// plane slots; optionally the dipole partial sums of the tile.  One wave per
+// (tile, state), a lane pair per line; LDS buffers private to the wave.
 template <typename T, int N0>
 __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                     int total, HamDev hd, int lds_per_state) {
+                                                     int total, HamDev hd, int lds_per_wave) {
   using C = cplx<T>;
   constexpr int H = N0 / 2;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ns = blockDim.x >> 7, sw = wave >> 1, part = wave & 1, ptid = threadIdx.x & 127;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
+  const int ns = blockDim.x >> 6, ptid = lane;
   const bool upper = lane >= 32;
-  // one point accumulator per wave of the pair (added in a fixed order)
-  C *pts0 = reinterpret_cast<C *>(smem_raw) + sw * lds_per_state;
-  C *pts1 = pts0 + hd.maxpts;
-  C *dpl = pts1 + hd.maxpts;
-  __shared__ double dipw[2][2][4];  // [state slot][wave of the pair][4]
+  C *pts = reinterpret_cast<C *>(smem_raw) + wave * lds_per_wave;
+  C *dpl = pts + hd.maxpts;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
     const int local = wg - gp(prefix)[b];
     const int ngrp = (B.nst + ns - 1) / ns;
     const int tile = local / ngrp;
-    const int s = (local - tile * ngrp) * ns + sw;
-    const int L = tile * kTile + part * 32 + (lane & 31);
+    const int s = (local - tile * ngrp) * ns + wave;
+    const int L = tile * kTile + (lane & 31);
     const bool valid = L < hd.n12;
     const int Lc = valid ? L : hd.n12 - 1;
     int npts = 0, nsl = 0, slot_base = 0, ent_base = 0, nent = 0;
@@ -570,9 +623,9 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
     if (act) {
       if (npts > 0) {
         const GAS C *dp = gp(B.dp);
-        for (int i = ptid; i < nsl * kMaxNlm; i += 128)
+        for (int i = ptid; i < nsl * kMaxNlm; i += 64)
           dpl[i] = dp[((size_t)gp(hd.slot_atom)[slot_base + i / kMaxNlm] * kMaxNlm + (i % kMaxNlm)) * B.nst + s];
-        for (int i = ptid; i < 2 * hd.maxpts; i += 128) pts0[i] = C{0, 0};
+        for (int i = ptid; i < npts; i += 64) pts[i] = C{0, 0};
       }
       load_line_inverse_split<T, N0>(B, hd.n12, s, Lc, upper, v);
       if (B.dip) {
@@ -598,9 +651,11 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
         for (int c = 0; c < 4; ++c)
 #pragma unroll
           for (int off = 32; off > 0; off >>= 1) d[c] += __shfl_xor(d[c], off, 64);
-        if (lane == 0)
+        if (lane == 0) {
+          GAS double *dq = gw(B.dip) + ((size_t)tile * B.nst + s) * 4;
 #pragma unroll
-          for (int c = 0; c < 4; ++c) dipw[sw][part][c] = d[c];
+          for (int c = 0; c < 4; ++c) dq[c] = d[c];
+        }
       }
       // V/N at i0 = k + half N0/2
       const unsigned ps = (unsigned)(hd.n12 * sizeof(T));
@@ -615,13 +670,8 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
       }
     }
     __syncthreads();
-    if (act && B.dip && part == 0 && lane == 0) {
-      GAS double *dq = gw(B.dip) + ((size_t)tile * B.nst + s) * 4;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) dq[c] = dipw[sw][0][c] + dipw[sw][1][c];
-    }
     if (act && npts > 0) {
-      for (int e = ent_base + ptid; e < ent_base + nent; e += 128) {
+      for (int e = ent_base + ptid; e < ent_base + nent; e += 64) {
         const V2<int> cs = ld2<int>(hd.ent_cs, e);
         const V4<T> bt = ld4<T>(hd.ent_b, e);
         const C *dd = dpl + cs.y * kMaxNlm;
@@ -629,7 +679,7 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
         t.x = bt.x * dd[0].x + bt.y * dd[1].x + bt.z * dd[2].x + bt.w * dd[3].x;
         t.y = bt.x * dd[0].y + bt.y * dd[1].y + bt.z * dd[2].y + bt.w * dd[3].y;
         if (!B.gamma) t = cmul(t, kb_phase<T>(B.kcov, ld4<float>(hd.ent_pos, e), (T)-1));
-        T *q = reinterpret_cast<T *>((part ? pts1 : pts0) + cs.x);
+        T *q = reinterpret_cast<T *>(pts + cs.x);
         lds_add(q + 0, t.x);
         lds_add(q + 1, t.y);
       }
@@ -643,13 +693,13 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
 #pragma unroll
         for (int k = 0; k < H; ++k) {
           if (mask_bit<N0>(m, k, upper)) {
-            v[k] = cadd(cadd(v[k], pts0[cnt]), pts1[cnt]);
+            v[k] = cadd(v[k], pts[cnt]);
             ++cnt;
           }
         }
       }
       fft_split<N0, -1, C>::dif(v, upper);
-      if (valid) store_line_slots_split<T, N0>(B, hd.n12, s, L, upper, v);
+      store_line_slots_split<T, N0>(B, hd.n12, s, L, upper, valid, v);
     }
     __syncthreads();
   }
@@ -989,7 +1039,7 @@ __global__ void __launch_bounds__(256) k_line_to_real(const SphereBlock<T> *bloc
                                                       int total, int n12, int ntile, T scale) {
   using C = cplx<T>;
   constexpr int H = N0 / 2;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
   const bool upper = lane >= 32;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
@@ -1003,14 +1053,14 @@ __global__ void __launch_bounds__(256) k_line_to_real(const SphereBlock<T> *bloc
     const int Lc = L < n12 ? L : n12 - 1;
     C v[H];
     load_line_inverse_split<T, N0>(B, n12, s, Lc, upper, v);
-    if (L < n12) {
-      if (B.rs_layout == 0) {
-        const unsigned pl = (unsigned)(n12 * sizeof(C));
-        const auto rs = make_rsrc(B.rs + (size_t)s * N0 * n12, (unsigned)N0 * pl);
-        const unsigned vo = (unsigned)(L * sizeof(C)) + (upper ? (unsigned)H * pl : 0u);
+    if (B.rs_layout == 0) {
+      const unsigned pl = (unsigned)(n12 * sizeof(C));
+      const auto rs = make_rsrc(B.rs + (size_t)s * N0 * n12, (unsigned)N0 * pl);
+      const unsigned vo = L < n12 ? (unsigned)(L * sizeof(C)) + (upper ? (unsigned)H * pl : 0u) : kOOB;
 #pragma unroll
-        for (int k = 0; k < H; ++k) bstore(rs, vo, (unsigned)k * pl, C{v[k].x * scale, v[k].y * scale});
-      } else {
+      for (int k = 0; k < H; ++k) bstore(rs, vo, (unsigned)k * pl, C{v[k].x * scale, v[k].y * scale});
+    } else if (L < n12) {
+      {
         GAS C *out = gw(B.rs) + ((size_t)L + (upper ? (size_t)H * n12 : 0)) * B.nst + s;
 #pragma unroll
         for (int k = 0; k < H; ++k) out[(size_t)k * n12 * B.nst] = C{v[k].x * scale, v[k].y * scale};
@@ -1027,7 +1077,7 @@ __global__ void __launch_bounds__(256) k_line_from_real(const SphereBlock<T> *bl
                                                         int nblocks, int total, int n12, int ntile) {
   using C = cplx<T>;
   constexpr int H = N0 / 2;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
   const bool upper = lane >= 32;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int b = find_block(prefix, nblocks, wg);
@@ -1052,7 +1102,7 @@ __global__ void __launch_bounds__(256) k_line_from_real(const SphereBlock<T> *bl
       for (int k = 0; k < H; ++k) v[k] = in[(size_t)k * n12 * B.nst];
     }
     fft_split<N0, -1, C>::dif(v, upper);
-    if (L < n12) store_line_slots_split<T, N0>(B, n12, s, L, upper, v);
+    store_line_slots_split<T, N0>(B, n12, s, L, upper, L < n12, v);
   }
 }
 
@@ -1120,7 +1170,7 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
     INQX_HIP(hipGetLastError());
     return;
   }
-  const int xthreads = 128 * ld.nwave;  // a wave pair per state
+  const int xthreads = 64 * ld.nwave;  // one wave per (tile, state)
   const int xlds = ld.nwave * ld.lds_per_wave * (int)sizeof(cplx<T>);
   set_x_attr<T, N0>(xlds);
   const int grid_x = std::min(ld.total_x, cu * 16);
@@ -1231,12 +1281,11 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     }
     // waves (states) per x-pass workgroup within the LDS budget
     const int cs = (int)sizeof(cplx<T>);
-    int nw = std::max(1, std::min(2, h->xwaves[prec]));  // states per x-pass work-group
+    int nw = std::max(1, std::min(4, h->xwaves[prec]));  // states (waves) per x-pass work-group
     ld->mode = mode;
     ld->ntile_line = (g.n[1] * g.n[2] + 31) / 32;
     if (mode == 0) {
-      // per state: two point sets + D P (apply), or points + two accumulator sets (project)
-      ld->lds_per_wave = 2 * h->kb.maxpts + 2 * kMaxNlm * h->kb.maxslot;
+      ld->lds_per_wave = h->kb.maxpts + kMaxNlm * h->kb.maxslot;
       while (nw > 1 && nw * ld->lds_per_wave * cs > 160 * 1024) --nw;
       INQX_CHECK(ld->lds_per_wave * cs <= 160 * 1024, INQX_ERR_UNSUPPORTED, "KB tile too dense for LDS");
     }

@@ -137,7 +137,7 @@ struct Projector {
 #define INQX_SIZES(X) X(12) X(16) X(18) X(20) X(24) X(28) X(30) X(32) X(36) X(40) X(48) X(54) X(60) X(64) X(72) X(80) X(96)
 
 constexpr int kMaxNlm = 4;   // s + 3 p per atom, padded
-constexpr int kTile = 64;    // x-pencils (lines) per tile: one wavefront
+constexpr int kTile = 32;    // x-pencils (lines) per tile: one wavefront, a lane pair per line
 
 // tables of the real-space KB pass, shared by all blocks of a Hamiltonian
 // A tile is 64 consecutive lines L = i1*n2 + i2.  Every (atom, sphere point)
@@ -244,9 +244,8 @@ struct inqx_ham_s {
   inqx::KBTables kb;
   std::vector<inqx_basis> bases;
   int pinned = 0;  // > 0 while a captured graph uses the projector tables
-  // launch shape of the x-pass kernels per precision: states per work-group
-  // (a wave pair each)
-  int xwaves[2] = {2, 2};
+  // launch shape of the x-pass kernels per precision: states (waves) per work-group
+  int xwaves[2] = {1, 1};
   // exact-exchange workspaces: real-space states, hf states, accumulator, pair planes
   inqx::DevMem exx_phi, exx_hf, exx_w, exx_t, exx_jlist, exx_coef;
   // real-space apply and density workspaces (sphere arrays of a full-cube basis, occupations)
