@@ -320,11 +320,10 @@ template <typename T, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
   }
   attr = true;
   const bool f64 = std::is_same<T, double>::value;
-  const int cap = cx.num_cu * 16;
   const int tot1 = d.nj * S * d.nst;
   {
     ProfScope ps(cx, f64 ? "pair_plane_fwd_f64" : "pair_plane_fwd_f32");
-    hipLaunchKernelGGL((k_pair_plane_fwd<T, S, S, S>), dim3(std::min(tot1, cap)), dim3(Cfg::THREADS), Cfg::LDS,
+    hipLaunchKernelGGL((k_pair_plane_fwd<T, S, S, S>), dim3(tot1), dim3(Cfg::THREADS), Cfg::LDS,
                        cx.stream, d, tot1);
   }
   INQX_HIP(hipGetLastError());
@@ -332,13 +331,13 @@ template <typename T, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
   const int tot2 = d.nj * d.nst * ngrp;
   {
     ProfScope ps(cx, f64 ? "pair_xline_f64" : "pair_xline_f32");
-    hipLaunchKernelGGL((k_pair_xline<T, S, S, S>), dim3(std::min(tot2, cap)), dim3(256), 0, cx.stream, d, tot2);
+    hipLaunchKernelGGL((k_pair_xline<T, S, S, S>), dim3(tot2), dim3(256), 0, cx.stream, d, tot2);
   }
   INQX_HIP(hipGetLastError());
   const int tot3 = S * d.nst;
   {
     ProfScope ps(cx, f64 ? "pair_plane_inv_f64" : "pair_plane_inv_f32");
-    hipLaunchKernelGGL((k_pair_plane_inv<T, S, S, S>), dim3(std::min(tot3, cap)), dim3(Cfg::THREADS), Cfg::LDS,
+    hipLaunchKernelGGL((k_pair_plane_inv<T, S, S, S>), dim3(tot3), dim3(Cfg::THREADS), Cfg::LDS,
                        cx.stream, d, tot3);
   }
   INQX_HIP(hipGetLastError());

@@ -1137,11 +1137,11 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   const int *pref_red = reinterpret_cast<const int *>(ld.desc.as<char>() + ld.off_pref[2]);
   set_plane_attr<T, N12, N12>();
   const int cu = h->ctx->c.num_cu;
-  const int grid_plane = std::min(ld.total_plane, cu * 16);
+  const int grid_plane = ld.total_plane;  // one item per work-group
   Ctx &cx = h->ctx->c;
   const bool f64 = std::is_same<T, double>::value;
   if (ld.mode == 2) {  // real space -> sphere
-    const int grid_l = std::min(ld.total_line, cu * 16);
+    const int grid_l = ld.total_line;
     {
       ProfScope ps(cx, f64 ? "line_from_real_f64" : "line_from_real_f32");
       hipLaunchKernelGGL((k_line_from_real<T, N0>), dim3(grid_l), dim3(256), 0, st, blk, pref_x, nblocks,
@@ -1163,7 +1163,7 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   }
   INQX_HIP(hipGetLastError());
   if (ld.mode == 1) {  // sphere -> real space
-    const int grid_l = std::min(ld.total_line, cu * 16);
+    const int grid_l = ld.total_line;
     ProfScope ps(cx, f64 ? "line_to_real_f64" : "line_to_real_f32");
     hipLaunchKernelGGL((k_line_to_real<T, N0>), dim3(grid_l), dim3(256), 0, st, blk, pref_x, nblocks, ld.total_line,
                        hd.n12, ld.ntile_line, (T)(1.0 / (double)hd.N));
@@ -1173,7 +1173,7 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   const int xthreads = 64 * ld.nwave;  // one wave per (tile, state)
   const int xlds = ld.nwave * ld.lds_per_wave * (int)sizeof(cplx<T>);
   set_x_attr<T, N0>(xlds);
-  const int grid_x = std::min(ld.total_x, cu * 16);
+  const int grid_x = ld.total_x;
   if (hd.has_kb) {
     {
       ProfScope ps(cx, f64 ? "xpass_project_f64" : "xpass_project_f32");
@@ -1183,7 +1183,7 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
     INQX_HIP(hipGetLastError());
     {
       ProfScope ps(cx, f64 ? "kb_reduce_f64" : "kb_reduce_f32");
-      hipLaunchKernelGGL((k_kb_reduce<T>), dim3(std::min(ld.total_red, cu * 16)), dim3(256), 0, st, blk, pref_red,
+      hipLaunchKernelGGL((k_kb_reduce<T>), dim3(ld.total_red), dim3(256), 0, st, blk, pref_red,
                          nblocks, ld.total_red, hd);
     }
     INQX_HIP(hipGetLastError());

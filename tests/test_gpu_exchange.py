@@ -192,3 +192,70 @@ def test_chunked_pairs_are_bitwise_identical(ctx):
     many = ex.direct(d, d, occ).numpy()
     gh.set_exchange_workspace(0)
     assert np.array_equal(one, many)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("n", [72, 96])
+def test_large_grid_paths(ctx, dtype, n):
+    """the largest line lengths (their kernels run close to the register limit):
+    transforms and Poisson at n^3 against numpy / the oracle, and the direct
+    exchange of 2 states with 1 occupied state"""
+    say("case: large grid", n, np.dtype(dtype).name)
+    lat = np.eye(3) * (n * 0.3)
+    ng = (n, n, n)
+    N = n ** 3
+    gh = KSHamiltonian(ctx, lat, ng)
+    b = gh.basis((0, 0, 0), 4.0)
+    g = b.gindex()
+    vol = abs(np.linalg.det(lat))
+    c = S.random_states(b.npw, 2, 50 + n, vol / float(N) ** 2).astype(dtype)
+    rs = gh.to_real(b, DeviceArray.from_numpy(ctx, c)).numpy()
+    cube = np.zeros((2, N), dtype=np.complex128)
+    cube[:, g] = c
+    ref = np.fft.ifftn(cube.reshape((2,) + ng), axes=(1, 2, 3)).reshape(2, N)
+    assert rel_err(rs, ref) < TOL_T[dtype], "to_real"
+    back = gh.to_sphere(b, DeviceArray.from_numpy(ctx, rs), 2).numpy()
+    assert rel_err(back, c) < TOL_T[dtype], "to_sphere"
+    rng = np.random.default_rng(n)
+    rho = (rng.standard_normal((2, N)) + 1j * rng.standard_normal((2, N))).astype(dtype)
+    got = gh.poisson(DeviceArray.from_numpy(ctx, rho)).numpy()
+    G = 2 * np.pi * np.fft.fftfreq(n, d=lat[0, 0] / n)
+    g2 = G[:, None, None] ** 2 + G[None, :, None] ** 2 + G[None, None, :] ** 2
+    kern = np.where(g2 < 1e-6, 0.0, 4 * np.pi / np.where(g2 < 1e-6, 1.0, g2))
+    ref = np.fft.ifftn(np.fft.fftn(rho.astype(np.complex128).reshape((2,) + ng), axes=(1, 2, 3)) * kern,
+                       axes=(1, 2, 3)).reshape(2, N)
+    assert rel_err(got, ref) < TOL_T[dtype], "poisson"
+    ex = ExchangeOperator(gh, b, exchange_coefficient=1.0)
+    d = DeviceArray.from_numpy(ctx, c)
+    hf = DeviceArray.from_numpy(ctx, np.ascontiguousarray(c[:1]))
+    k = ex.direct(d, hf, np.array([2.0])).numpy()
+    rs64 = ref_rs = np.fft.ifftn(cube.reshape((2,) + ng), axes=(1, 2, 3)).reshape(2, N)
+    v = np.fft.ifftn(np.fft.fftn((np.conj(rs64[0])[None, :] * rs64).reshape((2,) + ng), axes=(1, 2, 3)) * kern,
+                     axes=(1, 2, 3)).reshape(2, N)
+    kr = -0.5 * 2.0 * rs64[0][None, :] * v
+    kref = np.fft.fftn(kr.reshape((2,) + ng), axes=(1, 2, 3)).reshape(2, N)[:, g]
+    assert rel_err(k, kref) < TOL_X[dtype], "exchange"
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_many_states_large_grid_transforms(ctx, dtype):
+    """regression: 60 states on a 96^3 grid (thousands of line work-groups);
+    a few states are checked against numpy, all against the round trip"""
+    say("case: 60 states 96^3 transforms", np.dtype(dtype).name)
+    n, nst = 96, 60
+    lat = np.eye(3) * 28.8
+    ng = (n, n, n)
+    N = n ** 3
+    gh = KSHamiltonian(ctx, lat, ng)
+    b = gh.basis((0, 0, 0), 3.0)
+    g = b.gindex()
+    c = S.random_states(b.npw, nst, 77, 1.0).astype(dtype)
+    rs = gh.to_real(b, DeviceArray.from_numpy(ctx, c))
+    pick = [0, 1, 31, 59]
+    cube = np.zeros((len(pick), N), dtype=np.complex128)
+    cube[:, g] = c[pick]
+    ref = np.fft.ifftn(cube.reshape((len(pick),) + ng), axes=(1, 2, 3)).reshape(len(pick), N)
+    got = rs.numpy()
+    assert rel_err(got[pick], ref) < TOL_T[dtype]
+    back = gh.to_sphere(b, rs, nst).numpy()
+    assert rel_err(back, c) < TOL_T[dtype]
