@@ -160,7 +160,7 @@ def c4_tddft(ctx, nsteps):
         b = gh.basis((0, 0, 0), sysm.ecut)
         c = S.random_states(b.npw, sysm.nst, sysm.seed, S.sphere_volume_element(sysm))
         d = DeviceArray.from_numpy(ctx, c.astype(dtype))
-        prop = rt.Propagator(gh, [b], [d], [np.full(sysm.nst, 2.0)], dt=0.05)
+        prop = rt.Propagator(gh, [b], [d], [np.full(sysm.nst, 2.0)], dt=0.05, ring_capacity=nsteps + 8)
         prop.energy()
         prop.step(2, graph=True)  # eager first step + capture
         ctx.sync()
@@ -168,7 +168,7 @@ def c4_tddft(ctx, nsteps):
         prop.step(nsteps, graph=True)
         ctx.sync()
         dt = (time.perf_counter() - t0) / nsteps
-        recs[name] = prop.records()
+        recs[name] = prop.records(cap=nsteps + 8)
         out[name] = {"ms_per_step": dt * 1e3, "steps_per_s": 1.0 / dt,
                      "state_applies_per_s": 9 * sysm.nst / dt}
         prop.close()
@@ -180,10 +180,15 @@ def c4_tddft(ctx, nsteps):
     e64 = r64[:m, 1]
     out["steps"] = int(m - 1)
     out["dipole_trace_max_abs_err_fp32_vs_fp64"] = dip_err
-    out["energy_drift_fp64_Ha"] = float(abs(e64[-1] - e64[0]))
-    out["energy_drift_fp32_Ha"] = float(abs(e32[-1] - e32[0]))
+    out["dipole_trace_max_abs"] = float(np.max(np.abs(r64[:m, 4:7])))
+    # the seeded states fill the whole sphere up to 30 Ha, where the Taylor-4
+    # step (dt/2 * 30 Ha = 0.75) is slightly dissipative: the drift is the
+    # propagator's (identical in FP32 and FP64), the FP32-vs-FP64 gap is precision
+    out["energy_drift_fp64_rel"] = float(abs(e64[-1] - e64[0]) / abs(e64[0]))
+    out["energy_drift_fp32_rel"] = float(abs(e32[-1] - e32[0]) / abs(e32[0]))
     out["band_energy_err_per_atom_Ha"] = float(np.max(np.abs(e32 - e64)) / sysm.natoms)
-    out["norm_err_fp32"] = float(np.max(np.abs(r32[:m, 3] - sysm.nst)))
+    out["norm_drift_fp64"] = float(np.max(np.abs(r64[:m, 3] - sysm.nst)) / sysm.nst)
+    out["norm_fp32_vs_fp64"] = float(np.max(np.abs(r32[:m, 3] - r64[:m, 3])) / sysm.nst)
     return out
 
 
@@ -327,6 +332,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--copy", action="store_true", help="run the HBM copy reference even with --no-extras")
     ap.add_argument("--only-c5", action="store_true", help="run only the Si216 ACE line (profiling)")
+    ap.add_argument("--c4-steps", type=int, default=2000, help="ETRS steps of the Si64 TDDFT line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -430,7 +436,7 @@ def main():
         log(f"bench: hbm copy {out['hbm_copy_gbs']:.0f} GB/s")
         out["c3_kpoints"] = c3_kpoints(ctx, max(3, args.steps // 4))
         log(f"bench: c3 {out['c3_kpoints']}")
-        out["c4_tddft"] = c4_tddft(ctx, max(4, args.steps // 2))
+        out["c4_tddft"] = c4_tddft(ctx, args.c4_steps)
         log(f"bench: c4 {out['c4_tddft']}")
         out["c1_diamond"] = c1_diamond(ctx, not args.no_cpu)
         log(f"bench: c1 {out['c1_diamond']}")
