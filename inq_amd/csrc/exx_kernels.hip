@@ -88,12 +88,36 @@ template <typename T> struct PairDev {
   T zero;              // value of the kernel at |G + q|^2 < 1e-6: zeroterm / N
 };
 
+// a plane pass works on one N1 x N2 plane in LDS (rows padded to N2 + 1); every
+// row or column of the plane is split over a lane pair (lane, lane + 32), so
+// a wave covers 32 lines and the work-group has ceil(max(N1, N2) / 32) waves
 template <typename T, int N1, int N2> struct PairPlaneCfg {
-  static constexpr int STRIDE = N2 + 1;  // padded LDS row
-  static constexpr int THREADS = ((N1 > N2 ? N1 : N2) + 63) / 64 * 64;  // <= 128 (launch bound)
-  static_assert(THREADS <= 128, "plane pass sized for at most 128 lines");
+  static constexpr int STRIDE = N2 + 1;
+  static constexpr int NL = N1 > N2 ? N1 : N2;
+  static constexpr int THREADS = (NL + 31) / 32 * 64;
+  static_assert(THREADS <= 256, "plane pass sized for at most 128 lines");
   static constexpr int LDS = N1 * STRIDE * (int)sizeof(cplx<T>);
 };
+
+// line index of the lane pair and the pair half of this thread
+__device__ __forceinline__ int pair_line(int tid) { return (tid >> 6) * 32 + (tid & 31); }
+
+// FFT along i2 of row r of the LDS plane, split over the lane pair; rows >= N1 run on a clamped row and do not store
+template <typename C, int N1, int N2, int SIGN>
+__device__ __forceinline__ void lds_row_fft_split(C *lds, int r, bool upper) {
+  constexpr int H = N2 / 2;
+  const int rc = r < N1 ? r : N1 - 1;
+  C *row = lds + rc * (N2 + 1);
+  C v[H];
+#pragma unroll
+  for (int n = 0; n < H; ++n) v[n] = row[2 * n + (upper ? 1 : 0)];
+  fft_split<N2, SIGN, C>::dit(v, upper);
+  __builtin_amdgcn_wave_barrier();
+  if (r < N1) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) row[k + (upper ? H : 0)] = v[k];
+  }
+}
 
 template <int N> __host__ __device__ constexpr int symi(int i) { return i >= (N + 1) / 2 ? i - N : i; }
 
@@ -103,14 +127,18 @@ __device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x
 __device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
 
 template <typename T, int N0, int N1, int N2>
-__global__ void __launch_bounds__(128) k_pair_plane_fwd(PairDev<T> d, int total) {
+__global__ void __launch_bounds__(256) k_pair_plane_fwd(PairDev<T> d, int total) {
   using C = cplx<T>;
   using Cfg = PairPlaneCfg<T, N1, N2>;
   constexpr int NP = N1 * N2;
+  constexpr int H1 = N1 / 2;
   constexpr long N = (long)N0 * NP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x;
+  const int line = pair_line(tid);
+  const bool upper = (tid & 63) >= 32;
+  const int half = upper ? 1 : 0;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int i = wg % d.nst;
     const int r = wg / d.nst;
@@ -120,33 +148,29 @@ __global__ void __launch_bounds__(128) k_pair_plane_fwd(PairDev<T> d, int total)
     const GAS C *ph = xg(d.phi) + (size_t)i * N + plane;
     const GAS C *hf = nullptr;
     if (d.hf) hf = xg(d.hf) + (size_t)xg(d.jlist)[jj] * N + plane;
-    if (tid < N2) {
-      const int z = tid;
-      C x[N1];
+    {
+      // column z = line along i1: lower lanes y = 2n, upper y = 2n + 1
+      const int z = line < N2 ? line : N2 - 1;
+      C v[H1];
       if (hf) {
 #pragma unroll
-        for (int y = 0; y < N1; ++y) {
+        for (int n = 0; n < H1; ++n) {
+          const int y = 2 * n + half;
           const C a = ph[y * N2 + z], h = hf[y * N2 + z];
-          x[y] = C{h.x * a.x + h.y * a.y, h.x * a.y - h.y * a.x};  // conj(h) a
+          v[n] = C{h.x * a.x + h.y * a.y, h.x * a.y - h.y * a.x};  // conj(h) a
         }
       } else {
 #pragma unroll
-        for (int y = 0; y < N1; ++y) x[y] = ph[y * N2 + z];
+        for (int n = 0; n < H1; ++n) v[n] = ph[(2 * n + half) * N2 + z];
       }
-      fft_line<N1, -1, C>::run(x);
+      fft_split<N1, -1, C>::dit(v, upper);
+      if (line < N2) {
 #pragma unroll
-      for (int y = 0; y < N1; ++y) lds[y * Cfg::STRIDE + z] = x[y];
+        for (int k = 0; k < H1; ++k) lds[(k + half * H1) * Cfg::STRIDE + z] = v[k];
+      }
     }
     __syncthreads();
-    if (tid < N1) {
-      C *row = lds + tid * Cfg::STRIDE;
-      C x[N2];
-#pragma unroll
-      for (int z = 0; z < N2; ++z) x[z] = row[z];
-      fft_line<N2, -1, C>::run(x);
-#pragma unroll
-      for (int z = 0; z < N2; ++z) row[z] = x[z];
-    }
+    lds_row_fft_split<C, N1, N2, -1>(lds, line, upper);
     __syncthreads();
     GAS C *tp = xw(d.t) + ((size_t)jj * d.nst + i) * N + plane;
     for (int e = tid; e < NP; e += Cfg::THREADS) {
@@ -207,89 +231,105 @@ __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
   }
 }
 
-// the column of w_i that a lane owns stays in registers over the whole chunk
-// of j when it fits beside the FFT line (complex64 always, complex128 up to
-// 48 points); otherwise w is read, updated and written back for every j
-template <typename T, int N1> struct PairAccInRegs {
-  static constexpr bool value = sizeof(T) == 4 || N1 <= 48;
-};
-
+// inverse 2D transform of one pair plane in place, the same decomposition as
+// the forward pass: the column transform reads t straight from memory, the
+// row transform runs from LDS.  k_pair_reduce then forms sum_j c_j hf_j t_j.
 template <typename T, int N0, int N1, int N2>
-__global__ void __launch_bounds__(128) k_pair_plane_inv(PairDev<T> d, int total) {
+__global__ void __launch_bounds__(256) k_pair_plane_inv(PairDev<T> d, int total) {
   using C = cplx<T>;
   using Cfg = PairPlaneCfg<T, N1, N2>;
   constexpr int NP = N1 * N2;
+  constexpr int H1 = N1 / 2;
   constexpr long N = (long)N0 * NP;
-  constexpr bool ACC = PairAccInRegs<T, N1>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x;
-  const bool acc_regs = ACC && d.hf != nullptr;
+  const int line = pair_line(tid);
+  const bool upper = (tid & 63) >= 32;
+  const int half = upper ? 1 : 0;
   for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
     const int i = wg % d.nst;
-    const int i0 = wg / d.nst;
+    const int r = wg / d.nst;
+    const int i0 = r % N0;
+    const int jj = r / N0;
     const size_t plane = (size_t)i0 * NP;
-    GAS C *wp = xw(d.w) + (size_t)i * N + plane;
-    C acc[N1];
-    if (acc_regs && tid < N2) {
+    GAS C *tp = xw(d.t) + ((size_t)jj * d.nst + i) * N + plane;
+    {
+      const int z = line < N2 ? line : N2 - 1;
+      C v[H1];
 #pragma unroll
-      for (int y = 0; y < N1; ++y) acc[y] = wp[y * N2 + tid];
-    }
-    for (int jj = 0; jj < d.nj; ++jj) {
-      const GAS C *tp = xg(d.t) + ((size_t)jj * d.nst + i) * N + plane;
-      for (int e = tid; e < NP; e += Cfg::THREADS) {
-        const int y = e / N2, z = e - y * N2;
-        lds[y * Cfg::STRIDE + z] = tp[e];
+      for (int n = 0; n < H1; ++n) v[n] = tp[(2 * n + half) * N2 + z];
+      fft_split<N1, +1, C>::dit(v, upper);
+      if (line < N2) {
+#pragma unroll
+        for (int k = 0; k < H1; ++k) lds[(k + half * H1) * Cfg::STRIDE + z] = v[k];
       }
-      __syncthreads();
-      if (tid < N1) {
-        C *row = lds + tid * Cfg::STRIDE;
-        C x[N2];
-#pragma unroll
-        for (int z = 0; z < N2; ++z) x[z] = row[z];
-        fft_line<N2, +1, C>::run(x);
-#pragma unroll
-        for (int z = 0; z < N2; ++z) row[z] = x[z];
-      }
-      __syncthreads();
-      if (tid < N2) {
-        const int z = tid;
-        C x[N1];
-#pragma unroll
-        for (int y = 0; y < N1; ++y) x[y] = lds[y * Cfg::STRIDE + z];
-        fft_line<N1, +1, C>::run(x);
-        if (d.hf) {
-          const int j = xg(d.jlist)[jj];
-          const T c = (T)xg(d.coef)[j];
-          const GAS C *hf = xg(d.hf) + (size_t)j * N + plane;
-          if (ACC) {
-#pragma unroll
-            for (int y = 0; y < N1; ++y) {
-              const C h = hf[y * N2 + z];
-              acc[y].x += c * (h.x * x[y].x - h.y * x[y].y);
-              acc[y].y += c * (h.x * x[y].y + h.y * x[y].x);
-            }
-          } else {
-#pragma unroll
-            for (int y = 0; y < N1; ++y) {
-              const C h = hf[y * N2 + z];
-              C o = wp[y * N2 + z];
-              o.x += c * (h.x * x[y].x - h.y * x[y].y);
-              o.y += c * (h.x * x[y].y + h.y * x[y].x);
-              wp[y * N2 + z] = o;
-            }
-          }
-        } else {
-#pragma unroll
-          for (int y = 0; y < N1; ++y) wp[y * N2 + z] = x[y];
-        }
-      }
-      __syncthreads();
     }
-    if (acc_regs && tid < N2) {
-#pragma unroll
-      for (int y = 0; y < N1; ++y) wp[y * N2 + tid] = acc[y];
+    __syncthreads();
+    lds_row_fft_split<C, N1, N2, +1>(lds, line, upper);
+    __syncthreads();
+    for (int e = tid; e < NP; e += Cfg::THREADS) {
+      const int y = e / N2, z = e - y * N2;
+      tp[e] = lds[y * Cfg::STRIDE + z];
     }
+    __syncthreads();
+  }
+}
+
+// w_i += sum_jj c_j hf_j t[jj][i] over the chunk, added in jj order so that
+// splitting the j range into chunks gives the same bits.  16-byte vectors
+// (two complex64 or one complex128), 4 per thread; the work-groups of one
+// point range run over consecutive i so that hf_j is read once from memory.
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; };
+template <> struct Vec16<double> { using type = double2; };
+
+__device__ __forceinline__ void cmac(float4 &a, float c, const float4 &h, const float4 &t) {
+  a.x += c * (h.x * t.x - h.y * t.y);
+  a.y += c * (h.x * t.y + h.y * t.x);
+  a.z += c * (h.z * t.z - h.w * t.w);
+  a.w += c * (h.z * t.w + h.w * t.z);
+}
+__device__ __forceinline__ void cmac(double2 &a, double c, const double2 &h, const double2 &t) {
+  a.x += c * (h.x * t.x - h.y * t.y);
+  a.y += c * (h.x * t.y + h.y * t.x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_pair_reduce(PairDev<T> d, long nvec) {
+  using V = typename Vec16<T>::type;
+  constexpr int U = 4;
+  const int i = (int)(blockIdx.x % (unsigned)d.nst);
+  const long v0 = (long)(blockIdx.x / (unsigned)d.nst) * 256 * U + threadIdx.x;
+  GAS V *wv = xw(reinterpret_cast<V *>(d.w)) + (size_t)i * nvec;
+  const GAS V *tv = xg(reinterpret_cast<const V *>(d.t)) + (size_t)i * nvec;
+  const GAS V *hv = xg(reinterpret_cast<const V *>(d.hf));
+  const size_t jstride = (size_t)d.nst * nvec;
+  V acc[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const long v = v0 + u * 256;
+    if (v < nvec) acc[u] = wv[v];
+  }
+  for (int jj = 0; jj < d.nj; ++jj) {
+    const int j = xg(d.jlist)[jj];
+    const T c = (T)xg(d.coef)[j];
+    V x[U], h[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long v = v0 + u * 256;
+      if (v < nvec) {
+        x[u] = tv[jj * jstride + v];
+        h[u] = hv[(size_t)j * nvec + v];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) cmac(acc[u], c, h[u], x[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const long v = v0 + u * 256;
+    if (v < nvec) wv[v] = acc[u];
   }
 }
 
@@ -334,13 +374,21 @@ template <typename T, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
     hipLaunchKernelGGL((k_pair_xline<T, S, S, S>), dim3(tot2), dim3(256), 0, cx.stream, d, tot2);
   }
   INQX_HIP(hipGetLastError());
-  const int tot3 = S * d.nst;
   {
     ProfScope ps(cx, f64 ? "pair_plane_inv_f64" : "pair_plane_inv_f32");
-    hipLaunchKernelGGL((k_pair_plane_inv<T, S, S, S>), dim3(tot3), dim3(Cfg::THREADS), Cfg::LDS,
-                       cx.stream, d, tot3);
+    hipLaunchKernelGGL((k_pair_plane_inv<T, S, S, S>), dim3(tot1), dim3(Cfg::THREADS), Cfg::LDS,
+                       cx.stream, d, tot1);
   }
   INQX_HIP(hipGetLastError());
+  if (d.hf) {
+    // 16-byte vectors per state: N complex values of 2 sizeof(T) bytes
+    const long nvec = (long)g.N * 2 * (long)sizeof(T) / 16;
+    const long per = (nvec + 1023) / 1024;
+    const long tot4 = per * d.nst;
+    ProfScope ps(cx, f64 ? "pair_reduce_f64" : "pair_reduce_f32");
+    hipLaunchKernelGGL((k_pair_reduce<T>), dim3((unsigned)tot4), dim3(256), 0, cx.stream, d, nvec);
+    INQX_HIP(hipGetLastError());
+  }
 }
 
 }  // namespace
@@ -361,6 +409,7 @@ void launch_pairs(inqx_ham_s *h, int prec, const PairArgs &a) {
   INQX_CHECK(pair_grid_supported(n), INQX_ERR_UNSUPPORTED, "pair Poisson solves need a cubic grid of a built size");
   INQX_CHECK(a.nst > 0 && a.nj > 0 && a.phi && a.t && a.w, INQX_ERR_ARG, "bad pair arguments");
   INQX_CHECK(!a.hf || (a.jlist && a.coef), INQX_ERR_ARG, "pair chunk without j list");
+  INQX_CHECK(a.hf || a.t == a.w, INQX_ERR_ARG, "a plain Poisson solve works in place");
   switch (n[0]) {
 #define CASE(S)                                    \
   case S:                                          \
