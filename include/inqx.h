@@ -178,6 +178,20 @@ int inqx_orthogonalize(inqx_ctx ctx, int precision, int n, long npw, double vol_
  * H_sub = phi^H H phi, FP64 eigendecomposition (eigenvalues ascending, host
  * array), phi <- phi U */
 int inqx_subspace_diag(inqx_ham ham, inqx_basis basis, int nst, int precision, void *phi, double *eigenvalues);
+/* operations::preconditioner on Fourier-space states (src/operations/preconditioner.hpp:26-58),
+ * Teter-Payne-Allan: <T>_i = Re(<(1/2)|G|^2 phi_i|phi_i> / <phi_i|phi_i>), then
+ * phi_i(G) *= K((1/2)|G|^2 / <T>_i), K(x) = (27+18x+12x^2+8x^3)/(27+18x+12x^2+8x^3+16x^4);
+ * |G| without the k shift (point_op.g2).  In place on the sphere states. */
+int inqx_precondition(inqx_ham ham, inqx_basis basis, int nst, int precision, void *phi);
+/* eigensolvers::steepest_descent (src/eigensolvers/steepest_descent.hpp:23-86):
+ * num_steps (the reference uses 5) line minimisations of each state along its
+ * (with precondition != 0 TPA-preconditioned) residual H phi - <H>phi, then
+ * operations::orthogonalize (complex64 states: two Cholesky passes, so that
+ * the FP32 rotation keeps them orthonormal); in place.  Overlaps and the step lengths are FP64
+ * and stay on the device; the only host synchronisation is the Cholesky status
+ * of the final orthogonalisation (INQX_ERR_NOT_PD if it fails). */
+int inqx_steepest_descent(inqx_ham ham, inqx_basis basis, int nst, int precision, int num_steps, int precondition,
+                          void *phi);
 
 /* ------------------------------------------------------------ real-time propagation */
 /* operations::exponential_in_place (src/operations/exponential.hpp:21-34):

@@ -5,8 +5,9 @@
 //
 //   inqx::ks_hamiltonian     hamiltonian::ks_hamiltonian (src/hamiltonian/ks_hamiltonian.hpp:35-275)
 //   inqx::orbital_block      the G-sphere form of states::orbital_set (src/states/orbital_set.hpp:18-192)
-//   inqx::operations::*      operations::overlap / orthogonalize / rotate / exponential
-//                            (src/operations/*.hpp)
+//   inqx::operations::*      operations::overlap / orthogonalize / rotate / exponential /
+//                            preconditioner (src/operations/*.hpp)
+//   inqx::eigensolvers::*    eigensolvers::steepest_descent (src/eigensolvers/steepest_descent.hpp)
 //
 // Errors: every C status < 0 becomes inqx::error (std::runtime_error) with the
 // library's message; nothing aborts.
@@ -143,5 +144,43 @@ template <typename Complex> void orthogonalize(orbital_block<Complex> &phi, doub
         phi.ctx(), "orthogonalize");
 }
 
+// operations::preconditioner (preconditioner.hpp:26-58), Teter-Payne-Allan, in
+// place on Fourier-space states; it reads |G|^2 from the Hamiltonian's grid
+class preconditioner {
+  const ks_hamiltonian &ham_;
+
+ public:
+  explicit preconditioner(const ks_hamiltonian &ham) : ham_(ham) {}
+  const ks_hamiltonian &hamiltonian() const { return ham_; }
+  template <typename Complex> void operator()(orbital_block<Complex> &phi) const {
+    check(inqx_precondition(ham_.get(), phi.basis(), phi.set_size(), orbital_block<Complex>::precision, phi.data()),
+          phi.ctx(), "precondition");
+  }
+};
+
+// operations::no_preconditioner (preconditioner.hpp:68-73)
+struct no_preconditioner {
+  template <typename Block> void operator()(Block &) const {}
+};
+
 }  // namespace operations
+
+namespace eigensolvers {
+
+// eigensolvers::steepest_descent(ham, prec, phi) (steepest_descent.hpp:23-86):
+// five preconditioned line minimisations per state, then orthogonalize
+template <typename Complex>
+void steepest_descent(const ks_hamiltonian &ham, const operations::preconditioner &, orbital_block<Complex> &phi) {
+  check(inqx_steepest_descent(ham.get(), phi.basis(), phi.set_size(), orbital_block<Complex>::precision, 5, 1,
+                              phi.data()),
+        phi.ctx(), "steepest_descent");
+}
+template <typename Complex>
+void steepest_descent(const ks_hamiltonian &ham, const operations::no_preconditioner &, orbital_block<Complex> &phi) {
+  check(inqx_steepest_descent(ham.get(), phi.basis(), phi.set_size(), orbital_block<Complex>::precision, 5, 0,
+                              phi.data()),
+        phi.ctx(), "steepest_descent");
+}
+
+}  // namespace eigensolvers
 }  // namespace inqx

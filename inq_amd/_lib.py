@@ -75,6 +75,8 @@ SIGNATURES = {
                                   C.c_double]),
     "inqx_orthogonalize": (C.c_int, [P, C.c_int, C.c_int, C.c_long, C.c_double, P]),
     "inqx_subspace_diag": (C.c_int, [P, P, C.c_int, C.c_int, P, DP]),
+    "inqx_precondition": (C.c_int, [P, P, C.c_int, C.c_int, P]),
+    "inqx_steepest_descent": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, C.c_int, P]),
     "inqx_exponential": (C.c_int, [P, P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, P]),
     "inqx_exponential_2_for_1": (C.c_int, [P, P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.c_int, P, P]),

@@ -832,10 +832,7 @@ int inqx_rotate_add(inqx_ctx ctx, int precision, int n, long npw, const void *ro
   API_END(ctx)
 }
 
-int inqx_orthogonalize(inqx_ctx ctx, int precision, int n, long npw, double vol_el, void *phi) {
-  API_BEGIN
-  check_states(ctx, precision, n, npw);
-  INQX_CHECK(phi, INQX_ERR_ARG, "null argument");
+static void orthogonalize_impl(inqx_ctx ctx, int precision, int n, long npw, double vol_el, void *phi) {
   ctx->c.ws_S.ensure(16 * (size_t)n * n);
   ctx->c.ws_X.ensure(16 * (size_t)n * n);
   double *S = ctx->c.ws_S.as<double>(), *X = ctx->c.ws_X.as<double>();
@@ -844,6 +841,13 @@ int inqx_orthogonalize(inqx_ctx ctx, int precision, int n, long npw, double vol_
   INQX_CHECK(info == 0, INQX_ERR_NOT_PD, "orthogonalize: overlap not positive definite (minor " + std::to_string(info) + ")");
   launch_trinv(ctx->c, n, S, X);
   rotate_inplace(ctx, precision, n, npw, X, 0, phi);  // phi <- phi Linv^H
+}
+
+int inqx_orthogonalize(inqx_ctx ctx, int precision, int n, long npw, double vol_el, void *phi) {
+  API_BEGIN
+  check_states(ctx, precision, n, npw);
+  INQX_CHECK(phi, INQX_ERR_ARG, "null argument");
+  orthogonalize_impl(ctx, precision, n, npw, vol_el, phi);
   API_END(ctx)
 }
 
@@ -878,6 +882,112 @@ int inqx_subspace_diag(inqx_ham h, inqx_basis bs, int nst, int precision, void *
                 ctx->c.ws_ops);
   INQX_HIP(hipMemcpyAsync(phi, ctx->c.ws_tmp.p, cs * (size_t)nst * npw, hipMemcpyDeviceToDevice, ctx->c.stream));
   for (int k = 0; k < nst; ++k) eigenvalues[k] = w[k];
+  API_END(ctx)
+}
+
+// ---------------------------------------------------------------- steepest descent
+// (1/2)|G|^2 of the sphere points without the k shift (fourier_space point_op.g2)
+static void ensure_g2h(inqx_ham h, Basis &b) {
+  if (b.d_g2h.p) return;
+  std::vector<double> g2h(b.npw);
+  const Grid &g = h->g;
+  for (long i = 0; i < b.npw; ++i) {
+    const int f = b.gidx[i];
+    const int i2 = f % g.n[2], i1 = (f / g.n[2]) % g.n[1], i0 = f / (g.n[2] * g.n[1]);
+    const double v[3] = {2.0 * M_PI * Grid::sym(i0, g.n[0]), 2.0 * M_PI * Grid::sym(i1, g.n[1]),
+                         2.0 * M_PI * Grid::sym(i2, g.n[2])};
+    double c[3];
+    g.cov_to_cart(v, c);
+    g2h[i] = 0.5 * (c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  }
+  upload(b.d_g2h, g2h, h->ctx->c.stream);
+  INQX_HIP(hipStreamSynchronize(h->ctx->c.stream));
+}
+
+// scalar workspace: dot partials, then per state the (H phi, phi) and (phi, phi)
+// pair, the preconditioner pair, the four residual overlaps and lambda
+struct SdScal {
+  double *part, *en, *kn, *m03, *lam;
+};
+static SdScal sd_scalars(inqx_ham h, int nst, long npw) {
+  const size_t pb = (sd_dots_partial_bytes(nst, npw) + 255) / 256 * 256;
+  const size_t per = (size_t)nst * sizeof(double);
+  h->sd_scal.ensure(pb + per * (4 + 4 + 8 + 1));
+  char *base = (char *)h->sd_scal.p;
+  SdScal r;
+  r.part = (double *)base;
+  r.en = (double *)(base + pb);
+  r.kn = r.en + 4 * (size_t)nst;
+  r.m03 = r.kn + 4 * (size_t)nst;
+  r.lam = r.m03 + 8 * (size_t)nst;
+  return r;
+}
+
+static void precondition_impl(inqx_ham h, Basis &B, int nst, int prec, void *phi, const SdScal &sc) {
+  Ctx &cx = h->ctx->c;
+  ensure_g2h(h, B);
+  const void *a[2] = {phi, phi};
+  launch_sd_dots(cx, prec, nst, B.npw, 2, a, a, B.d_g2h.as<double>(), 1, 1.0, sc.part, sc.kn);
+  launch_tpa_scale(cx, prec, nst, B.npw, phi, B.d_g2h.as<double>(), sc.kn);
+}
+
+static void check_sd(inqx_ham h, inqx_basis bs, int nst, int precision, const void *phi) {
+  INQX_CHECK(h && bs && phi && nst > 0, INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!h->ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(bs->b.ham == h, INQX_ERR_ARG, "basis belongs to another Hamiltonian");
+  INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
+  INQX_CHECK(nst <= 65535, INQX_ERR_ARG, "at most 65535 states per block");
+}
+
+int inqx_precondition(inqx_ham h, inqx_basis bs, int nst, int precision, void *phi) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_sd(h, bs, nst, precision, phi);
+  precondition_impl(h, bs->b, nst, precision, phi, sd_scalars(h, nst, bs->b.npw));
+  API_END(ctx)
+}
+
+int inqx_steepest_descent(inqx_ham h, inqx_basis bs, int nst, int precision, int num_steps, int precondition,
+                          void *phi) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_sd(h, bs, nst, precision, phi);
+  INQX_CHECK(num_steps >= 0, INQX_ERR_ARG, "bad step count");
+  Ctx &cx = ctx->c;
+  Basis &B = bs->b;
+  const long npw = B.npw;
+  const size_t cs = precision == INQX_FP64 ? 16 : 8;
+  const size_t sb = cs * (size_t)nst * npw;
+  h->sd_states.ensure(3 * sb);
+  void *hphi = h->sd_states.p;
+  void *res = (char *)h->sd_states.p + sb;
+  void *hres = (char *)h->sd_states.p + 2 * sb;
+  const SdScal sc = sd_scalars(h, nst, npw);
+  const double vol = h->g.vol / ((double)h->g.N * (double)h->g.N);
+  Basis *bb = &B;
+  auto apply = [&](const void *in, void *out) {
+    if (precision == INQX_FP64) launch_apply<double>(h, 1, &bb, &nst, &in, &out);
+    else launch_apply<float>(h, 1, &bb, &nst, &in, &out);
+  };
+  apply(phi, hphi);
+  for (int step = 0; step < num_steps; ++step) {
+    // eigenvalue (H phi, phi) and norm (phi, phi), then the residual
+    const void *a1[2] = {hphi, phi}, *b1[2] = {phi, phi};
+    launch_sd_dots(cx, precision, nst, npw, 2, a1, b1, nullptr, 0, vol, sc.part, sc.en);
+    launch_sd_residual(cx, precision, nst, npw, phi, hphi, res, sc.en);
+    if (precondition) precondition_impl(h, B, nst, precision, res, sc);
+    apply(res, hres);
+    const void *a2[4] = {res, phi, res, phi}, *b2[4] = {res, res, hres, hres};
+    launch_sd_dots(cx, precision, nst, npw, 4, a2, b2, nullptr, 0, vol, sc.part, sc.m03);
+    launch_sd_lambda(cx, nst, sc.m03, sc.en, sc.lam);
+    launch_sd_update(cx, precision, nst, npw, phi, step != num_steps - 1 ? hphi : nullptr, res, hres, sc.lam);
+  }
+  orthogonalize_impl(ctx, precision, nst, npw, vol, phi);
+  // complex64 states: a second Cholesky pass (CholeskyQR2).  After the line
+  // minimisations the overlap can be ill-conditioned, and one pass with an FP32
+  // rotation leaves an orthonormality error of eps_32 cond(S); the FP64 path
+  // keeps the reference's single pass.
+  if (precision == INQX_FP32) orthogonalize_impl(ctx, precision, nst, npw, vol, phi);
   API_END(ctx)
 }
 

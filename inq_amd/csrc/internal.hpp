@@ -218,6 +218,7 @@ struct Basis {
   std::vector<int> slot_seg;  // [nslot][4] segment start, length, row start, row count
   std::vector<int> rows, pw_yz;
   DevMem d_slot_seg, d_rows, d_pw_yz, d_kin64, d_kin32;
+  DevMem d_g2h;  // (1/2)|G|^2 per sphere point, FP64, no k shift (preconditioner)
   double kin_A[3] = {1e300, 0, 0};  // A for which d_kin* were built
   // workspaces per precision
   DevMem t1[2], partial[2], dp[2], proj[2], dip, dots, hn[2][2], save[2];
@@ -250,6 +251,8 @@ struct inqx_ham_s {
   inqx::DevMem exx_phi, exx_hf, exx_w, exx_t, exx_jlist, exx_coef;
   // real-space apply and density workspaces (sphere arrays of a full-cube basis, occupations)
   inqx::DevMem rs_a, rs_b, dens_occ;
+  // steepest-descent workspace: H phi, residual, H residual, dot partials and results
+  inqx::DevMem sd_states, sd_scal;
   size_t exx_pair_bytes = 0;  // budget of the pair workspace (one j-chunk); 0: a quarter of free memory, <= 64 GiB
 };
 
@@ -284,6 +287,19 @@ void launch_overlap_diagonal(Ctx &cx, int prec, int n, long K, double vol, const
                              int normalized, double *out);
 void launch_density_add(Ctx &cx, int prec, const void *phi, const double *occ, int nst, long N, int ncomp,
                         int ispin, double *den);
+// eigensolver.hip: per-state passes of steepest descent and the TPA
+// preconditioner over band-major sphere states [nst][K].  Dots: out[s][k] =
+// vol sum_G w conj(a_k) b_k as (re, im), n <= 4 pairs, w applied to the pairs
+// in wmask; part holds sd_dots_partial_bytes(nst, K).
+size_t sd_dots_partial_bytes(int nst, long K);
+void launch_sd_dots(Ctx &cx, int prec, int nst, long K, int n, const void *const *a, const void *const *b,
+                    const double *w, int wmask, double vol, double *part, double *out);
+void launch_sd_residual(Ctx &cx, int prec, int nst, long K, const void *phi, const void *hphi, void *res,
+                        const double *en);
+void launch_tpa_scale(Ctx &cx, int prec, int nst, long K, void *res, const double *g2h, const double *kn);
+void launch_sd_lambda(Ctx &cx, int nst, const double *m03, const double *m45, double *lam);
+void launch_sd_update(Ctx &cx, int prec, int nst, long K, void *phi, void *hphi, const void *res, const void *hres,
+                      const double *lam);
 // exx_kernels.hip: batched pair-density Poisson solves on real-space arrays in
 // state-major layout [n][N].  With hf: for every chunk entry jj (hf index
 // j = jlist[jj]) and state i, rho = conj(hf_j) phi_i -> Poisson -> w_i +=

@@ -16,6 +16,8 @@
 // orbitals; everything is reduced in FP64 into a device ring of records.
 #include "internal.hpp"
 
+#include <memory>
+
 #include <algorithm>
 #include <complex>
 #include <cstring>
@@ -236,17 +238,16 @@ int inqx_prop_create(inqx_ham h, int nblocks, const inqx_basis *bases, const int
   INQX_CHECK(h && bases && nst && phi && occ && out && nblocks > 0 && ring_capacity > 0, INQX_ERR_ARG, "null argument");
   INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
   INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
-  auto *p = new inqx_prop_s();
+  for (int i = 0; i < nblocks; ++i)
+    INQX_CHECK(bases[i] && bases[i]->b.ham == h && nst[i] > 0 && phi[i] && occ[i], INQX_ERR_ARG, "bad block");
+  std::unique_ptr<inqx_prop_s> p(new inqx_prop_s());
   p->ham = h;
   p->prec = precision;
   p->dt = dt;
-  size_t tot = 0;
   for (int i = 0; i < nblocks; ++i) {
-    INQX_CHECK(bases[i] && bases[i]->b.ham == h && nst[i] > 0 && phi[i] && occ[i], INQX_ERR_ARG, "bad block");
     p->bases.push_back(&bases[i]->b);
     p->nst.push_back(nst[i]);
     p->phi.push_back(phi[i]);
-    tot += nst[i];
   }
   std::vector<double> occ_all;
   for (int i = 0; i < nblocks; ++i) occ_all.insert(occ_all.end(), occ[i], occ[i] + nst[i]);
@@ -262,7 +263,7 @@ int inqx_prop_create(inqx_ham h, int nblocks, const inqx_basis *bases, const int
   const int zero = 0;
   INQX_HIP(hipMemcpyAsync(p->counter.p, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->c.stream));
   INQX_HIP(hipStreamSynchronize(ctx->c.stream));
-  *out = p;
+  *out = p.release();
   API_END(ctx)
 }
 

@@ -403,3 +403,57 @@ def sing_correction(lattice, kcov, kweight):
     fz = np.zeros(1)
     lib().orc_sing_correction(_f64(lattice).ravel(), nk, kcov.ravel(), _f64(kweight), fk, fz)
     return fk, fz[0]
+
+
+# ---- eigensolver ----
+def overlap_diagonal(phi1, phi2, vol_el):
+    """operations::overlap_diagonal (overlap_diagonal.hpp:25-60), band-major [nst][npnt]."""
+    return vol_el * np.einsum("ip,ip->i", np.conj(phi1), phi2)
+
+
+def tpa_k(x):
+    """Teter-Payne-Allan preconditioning function (preconditioner.hpp:31-38)."""
+    num = ((8.0 * x + 12.0) * x + 18.0) * x + 27.0
+    den = (((16.0 * x + 8.0) * x + 12.0) * x + 18.0) * x + 27.0
+    return num / den
+
+
+def tpa_precondition(phi, g2half):
+    """operations::preconditioner on Fourier-space states (preconditioner.hpp:40-58):
+    <T>_i = Re(<(1/2)|G|^2 phi_i|phi_i> / <phi_i|phi_i>), phi_i(G) *= K((1/2)|G|^2 / <T>_i).
+    g2half holds (1/2)|G|^2 per point (no k shift: point_op.g2)."""
+    phi = np.asarray(phi, dtype=np.complex128)
+    w = np.abs(phi) ** 2
+    kin = (w @ g2half) / w.sum(axis=1)
+    return phi * tpa_k(g2half[None, :] / kin[:, None])
+
+
+def steepest_descent(apply, precond, phi, vol_el, num_steps=5):
+    """eigensolvers::steepest_descent (steepest_descent.hpp:23-86) on band-major
+    states: num_steps line-minimisation steps along the preconditioned residual,
+    then operations::orthogonalize.  apply(phi) -> H phi; precond(res) -> res or None."""
+    phi = np.array(phi, dtype=np.complex128)
+    hphi = np.asarray(apply(phi), dtype=np.complex128).copy()
+    for istep in range(num_steps):
+        ev = overlap_diagonal(hphi, phi, vol_el)
+        norm = overlap_diagonal(phi, phi, vol_el)
+        res = hphi - (ev / norm.real)[:, None] * phi
+        if precond is not None:
+            res = precond(res)
+        hres = np.asarray(apply(res), dtype=np.complex128)
+        m0 = overlap_diagonal(res, res, vol_el)
+        m1 = overlap_diagonal(phi, res, vol_el)
+        m2 = overlap_diagonal(res, hres, vol_el)
+        m3 = overlap_diagonal(phi, hres, vol_el)
+        m4, m5 = ev, norm
+        ca = (m0 * m3 - m2 * m1).real
+        cb = (m5 * m2 - m4 * m0).real
+        cc = (m4 * m1 - m3 * m5).real
+        with np.errstate(invalid="ignore", divide="ignore"):
+            den = cb + np.sqrt(cb * cb - 4.0 * ca * cc)
+            lam = np.where(np.abs(den) < 1e-15, 0.0, 2.0 * cc / den)
+        phi += lam[:, None] * res
+        if istep != num_steps - 1:
+            hphi += lam[:, None] * hres
+    phi, info = orthogonalize(phi, vol_el, True)
+    return phi, info

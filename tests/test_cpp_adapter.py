@@ -25,6 +25,10 @@ int main(int argc, char **) {
     auto hphi = ham(phi);
     half_step(ham, phi, 0.05);
     inqx::operations::orthogonalize(phi, 1e-3);
+    // the ground-state caller's shape (src/ground_state/calculator.hpp:160-168)
+    inqx::operations::preconditioner prec(ham);
+    inqx::eigensolvers::steepest_descent(ham, prec, phi);
+    inqx::eigensolvers::steepest_descent(ham, inqx::operations::no_preconditioner{}, phi);
   }
   return inqx_version() >= 1 ? 0 : 1;
 }
