@@ -193,6 +193,37 @@ int inqx_precondition(inqx_ham ham, inqx_basis basis, int nst, int precision, vo
 int inqx_steepest_descent(inqx_ham ham, inqx_basis basis, int nst, int precision, int num_steps, int precondition,
                           void *phi);
 
+/* The Hartree step of hamiltonian::self_consistency::update_hamiltonian
+ * (src/hamiltonian/self_consistency.hpp:115-120): vh = poisson::solve(density)
+ * for a real field (src/solvers/poisson.hpp:217-223; periodic cell, G = 0 term
+ * dropped) and energy = 1/2 sum density vh dV with dV = V/N
+ * (operations::integral_product).  density, vh: device double arrays [N], point
+ * order (i0, i1, i2); vh or energy may be NULL; energy is a host double (the
+ * call then synchronises).  FP64 throughout. */
+int inqx_hartree(inqx_ham ham, const double *density, double *vh, double *energy);
+
+/* ------------------------------------------------------------ restart files */
+/* The reference's restart format (src/operations/io.hpp:41-211,
+ * src/systems/electrons.hpp:323-381): raw native-endian files without headers,
+ * named by utils::num_to_str ("%010ld", src/utils/num_str.hpp:20-24).
+ * io::save / io::load of an orbital_set (io.hpp:168-211): file
+ * dirname/<first_state+ist>_0000000000 per state, N complex128 real-space values
+ * in point order (i0, i1, i2).  Save goes through to_real (x 1/N) from the
+ * sphere; load through the masked forward transform onto the sphere.
+ * complex64 states are widened to complex128 in the files.  Load returns
+ * INQX_ERR_ARG when a file is missing or short (the reference returns false). */
+int inqx_states_save(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *psi, const char *dirname,
+                     int first_state);
+int inqx_states_load(inqx_ham ham, inqx_basis basis, int nst, int precision, const char *dirname, int first_state,
+                     void *psi);
+/* io::save / io::load of a real field_set (io.hpp:124-162), e.g. the spin density:
+ * dirname/<icomp> with npoints doubles each; v is a host array [npoints][ncomp] */
+int inqx_field_save(inqx_ctx ctx, const char *dirname, long npoints, int ncomp, const double *v);
+int inqx_field_load(inqx_ctx ctx, const char *dirname, long npoints, int ncomp, double *v);
+/* io::save / io::load of a plain array (io.hpp:104-120), e.g. the occupations: dirname/array */
+int inqx_array_save(inqx_ctx ctx, const char *dirname, long n, const double *v);
+int inqx_array_load(inqx_ctx ctx, const char *dirname, long n, double *v);
+
 /* ------------------------------------------------------------ real-time propagation */
 /* operations::exponential_in_place (src/operations/exponential.hpp:21-34):
  * phi <- sum_{n<=order} (f H)^n / n! phi, f = f_re + i f_im, in place */

@@ -76,6 +76,13 @@ SIGNATURES = {
     "inqx_orthogonalize": (C.c_int, [P, C.c_int, C.c_int, C.c_long, C.c_double, P]),
     "inqx_subspace_diag": (C.c_int, [P, P, C.c_int, C.c_int, P, DP]),
     "inqx_precondition": (C.c_int, [P, P, C.c_int, C.c_int, P]),
+    "inqx_hartree": (C.c_int, [P, P, P, DP]),
+    "inqx_states_save": (C.c_int, [P, P, C.c_int, C.c_int, P, C.c_char_p, C.c_int]),
+    "inqx_states_load": (C.c_int, [P, P, C.c_int, C.c_int, C.c_char_p, C.c_int, P]),
+    "inqx_field_save": (C.c_int, [P, C.c_char_p, C.c_long, C.c_int, DP]),
+    "inqx_field_load": (C.c_int, [P, C.c_char_p, C.c_long, C.c_int, DP]),
+    "inqx_array_save": (C.c_int, [P, C.c_char_p, C.c_long, DP]),
+    "inqx_array_load": (C.c_int, [P, C.c_char_p, C.c_long, DP]),
     "inqx_steepest_descent": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, C.c_int, P]),
     "inqx_exponential": (C.c_int, [P, P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, P]),
     "inqx_exponential_2_for_1": (C.c_int, [P, P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,

@@ -287,20 +287,6 @@ static void hermitian_eig(int n, std::vector<std::complex<double>> a, std::vecto
 
 }  // namespace inqx
 
-// ---------------------------------------------------------------- error plumbing
-#define API_BEGIN try {
-#define API_END(ctxp)                                    \
-  }                                                      \
-  catch (const inqx::Error &e) {                         \
-    if (ctxp) (ctxp)->c.err = e.msg;                     \
-    return e.code;                                       \
-  }                                                      \
-  catch (const std::exception &e) {                      \
-    if (ctxp) (ctxp)->c.err = e.what();                  \
-    return INQX_ERR_ARG;                                 \
-  }                                                      \
-  return INQX_OK;
-
 extern "C" {
 
 int inqx_version(void) { return 1; }
@@ -1056,6 +1042,29 @@ int inqx_poisson(inqx_ham h, int n, int precision, void *rho, const double gshif
     for (int d = 0; d < 3; ++d) a.gshift[d] = gshift_cart[d];
   a.zeroterm = zeroterm;
   launch_pairs(h, precision, a);
+  API_END(ctx)
+}
+
+int inqx_hartree(inqx_ham h, const double *density, double *vh, double *energy) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && density && (vh || energy), INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "Poisson solves need a cubic grid of a built size");
+  const long N = h->g.N;
+  const int nch = hartree_chunks(N);
+  h->hartree_work.ensure(16 * (size_t)N);
+  h->hartree_part.ensure(sizeof(double) * (size_t)nch);
+  launch_hartree(h, density, vh, h->hartree_work.p, h->hartree_part.as<double>());
+  if (energy) {
+    std::vector<double> part(nch);
+    INQX_HIP(hipMemcpyAsync(part.data(), h->hartree_part.p, sizeof(double) * nch, hipMemcpyDeviceToHost,
+                            ctx->c.stream));
+    INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+    double e = 0.0;
+    for (int c = 0; c < nch; ++c) e += part[c];
+    *energy = 0.5 * e * h->g.vol / (double)N;
+  }
   API_END(ctx)
 }
 

@@ -393,6 +393,63 @@ template <typename T, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- Hartree
+namespace {
+constexpr int kHChunk = 8192;
+
+__global__ void __launch_bounds__(256) k_real_to_cplx(const double *r, double2 *c, long N) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p < N) xw(c)[p] = double2{xg(r)[p], 0.0};
+}
+
+// vh = Re(c); part[chunk] = sum over the chunk of rho vh (fixed tree order)
+__global__ void __launch_bounds__(256) k_hartree_out(const double2 *c, const double *rho, double *vh, long N,
+                                                     double *part) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  const long p0 = (long)blockIdx.x * kHChunk;
+  double acc = 0.0;
+  for (long p = p0 + tid; p < p0 + kHChunk && p < N; p += 256) {
+    const double v = xg(c)[p].x;
+    if (vh) xw(vh)[p] = v;
+    acc += xg(rho)[p] * v;
+  }
+  red[tid] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) red[tid] += red[tid + h];
+    __syncthreads();
+  }
+  if (tid == 0) xw(part)[blockIdx.x] = red[0];
+}
+}  // namespace
+
+int hartree_chunks(long N) { return (int)((N + kHChunk - 1) / kHChunk); }
+
+void launch_hartree(inqx_ham_s *h, const double *rho, double *vh, void *work, double *part) {
+  Ctx &cx = h->ctx->c;
+  const long N = h->g.N;
+  {
+    ProfScope ps(cx, "hartree_in");
+    hipLaunchKernelGGL(k_real_to_cplx, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, cx.stream, rho,
+                       reinterpret_cast<double2 *>(work), N);
+  }
+  INQX_HIP(hipGetLastError());
+  PairArgs a;
+  a.phi = work;
+  a.t = work;
+  a.w = work;
+  a.nst = 1;
+  a.nj = 1;
+  launch_pairs(h, INQX_FP64, a);
+  {
+    ProfScope ps(cx, "hartree_out");
+    hipLaunchKernelGGL(k_hartree_out, dim3((unsigned)hartree_chunks(N)), dim3(256), 0, cx.stream,
+                       reinterpret_cast<const double2 *>(work), rho, vh, N, part);
+  }
+  INQX_HIP(hipGetLastError());
+}
+
 bool pair_grid_supported(const int *n) {
   if (n[0] != n[1] || n[1] != n[2]) return false;
   switch (n[0]) {

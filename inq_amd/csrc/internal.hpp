@@ -229,6 +229,21 @@ struct Basis {
 
 }  // namespace inqx
 
+// C entry points: every inqx::Error becomes its status code and the context's
+// last-error message; nothing propagates across the boundary
+#define API_BEGIN try {
+#define API_END(ctxp)                                    \
+  }                                                      \
+  catch (const inqx::Error &e) {                         \
+    if (ctxp) (ctxp)->c.err = e.msg;                     \
+    return e.code;                                       \
+  }                                                      \
+  catch (const std::exception &e) {                      \
+    if (ctxp) (ctxp)->c.err = e.what();                  \
+    return INQX_ERR_ARG;                                 \
+  }                                                      \
+  return INQX_OK;
+
 struct inqx_ctx_s {
   inqx::Ctx c;
 };
@@ -253,6 +268,7 @@ struct inqx_ham_s {
   inqx::DevMem rs_a, rs_b, dens_occ;
   // steepest-descent workspace: H phi, residual, H residual, dot partials and results
   inqx::DevMem sd_states, sd_scal;
+  inqx::DevMem hartree_work, hartree_part;  // Hartree solve: complex128 [N] and the energy partials
   size_t exx_pair_bytes = 0;  // budget of the pair workspace (one j-chunk); 0: a quarter of free memory, <= 64 GiB
 };
 
@@ -317,4 +333,8 @@ struct PairArgs {
 };
 void launch_pairs(inqx_ham_s *h, int prec, const PairArgs &a);
 bool pair_grid_supported(const int *n);
+// Hartree potential of a real density [N] (FP64): work holds N complex128,
+// part hartree_chunks(N) doubles of sum rho vh partials (fixed order)
+int hartree_chunks(long N);
+void launch_hartree(inqx_ham_s *h, const double *rho, double *vh, void *work, double *part);
 }  // namespace inqx

@@ -188,19 +188,6 @@ template <typename T> static void prop_step_eager(inqx_prop_s *p) {
                    p->cap, p->counter.as<int>(), p->obs_desc, p->obs_host, p->capturing);
 }
 
-#define API_BEGIN try {
-#define API_END(ctxp)                                    \
-  }                                                      \
-  catch (const inqx::Error &e) {                         \
-    if (ctxp) (ctxp)->c.err = e.msg;                     \
-    return e.code;                                       \
-  }                                                      \
-  catch (const std::exception &e) {                      \
-    if (ctxp) (ctxp)->c.err = e.what();                  \
-    return INQX_ERR_ARG;                                 \
-  }                                                      \
-  return INQX_OK;
-
 extern "C" {
 
 int inqx_exponential(inqx_ham h, inqx_basis bs, int nst, int precision, double f_re, double f_im, int order,

@@ -262,6 +262,16 @@ class KSHamiltonian:
               self.ctx.h, "inqx_poisson")
         return rho
 
+    def hartree(self, density, vh=None, energy=True):
+        """Hartree potential and energy of a real density (self_consistency.hpp:115-120):
+        density, vh device float64 arrays [N]; returns (vh, E_H or None)."""
+        if vh is None:
+            vh = DeviceArray(self.ctx, density.shape, np.float64)
+        e = C.c_double()
+        check(lib().inqx_hartree(self.h, density.ptr, vh.ptr, C.byref(e) if energy else None), self.ctx.h,
+              "inqx_hartree")
+        return vh, (e.value if energy else None)
+
     def autotune(self, basis, nst, dtype):
         """times the x-pass launch shapes on this block size and keeps the fastest;
         returns (states per work-group, ms per apply)."""

@@ -1,0 +1,79 @@
+"""Restart files of sphere states through the C ABI (io::save / io::load of an
+orbital_set, src/operations/io.hpp:168-211).
+
+Checker: the FP64 oracle's real-space values of the same sphere states
+(oracle sphere_to_cube + to_real, x 1/N), compared with the raw files.
+File names follow utils::num_to_str ("%010ld"; the reference's known answer
+"0000001024", src/utils/num_str.hpp:76).  Tolerances: complex128 1e-13
+relative for the file values and the load round trip; complex64 states are
+widened to complex128 and agree with the FP64 files to 2e-6.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from inq_amd import Context, DeviceArray, InqxError
+from inq_amd import io as inqio
+from inq_amd import systems as S
+from tests.helpers import gpu_ham, oracle_ham, rel_err, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return Context(0)
+
+
+@pytest.fixture(scope="module")
+def setup(ctx):
+    sysm = S.small_config(seed=9, n=(20, 20, 20), natoms=3, nst=8, ecut=7.0)
+    gh = gpu_ham(ctx, sysm)
+    b = gh.basis((0.2, -0.1, 0.05), sysm.ecut)
+    c = S.random_states(b.npw, sysm.nst, 21, S.sphere_volume_element(sysm))
+    return sysm, gh, b, c
+
+
+def test_states_files_match_oracle_real_space(ctx, setup, tmp_path):
+    sysm, gh, b, c = setup
+    say("case: states save, complex128")
+    d = DeviceArray.from_numpy(ctx, c)
+    inqio.save_states(gh, b, d, tmp_path / "states", first_state=1024)
+    names = sorted(os.listdir(tmp_path / "states"))
+    assert names[0] == "0000001024_0000000000" and len(names) == sysm.nst
+    oh = oracle_ham(sysm)
+    ref = oh.to_real(oh.sphere_to_cube(c, b.gindex()))  # [N][nst]
+    N = int(np.prod(sysm.n))
+    for i in range(sysm.nst):
+        raw = np.fromfile(tmp_path / "states" / f"{1024 + i:010d}_{0:010d}", dtype=np.complex128)
+        assert raw.shape == (N,)
+        assert rel_err(raw, ref[:, i]) < 1e-13
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_states_round_trip(ctx, setup, tmp_path, dtype):
+    sysm, gh, b, c = setup
+    say("case: states save/load", np.dtype(dtype).name)
+    d = DeviceArray.from_numpy(ctx, c.astype(dtype))
+    inqio.save_states(gh, b, d, tmp_path / "s")
+    back = DeviceArray(ctx, c.shape, dtype)
+    inqio.load_states(gh, b, back, tmp_path / "s")
+    assert rel_err(back.numpy().astype(np.complex128), c) < (1e-13 if dtype == np.complex128 else 2e-6)
+    if dtype == np.complex64:
+        d64 = DeviceArray.from_numpy(ctx, c)
+        inqio.save_states(gh, b, d64, tmp_path / "s64")
+        for i in range(sysm.nst):
+            name = f"{i:010d}_{0:010d}"
+            a = np.fromfile(tmp_path / "s" / name, dtype=np.complex128)
+            r = np.fromfile(tmp_path / "s64" / name, dtype=np.complex128)
+            assert rel_err(a, r) < 2e-6
+
+
+def test_missing_state_file_is_a_status(ctx, setup, tmp_path):
+    sysm, gh, b, c = setup
+    d = DeviceArray.from_numpy(ctx, c)
+    inqio.save_states(gh, b, d, tmp_path / "s")
+    os.remove(tmp_path / "s" / f"{3:010d}_{0:010d}")
+    with pytest.raises(InqxError):
+        inqio.load_states(gh, b, d, tmp_path / "s")
