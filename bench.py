@@ -296,9 +296,13 @@ def c5_ace(ctx, cpu, cpu_seconds=8.0):
     d32h = DeviceArray.from_numpy(ctx, c[:hf_idx].astype(np.complex64))
     d64a = DeviceArray.from_numpy(ctx, c.astype(np.complex64).astype(np.complex128))
     d32h_in = DeviceArray.from_numpy(ctx, c[:hf_idx].astype(np.complex64).astype(np.complex128))
-    k32 = ex.direct(d, d32h, occ[:hf_idx]).numpy().astype(np.complex128)
-    k64 = ex.direct(d64a, d32h_in, occ[:hf_idx]).numpy()
-    res["direct_fp32_vs_fp64_rel"] = float(np.max(np.abs(k32 - k64)) / np.max(np.abs(k64)))
+    # each precision twice: the repeats must agree bit for bit (fixed reduction orders)
+    k32 = [ex.direct(d, d32h, occ[:hf_idx]).numpy().astype(np.complex128) for _ in range(2)]
+    k64 = [ex.direct(d64a, d32h_in, occ[:hf_idx]).numpy() for _ in range(2)]
+    scale64 = np.max(np.abs(k64[0]))
+    res["direct_fp32_vs_fp64_rel"] = float(np.max(np.abs(k32[0] - k64[0])) / scale64)
+    res["direct_repeat_rel"] = {"fp32": float(np.max(np.abs(k32[1] - k32[0])) / scale64),
+                                "fp64": float(np.max(np.abs(k64[1] - k64[0])) / scale64)}
     for a in (d32h, d64a, d32h_in, out, d, d64):
         a.free()
     if cpu:
