@@ -14,6 +14,8 @@ Secondary lines in the same JSON object (same seeded inputs):
   c4_tddft      ETRS steps on Si64 with a velocity-gauge kick, hipGraph replay,
                 FP32 vs FP64 dipole trace and energy drift
   c1_diamond    C8 diamond apply + overlap + Cholesky, GPU and CPU oracle
+  c5_ace        Si216 (96^3, 432 states) ACE build: all pair-density Poisson solves,
+                W^H phi, FP64 Cholesky, xi; ACE apply; FP32 vs FP64 direct exchange
 The CPU baseline is the FP64 oracle (a C/OpenMP port of the reference CPU data
 flow) on a bounded subset of the Si64 states.
 
@@ -281,7 +283,8 @@ def c5_ace(ctx, cpu, cpu_seconds=8.0):
     t_apply = (time.perf_counter() - t0) / 5
     pairs = nst * nst
     sc = 8
-    # per pair (one j per chunk): phi_i read, t written, t read+written, t read, w read+written
+    # per pair: phi_i read and t written (plane_fwd), t read+written (xline), t read+written
+    # (plane_inv), t read (reduce); hf_j is shared by all i and w_i is amortised over the chunk
     pair_bytes = 7 * N * sc
     kern = {k: v[1] / v[0] * 1e3 for k, v in kt.items()}
     pk = {k: kern[k] for k in kern if k.startswith("pair_")}
