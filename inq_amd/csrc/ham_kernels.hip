@@ -102,205 +102,6 @@ template <> __device__ __forceinline__ void sincos_t<double>(double a, double *s
 // -------------------------------------------------------------------------------------------
 // plane passes
 // -------------------------------------------------------------------------------------------
-template <typename T, int N1, int N2> struct PlaneCfg {
-  static constexpr int NL = (N1 > N2) ? N1 : N2;
-  static constexpr int PLANE = N1 * (N2 + 1);  // padded row stride N2+1
-  static constexpr int PLANE_BYTES = PLANE * (int)sizeof(cplx<T>);
-  static constexpr int P0 = 256 / NL;
-  static constexpr int PL = (65536 / PLANE_BYTES) > 0 ? (65536 / PLANE_BYTES) : 1;
-  static constexpr int P = (P0 < PL ? P0 : PL) > 0 ? (P0 < PL ? P0 : PL) : 1;
-  static constexpr int THREADS = ((P * NL + 63) / 64) * 64;
-  static constexpr int LDS = P * PLANE_BYTES;
-};
-
-template <typename T, int N1, int N2>
-__global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                       int total) {
-  using C = cplx<T>;
-  using Cfg = PlaneCfg<T, N1, N2>;
-  constexpr int P = Cfg::P;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  C *lds = reinterpret_cast<C *>(smem_raw);
-  const int tid = threadIdx.x, nth = blockDim.x;
-  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
-    const int b = find_block(prefix, nblocks, wg);
-    const SphereBlock<T> B = blocks[b];
-    const int local = wg - gp(prefix)[b];
-    const int ngrp = (B.nst + P - 1) / P;
-    const int slot = local / ngrp;
-    const int s0 = (local - slot * ngrp) * P;
-    const int np = min(P, B.nst - s0);
-    const V4<int> sg = ld4<int>(B.slot_seg, slot);
-    const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
-    for (int i = tid; i < np * Cfg::PLANE; i += nth) lds[i] = C{0, 0};
-    __syncthreads();
-    const GAS C *psi = gp(B.psi);
-    const GAS int *pwyz = gp(B.pw_yz);
-    // four sphere points per thread and step: their loads are in flight together
-    const int tot = np * nseg;
-    for (int i0 = tid; i0 < tot; i0 += 4 * nth) {
-      int ldx[4];
-      C val[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = min(i0 + u * nth, tot - 1);
-        const int p = i / nseg, j = i - p * nseg;
-        const int yz = pwyz[seg0 + j];
-        const int y = yz / N2, z = yz - y * N2;
-        ldx[u] = p * Cfg::PLANE + y * (N2 + 1) + z;
-        val[u] = psi[(size_t)(s0 + p) * B.npw + seg0 + j];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (i0 + u * nth < tot) lds[ldx[u]] = val[u];
-    }
-    __syncthreads();
-    const GAS int *rows = gp(B.rows);
-    for (int i = tid; i < np * nrow; i += nth) {
-      const int p = i / nrow;
-      const int y = rows[r0 + i - p * nrow];
-      C *row = lds + p * Cfg::PLANE + y * (N2 + 1);
-      C x[N2];
-#pragma unroll
-      for (int z = 0; z < N2; ++z) x[z] = row[z];
-      fft_line<N2, +1, C>::run(x);
-#pragma unroll
-      for (int z = 0; z < N2; ++z) row[z] = x[z];
-    }
-    __syncthreads();
-    GAS C *t1 = gw(B.t1);
-    for (int i = tid; i < np * N2; i += nth) {
-      const int p = i / N2, z = i - p * N2;
-      const C *col = lds + p * Cfg::PLANE + z;
-      C x[N1];
-#pragma unroll
-      for (int y = 0; y < N1; ++y) x[y] = col[y * (N2 + 1)];
-      fft_line<N1, +1, C>::run(x);
-      GAS C *out = t1 + ((size_t)(s0 + p) * B.nslot + slot) * (N1 * N2) + z;
-#pragma unroll
-      for (int y = 0; y < N1; ++y) out[y * N2] = x[y];
-    }
-    __syncthreads();
-  }
-}
-
-template <typename T, int N1, int N2>
-__global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                       int total, T c1r, T c1i, T c2r, T c2i) {
-  using C = cplx<T>;
-  using Cfg = PlaneCfg<T, N1, N2>;
-  constexpr int P = Cfg::P;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  C *lds = reinterpret_cast<C *>(smem_raw);
-  const int tid = threadIdx.x, nth = blockDim.x;
-  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
-    const int b = find_block(prefix, nblocks, wg);
-    const SphereBlock<T> B = blocks[b];
-    const int local = wg - gp(prefix)[b];
-    const int ngrp = (B.nst + P - 1) / P;
-    const int slot = local / ngrp;
-    const int s0 = (local - slot * ngrp) * P;
-    const int np = min(P, B.nst - s0);
-    const V4<int> sg = ld4<int>(B.slot_seg, slot);
-    const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
-    if (nseg == 0) continue;  // uniform: nothing of this plane is on the sphere
-    const GAS C *t1 = gp(B.t1);
-    for (int i = tid; i < np * N2; i += nth) {
-      const int p = i / N2, z = i - p * N2;
-      const GAS C *in = t1 + ((size_t)(s0 + p) * B.nslot + slot) * (N1 * N2) + z;
-      C x[N1];
-#pragma unroll
-      for (int y = 0; y < N1; ++y) x[y] = in[y * N2];
-      fft_line<N1, -1, C>::run(x);
-      C *col = lds + p * Cfg::PLANE + z;
-#pragma unroll
-      for (int y = 0; y < N1; ++y) col[y * (N2 + 1)] = x[y];
-    }
-    __syncthreads();
-    const GAS int *rows = gp(B.rows);
-    for (int i = tid; i < np * nrow; i += nth) {
-      const int p = i / nrow;
-      const int y = rows[r0 + i - p * nrow];
-      C *row = lds + p * Cfg::PLANE + y * (N2 + 1);
-      C x[N2];
-#pragma unroll
-      for (int z = 0; z < N2; ++z) x[z] = row[z];
-      fft_line<N2, -1, C>::run(x);
-#pragma unroll
-      for (int z = 0; z < N2; ++z) row[z] = x[z];
-    }
-    __syncthreads();
-    const GAS C *psi = gp(B.psi);
-    GAS C *hpsi = gw(B.hpsi);
-    const GAS int *pwyz = gp(B.pw_yz);
-    const GAS T *kin = gp(B.kin);
-    const int tot = np * nseg;
-    for (int i0 = tid; i0 < tot; i0 += 4 * nth) {
-      // four sphere points per thread and step: loads first, then the updates
-      size_t oo[4];
-      C outv[4], cv[4];
-      T kv[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = min(i0 + u * nth, tot - 1);
-        const int p = i / nseg, j = i - p * nseg;
-        const int yz = pwyz[seg0 + j];
-        const int y = yz / N2, z = yz - y * N2;
-        oo[u] = (size_t)(s0 + p) * B.npw + seg0 + j;
-        outv[u] = lds[p * Cfg::PLANE + y * (N2 + 1) + z];
-        if (B.kin) {
-          cv[u] = psi[oo[u]];
-          kv[u] = kin[seg0 + j];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-      if (i0 + u * nth >= tot) continue;
-      const size_t o = oo[u];
-      C out = outv[u];
-      if (B.kin) {  // H application: + 1/2|G+k|^2 psi; without it a plain masked forward FFT
-        const C c = cv[u];
-        const T kf = kv[u];
-        out.x += kf * c.x;
-        out.y += kf * c.y;
-      }
-      if (B.write_h) {
-        if (B.accumulate) {
-          C q = hpsi[o];
-          q.x += out.x;
-          q.y += out.y;
-          hpsi[o] = q;
-        } else {
-          hpsi[o] = out;
-        }
-      }
-      if (B.acc1) {
-        GAS C *a = gw(B.acc1) + o;
-        C q = *a;
-        q.x += c1r * out.x - c1i * out.y;
-        q.y += c1r * out.y + c1i * out.x;
-        *a = q;
-      }
-      if (B.acc2) {
-        GAS C *a = gw(B.acc2) + o;
-        C q = *a;
-        q.x += c2r * out.x - c2i * out.y;
-        q.y += c2r * out.y + c2i * out.x;
-        *a = q;
-      }
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// -------------------------------------------------------------------------------------------
-// x passes: one lane per line (i1,i2), 64 consecutive lines per wave, one state per wave
-// -------------------------------------------------------------------------------------------
-
-// symmetric plane index of FFT index i (compile time)
-template <int N> constexpr int sym_of(int i) { return i >= (N + 1) / 2 ? i - N : i; }
-
 // ---- raw buffer access: per-lane byte offset in a VGPR, wave-uniform part in an SGPR
 // the base and size of a buffer resource are wave-uniform by construction;
 // readfirstlane makes that explicit so the resource stays in SGPRs
@@ -342,6 +143,290 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, 
 
 // a per-lane buffer offset beyond every record: loads return 0, stores are dropped
 constexpr unsigned kOOB = 0x80000000u;
+
+// FFT of one LDS row of N values split over the lane pair; lanes with ok ==
+// false transform a copy of a valid row and store nothing
+template <typename C, int N, int SIGN>
+__device__ __forceinline__ void plane_row_split(C *row, bool upper, bool ok) {
+  constexpr int H = N / 2;
+  C v[H];
+#pragma unroll
+  for (int n = 0; n < H; ++n) v[n] = row[2 * n + (upper ? 1 : 0)];
+  fft_split<N, SIGN, C>::dit(v, upper);
+  __builtin_amdgcn_wave_barrier();
+  if (ok) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) row[k + (upper ? H : 0)] = v[k];
+  }
+}
+
+// In the inverse pass every row and column of a plane is split over a lane
+// pair (lane, lane + 32): a wave transforms 32 lines; P planes share a
+// work-group.  The forward pass does the same for long lines (SPLIT_F) and
+// gives each line one lane otherwise.
+template <typename T, int N1, int N2> struct PlaneCfg {
+  static_assert(N1 % 2 == 0 && N2 % 2 == 0, "split lines need even lengths");
+  static constexpr int NL = (N1 > N2) ? N1 : N2;
+  static constexpr int PLANE = N1 * (N2 + 1);  // padded row stride N2+1
+  static constexpr int PLANE_BYTES = PLANE * (int)sizeof(cplx<T>);
+  static constexpr int P0 = 128 / NL > 0 ? 128 / NL : 1;  // 128 lines per 256 threads
+  static constexpr int PL = (65536 / PLANE_BYTES) > 0 ? (65536 / PLANE_BYTES) : 1;
+  static constexpr int P = (P0 < PL ? P0 : PL) > 0 ? (P0 < PL ? P0 : PL) : 1;
+  static constexpr int WAVES = (P * NL + 31) / 32 < 4 ? (P * NL + 31) / 32 : 4;
+  static constexpr int THREADS = 64 * WAVES;  // split lines
+  // forward pass: lines split when a whole line would hold more than 72
+  // complex64 values (complex128 from 40 points, complex64 above 72); measured
+  // on the 72^3 sphere: complex128 522 -> 417 us split, complex64 186 -> 192 us
+  static constexpr bool SPLIT_F = (int)sizeof(T) * NL > 4 * 72;
+  static constexpr int THREADS_F = SPLIT_F ? THREADS : (P * NL + 63) / 64 * 64;
+  static constexpr int LDS = P * PLANE_BYTES;
+};
+
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
+                                                       int total) {
+  using C = cplx<T>;
+  using Cfg = PlaneCfg<T, N1, N2>;
+  constexpr int P = Cfg::P;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C *lds = reinterpret_cast<C *>(smem_raw);
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool upper = lane >= 32;
+  const int half = upper ? 1 : 0;
+  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+    const int b = find_block(prefix, nblocks, wg);
+    const SphereBlock<T> B = blocks[b];
+    const int local = wg - gp(prefix)[b];
+    const int ngrp = (B.nst + P - 1) / P;
+    const int slot = local / ngrp;
+    const int s0 = (local - slot * ngrp) * P;
+    const int np = min(P, B.nst - s0);
+    const V4<int> sg = ld4<int>(B.slot_seg, slot);
+    const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
+    for (int i = tid; i < np * Cfg::PLANE; i += nth) lds[i] = C{0, 0};
+    __syncthreads();
+    const GAS C *psi = gp(B.psi);
+    const GAS int *pwyz = gp(B.pw_yz);
+    // four sphere points per thread and step: their loads are in flight together
+    const int tot = np * nseg;
+    for (int i0 = tid; i0 < tot; i0 += 4 * nth) {
+      int ldx[4];
+      C val[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = min(i0 + u * nth, tot - 1);
+        const int p = i / nseg, j = i - p * nseg;
+        const int yz = pwyz[seg0 + j];
+        const int y = yz / N2, z = yz - y * N2;
+        ldx[u] = p * Cfg::PLANE + y * (N2 + 1) + z;
+        val[u] = psi[(size_t)(s0 + p) * B.npw + seg0 + j];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (i0 + u * nth < tot) lds[ldx[u]] = val[u];
+    }
+    __syncthreads();
+    // rows along i2 (sphere rows only), a lane pair per row
+    const GAS int *rows = gp(B.rows);
+    for (int g = wave; g * 32 < np * nrow; g += Cfg::WAVES) {
+      const int li = g * 32 + (lane & 31);
+      const bool ok = li < np * nrow;
+      const int lc = ok ? li : np * nrow - 1;
+      const int p = lc / nrow;
+      const int y = rows[r0 + lc - p * nrow];
+      plane_row_split<C, N2, +1>(lds + p * Cfg::PLANE + y * (N2 + 1), upper, ok);
+    }
+    __syncthreads();
+    // columns along i1 to the plane slots, a lane pair per column; the np
+    // planes of this work-group are slot `slot` of states s0 .. s0+np-1
+    const size_t pstride = (size_t)B.nslot * (N1 * N2);  // between the states' planes
+    const auto rt = make_rsrc(B.t1 + (size_t)s0 * pstride + (size_t)slot * (N1 * N2),
+                              (unsigned)(((size_t)(np - 1) * pstride + N1 * N2) * sizeof(C)));
+    for (int g = wave; g * 32 < np * N2; g += Cfg::WAVES) {
+      const int li = g * 32 + (lane & 31);
+      const bool ok = li < np * N2;
+      const int lc = ok ? li : np * N2 - 1;
+      const int p = lc / N2, z = lc - p * N2;
+      const C *col = lds + p * Cfg::PLANE + z;
+      C v[N1 / 2];
+#pragma unroll
+      for (int n = 0; n < N1 / 2; ++n) v[n] = col[(2 * n + half) * (N2 + 1)];
+      fft_split<N1, +1, C>::dit(v, upper);
+      // lanes past the last column store out of range (dropped), no branch
+      const unsigned vo = ok ? (unsigned)(((size_t)p * pstride + z + half * (N1 / 2) * N2) * sizeof(C)) : kOOB;
+#pragma unroll
+      for (int k = 0; k < N1 / 2; ++k) bstore(rt, vo, (unsigned)(k * N2 * sizeof(C)), v[k]);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
+                                                       int total, T c1r, T c1i, T c2r, T c2i) {
+  using C = cplx<T>;
+  using Cfg = PlaneCfg<T, N1, N2>;
+  constexpr int P = Cfg::P;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C *lds = reinterpret_cast<C *>(smem_raw);
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool upper = lane >= 32;
+  const int half = upper ? 1 : 0;
+  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+    const int b = find_block(prefix, nblocks, wg);
+    const SphereBlock<T> B = blocks[b];
+    const int local = wg - gp(prefix)[b];
+    const int ngrp = (B.nst + P - 1) / P;
+    const int slot = local / ngrp;
+    const int s0 = (local - slot * ngrp) * P;
+    const int np = min(P, B.nst - s0);
+    const V4<int> sg = ld4<int>(B.slot_seg, slot);
+    const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
+    if (nseg == 0) continue;  // uniform: nothing of this plane is on the sphere
+    if constexpr (Cfg::SPLIT_F) {
+      // columns along i1 from the plane slots, then the sphere rows along i2,
+      // a lane pair per line; the np planes are slot `slot` of states s0..s0+np-1
+      const size_t pstride = (size_t)B.nslot * (N1 * N2);
+      const auto rt = make_rsrc(B.t1 + (size_t)s0 * pstride + (size_t)slot * (N1 * N2),
+                                (unsigned)(((size_t)(np - 1) * pstride + N1 * N2) * sizeof(C)));
+      for (int g = wave; g * 32 < np * N2; g += Cfg::WAVES) {
+        const int li = g * 32 + (lane & 31);
+        const bool ok = li < np * N2;
+        const int lc = ok ? li : np * N2 - 1;
+        const int p = lc / N2, z = lc - p * N2;
+        const unsigned vo = (unsigned)(((size_t)p * pstride + z + half * N2) * sizeof(C));
+        C v[N1 / 2];
+#pragma unroll
+        for (int n = 0; n < N1 / 2; ++n) bload(rt, vo, (unsigned)(2 * n * N2 * sizeof(C)), v[n]);
+        fft_split<N1, -1, C>::dit(v, upper);
+        C *col = lds + p * Cfg::PLANE + z;
+        if (ok) {
+#pragma unroll
+          for (int k = 0; k < N1 / 2; ++k) col[(k + half * (N1 / 2)) * (N2 + 1)] = v[k];
+        }
+      }
+      __syncthreads();
+      const GAS int *rows = gp(B.rows);
+      for (int g = wave; g * 32 < np * nrow; g += Cfg::WAVES) {
+        const int li = g * 32 + (lane & 31);
+        const bool ok = li < np * nrow;
+        const int lc = ok ? li : np * nrow - 1;
+        const int p = lc / nrow;
+        const int y = rows[r0 + lc - p * nrow];
+        plane_row_split<C, N2, -1>(lds + p * Cfg::PLANE + y * (N2 + 1), upper, ok);
+      }
+      __syncthreads();
+    } else {
+      // the same with one lane per line
+      const GAS C *t1 = gp(B.t1);
+      for (int i = tid; i < np * N2; i += nth) {
+        const int p = i / N2, z = i - p * N2;
+        const GAS C *in = t1 + ((size_t)(s0 + p) * B.nslot + slot) * (N1 * N2) + z;
+        C x[N1];
+#pragma unroll
+        for (int y = 0; y < N1; ++y) x[y] = in[y * N2];
+        fft_line<N1, -1, C>::run(x);
+        C *col = lds + p * Cfg::PLANE + z;
+#pragma unroll
+        for (int y = 0; y < N1; ++y) col[y * (N2 + 1)] = x[y];
+      }
+      __syncthreads();
+      const GAS int *rows = gp(B.rows);
+      for (int i = tid; i < np * nrow; i += nth) {
+        const int p = i / nrow;
+        const int y = rows[r0 + i - p * nrow];
+        C *row = lds + p * Cfg::PLANE + y * (N2 + 1);
+        C x[N2];
+#pragma unroll
+        for (int z = 0; z < N2; ++z) x[z] = row[z];
+        fft_line<N2, -1, C>::run(x);
+#pragma unroll
+        for (int z = 0; z < N2; ++z) row[z] = x[z];
+      }
+      __syncthreads();
+    }
+    // keep the gather's index arithmetic out of the transform phases' live range
+    __builtin_amdgcn_sched_barrier(0);
+    // the sphere values of this work-group's states s0 .. s0+np-1 through
+    // buffer resources based at state s0: one 32-bit offset per point, and
+    // points past the end use an offset beyond the buffer (loads give zero,
+    // stores are dropped) instead of a branch
+    const size_t base = (size_t)s0 * B.npw;
+    const unsigned nb = (unsigned)((size_t)np * B.npw * sizeof(C));
+    const auto rpsi = make_rsrc(B.psi + base, nb);
+    const auto rh = make_rsrc(B.hpsi + base, nb);
+    const auto ra1 = make_rsrc(B.acc1 ? B.acc1 + base : B.hpsi + base, B.acc1 ? nb : 0u);
+    const auto ra2 = make_rsrc(B.acc2 ? B.acc2 + base : B.hpsi + base, B.acc2 ? nb : 0u);
+    const auto rk = make_rsrc(B.kin ? (const void *)B.kin : (const void *)B.psi,
+                              B.kin ? (unsigned)(B.npw * sizeof(T)) : 0u);
+    const GAS int *pwyz = gp(B.pw_yz);
+    const int tot = np * nseg;
+    for (int i0 = tid; i0 < tot; i0 += 4 * nth) {
+      // four sphere points per thread and step: loads first, then the updates
+      unsigned oo[4];
+      C outv[4], cv[4];
+      T kv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * nth;
+        const int ic = i < tot ? i : tot - 1;
+        const int p = ic / nseg, j = ic - p * nseg;
+        const int yz = pwyz[seg0 + j];
+        const int y = yz / N2, z = yz - y * N2;
+        oo[u] = i < tot ? (unsigned)(((size_t)p * B.npw + seg0 + j) * sizeof(C)) : kOOB;
+        outv[u] = lds[p * Cfg::PLANE + y * (N2 + 1) + z];
+        if (B.kin) {
+          bload(rpsi, oo[u], 0u, cv[u]);
+          bload(rk, (unsigned)((seg0 + j) * sizeof(T)), 0u, kv[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        C out = outv[u];
+        if (B.kin) {  // H application: + 1/2|G+k|^2 psi; without it a plain masked forward FFT
+          out.x += kv[u] * cv[u].x;
+          out.y += kv[u] * cv[u].y;
+        }
+        if (B.write_h) {
+          if (B.accumulate) {
+            C q;
+            bload(rh, oo[u], 0u, q);
+            q.x += out.x;
+            q.y += out.y;
+            bstore(rh, oo[u], 0u, q);
+          } else {
+            bstore(rh, oo[u], 0u, out);
+          }
+        }
+        if (B.acc1) {
+          C q;
+          bload(ra1, oo[u], 0u, q);
+          q.x += c1r * out.x - c1i * out.y;
+          q.y += c1r * out.y + c1i * out.x;
+          bstore(ra1, oo[u], 0u, q);
+        }
+        if (B.acc2) {
+          C q;
+          bload(ra2, oo[u], 0u, q);
+          q.x += c2r * out.x - c2i * out.y;
+          q.y += c2r * out.y + c2i * out.x;
+          bstore(ra2, oo[u], 0u, q);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// -------------------------------------------------------------------------------------------
+// x passes: one lane per line (i1,i2), 64 consecutive lines per wave, one state per wave
+// -------------------------------------------------------------------------------------------
+
+// symmetric plane index of FFT index i (compile time)
+template <int N> constexpr int sym_of(int i) { return i >= (N + 1) / 2 ? i - N : i; }
+
 
 // A line (i1,i2) along i0 is shared by a lane pair (lane, lane + 32): the
 // lower lane holds i0 in [0, N0/2), the upper lane [N0/2, N0) after an
@@ -1150,7 +1235,7 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
     INQX_HIP(hipGetLastError());
     {
       ProfScope ps(cx, f64 ? "plane_forward_f64" : "plane_forward_f32");
-      hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), Cfg::LDS, st, blk,
+      hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS_F), Cfg::LDS, st, blk,
                          pref_plane, nblocks, ld.total_plane, (T)0, (T)0, (T)0, (T)0);
     }
     INQX_HIP(hipGetLastError());
@@ -1196,7 +1281,7 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   INQX_HIP(hipGetLastError());
   {
     ProfScope ps(cx, f64 ? "plane_forward_f64" : "plane_forward_f32");
-    hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), Cfg::LDS, st, blk,
+    hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS_F), Cfg::LDS, st, blk,
                        pref_plane, nblocks, ld.total_plane, (T)cf[0], (T)cf[1], (T)cf[2], (T)cf[3]);
   }
   INQX_HIP(hipGetLastError());
