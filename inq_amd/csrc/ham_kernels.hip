@@ -706,13 +706,15 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
     const bool act = s < B.nst;
     C v[H];
     if (act) {
+      // the line's loads go out first; the projection staging below overlaps them
+      load_line_slots_split<T, N0>(B, hd.n12, s, Lc, upper, v);
       if (npts > 0) {
         const GAS C *dp = gp(B.dp);
         for (int i = ptid; i < nsl * kMaxNlm; i += 64)
           dpl[i] = dp[((size_t)gp(hd.slot_atom)[slot_base + i / kMaxNlm] * kMaxNlm + (i % kMaxNlm)) * B.nst + s];
         for (int i = ptid; i < npts; i += 64) pts[i] = C{0, 0};
       }
-      load_line_inverse_split<T, N0>(B, hd.n12, s, Lc, upper, v);
+      fft_split<N0, +1, C>::dit(v, upper);
       if (B.dip) {
         // sum |u|^2 and sum f0 |u|^2 along the lane's half line, f0 = sym(i0)/n0
         double s0 = 0.0, s1 = 0.0;
@@ -756,17 +758,26 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
     }
     __syncthreads();
     if (act && npts > 0) {
-      for (int e = ent_base + ptid; e < ent_base + nent; e += 64) {
-        const V2<int> cs = ld2<int>(hd.ent_cs, e);
-        const V4<T> bt = ld4<T>(hd.ent_b, e);
-        const C *dd = dpl + cs.y * kMaxNlm;
-        C t;
-        t.x = bt.x * dd[0].x + bt.y * dd[1].x + bt.z * dd[2].x + bt.w * dd[3].x;
-        t.y = bt.x * dd[0].y + bt.y * dd[1].y + bt.z * dd[2].y + bt.w * dd[3].y;
-        if (!B.gamma) t = cmul(t, kb_phase<T>(B.kcov, ld4<float>(hd.ent_pos, e), (T)-1));
-        T *q = reinterpret_cast<T *>(pts + cs.x);
-        lds_add(q + 0, t.x);
-        lds_add(q + 1, t.y);
+      // two entries per lane and step: both table loads are in flight together
+      const int eend = ent_base + nent;
+      for (int e = ent_base + ptid; e < eend; e += 128) {
+        const int e1 = e + 64 < eend ? e + 64 : e;  // clamped: the second add is skipped below
+        const V2<int> cs0 = ld2<int>(hd.ent_cs, e), cs1 = ld2<int>(hd.ent_cs, e1);
+        const V4<T> bt0 = ld4<T>(hd.ent_b, e), bt1 = ld4<T>(hd.ent_b, e1);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          if (u == 1 && e + 64 >= eend) break;
+          const V2<int> cs = u ? cs1 : cs0;
+          const V4<T> bt = u ? bt1 : bt0;
+          const C *dd = dpl + cs.y * kMaxNlm;
+          C t;
+          t.x = bt.x * dd[0].x + bt.y * dd[1].x + bt.z * dd[2].x + bt.w * dd[3].x;
+          t.y = bt.x * dd[0].y + bt.y * dd[1].y + bt.z * dd[2].y + bt.w * dd[3].y;
+          if (!B.gamma) t = cmul(t, kb_phase<T>(B.kcov, ld4<float>(hd.ent_pos, u ? e1 : e), (T)-1));
+          T *q = reinterpret_cast<T *>(pts + cs.x);
+          lds_add(q + 0, t.x);
+          lds_add(q + 1, t.y);
+        }
       }
     }
     __syncthreads();
