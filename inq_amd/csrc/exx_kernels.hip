@@ -351,14 +351,13 @@ template <typename T, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
   for (int c = 0; c < 3; ++c) d.q[c] = (T)a.gshift[c];
   d.scal = (T)(4.0 * M_PI / (double)g.N);
   d.zero = (T)(a.zeroterm / (double)g.N);
-  static bool attr = false;
-  if (!attr && Cfg::LDS > 65536) {
+  static DeviceOnce attr;
+  if (Cfg::LDS > 65536 && !attr.test_and_set(cx.device)) {
     INQX_HIP(hipFuncSetAttribute((const void *)k_pair_plane_fwd<T, S, S, S>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
     INQX_HIP(hipFuncSetAttribute((const void *)k_pair_plane_inv<T, S, S, S>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
   }
-  attr = true;
   const bool f64 = std::is_same<T, double>::value;
   const int tot1 = d.nj * S * d.nst;
   {

@@ -1202,25 +1202,27 @@ __global__ void __launch_bounds__(256) k_line_from_real(const SphereBlock<T> *bl
   }
 }
 
-template <typename T, int N1, int N2> static void set_plane_attr() {
-  static bool done = false;
-  if (done) return;
+template <typename T, int N1, int N2> static void set_plane_attr(int device) {
+  static DeviceOnce once;
   using Cfg = PlaneCfg<T, N1, N2>;
-  if (Cfg::LDS > 65536) {
-    INQX_HIP(hipFuncSetAttribute((const void *)k_plane_inverse<T, N1, N2>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
-    INQX_HIP(hipFuncSetAttribute((const void *)k_plane_forward<T, N1, N2>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
-  }
-  done = true;
+  if (Cfg::LDS <= 65536 || once.test_and_set(device)) return;
+  INQX_HIP(hipFuncSetAttribute((const void *)k_plane_inverse<T, N1, N2>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
+  INQX_HIP(hipFuncSetAttribute((const void *)k_plane_forward<T, N1, N2>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
 }
 
-template <typename T, int N0> static void set_x_attr(int lds) {
-  static int done = 65536;
-  if (lds <= done) return;
+// the x-pass LDS depends on the KB tables: the attribute grows to the largest
+// size requested on each device
+template <typename T, int N0> static void set_x_attr(int device, int lds) {
+  static int done[64];
+  static DeviceOnce init;
+  int &d = done[device & 63];
+  if (!init.test_and_set(device)) d = 65536;
+  if (lds <= d) return;
   INQX_HIP(hipFuncSetAttribute((const void *)k_xpass_project<T, N0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   INQX_HIP(hipFuncSetAttribute((const void *)k_xpass_apply<T, N0>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  done = lds;
+  d = lds;
 }
 
 template <typename T, int N0, int N12>
@@ -1231,7 +1233,7 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   const int *pref_plane = reinterpret_cast<const int *>(ld.desc.as<char>() + ld.off_pref[0]);
   const int *pref_x = reinterpret_cast<const int *>(ld.desc.as<char>() + ld.off_pref[1]);
   const int *pref_red = reinterpret_cast<const int *>(ld.desc.as<char>() + ld.off_pref[2]);
-  set_plane_attr<T, N12, N12>();
+  set_plane_attr<T, N12, N12>(h->ctx->c.device);
   const int cu = h->ctx->c.num_cu;
   const int grid_plane = ld.total_plane;  // one item per work-group
   Ctx &cx = h->ctx->c;
@@ -1268,7 +1270,7 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   }
   const int xthreads = 64 * ld.nwave;  // one wave per (tile, state)
   const int xlds = ld.nwave * ld.lds_per_wave * (int)sizeof(cplx<T>);
-  set_x_attr<T, N0>(xlds);
+  set_x_attr<T, N0>(h->ctx->c.device, xlds);
   const int grid_x = ld.total_x;
   if (hd.has_kb) {
     {

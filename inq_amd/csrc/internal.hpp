@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -34,6 +36,16 @@ struct Error {
   } while (0)
 
 // ------------------------------------------------------------------ device buffers
+// one-time per-device settings (function attributes are set for the current
+// device): test_and_set(dev) is false the first time for a device
+struct DeviceOnce {
+  std::atomic<unsigned long long> bits{0};
+  bool test_and_set(int dev) {
+    const unsigned long long m = 1ull << (dev & 63);
+    return (bits.fetch_or(m) & m) != 0;
+  }
+};
+
 struct DevMem {
   void *p = nullptr;
   size_t bytes = 0;
