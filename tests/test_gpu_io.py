@@ -77,3 +77,28 @@ def test_missing_state_file_is_a_status(ctx, setup, tmp_path):
     os.remove(tmp_path / "s" / f"{3:010d}_{0:010d}")
     with pytest.raises(InqxError):
         inqio.load_states(gh, b, d, tmp_path / "s")
+
+
+def test_states_round_trip_across_transform_chunks(ctx, tmp_path):
+    """96^3: the files are written and read through 256 MiB chunks of real-space
+    states (18 complex128 states per chunk), so 19 states cross a chunk boundary"""
+    from inq_amd import KSHamiltonian
+    say("case: states save/load across chunks, 96^3, 19 states")
+    n = (96, 96, 96)
+    lat = np.diag([12.0, 12.0, 12.0])
+    gh = KSHamiltonian(ctx, lat, n)
+    gh.set_scalar_potential(np.zeros(int(np.prod(n))))
+    b = gh.basis((0, 0, 0), 8.0)
+    rng = np.random.default_rng(11)
+    c = rng.standard_normal((19, b.npw)) + 1j * rng.standard_normal((19, b.npw))
+    d = DeviceArray.from_numpy(ctx, c)
+    inqio.save_states(gh, b, d, tmp_path / "s", first_state=5)
+    assert len(os.listdir(tmp_path / "s")) == 19
+    back = DeviceArray(ctx, c.shape, np.complex128)
+    inqio.load_states(gh, b, back, tmp_path / "s", first_state=5)
+    assert rel_err(back.numpy(), c) < 1e-13
+    # the last file holds state 18 (the second chunk's only state)
+    last = np.fromfile(tmp_path / "s" / f"{5 + 18:010d}_{0:010d}", dtype=np.complex128)
+    one = DeviceArray.from_numpy(ctx, c[18:19].copy())
+    ref = gh.to_real(b, one, layout=0).numpy()[0]
+    assert np.array_equal(last, ref)
