@@ -553,9 +553,12 @@ __global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blo
     if (active) {
       const bool valid = L < hd.n12;
       C v[H];
-      load_line_inverse_split<T, N0>(B, hd.n12, s, valid ? L : hd.n12 - 1, upper, v);
+      // line loads, then the line's point mask, then the transform: both loads in flight together
+      load_line_slots_split<T, N0>(B, hd.n12, s, valid ? L : hd.n12 - 1, upper, v);
+      const V4<uint32_t> w = ld4<uint32_t>(hd.line_mask, valid ? L : hd.n12 - 1);
+      for (int i = ptid; i < nsl * kMaxNlm; i += 64) acc[i] = C{0, 0};
+      fft_split<N0, +1, C>::dit(v, upper);
       if (valid) {
-        const V4<uint32_t> w = ld4<uint32_t>(hd.line_mask, L);
         const uint32_t m[3] = {w.x, w.y, w.z};
         int cnt = (int)w.w + (upper ? mask_count_low<N0>(m) : 0);
 #pragma unroll
@@ -566,7 +569,6 @@ __global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blo
           }
         }
       }
-      for (int i = ptid; i < nsl * kMaxNlm; i += 64) acc[i] = C{0, 0};
     }
     __syncthreads();
     if (active) {
