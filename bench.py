@@ -16,6 +16,8 @@ Secondary lines in the same JSON object (same seeded inputs):
   c1_diamond    C8 diamond apply + overlap + Cholesky, GPU and CPU oracle
   c5_ace        Si216 (96^3, 432 states) ACE build: all pair-density Poisson solves,
                 W^H phi, FP64 Cholesky, xi; ACE apply; FP32 vs FP64 direct exchange
+  contractions  overlap and rotate on the matrix cores at the Si64 and Si216 block
+                sizes, TFLOP/s and fraction of the dense MFMA peak
 The CPU baseline is the FP64 oracle (a C/OpenMP port of the reference CPU data
 flow) on a bounded subset of the Si64 states.
 
@@ -242,6 +244,47 @@ def c1_diamond(ctx, cpu):
     return res
 
 
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}  # MI355X dense matrix-core peaks
+
+
+def contractions(ctx, reps=10):
+    """The dense contractions on the matrix cores: the overlap S = vol phi^H phi
+    (operations::overlap) and the rotation phi R (operations::rotate) at the
+    Si64 block (128 x 67 691) and the Si216 block (432 x 228 487).  8 n^2 K flop
+    each (complex multiply-add), timed per kernel with the library's HIP events;
+    utilisation is against the dense FP32 / FP64 MFMA peak."""
+    from inq_amd import DeviceArray
+    from inq_amd import operations as ops
+    out = {}
+    rng = np.random.default_rng(3)
+    for name, n, K in (("si64", 128, 67691), ("si216", 432, 228487)):
+        for dtype, prec in ((np.complex64, "fp32"), (np.complex128, "fp64")):
+            a = (rng.standard_normal((n, K)) + 1j * rng.standard_normal((n, K))).astype(dtype)
+            d = DeviceArray.from_numpy(ctx, a)
+            rot = DeviceArray.from_numpy(ctx, np.eye(n, dtype=np.complex128))
+            ops.overlap(ctx, d, d, 1.0).free()
+            ops.rotate(ctx, rot, d)
+            ctx.sync()
+            ctx.profile(True)
+            ctx.profile_reset()
+            for _ in range(reps):
+                ops.overlap(ctx, d, d, 1.0).free()
+                ops.rotate(ctx, rot, d)
+            kt = ctx.kernel_times()
+            ctx.profile(False)
+            flop = 8.0 * n * n * K
+            res = {}
+            sfx = "f32" if prec == "fp32" else "f64"
+            for kind, key in (("overlap", f"overlap_mfma_{sfx}"), ("rotate", f"rotate_mfma_{sfx}")):
+                cnt, ms = kt[key]
+                tf = flop / (ms / cnt * 1e-3) / 1e12
+                res[kind] = {"us": ms / cnt * 1e3, "tflops": tf, "frac_of_mfma_peak": tf / MFMA_PEAK_TFLOPS[prec]}
+            out[f"{name}_{prec}"] = res
+            d.free()
+            rot.free()
+    return out
+
+
 def c5_ace(ctx, cpu, cpu_seconds=8.0):
     """ACE build on Si216 (96^3, 432 complex64 states, Gamma): nst^2 pair-density
     Poisson solves + W^H phi + FP64 Cholesky + xi = W L^-H, and one ACE apply."""
@@ -449,6 +492,8 @@ def main():
         log(f"bench: c1 {out['c1_diamond']}")
         out["c5_ace"] = c5_ace(ctx, not args.no_cpu)
         log(f"bench: c5 {out['c5_ace']}")
+        out["contractions"] = contractions(ctx)
+        log(f"bench: contractions {out['contractions']}")
 
     cpu = None
     if not args.no_cpu:
