@@ -279,6 +279,8 @@ def contractions(ctx, reps=10):
                 cnt, ms = kt[key]
                 tf = flop / (ms / cnt * 1e-3) / 1e12
                 res[kind] = {"us": ms / cnt * 1e3, "tflops": tf, "frac_of_mfma_peak": tf / MFMA_PEAK_TFLOPS[prec]}
+            cnt, ms = kt[f"overlap_reduce_{sfx}"]
+            res["overlap"]["reduce_us"] = ms / cnt * 1e3
             out[f"{name}_{prec}"] = res
             d.free()
             rot.free()
@@ -382,6 +384,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--copy", action="store_true", help="run the HBM copy reference even with --no-extras")
     ap.add_argument("--only-c5", action="store_true", help="run only the Si216 ACE line (profiling)")
+    ap.add_argument("--only-contractions", action="store_true", help="run only the overlap / rotate timing")
     ap.add_argument("--c4-steps", type=int, default=2000, help="ETRS steps of the Si64 TDDFT line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -395,6 +398,9 @@ def main():
     if args.only_c5:
         res = c5_ace(Context(0), not args.no_cpu)
         print(json.dumps({"c5_ace": res}), flush=True)
+        return
+    if args.only_contractions:
+        print(json.dumps({"contractions": contractions(Context(0))}), flush=True)
         return
 
     sysm = S.config(2)

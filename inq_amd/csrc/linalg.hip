@@ -41,11 +41,14 @@ struct alignas(16) V4f {
   float x, y, z, w;
 };
 typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+typedef double doublex2 __attribute__((ext_vector_type(2)));
 
 // row of the 16x16 output tile held in accumulator r of a lane (column = lane % 16)
 template <typename T> struct mfma_t;
 template <> struct mfma_t<float> {
   using acc = floatx4;
+  using pair = floatx2;
   static __device__ __forceinline__ acc run(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -53,6 +56,7 @@ template <> struct mfma_t<float> {
 };
 template <> struct mfma_t<double> {
   using acc = doublex4;
+  using pair = doublex2;
   static __device__ __forceinline__ acc run(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -62,19 +66,34 @@ template <> struct mfma_t<double> {
 // ------------------------------------------------------------------------ overlap
 constexpr int kOvTile = 64;   // output tile edge per work-group
 constexpr int kOvKC = 32;     // real K per LDS stage
-constexpr int kOvChunk = 2048;  // real K per work-group (split K)
+constexpr int kOvChunkMax = 2048;  // real K per work-group (split K), upper bound
+constexpr int kOvChunkMin = 256;   // lower bound when few output tiles call for more chunks
+constexpr int kOvGridTarget = 1024;  // work-groups that keep every CU busy with 4 of them
 
-// partial[chunk][tile_i][tile_j] holds a 64x64 complex tile in T
+// real K per work-group: the full 2048 when the output has enough tiles, else
+// shorter chunks (a multiple of the LDS stage) so that the grid fills the chip
+static int overlap_chunk(long K2, int tiles) {
+  const long want = std::max<long>(1, (kOvGridTarget + tiles - 1) / tiles);
+  long c = (K2 + want - 1) / want;
+  c = (c + kOvKC - 1) / kOvKC * kOvKC;
+  return (int)std::min<long>(kOvChunkMax, std::max<long>(kOvChunkMin, c));
+}
+
+// partial[chunk][tile_i][tile_j] holds a 64x64 complex tile in T.  Each stage
+// loads (re, im) pairs (rows start at even real offsets, so pairs are aligned),
+// and the next stage's pairs are loaded into registers while the matrix cores
+// work on the current one.
 template <typename T>
 __global__ void __launch_bounds__(256) k_overlap_partial(const T *A, const T *B, int n1, int n2, long K2,
-                                                         int tiles_i, int tiles_j, T *partial) {
+                                                         int tiles_i, int tiles_j, int kchunk, T *partial) {
   using M = mfma_t<T>;
+  using P2 = typename M::pair;
   __shared__ T At[kOvTile][kOvKC + 1];
   __shared__ T Bt[kOvTile][kOvKC + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int tj = blockIdx.x % tiles_j, ti = (blockIdx.x / tiles_j) % tiles_i, chunk = blockIdx.x / (tiles_j * tiles_i);
-  const long k_begin = (long)chunk * kOvChunk;
-  const long k_end = min(k_begin + kOvChunk, K2);
+  const long k_begin = (long)chunk * kchunk;
+  const long k_end = min(k_begin + kchunk, K2);
   const int i0 = ti * kOvTile, j0 = tj * kOvTile;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;  // wave sub-tile 32x32
   typename M::acc re[2][2], im[2][2];
@@ -86,16 +105,32 @@ __global__ void __launch_bounds__(256) k_overlap_partial(const T *A, const T *B,
       im[a][b] = typename M::acc{0, 0, 0, 0};
     }
   const GAS T *Ag = lgp(A), *Bg = lgp(B);
-  for (long kb = k_begin; kb < k_end; kb += kOvKC) {
-    // stage 64 x 32 of A and B (real index k' along rows of the complex arrays)
-    for (int e = tid; e < kOvTile * kOvKC; e += 256) {
-      const int r = e / kOvKC, c = e - r * kOvKC;
-      const long k = kb + c;
+  // stage = 64 rows x 32 real k' of A and of B = 64 x 16 pairs each, 4 pairs per lane;
+  // K2 and kchunk are even, so k < k_end implies k + 1 < k_end
+  constexpr int kPairs = kOvTile * kOvKC / 2 / 256;
+  P2 ra[kPairs], rb[kPairs];
+  auto load_stage = [&](long kb) {
+#pragma unroll
+    for (int q = 0; q < kPairs; ++q) {
+      const int e = tid + 256 * q, r = e >> 4;
+      const long k = kb + 2 * (e & 15);
       const int ia = i0 + r, jb = j0 + r;
-      At[r][c] = (ia < n1 && k < k_end) ? Ag[(long)ia * K2 + k] : (T)0;
-      Bt[r][c] = (jb < n2 && k < k_end) ? Bg[(long)jb * K2 + k] : (T)0;
+      ra[q] = (ia < n1 && k < k_end) ? *(const GAS P2 *)(Ag + (long)ia * K2 + k) : P2{0, 0};
+      rb[q] = (jb < n2 && k < k_end) ? *(const GAS P2 *)(Bg + (long)jb * K2 + k) : P2{0, 0};
+    }
+  };
+  load_stage(k_begin);
+  for (long kb = k_begin; kb < k_end; kb += kOvKC) {
+#pragma unroll
+    for (int q = 0; q < kPairs; ++q) {
+      const int e = tid + 256 * q, r = e >> 4, c = 2 * (e & 15);
+      At[r][c] = ra[q].x;
+      At[r][c + 1] = ra[q].y;
+      Bt[r][c] = rb[q].x;
+      Bt[r][c + 1] = rb[q].y;
     }
     __syncthreads();
+    if (kb + kOvKC < k_end) load_stage(kb + kOvKC);
 #pragma unroll
     for (int ks = 0; ks < kOvKC; ks += 4) {
       const int kk = ks + (lane >> 4);
@@ -131,25 +166,42 @@ __global__ void __launch_bounds__(256) k_overlap_partial(const T *A, const T *B,
       }
 }
 
-// S[i][j] = vol * sum_chunks partial, FP64, fixed chunk order
+// S[i][j] = vol * sum_chunks partial in FP64, in a fixed order: work-group = one
+// 64-wide row of one output tile; lane group g (of 4) sums chunks g, g+4, g+8, ...
+// in ascending order, and the four group sums are added g = 0..3.
+constexpr int kOvRedGroups = 4;
 template <typename T>
-__global__ void k_overlap_reduce(const T *partial, int n1, int n2, int tiles_i, int tiles_j, int nchunk, double vol,
-                                 double *S) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n1 * n2) return;
-  const int i = idx / n2, j = idx - i * n2;
-  const int ti = i / kOvTile, tj = j / kOvTile, ii = i % kOvTile, jj = j % kOvTile;
+__global__ void __launch_bounds__(256) k_overlap_reduce(const T *partial, int n1, int n2, int tiles_i, int tiles_j,
+                                                        int nchunk, double vol, double *S) {
+  using P2 = typename mfma_t<T>::pair;
+  __shared__ double red[kOvRedGroups][kOvTile][2];
+  const int jj = threadIdx.x & (kOvTile - 1), g = threadIdx.x >> 6;
+  const int ii = blockIdx.x % kOvTile, tile = blockIdx.x / kOvTile;
+  const int tj = tile % tiles_j, ti = tile / tiles_j;
+  const long tile_stride = (long)tiles_i * tiles_j * kOvTile * kOvTile;  // P2 per chunk
+  const GAS P2 *P = (const GAS P2 *)lgp(partial) + (long)tile * kOvTile * kOvTile + ii * kOvTile + jj;
   double sr = 0.0, si = 0.0;
-  const GAS T *P = lgp(partial);
-  for (int c = 0; c < nchunk; ++c) {
-    const long blk = ((long)c * tiles_i + ti) * tiles_j + tj;
-    const long o = (blk * kOvTile * kOvTile + ii * kOvTile + jj) * 2;
-    sr += (double)P[o];
-    si += (double)P[o + 1];
+#pragma unroll 8
+  for (int c = g; c < nchunk; c += kOvRedGroups) {
+    const P2 v = P[(long)c * tile_stride];
+    sr += (double)v.x;
+    si += (double)v.y;
   }
-  GAS double *Sg = lgw(S);
-  Sg[2 * idx] = vol * sr;
-  Sg[2 * idx + 1] = vol * si;
+  red[g][jj][0] = sr;
+  red[g][jj][1] = si;
+  __syncthreads();
+  const int i = ti * kOvTile + ii, j = tj * kOvTile + jj;
+  if (g == 0 && i < n1 && j < n2) {
+    double tr = red[0][jj][0], tim = red[0][jj][1];
+#pragma unroll
+    for (int q = 1; q < kOvRedGroups; ++q) {
+      tr += red[q][jj][0];
+      tim += red[q][jj][1];
+    }
+    GAS double *Sg = lgw(S);
+    Sg[2 * ((long)i * n2 + j)] = vol * tr;
+    Sg[2 * ((long)i * n2 + j) + 1] = vol * tim;
+  }
 }
 
 // ------------------------------------------------------------------------ rotate
@@ -374,30 +426,32 @@ void launch_overlap(Ctx &cx, int prec, int n1, int n2, long K, const void *A, co
                     DevMem &partial) {
   const long K2 = 2 * K;
   const int ti = (n1 + kOvTile - 1) / kOvTile, tj = (n2 + kOvTile - 1) / kOvTile;
-  const int nchunk = (int)((K2 + kOvChunk - 1) / kOvChunk);
+  const int kchunk = overlap_chunk(K2, ti * tj);
+  const int nchunk = (int)((K2 + kchunk - 1) / kchunk);
   const size_t es = prec == INQX_FP64 ? 8 : 4;
   partial.ensure(es * (size_t)nchunk * ti * tj * kOvTile * kOvTile * 2);
   const int grid = nchunk * ti * tj;
+  const int rgrid = ti * tj * kOvTile;
   if (prec == INQX_FP64) {
     {
       ProfScope ps(cx, "overlap_mfma_f64");
       hipLaunchKernelGGL(k_overlap_partial<double>, dim3(grid), dim3(256), 0, cx.stream, (const double *)A,
-                         (const double *)B, n1, n2, K2, ti, tj, partial.as<double>());
+                         (const double *)B, n1, n2, K2, ti, tj, kchunk, partial.as<double>());
     }
     INQX_HIP(hipGetLastError());
     ProfScope ps(cx, "overlap_reduce_f64");
-    hipLaunchKernelGGL(k_overlap_reduce<double>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, cx.stream,
-                       partial.as<double>(), n1, n2, ti, tj, nchunk, vol, S);
+    hipLaunchKernelGGL(k_overlap_reduce<double>, dim3(rgrid), dim3(256), 0, cx.stream, partial.as<double>(), n1, n2,
+                       ti, tj, nchunk, vol, S);
   } else {
     {
       ProfScope ps(cx, "overlap_mfma_f32");
       hipLaunchKernelGGL(k_overlap_partial<float>, dim3(grid), dim3(256), 0, cx.stream, (const float *)A,
-                         (const float *)B, n1, n2, K2, ti, tj, partial.as<float>());
+                         (const float *)B, n1, n2, K2, ti, tj, kchunk, partial.as<float>());
     }
     INQX_HIP(hipGetLastError());
     ProfScope ps(cx, "overlap_reduce_f32");
-    hipLaunchKernelGGL(k_overlap_reduce<float>, dim3((n1 * n2 + 255) / 256), dim3(256), 0, cx.stream,
-                       partial.as<float>(), n1, n2, ti, tj, nchunk, vol, S);
+    hipLaunchKernelGGL(k_overlap_reduce<float>, dim3(rgrid), dim3(256), 0, cx.stream, partial.as<float>(), n1, n2, ti,
+                       tj, nchunk, vol, S);
   }
   INQX_HIP(hipGetLastError());
 }
