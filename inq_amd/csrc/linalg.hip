@@ -49,6 +49,7 @@ template <typename T> struct mfma_t;
 template <> struct mfma_t<float> {
   using acc = floatx4;
   using pair = floatx2;
+  using v16 = floatx4;
   static __device__ __forceinline__ acc run(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -57,6 +58,7 @@ template <> struct mfma_t<float> {
 template <> struct mfma_t<double> {
   using acc = doublex4;
   using pair = doublex2;
+  using v16 = doublex2;
   static __device__ __forceinline__ acc run(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -205,65 +207,61 @@ __global__ void __launch_bounds__(256) k_overlap_reduce(const T *partial, int n1
 }
 
 // ------------------------------------------------------------------------ rotate
-// operand matrices Ap = [Mr, -Mi], Aq = [Mi, Mr] interleaved along j: [nr][2 nc] in T
+// operand matrices Ap = [Mr, -Mi], Aq = [Mi, Mr] interleaved along j: [nr_pad][ld] in T,
+// rows padded to a multiple of 64 and ld = 2 nc rounded up to 32, the padding zero
 // mode 0: M = conj(R) (Phi <- Phi R^H);  mode 1: M = alpha R^T (5-argument rotate,
 // R is [nc][nr]); mode 2: M = R
-__global__ void k_rot_prep32(const double *R, int nr, int nc, int mode, double ar, double ai, float *Ap, float *Aq) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nr * nc) return;
-  const int i = idx / nc, j = idx - i * nc;
-  const int n = nc;
-  double mr, mi;
-  if (mode == 0) {
-    mr = R[2 * ((long)i * nc + j)];
-    mi = -R[2 * ((long)i * nc + j) + 1];
-  } else if (mode == 1) {
-    const double rr = R[2 * ((long)j * nr + i)], ri = R[2 * ((long)j * nr + i) + 1];
-    mr = ar * rr - ai * ri;
-    mi = ar * ri + ai * rr;
-  } else {
-    mr = R[2 * ((long)i * nc + j)];
-    mi = R[2 * ((long)i * nc + j) + 1];
+template <typename T>
+__global__ void k_rot_prep(const double *R, int nr, int nc, int mode, double ar, double ai, int nr_pad, int ld, T *Ap,
+                           T *Aq) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int half = ld / 2;
+  if (idx >= (long)nr_pad * half) return;
+  const int i = (int)(idx / half), j = (int)(idx - (long)i * half);
+  double mr = 0.0, mi = 0.0;
+  if (i < nr && j < nc) {
+    if (mode == 0) {
+      mr = R[2 * ((long)i * nc + j)];
+      mi = -R[2 * ((long)i * nc + j) + 1];
+    } else if (mode == 1) {
+      const double rr = R[2 * ((long)j * nr + i)], ri = R[2 * ((long)j * nr + i) + 1];
+      mr = ar * rr - ai * ri;
+      mi = ar * ri + ai * rr;
+    } else {
+      mr = R[2 * ((long)i * nc + j)];
+      mi = R[2 * ((long)i * nc + j) + 1];
+    }
   }
-  Ap[(long)i * 2 * n + 2 * j] = (float)mr;
-  Ap[(long)i * 2 * n + 2 * j + 1] = (float)-mi;
-  Aq[(long)i * 2 * n + 2 * j] = (float)mi;
-  Aq[(long)i * 2 * n + 2 * j + 1] = (float)mr;
-}
-__global__ void k_rot_prep64(const double *R, int nr, int nc, int mode, double ar, double ai, double *Ap, double *Aq) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nr * nc) return;
-  const int i = idx / nc, j = idx - i * nc;
-  const int n = nc;
-  double mr, mi;
-  if (mode == 0) {
-    mr = R[2 * ((long)i * nc + j)];
-    mi = -R[2 * ((long)i * nc + j) + 1];
-  } else if (mode == 1) {
-    const double rr = R[2 * ((long)j * nr + i)], ri = R[2 * ((long)j * nr + i) + 1];
-    mr = ar * rr - ai * ri;
-    mi = ar * ri + ai * rr;
-  } else {
-    mr = R[2 * ((long)i * nc + j)];
-    mi = R[2 * ((long)i * nc + j) + 1];
-  }
-  Ap[(long)i * 2 * n + 2 * j] = mr;
-  Ap[(long)i * 2 * n + 2 * j + 1] = -mi;
-  Aq[(long)i * 2 * n + 2 * j] = mi;
-  Aq[(long)i * 2 * n + 2 * j + 1] = mr;
+  Ap[(long)i * ld + 2 * j] = (T)mr;
+  Ap[(long)i * ld + 2 * j + 1] = (T)-mi;
+  Aq[(long)i * ld + 2 * j] = (T)mi;
+  Aq[(long)i * ld + 2 * j + 1] = (T)mr;
 }
 
 constexpr int kRotJC = 16;  // complex j per LDS stage (32 real k')
 
-// Out[i][p] = sum_j M[i][j] C[j][p] (+ beta Out[i][p]), i < nr, j < n; work-group = 64 rows x 64 columns p
+// Out[i][p] = sum_j M[i][j] C[j][p] (+ beta Out[i][p]), i < nr, j < n; work-group = 64 rows x 64 columns p.
+// Within a stage, lane group q = lane / 16 of an MFMA step s holds real k' = 8q + s (any
+// order of k' is the same sum), so a lane's eight A values per stage are contiguous and
+// come in 16-byte loads.  The next stage's A values and C tile are loaded into registers
+// while the matrix cores work on the current one.
 template <typename T>
-__global__ void __launch_bounds__(256) k_rotate(const T *Ap, const T *Aq, const cplx<T> *Cm, int nr, int n, long K,
-                                                cplx<T> *Out, double br, double bi, int tiles_p) {
+__global__ void __launch_bounds__(256) k_rotate(const T *Ap, const T *Aq, int ld, const cplx<T> *Cm, int nr, int n,
+                                                long K, cplx<T> *Out, double br, double bi, int tiles_p, int tiles_i) {
   using M = mfma_t<T>;
   using C = cplx<T>;
-  __shared__ C Ct[kRotJC][64 + 1];
+  using V = typename M::v16;
+  using P2 = typename M::pair;  // one complex value as a vector (loads stay in registers)
+  constexpr int PV = 16 / sizeof(T);  // T per 16-byte load
+  constexpr int NV = 8 / PV;          // 16-byte loads per lane per operand and stage
+  __shared__ P2 Ct[kRotJC][64 + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tp = blockIdx.x % tiles_p, ti = blockIdx.x / tiles_p;
+  // Work-groups are dealt round-robin to the 8 XCDs; renumber so that each XCD runs a
+  // contiguous range, in which the tiles_i row tiles of one column block p are
+  // neighbours and read that block of C through the same L2.
+  const int G = gridDim.x, b = blockIdx.x, xq = G / 8, xr = G % 8, x = b % 8;
+  const int logical = (x < xr ? x * (xq + 1) : xr * (xq + 1) + (x - xr) * xq) + b / 8;
+  const int ti = logical % tiles_i, tp = logical / tiles_i;
   const long p0 = (long)tp * 64;
   const int i0 = ti * 64 + wave * 16;  // this wave's 16 rows
   typename M::acc re[4], im[4];
@@ -272,34 +270,52 @@ __global__ void __launch_bounds__(256) k_rotate(const T *Ap, const T *Aq, const 
     re[c] = typename M::acc{0, 0, 0, 0};
     im[c] = typename M::acc{0, 0, 0, 0};
   }
-  const GAS T *Apg = lgp(Ap), *Aqg = lgp(Aq);
-  const GAS C *Cg = lgp(Cm);
-  const int arow = i0 + (lane & 15);
-  for (int jb = 0; jb < n; jb += kRotJC) {
-    for (int e = tid; e < kRotJC * 64; e += 256) {
-      const int r = e / 64, c = e - r * 64;
+  const int q = lane >> 4;
+  // rows are padded to the tile and ld covers every stage, so these loads need no check
+  const GAS V *apv = (const GAS V *)(lgp(Ap) + (long)(i0 + (lane & 15)) * ld + 8 * q);
+  const GAS V *aqv = (const GAS V *)(lgp(Aq) + (long)(i0 + (lane & 15)) * ld + 8 * q);
+  const GAS P2 *Cg = (const GAS P2 *)lgp(Cm);
+  V an[NV], bn[NV];
+  P2 cn[4];
+  auto load_stage = [&](int jb) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      an[v] = apv[(2 * jb) / PV + v];
+      bn[v] = aqv[(2 * jb) / PV + v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u, r = e >> 6, c = e & 63;
       const int j = jb + r;
       const long p = p0 + c;
-      C v{0, 0};
-      if (j < n && p < K) v = Cg[(long)j * K + p];
-      Ct[r][c] = v;
+      cn[u] = (j < n && p < K) ? Cg[(long)j * K + p] : P2{0, 0};
+    }
+  };
+  load_stage(0);
+  for (int jb = 0; jb < n; jb += kRotJC) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;
+      Ct[e >> 6][e & 63] = cn[u];
+    }
+    V ac[NV], bc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      ac[v] = an[v];
+      bc[v] = bn[v];
     }
     __syncthreads();
+    if (jb + kRotJC < n) load_stage(jb + kRotJC);
 #pragma unroll
-    for (int ks = 0; ks < 2 * kRotJC; ks += 4) {
-      const int kk = ks + (lane >> 4);  // real k' within the stage
-      const int jglob = jb + (kk >> 1);
-      T ap = 0, aq = 0;
-      if (arow < nr && jglob < n) {
-        ap = Apg[(long)arow * 2 * n + 2 * jb + kk];
-        aq = Aqg[(long)arow * 2 * n + 2 * jb + kk];
-      }
+    for (int s = 0; s < 8; ++s) {
+      const T ap = ac[s / PV][s % PV], aq = bc[s / PV][s % PV];
+      const int jl = 4 * q + (s >> 1);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const C v = Ct[kk >> 1][16 * c + (lane & 15)];
-        const T b = (kk & 1) ? v.y : v.x;
-        re[c] = M::run(ap, b, re[c]);
-        im[c] = M::run(aq, b, im[c]);
+        const P2 v = Ct[jl][16 * c + (lane & 15)];
+        const T bv = (s & 1) ? v.y : v.x;
+        re[c] = M::run(ap, bv, re[c]);
+        im[c] = M::run(aq, bv, im[c]);
       }
     }
     __syncthreads();
@@ -315,9 +331,9 @@ __global__ void __launch_bounds__(256) k_rotate(const T *Ap, const T *Aq, const 
       if (i < nr && p < K) {
         C o{re[c][r], im[c][r]};
         if (beta) {
-          const C q = Og[(long)i * K + p];
-          o.x += (T)(br * (double)q.x - bi * (double)q.y);
-          o.y += (T)(br * (double)q.y + bi * (double)q.x);
+          const C qv = Og[(long)i * K + p];
+          o.x += (T)(br * (double)qv.x - bi * (double)qv.y);
+          o.y += (T)(br * (double)qv.y + bi * (double)qv.x);
         }
         Og[(long)i * K + p] = o;
       }
@@ -460,24 +476,28 @@ void launch_rotate(Ctx &cx, int prec, int nr, int nc, long K, const double *R, i
                    const void *Cm, void *Out, double br, double bi, DevMem &ops) {
   INQX_CHECK(mode == 1 || nr == nc, INQX_ERR_ARG, "rotate: modes 0 and 2 need a square matrix");
   const size_t es = prec == INQX_FP64 ? 8 : 4;
-  const size_t mat = (size_t)nr * 2 * nc;
-  ops.ensure(es * mat * 2);
   const int tiles_p = (int)((K + 63) / 64), tiles_i = (nr + 63) / 64;
-  const int gprep = (int)(((size_t)nr * nc + 255) / 256);
+  const int nr_pad = tiles_i * 64, ld = (2 * nc + 2 * kRotJC - 1) / (2 * kRotJC) * (2 * kRotJC);
+  const size_t mat = (size_t)nr_pad * ld;
+  ops.ensure(es * mat * 2);
+  const int gprep = (int)(((size_t)nr_pad * (ld / 2) + 255) / 256);
+  const int grid = tiles_p * tiles_i;
   if (prec == INQX_FP64) {
     double *Ap = ops.as<double>(), *Aq = Ap + mat;
-    hipLaunchKernelGGL(k_rot_prep64, dim3(gprep), dim3(256), 0, cx.stream, R, nr, nc, mode, ar, ai, Ap, Aq);
+    hipLaunchKernelGGL(k_rot_prep<double>, dim3(gprep), dim3(256), 0, cx.stream, R, nr, nc, mode, ar, ai, nr_pad, ld,
+                       Ap, Aq);
     INQX_HIP(hipGetLastError());
     ProfScope ps(cx, "rotate_mfma_f64");
-    hipLaunchKernelGGL(k_rotate<double>, dim3(tiles_p * tiles_i), dim3(256), 0, cx.stream, Ap, Aq,
-                       (const cplx<double> *)Cm, nr, nc, K, (cplx<double> *)Out, br, bi, tiles_p);
+    hipLaunchKernelGGL(k_rotate<double>, dim3(grid), dim3(256), 0, cx.stream, Ap, Aq, ld, (const cplx<double> *)Cm, nr,
+                       nc, K, (cplx<double> *)Out, br, bi, tiles_p, tiles_i);
   } else {
     float *Ap = ops.as<float>(), *Aq = Ap + mat;
-    hipLaunchKernelGGL(k_rot_prep32, dim3(gprep), dim3(256), 0, cx.stream, R, nr, nc, mode, ar, ai, Ap, Aq);
+    hipLaunchKernelGGL(k_rot_prep<float>, dim3(gprep), dim3(256), 0, cx.stream, R, nr, nc, mode, ar, ai, nr_pad, ld,
+                       Ap, Aq);
     INQX_HIP(hipGetLastError());
     ProfScope ps(cx, "rotate_mfma_f32");
-    hipLaunchKernelGGL(k_rotate<float>, dim3(tiles_p * tiles_i), dim3(256), 0, cx.stream, Ap, Aq,
-                       (const cplx<float> *)Cm, nr, nc, K, (cplx<float> *)Out, br, bi, tiles_p);
+    hipLaunchKernelGGL(k_rotate<float>, dim3(grid), dim3(256), 0, cx.stream, Ap, Aq, ld, (const cplx<float> *)Cm, nr, nc,
+                       K, (cplx<float> *)Out, br, bi, tiles_p, tiles_i);
   }
   INQX_HIP(hipGetLastError());
 }
