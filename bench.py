@@ -284,6 +284,22 @@ def contractions(ctx, reps=10):
             out[f"{name}_{prec}"] = res
             d.free()
             rot.free()
+        # the FP64 steps between overlap and rotate in orthogonalize: Cholesky and L^-1
+        s = a[:, :n].astype(np.complex128)
+        s = s @ s.conj().T + n * np.eye(n)
+        lmat = DeviceArray.from_numpy(ctx, np.linalg.cholesky(s))
+        ops.invert_triangular(ctx, lmat).free()
+        ctx.sync()
+        ctx.profile(True)
+        ctx.profile_reset()
+        for _ in range(reps):
+            ops.cholesky(ctx, DeviceArray.from_numpy(ctx, s)).free()
+            ops.invert_triangular(ctx, lmat).free()
+        kt = ctx.kernel_times()
+        ctx.profile(False)
+        out[f"{name}_fp64"]["cholesky_us"] = kt["cholesky_f64"][1] / kt["cholesky_f64"][0] * 1e3
+        out[f"{name}_fp64"]["trinv_us"] = kt["trinv_f64"][1] / kt["trinv_f64"][0] * 1e3
+        lmat.free()
     return out
 
 

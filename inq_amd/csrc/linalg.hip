@@ -12,9 +12,9 @@
 //             (operations::rotate, src/operations/rotate.hpp:26-156), MFMA with
 //             the real operands A' = [Mr, -Mi], A'' = [Mi, Mr] interleaved along j.
 //   cholesky  FP64 lower Cholesky S = L L^H, upper zeroed (matrix::cholesky,
-//             src/matrix/cholesky.hpp:28-100), one work-group.
+//             src/matrix/cholesky.hpp:28-100), blocked in 32-wide panels.
 //   trinv     FP64 inverse of a lower-triangular matrix (matrix::invert_triangular,
-//             src/matrix/invert_triangular.hpp:20-50), one lane per column.
+//             src/matrix/invert_triangular.hpp:20-50), blocked by 32x32 block rows.
 //
 // MFMA 16x16x4 lane layout (CDNA4): lane l supplies A[l%16][l/16] and
 // B[l/16][l%16] and receives column l%16 of D in its four accumulators; the
@@ -341,91 +341,260 @@ __global__ void __launch_bounds__(256) k_rotate(const T *Ap, const T *Aq, int ld
 }
 
 // ------------------------------------------------------------------------ Cholesky / inverse (FP64)
-// in place on S (complex double, row-major n x n); info[0] = 0 or the first
-// non-positive leading minor (LAPACK convention)
-__global__ void __launch_bounds__(1024) k_cholesky(double *S, int n, int *info) {
-  __shared__ double red[1024];
+// Blocked in 32-wide panels, one launch per step so that every read of a value
+// another work-group wrote happens in a later launch.
+//
+// Cholesky S = L L^H in place (lower; the strict upper triangle is zeroed), per panel c0:
+//   k_chol_diag    factor the diagonal block (already updated by earlier panels) in LDS
+//   k_chol_panel   rows below: L[i][c0+j] = (S[i][c0+j] - sum_{k<j} L[i][c0+k] conj(L_dd[j][k])) / L_dd[j][j]
+//   k_chol_update  S[i][j] -= sum_k L[i][c0+k] conj(L[j][c0+k]) on the lower trailing matrix
+// info[0] = 0 or the first non-positive leading minor (LAPACK convention); the first
+// diagonal launch writes it, and every later launch returns at once when it is set.
+constexpr int kChB = 32;
+
+__global__ void __launch_bounds__(256) k_chol_diag(double *S, int n, int c0, int *info) {
+  __shared__ double Dr[kChB][kChB + 1], Di[kChB][kChB + 1];
   __shared__ int bad;
-  const int tid = threadIdx.x, nth = blockDim.x;
+  const int tid = threadIdx.x;
+  if (c0 > 0 && info[0] != 0) return;
+  const int b = min(kChB, n - c0);
+  for (int e = tid; e < kChB * kChB; e += 256) {
+    const int i = e / kChB, j = e - i * kChB;
+    double vr = 0.0, vi = 0.0;
+    if (i < b && j <= i) {
+      vr = S[2 * ((long)(c0 + i) * n + c0 + j)];
+      vi = S[2 * ((long)(c0 + i) * n + c0 + j) + 1];
+    }
+    Dr[i][j] = vr;
+    Di[i][j] = vi;
+  }
   if (tid == 0) bad = 0;
   __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    double part = 0.0;
-    for (int k = tid; k < j; k += nth) {
-      const double lr = S[2 * (j * n + k)], li = S[2 * (j * n + k) + 1];
-      part += lr * lr + li * li;
-    }
-    red[tid] = part;
-    __syncthreads();
-    for (int w = nth >> 1; w > 0; w >>= 1) {
-      if (tid < w) red[tid] += red[tid + w];
-      __syncthreads();
-    }
+  for (int j = 0; j < b; ++j) {
     if (tid == 0) {
-      const double d = S[2 * (j * n + j)] - red[0];
+      double d = Dr[j][j];
+      for (int k = 0; k < j; ++k) d -= Dr[j][k] * Dr[j][k] + Di[j][k] * Di[j][k];
       if (!(d > 0.0)) {
-        bad = j + 1;
+        bad = c0 + j + 1;
       } else {
-        S[2 * (j * n + j)] = sqrt(d);
-        S[2 * (j * n + j) + 1] = 0.0;
+        Dr[j][j] = sqrt(d);
+        Di[j][j] = 0.0;
       }
     }
     __syncthreads();
     if (bad) break;
-    const double ljj = S[2 * (j * n + j)];
-    for (int i = j + 1 + tid; i < n; i += nth) {
-      double sr = S[2 * (i * n + j)], si = S[2 * (i * n + j) + 1];
-      for (int k = 0; k < j; ++k) {
-        const double ar = S[2 * (i * n + k)], ai = S[2 * (i * n + k) + 1];
-        const double br = S[2 * (j * n + k)], bi = S[2 * (j * n + k) + 1];
-        // a * conj(b)
-        sr -= ar * br + ai * bi;
-        si -= ai * br - ar * bi;
+    const double ljj = Dr[j][j];
+    const int i = j + 1 + tid;
+    if (i < b) {
+      double sr = Dr[i][j], si = Di[i][j];
+      for (int k = 0; k < j; ++k) {  // L[i][k] conj(L[j][k])
+        sr -= Dr[i][k] * Dr[j][k] + Di[i][k] * Di[j][k];
+        si -= Di[i][k] * Dr[j][k] - Dr[i][k] * Di[j][k];
       }
-      S[2 * (i * n + j)] = sr / ljj;
-      S[2 * (i * n + j) + 1] = si / ljj;
+      Dr[i][j] = sr / ljj;
+      Di[i][j] = si / ljj;
     }
     __syncthreads();
   }
-  if (tid == 0) info[0] = bad;
-  if (!bad) {
-    for (int e = tid; e < n * n; e += nth) {
-      const int i = e / n, j = e - i * n;
-      if (j > i) {
-        S[2 * e] = 0.0;
-        S[2 * e + 1] = 0.0;
+  if (bad) {
+    if (tid == 0) info[0] = bad;
+    return;
+  }
+  if (c0 == 0 && tid == 0) info[0] = 0;
+  for (int e = tid; e < b * b; e += 256) {
+    const int i = e / b, j = e - i * b;
+    const bool lower = j <= i;
+    S[2 * ((long)(c0 + i) * n + c0 + j)] = lower ? Dr[i][j] : 0.0;
+    S[2 * ((long)(c0 + i) * n + c0 + j) + 1] = lower ? Di[i][j] : 0.0;
+  }
+  // rows of this panel right of the diagonal block are upper triangle: zero
+  const int w = n - c0 - b;
+  for (int e = tid; e < b * w; e += 256) {
+    const int i = e / w, j = e - i * w;
+    S[2 * ((long)(c0 + i) * n + c0 + b + j)] = 0.0;
+    S[2 * ((long)(c0 + i) * n + c0 + b + j) + 1] = 0.0;
+  }
+}
+
+// one lane per row i >= c0 + 32 (launched only when such rows exist, so the panel is 32 wide)
+__global__ void __launch_bounds__(256) k_chol_panel(double *S, int n, int c0, const int *info) {
+  __shared__ double Lr[kChB][kChB + 1], Li[kChB][kChB + 1];
+  if (info[0] != 0) return;
+  for (int e = threadIdx.x; e < kChB * kChB; e += 256) {
+    const int i = e / kChB, j = e - i * kChB;
+    Lr[i][j] = j <= i ? S[2 * ((long)(c0 + i) * n + c0 + j)] : 0.0;
+    Li[i][j] = j <= i ? S[2 * ((long)(c0 + i) * n + c0 + j) + 1] : 0.0;
+  }
+  __syncthreads();
+  const int i = c0 + kChB + blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double *row = S + 2 * ((long)i * n + c0);
+  double xr[kChB], xi[kChB];
+#pragma unroll
+  for (int j = 0; j < kChB; ++j) {
+    xr[j] = row[2 * j];
+    xi[j] = row[2 * j + 1];
+  }
+#pragma unroll
+  for (int j = 0; j < kChB; ++j) {
+    double sr = xr[j], si = xi[j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) {  // x_k conj(L_dd[j][k])
+      sr -= xr[k] * Lr[j][k] + xi[k] * Li[j][k];
+      si -= xi[k] * Lr[j][k] - xr[k] * Li[j][k];
+    }
+    xr[j] = sr / Lr[j][j];
+    xi[j] = si / Lr[j][j];
+  }
+#pragma unroll
+  for (int j = 0; j < kChB; ++j) {
+    row[2 * j] = xr[j];
+    row[2 * j + 1] = xi[j];
+  }
+}
+
+// 32x32 output tile (ti, tj), tj <= ti, of the trailing matrix starting at c0 + 32;
+// lane: row tid / 8, columns tid % 8 + 8 q
+__global__ void __launch_bounds__(256) k_chol_update(double *S, int n, int c0, const int *info) {
+  __shared__ double Ar[kChB][kChB + 1], Ai[kChB][kChB + 1], Br[kChB][kChB + 1], Bi[kChB][kChB + 1];
+  const int ti = blockIdx.y, tj = blockIdx.x;
+  if (tj > ti || info[0] != 0) return;
+  const int r0 = c0 + kChB, i0 = r0 + ti * kChB, j0 = r0 + tj * kChB;
+  for (int e = threadIdx.x; e < kChB * kChB; e += 256) {
+    const int r = e / kChB, k = e - r * kChB;
+    const bool ia = i0 + r < n, jb = j0 + r < n;
+    Ar[r][k] = ia ? S[2 * ((long)(i0 + r) * n + c0 + k)] : 0.0;
+    Ai[r][k] = ia ? S[2 * ((long)(i0 + r) * n + c0 + k) + 1] : 0.0;
+    Br[r][k] = jb ? S[2 * ((long)(j0 + r) * n + c0 + k)] : 0.0;
+    Bi[r][k] = jb ? S[2 * ((long)(j0 + r) * n + c0 + k) + 1] : 0.0;
+  }
+  __syncthreads();
+  const int r = threadIdx.x >> 3, cb = threadIdx.x & 7, i = i0 + r;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int c = cb + 8 * q, j = j0 + c;
+    if (i < n && j <= i) {
+      double sr = 0.0, si = 0.0;
+#pragma unroll 8
+      for (int k = 0; k < kChB; ++k) {  // A[r][k] conj(B[c][k])
+        sr += Ar[r][k] * Br[c][k] + Ai[r][k] * Bi[c][k];
+        si += Ai[r][k] * Br[c][k] - Ar[r][k] * Bi[c][k];
       }
+      S[2 * ((long)i * n + j)] -= sr;
+      S[2 * ((long)i * n + j) + 1] -= si;
     }
   }
 }
 
-// X = L^{-1} for lower-triangular L (complex double), one lane per column
-__global__ void k_trinv(const double *L, int n, double *X) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  for (int i = 0; i < c; ++i) {
-    X[2 * (i * n + c)] = 0.0;
-    X[2 * (i * n + c) + 1] = 0.0;
-  }
-  {
-    const double lr = L[2 * (c * n + c)], li = L[2 * (c * n + c) + 1];
-    const double d = lr * lr + li * li;
-    X[2 * (c * n + c)] = lr / d;
-    X[2 * (c * n + c) + 1] = -li / d;
-  }
-  for (int i = c + 1; i < n; ++i) {
-    double sr = 0.0, si = 0.0;
-    for (int k = c; k < i; ++k) {
-      const double ar = L[2 * (i * n + k)], ai = L[2 * (i * n + k) + 1];
-      const double xr = X[2 * (k * n + c)], xi = X[2 * (k * n + c) + 1];
-      sr += ar * xr - ai * xi;
-      si += ar * xi + ai * xr;
+// X = L^{-1} for lower-triangular L (complex double), blocked in 32x32 blocks:
+//   k_trinv_diag  X_bb = L_bb^{-1} (one lane per column, forward substitution in LDS),
+//                 blocks above the diagonal zeroed
+//   k_trinv_row   block row r, one work-group per block column jb < r:
+//                 X_r,jb = -X_rr sum_{k=jb}^{r-1} L_rk X_k,jb  (every X read was written
+//                 by an earlier launch)
+__global__ void __launch_bounds__(256) k_trinv_diag(const double *L, int n, double *X) {
+  __shared__ double Lr[kChB][kChB + 1], Li[kChB][kChB + 1], Xr[kChB][kChB + 1], Xi[kChB][kChB + 1];
+  const int bi = blockIdx.y, bj = blockIdx.x, tid = threadIdx.x;
+  const int r0 = bi * kChB, q0 = bj * kChB;
+  const int b = min(kChB, n - r0), bc = min(kChB, n - q0);
+  if (bj > bi) {
+    for (int e = tid; e < b * bc; e += 256) {
+      const int i = e / bc, j = e - i * bc;
+      X[2 * ((long)(r0 + i) * n + q0 + j)] = 0.0;
+      X[2 * ((long)(r0 + i) * n + q0 + j) + 1] = 0.0;
     }
-    const double lr = L[2 * (i * n + i)], li = L[2 * (i * n + i) + 1];
-    const double d = lr * lr + li * li;
-    // -(s / l)
-    X[2 * (i * n + c)] = -(sr * lr + si * li) / d;
-    X[2 * (i * n + c) + 1] = -(si * lr - sr * li) / d;
+    return;
+  }
+  if (bj < bi) return;
+  for (int e = tid; e < kChB * kChB; e += 256) {
+    const int i = e / kChB, j = e - i * kChB;
+    const bool in = i < b && j <= i;
+    Lr[i][j] = in ? L[2 * ((long)(r0 + i) * n + r0 + j)] : 0.0;
+    Li[i][j] = in ? L[2 * ((long)(r0 + i) * n + r0 + j) + 1] : 0.0;
+    Xr[i][j] = 0.0;
+    Xi[i][j] = 0.0;
+  }
+  __syncthreads();
+  const int c = tid;
+  if (c < b) {
+    {
+      const double lr = Lr[c][c], li = Li[c][c], d = lr * lr + li * li;
+      Xr[c][c] = lr / d;
+      Xi[c][c] = -li / d;
+    }
+    for (int i = c + 1; i < b; ++i) {
+      double sr = 0.0, si = 0.0;
+      for (int k = c; k < i; ++k) {
+        sr += Lr[i][k] * Xr[k][c] - Li[i][k] * Xi[k][c];
+        si += Lr[i][k] * Xi[k][c] + Li[i][k] * Xr[k][c];
+      }
+      const double lr = Lr[i][i], li = Li[i][i], d = lr * lr + li * li;
+      Xr[i][c] = -(sr * lr + si * li) / d;  // -(s / l)
+      Xi[i][c] = -(si * lr - sr * li) / d;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < b * b; e += 256) {
+    const int i = e / b, j = e - i * b;
+    X[2 * ((long)(r0 + i) * n + r0 + j)] = Xr[i][j];
+    X[2 * ((long)(r0 + i) * n + r0 + j) + 1] = Xi[i][j];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_trinv_row(const double *L, int n, int r, double *X) {
+  __shared__ double Ar[kChB][kChB + 1], Ai[kChB][kChB + 1], Br[kChB][kChB + 1], Bi[kChB][kChB + 1];
+  const int jb = blockIdx.x, tid = threadIdx.x;
+  const int r0 = r * kChB, q0 = jb * kChB;
+  const int row = tid >> 3, cb = tid & 7;
+  double tr[4] = {0, 0, 0, 0}, tim[4] = {0, 0, 0, 0};
+  for (int k = jb; k < r; ++k) {
+    const int k0 = k * kChB;
+    for (int e = tid; e < kChB * kChB; e += 256) {
+      const int a = e / kChB, m = e - a * kChB;
+      const bool la = r0 + a < n && k0 + m < n, xb = k0 + a < n && q0 + m < n;
+      Ar[a][m] = la ? L[2 * ((long)(r0 + a) * n + k0 + m)] : 0.0;
+      Ai[a][m] = la ? L[2 * ((long)(r0 + a) * n + k0 + m) + 1] : 0.0;
+      Br[a][m] = xb ? X[2 * ((long)(k0 + a) * n + q0 + m)] : 0.0;
+      Bi[a][m] = xb ? X[2 * ((long)(k0 + a) * n + q0 + m) + 1] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = cb + 8 * q;
+#pragma unroll 8
+      for (int m = 0; m < kChB; ++m) {
+        tr[q] += Ar[row][m] * Br[m][c] - Ai[row][m] * Bi[m][c];
+        tim[q] += Ar[row][m] * Bi[m][c] + Ai[row][m] * Br[m][c];
+      }
+    }
+    __syncthreads();
+  }
+  // Y = -X_rr T: T to LDS (B), X_rr to LDS (A)
+  for (int e = tid; e < kChB * kChB; e += 256) {
+    const int a = e / kChB, m = e - a * kChB;
+    const bool xa = r0 + a < n && r0 + m < n;
+    Ar[a][m] = xa ? X[2 * ((long)(r0 + a) * n + r0 + m)] : 0.0;
+    Ai[a][m] = xa ? X[2 * ((long)(r0 + a) * n + r0 + m) + 1] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    Br[row][cb + 8 * q] = tr[q];
+    Bi[row][cb + 8 * q] = tim[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int c = cb + 8 * q;
+    double yr = 0.0, yi = 0.0;
+#pragma unroll 8
+    for (int m = 0; m < kChB; ++m) {
+      yr += Ar[row][m] * Br[m][c] - Ai[row][m] * Bi[m][c];
+      yi += Ar[row][m] * Bi[m][c] + Ai[row][m] * Br[m][c];
+    }
+    if (r0 + row < n && q0 + c < n) {
+      X[2 * ((long)(r0 + row) * n + q0 + c)] = -yr;
+      X[2 * ((long)(r0 + row) * n + q0 + c) + 1] = -yi;
+    }
   }
 }
 
@@ -663,8 +832,17 @@ void launch_fill_zero(Ctx &cx, void *p, size_t bytes) {
 
 void launch_cholesky(Ctx &cx, int n, double *S, int *info) {
   ProfScope ps(cx, "cholesky_f64");
-  hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(n >= 1024 ? 1024 : (n >= 256 ? 256 : 64)), 0, cx.stream, S, n, info);
-  INQX_HIP(hipGetLastError());
+  for (int c0 = 0; c0 < n; c0 += kChB) {
+    hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, cx.stream, S, n, c0, info);
+    INQX_HIP(hipGetLastError());
+    const int rest = n - c0 - kChB;
+    if (rest <= 0) break;
+    hipLaunchKernelGGL(k_chol_panel, dim3((rest + 255) / 256), dim3(256), 0, cx.stream, S, n, c0, (const int *)info);
+    INQX_HIP(hipGetLastError());
+    const int m = (rest + kChB - 1) / kChB;
+    hipLaunchKernelGGL(k_chol_update, dim3(m, m), dim3(256), 0, cx.stream, S, n, c0, (const int *)info);
+    INQX_HIP(hipGetLastError());
+  }
 }
 
 void launch_stream_copy(Ctx &cx, const void *src, void *dst, size_t bytes) {
@@ -677,8 +855,13 @@ void launch_stream_copy(Ctx &cx, const void *src, void *dst, size_t bytes) {
 
 void launch_trinv(Ctx &cx, int n, const double *L, double *X) {
   ProfScope ps(cx, "trinv_f64");
-  hipLaunchKernelGGL(k_trinv, dim3((n + 63) / 64), dim3(64), 0, cx.stream, L, n, X);
+  const int nb = (n + kChB - 1) / kChB;
+  hipLaunchKernelGGL(k_trinv_diag, dim3(nb, nb), dim3(256), 0, cx.stream, L, n, X);
   INQX_HIP(hipGetLastError());
+  for (int r = 1; r < nb; ++r) {
+    hipLaunchKernelGGL(k_trinv_row, dim3(r), dim3(256), 0, cx.stream, L, n, r, X);
+    INQX_HIP(hipGetLastError());
+  }
 }
 
 }  // namespace inqx

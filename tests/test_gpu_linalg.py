@@ -67,6 +67,38 @@ def test_cholesky_not_positive_definite_is_a_status(ctx):
     assert "status -4" in str(e.value)
 
 
+@pytest.mark.parametrize("n", [32, 33, 100, 433])
+def test_blocked_cholesky_and_inverse_vs_scipy(ctx, n):
+    """Sizes across several 32-wide panels, a ragged last panel included; the
+    strict upper triangle of the input holds garbage that must be ignored."""
+    say("case: blocked cholesky / invert_triangular", n)
+    a = _rand((n, n), 10 + n)
+    s = a @ a.conj().T + n * np.eye(n)
+    s_in = s.copy()
+    s_in[np.triu_indices(n, 1)] = 7.0 - 3.0j
+    L = ops.cholesky(ctx, DeviceArray.from_numpy(ctx, s_in)).numpy()
+    L_ref = sla.cholesky(s, lower=True)
+    assert np.max(np.abs(L - L_ref)) < 1e-12 * np.max(np.abs(L_ref))
+    assert np.all(L[np.triu_indices(n, 1)] == 0.0)
+    X = ops.invert_triangular(ctx, DeviceArray.from_numpy(ctx, L_ref)).numpy()
+    assert np.max(np.abs(X @ L_ref - np.eye(n))) < 1e-12
+    assert np.all(X[np.triu_indices(n, 1)] == 0.0)
+
+
+def test_cholesky_not_positive_definite_in_a_later_panel(ctx):
+    from inq_amd import InqxError
+    a = _rand((100, 100), 11)
+    s = a @ a.conj().T + 100 * np.eye(100)
+    s[70, 70] = -5.0
+    with pytest.raises(InqxError) as e:
+        ops.cholesky(ctx, DeviceArray.from_numpy(ctx, s))
+    assert "minor 71 " in str(e.value)
+    # the context stays usable, and the next factorisation starts clean
+    s[70, 70] = s[69, 69]
+    L = ops.cholesky(ctx, DeviceArray.from_numpy(ctx, s)).numpy()
+    assert np.max(np.abs(L - sla.cholesky(s, lower=True))) < 1e-11
+
+
 def test_invert_triangular(ctx):
     say("case: invert_triangular")
     m = np.array([[4.0, 0.0], [-1.0, 2.0]], dtype=np.complex128)
