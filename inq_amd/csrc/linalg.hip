@@ -352,9 +352,11 @@ __global__ void __launch_bounds__(256) k_rotate(const T *Ap, const T *Aq, int ld
 // diagonal launch writes it, and every later launch returns at once when it is set.
 constexpr int kChB = 32;
 
+// right-looking in LDS with one barrier per column: every lane reads the same pivot d,
+// column j (scaled by 1/sqrt(d)) goes straight to S, and the lanes update the lower
+// block right of column j from the unscaled column (column j is not written in step j)
 __global__ void __launch_bounds__(256) k_chol_diag(double *S, int n, int c0, int *info) {
   __shared__ double Dr[kChB][kChB + 1], Di[kChB][kChB + 1];
-  __shared__ int bad;
   const int tid = threadIdx.x;
   if (c0 > 0 && info[0] != 0) return;
   const int b = min(kChB, n - c0);
@@ -368,31 +370,30 @@ __global__ void __launch_bounds__(256) k_chol_diag(double *S, int n, int c0, int
     Dr[i][j] = vr;
     Di[i][j] = vi;
   }
-  if (tid == 0) bad = 0;
   __syncthreads();
+  int bad = 0;
   for (int j = 0; j < b; ++j) {
-    if (tid == 0) {
-      double d = Dr[j][j];
-      for (int k = 0; k < j; ++k) d -= Dr[j][k] * Dr[j][k] + Di[j][k] * Di[j][k];
-      if (!(d > 0.0)) {
-        bad = c0 + j + 1;
-      } else {
-        Dr[j][j] = sqrt(d);
-        Di[j][j] = 0.0;
-      }
+    const double d = Dr[j][j];
+    if (!(d > 0.0)) {
+      bad = c0 + j + 1;
+      break;
     }
-    __syncthreads();
-    if (bad) break;
-    const double ljj = Dr[j][j];
-    const int i = j + 1 + tid;
-    if (i < b) {
-      double sr = Dr[i][j], si = Di[i][j];
-      for (int k = 0; k < j; ++k) {  // L[i][k] conj(L[j][k])
-        sr -= Dr[i][k] * Dr[j][k] + Di[i][k] * Di[j][k];
-        si -= Di[i][k] * Dr[j][k] - Dr[i][k] * Di[j][k];
+    const double ljj = sqrt(d), rl = 1.0 / ljj;
+    if (tid < b - j) {
+      const int i = j + tid;
+      const long o = 2 * ((long)(c0 + i) * n + c0 + j);
+      S[o] = tid == 0 ? ljj : Dr[i][j] * rl;
+      S[o + 1] = tid == 0 ? 0.0 : Di[i][j] * rl;
+    }
+    const double rd = rl * rl;  // L[i][j] conj(L[k][j]) = D[i][j] conj(D[k][j]) / d
+#pragma unroll
+    for (int e = tid; e < kChB * kChB; e += 256) {
+      const int i = e / kChB, k = e - i * kChB;
+      if (k > j && k <= i && i < b) {
+        const double ar = Dr[i][j] * rd, ai = Di[i][j] * rd, br = Dr[k][j], bi = Di[k][j];
+        Dr[i][k] -= ar * br + ai * bi;
+        Di[i][k] -= ai * br - ar * bi;
       }
-      Dr[i][j] = sr / ljj;
-      Di[i][j] = si / ljj;
     }
     __syncthreads();
   }
@@ -401,11 +402,12 @@ __global__ void __launch_bounds__(256) k_chol_diag(double *S, int n, int c0, int
     return;
   }
   if (c0 == 0 && tid == 0) info[0] = 0;
-  for (int e = tid; e < b * b; e += 256) {
+  for (int e = tid; e < b * b; e += 256) {  // upper part of the diagonal block
     const int i = e / b, j = e - i * b;
-    const bool lower = j <= i;
-    S[2 * ((long)(c0 + i) * n + c0 + j)] = lower ? Dr[i][j] : 0.0;
-    S[2 * ((long)(c0 + i) * n + c0 + j) + 1] = lower ? Di[i][j] : 0.0;
+    if (j > i) {
+      S[2 * ((long)(c0 + i) * n + c0 + j)] = 0.0;
+      S[2 * ((long)(c0 + i) * n + c0 + j) + 1] = 0.0;
+    }
   }
   // rows of this panel right of the diagonal block are upper triangle: zero
   const int w = n - c0 - b;
@@ -418,13 +420,14 @@ __global__ void __launch_bounds__(256) k_chol_diag(double *S, int n, int c0, int
 
 // one lane per row i >= c0 + 32 (launched only when such rows exist, so the panel is 32 wide)
 __global__ void __launch_bounds__(256) k_chol_panel(double *S, int n, int c0, const int *info) {
-  __shared__ double Lr[kChB][kChB + 1], Li[kChB][kChB + 1];
+  __shared__ double Lr[kChB][kChB + 1], Li[kChB][kChB + 1], rdiag[kChB];
   if (info[0] != 0) return;
   for (int e = threadIdx.x; e < kChB * kChB; e += 256) {
     const int i = e / kChB, j = e - i * kChB;
     Lr[i][j] = j <= i ? S[2 * ((long)(c0 + i) * n + c0 + j)] : 0.0;
     Li[i][j] = j <= i ? S[2 * ((long)(c0 + i) * n + c0 + j) + 1] : 0.0;
   }
+  if (threadIdx.x < kChB) rdiag[threadIdx.x] = 1.0 / S[2 * ((long)(c0 + threadIdx.x) * n + c0 + threadIdx.x)];
   __syncthreads();
   const int i = c0 + kChB + blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -435,16 +438,17 @@ __global__ void __launch_bounds__(256) k_chol_panel(double *S, int n, int c0, co
     xr[j] = row[2 * j];
     xi[j] = row[2 * j + 1];
   }
+  // right-looking: once x_j is final, x_m -= x_j conj(L_dd[m][j]) for every m > j
+  // (independent updates, short dependency chains)
 #pragma unroll
   for (int j = 0; j < kChB; ++j) {
-    double sr = xr[j], si = xi[j];
+    xr[j] *= rdiag[j];
+    xi[j] *= rdiag[j];
 #pragma unroll
-    for (int k = 0; k < j; ++k) {  // x_k conj(L_dd[j][k])
-      sr -= xr[k] * Lr[j][k] + xi[k] * Li[j][k];
-      si -= xi[k] * Lr[j][k] - xr[k] * Li[j][k];
+    for (int m = j + 1; m < kChB; ++m) {
+      xr[m] -= xr[j] * Lr[m][j] + xi[j] * Li[m][j];
+      xi[m] -= xi[j] * Lr[m][j] - xr[j] * Li[m][j];
     }
-    xr[j] = sr / Lr[j][j];
-    xi[j] = si / Lr[j][j];
   }
 #pragma unroll
   for (int j = 0; j < kChB; ++j) {
@@ -470,18 +474,23 @@ __global__ void __launch_bounds__(256) k_chol_update(double *S, int n, int c0, c
   }
   __syncthreads();
   const int r = threadIdx.x >> 3, cb = threadIdx.x & 7, i = i0 + r;
+  double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
+#pragma unroll 8
+  for (int k = 0; k < kChB; ++k) {  // A[r][k] conj(B[c][k]); out-of-range rows are zero
+    const double ar = Ar[r][k], ai = Ai[r][k];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = cb + 8 * q;
+      sr[q] += ar * Br[c][k] + ai * Bi[c][k];
+      si[q] += ai * Br[c][k] - ar * Bi[c][k];
+    }
+  }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int c = cb + 8 * q, j = j0 + c;
+    const int j = j0 + cb + 8 * q;
     if (i < n && j <= i) {
-      double sr = 0.0, si = 0.0;
-#pragma unroll 8
-      for (int k = 0; k < kChB; ++k) {  // A[r][k] conj(B[c][k])
-        sr += Ar[r][k] * Br[c][k] + Ai[r][k] * Bi[c][k];
-        si += Ai[r][k] * Br[c][k] - Ar[r][k] * Bi[c][k];
-      }
-      S[2 * ((long)i * n + j)] -= sr;
-      S[2 * ((long)i * n + j) + 1] -= si;
+      S[2 * ((long)i * n + j)] -= sr[q];
+      S[2 * ((long)i * n + j) + 1] -= si[q];
     }
   }
 }
@@ -514,23 +523,26 @@ __global__ void __launch_bounds__(256) k_trinv_diag(const double *L, int n, doub
     Xr[i][j] = 0.0;
     Xi[i][j] = 0.0;
   }
+  __shared__ double Ir[kChB], Ii[kChB];  // 1 / L[i][i]
+  if (tid < b) {
+    const double lr = L[2 * ((long)(r0 + tid) * n + r0 + tid)], li = L[2 * ((long)(r0 + tid) * n + r0 + tid) + 1];
+    const double rd = 1.0 / (lr * lr + li * li);
+    Ir[tid] = lr * rd;
+    Ii[tid] = -li * rd;
+  }
   __syncthreads();
   const int c = tid;
   if (c < b) {
-    {
-      const double lr = Lr[c][c], li = Li[c][c], d = lr * lr + li * li;
-      Xr[c][c] = lr / d;
-      Xi[c][c] = -li / d;
-    }
+    Xr[c][c] = Ir[c];
+    Xi[c][c] = Ii[c];
     for (int i = c + 1; i < b; ++i) {
       double sr = 0.0, si = 0.0;
       for (int k = c; k < i; ++k) {
         sr += Lr[i][k] * Xr[k][c] - Li[i][k] * Xi[k][c];
         si += Lr[i][k] * Xi[k][c] + Li[i][k] * Xr[k][c];
       }
-      const double lr = Lr[i][i], li = Li[i][i], d = lr * lr + li * li;
-      Xr[i][c] = -(sr * lr + si * li) / d;  // -(s / l)
-      Xi[i][c] = -(si * lr - sr * li) / d;
+      Xr[i][c] = -(sr * Ir[i] - si * Ii[i]);  // -(s / l)
+      Xi[i][c] = -(sr * Ii[i] + si * Ir[i]);
     }
   }
   __syncthreads();
@@ -547,24 +559,40 @@ __global__ void __launch_bounds__(256) k_trinv_row(const double *L, int n, int r
   const int r0 = r * kChB, q0 = jb * kChB;
   const int row = tid >> 3, cb = tid & 7;
   double tr[4] = {0, 0, 0, 0}, tim[4] = {0, 0, 0, 0};
-  for (int k = jb; k < r; ++k) {
+  // the next k block's L and X values are loaded into registers during the products
+  double par[4], pai[4], pbr[4], pbi[4];
+  auto fetch = [&](int k) {
     const int k0 = k * kChB;
-    for (int e = tid; e < kChB * kChB; e += 256) {
-      const int a = e / kChB, m = e - a * kChB;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u, a = e / kChB, m = e - a * kChB;
       const bool la = r0 + a < n && k0 + m < n, xb = k0 + a < n && q0 + m < n;
-      Ar[a][m] = la ? L[2 * ((long)(r0 + a) * n + k0 + m)] : 0.0;
-      Ai[a][m] = la ? L[2 * ((long)(r0 + a) * n + k0 + m) + 1] : 0.0;
-      Br[a][m] = xb ? X[2 * ((long)(k0 + a) * n + q0 + m)] : 0.0;
-      Bi[a][m] = xb ? X[2 * ((long)(k0 + a) * n + q0 + m) + 1] : 0.0;
+      par[u] = la ? L[2 * ((long)(r0 + a) * n + k0 + m)] : 0.0;
+      pai[u] = la ? L[2 * ((long)(r0 + a) * n + k0 + m) + 1] : 0.0;
+      pbr[u] = xb ? X[2 * ((long)(k0 + a) * n + q0 + m)] : 0.0;
+      pbi[u] = xb ? X[2 * ((long)(k0 + a) * n + q0 + m) + 1] : 0.0;
+    }
+  };
+  fetch(jb);
+  for (int k = jb; k < r; ++k) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u, a = e / kChB, m = e - a * kChB;
+      Ar[a][m] = par[u];
+      Ai[a][m] = pai[u];
+      Br[a][m] = pbr[u];
+      Bi[a][m] = pbi[u];
     }
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int c = cb + 8 * q;
+    if (k + 1 < r) fetch(k + 1);
 #pragma unroll 8
-      for (int m = 0; m < kChB; ++m) {
-        tr[q] += Ar[row][m] * Br[m][c] - Ai[row][m] * Bi[m][c];
-        tim[q] += Ar[row][m] * Bi[m][c] + Ai[row][m] * Br[m][c];
+    for (int m = 0; m < kChB; ++m) {
+      const double ar = Ar[row][m], ai = Ai[row][m];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = cb + 8 * q;
+        tr[q] += ar * Br[m][c] - ai * Bi[m][c];
+        tim[q] += ar * Bi[m][c] + ai * Br[m][c];
       }
     }
     __syncthreads();
@@ -582,18 +610,23 @@ __global__ void __launch_bounds__(256) k_trinv_row(const double *L, int n, int r
     Bi[row][cb + 8 * q] = tim[q];
   }
   __syncthreads();
+  double yr[4] = {0, 0, 0, 0}, yi[4] = {0, 0, 0, 0};
+#pragma unroll 8
+  for (int m = 0; m < kChB; ++m) {
+    const double ar = Ar[row][m], ai = Ai[row][m];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = cb + 8 * q;
+      yr[q] += ar * Br[m][c] - ai * Bi[m][c];
+      yi[q] += ar * Bi[m][c] + ai * Br[m][c];
+    }
+  }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int c = cb + 8 * q;
-    double yr = 0.0, yi = 0.0;
-#pragma unroll 8
-    for (int m = 0; m < kChB; ++m) {
-      yr += Ar[row][m] * Br[m][c] - Ai[row][m] * Bi[m][c];
-      yi += Ar[row][m] * Bi[m][c] + Ai[row][m] * Br[m][c];
-    }
     if (r0 + row < n && q0 + c < n) {
-      X[2 * ((long)(r0 + row) * n + q0 + c)] = -yr;
-      X[2 * ((long)(r0 + row) * n + q0 + c) + 1] = -yi;
+      X[2 * ((long)(r0 + row) * n + q0 + c)] = -yr[q];
+      X[2 * ((long)(r0 + row) * n + q0 + c) + 1] = -yi[q];
     }
   }
 }
