@@ -17,7 +17,8 @@ Secondary lines in the same JSON object (same seeded inputs):
   c5_ace        Si216 (96^3, 432 states) ACE build: all pair-density Poisson solves,
                 W^H phi, FP64 Cholesky, xi; ACE apply; FP32 vs FP64 direct exchange
   contractions  overlap and rotate on the matrix cores at the Si64 and Si216 block
-                sizes, TFLOP/s and fraction of the dense MFMA peak
+                sizes, TFLOP/s and fraction of the dense MFMA peak; FP64 Cholesky and L^-1
+  sd_si64       steepest_descent (5 preconditioned steps + orthogonalize) on the Si64 block
 The CPU baseline is the FP64 oracle (a C/OpenMP port of the reference CPU data
 flow) on a bounded subset of the Si64 states.
 
@@ -303,6 +304,41 @@ def contractions(ctx, reps=10):
     return out
 
 
+def sd_si64(ctx, gh, b, c, nst, reps=5):
+    """One ground-state eigensolver step of the Si64 block: steepest_descent(ham,
+    prec, phi) with 5 preconditioned line minimisations per state and the closing
+    orthogonalisation (src/eigensolvers/steepest_descent.hpp:23-86, called with
+    these arguments from src/ground_state/calculator.hpp:160-168).  Timed per call
+    around a device synchronisation; the kernel split comes from the library's HIP
+    events in a separate pass."""
+    from inq_amd import DeviceArray
+    from inq_amd import eigensolvers as es
+    out = {}
+    for dtype, prec in ((np.complex64, "fp32"), (np.complex128, "fp64")):
+        d = DeviceArray.from_numpy(ctx, c.astype(dtype))
+        gh.prepare(b, nst, dtype)
+        es.steepest_descent(gh, b, d)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            es.steepest_descent(gh, b, d)
+        ctx.sync()
+        dt = (time.perf_counter() - t0) / reps
+        ctx.profile(True)
+        ctx.profile_reset()
+        es.steepest_descent(gh, b, d)
+        kt = ctx.kernel_times()
+        ctx.profile(False)
+        groups = {"h_apply": ("plane_", "xpass_", "kb_reduce"), "sd_vector_ops": ("sd_", "tpa_", "precondition"),
+                  "orthogonalize": ("overlap_", "cholesky", "trinv", "rotate_")}
+        split = {g: sum(ms for k, (cnt, ms) in kt.items() if k.startswith(pre)) for g, pre in groups.items()}
+        out[prec] = {"ms_per_call": dt * 1e3, "states_per_s": nst / dt,
+                     "kernel_ms": {g: v for g, v in split.items()},
+                     "kernel_ms_other": sum(ms for _, ms in kt.values()) - sum(split.values())}
+        d.free()
+    return out
+
+
 def c5_ace(ctx, cpu, cpu_seconds=8.0):
     """ACE build on Si216 (96^3, 432 complex64 states, Gamma): nst^2 pair-density
     Poisson solves + W^H phi + FP64 Cholesky + xi = W L^-H, and one ACE apply."""
@@ -401,6 +437,7 @@ def main():
     ap.add_argument("--copy", action="store_true", help="run the HBM copy reference even with --no-extras")
     ap.add_argument("--only-c5", action="store_true", help="run only the Si216 ACE line (profiling)")
     ap.add_argument("--only-contractions", action="store_true", help="run only the overlap / rotate timing")
+    ap.add_argument("--only-sd", action="store_true", help="run only the Si64 steepest-descent line")
     ap.add_argument("--c4-steps", type=int, default=2000, help="ETRS steps of the Si64 TDDFT line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -431,6 +468,10 @@ def main():
     nslot = len(np.unique(gidx // (sysm.n[1] * sysm.n[2])))
     n12 = sysm.n[1] * sysm.n[2]
     nsph = sum(len(gh.projector_points(a)[0]) for a in range(sysm.natoms))
+
+    if args.only_sd:
+        print(json.dumps({"sd_si64": sd_si64(ctx, gh, b, c, sysm.nst)}), flush=True)
+        return
 
     res, outs = {}, {}
     precs = {"fp32": [np.complex64], "fp64": [np.complex128], "both": [np.complex64, np.complex128]}[args.precision]
@@ -516,6 +557,8 @@ def main():
         log(f"bench: c5 {out['c5_ace']}")
         out["contractions"] = contractions(ctx)
         log(f"bench: contractions {out['contractions']}")
+        out["sd_si64"] = sd_si64(ctx, gh, b, c, sysm.nst)
+        log(f"bench: sd {out['sd_si64']}")
 
     cpu = None
     if not args.no_cpu:
