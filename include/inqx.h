@@ -202,6 +202,21 @@ int inqx_steepest_descent(inqx_ham ham, inqx_basis basis, int nst, int precision
  * call then synchronises).  FP64 throughout. */
 int inqx_hartree(inqx_ham ham, const double *density, double *vh, double *energy);
 
+/* Atomic-sphere fields: the per-atom sphere sums of
+ * atomic_potential::local_potential (short-range radial functions added on each
+ * atom's sphere, src/hamiltonian/atomic_potential.hpp:116-165) and
+ * atomic_potential::ionic_density (valence charge x long-range density,
+ * atomic_potential.hpp:170-200):
+ *   field[p] += sum_a sum_T f_{species[a]}(|r_p - R_a - T|)
+ * over the periodic images T with |r_p - R_a - T| <= radius[a].  f_s is given at
+ * r = j dr, j < nr (tables [nspecies][nr], host) and interpolated linearly;
+ * radius[a] <= (nr - 1) dr.  positions: host cartesian [natoms][3]; radius,
+ * species: host [natoms]; field: device double [N] in point order (i0, i1, i2).
+ * Each grid point adds its atoms in a fixed order (no atomics), so results are
+ * reproducible bit for bit.  The call synchronises the context's stream. */
+int inqx_atomic_field_add(inqx_ham ham, int natoms, const double *positions, const double *radius,
+                          const int *species, int nspecies, int nr, double dr, const double *tables, double *field);
+
 /* ------------------------------------------------------------ restart files */
 /* The reference's restart format (src/operations/io.hpp:41-211,
  * src/systems/electrons.hpp:323-381): raw native-endian files without headers,

@@ -1068,6 +1068,48 @@ int inqx_hartree(inqx_ham h, const double *density, double *vh, double *energy) 
   API_END(ctx)
 }
 
+int inqx_atomic_field_add(inqx_ham h, int natoms, const double *positions, const double *radius, const int *species,
+                          int nspecies, int nr, double dr, const double *tables, double *field) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && field && tables && natoms >= 0 && nspecies > 0 && nr >= 2 && dr > 0.0 && std::isfinite(dr),
+             INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(natoms == 0 || (positions && radius && species), INQX_ERR_ARG, "null atom arrays");
+  const Grid &g = h->g;
+  double rln[3];
+  for (int k = 0; k < 3; ++k) rln[k] = std::sqrt(g.rl[k][0] * g.rl[k][0] + g.rl[k][1] * g.rl[k][1] + g.rl[k][2] * g.rl[k][2]);
+  std::vector<double> buf((size_t)kAtomRec * natoms + (size_t)nspecies * nr);
+  for (int a = 0; a < natoms; ++a) {
+    const double r = radius[a];
+    INQX_CHECK(species[a] >= 0 && species[a] < nspecies, INQX_ERR_ARG, "species index out of range");
+    INQX_CHECK(std::isfinite(r) && r >= 0.0 && r <= (nr - 1) * dr, INQX_ERR_ARG,
+               "atom radius must lie within the radial table (0 <= radius <= (nr - 1) dr)");
+    double images = 1.0;  // upper bound of the lattice images one grid point visits
+    for (int k = 0; k < 3; ++k) images *= std::floor(2.0 * r * rln[k]) + 2.0;
+    INQX_CHECK(images <= 4096.0, INQX_ERR_ARG, "atom radius spans too many periodic images of the cell");
+    double x[3];
+    for (int k = 0; k < 3; ++k) INQX_CHECK(std::isfinite(positions[3 * a + k]), INQX_ERR_ARG, "non-finite position");
+    g.cart_to_contra(positions + 3 * a, x);
+    double *rec = buf.data() + (size_t)kAtomRec * a;
+    rec[0] = x[0];
+    rec[1] = x[1];
+    rec[2] = x[2];
+    rec[3] = r;
+    rec[4] = (double)species[a];
+  }
+  std::copy(tables, tables + (size_t)nspecies * nr, buf.begin() + (size_t)kAtomRec * natoms);
+  if (natoms > 0) {
+    h->atom_work.ensure(sizeof(double) * buf.size());
+    INQX_HIP(hipMemcpyAsync(h->atom_work.p, buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice,
+                            ctx->c.stream));
+    const double *dev = h->atom_work.as<double>();
+    launch_atomic_field_add(h, natoms, dev, nr, dr, dev + (size_t)kAtomRec * natoms, field);
+    INQX_HIP(hipStreamSynchronize(ctx->c.stream));  // buf is released on return
+  }
+  API_END(ctx)
+}
+
 // exchange_operator::direct in sphere form: out (+)= mask(FFT(sum_j f occ_j hf_j Poisson(conj(hf_j) phi_i)))
 static void exchange_direct_impl(inqx_ham h, Basis &B, int nst, int prec, const void *phi, int nhf, const void *hf,
                                  const double *occ, double coeff, double scale, double zeroterm, void *out,

@@ -281,6 +281,7 @@ struct inqx_ham_s {
   // steepest-descent workspace: H phi, residual, H residual, dot partials and results
   inqx::DevMem sd_states, sd_scal;
   inqx::DevMem hartree_work, hartree_part;  // Hartree solve: complex128 [N] and the energy partials
+  inqx::DevMem atom_work;                   // atomic-sphere fields: atom records and radial tables
   size_t exx_pair_bytes = 0;  // budget of the pair workspace (one j-chunk); 0: a quarter of free memory, <= 64 GiB
 };
 
@@ -348,5 +349,10 @@ bool pair_grid_supported(const int *n);
 // Hartree potential of a real density [N] (FP64): work holds N complex128,
 // part hartree_chunks(N) doubles of sum rho vh partials (fixed order)
 int hartree_chunks(long N);
+// atomic-sphere fields (potential.hip): per atom record = contravariant position (3),
+// radius, radial table index
+constexpr int kAtomRec = 5;
+void launch_atomic_field_add(inqx_ham_s *h, int natoms, const double *atoms_dev, int nr, double dr,
+                             const double *tables_dev, double *field);
 void launch_hartree(inqx_ham_s *h, const double *rho, double *vh, void *work, double *part);
 }  // namespace inqx

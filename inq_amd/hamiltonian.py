@@ -272,6 +272,21 @@ class KSHamiltonian:
               "inqx_hartree")
         return vh, (e.value if energy else None)
 
+    def atomic_field_add(self, positions, radius, species, tables, dr, field):
+        """field[p] += sum_a sum_T f_{species[a]}(|r_p - R_a - T|) within radius[a]
+        (the sphere sums of atomic_potential::local_potential / ionic_density,
+        atomic_potential.hpp:116-200); tables [nspecies][nr] at r = j dr,
+        interpolated linearly; field a device float64 array [N]."""
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        nat = pos.shape[0]
+        rad = np.ascontiguousarray(np.broadcast_to(np.asarray(radius, dtype=np.float64), (nat,)))
+        spc = np.ascontiguousarray(np.broadcast_to(np.asarray(species, dtype=np.int32), (nat,)))
+        tab = np.ascontiguousarray(np.atleast_2d(tables), dtype=np.float64)
+        check(lib().inqx_atomic_field_add(self.h, nat, _dp(pos), _dp(rad), spc.ctypes.data_as(C.POINTER(C.c_int)),
+                                          tab.shape[0], tab.shape[1], float(dr), _dp(tab), field.ptr), self.ctx.h,
+              "inqx_atomic_field_add")
+        return field
+
     def autotune(self, basis, nst, dtype):
         """times the x-pass launch shapes on this block size and keeps the fastest;
         returns (states per work-group, ms per apply)."""

@@ -457,3 +457,32 @@ def steepest_descent(apply, precond, phi, vol_el, num_steps=5):
             hphi += lam[:, None] * hres
     phi, info = orthogonalize(phi, vol_el, True)
     return phi, info
+
+
+def atomic_field(lattice, n, positions, radius, species, tables, dr, images=2):
+    """Brute-force atomic-sphere sum (atomic_potential::local_potential,
+    src/hamiltonian/atomic_potential.hpp:116-165, and ::ionic_density, :170-200):
+    field[p] = sum_a sum_T f_{species[a]}(|r_p - R_a - T|) for |...| <= radius[a],
+    over every lattice image T = s . lattice with s in [-images, images]^3 (the
+    caller makes the range wide enough), f interpolated linearly on r = j dr.
+    Points in (i0, i1, i2) order, r_p = sum_d (i_d / n_d) a_d.  FP64 numpy."""
+    lat = np.asarray(lattice, dtype=np.float64).reshape(3, 3)
+    n = [int(x) for x in n]
+    pos = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
+    nat = pos.shape[0]
+    rad = np.broadcast_to(np.asarray(radius, dtype=np.float64), (nat,))
+    spc = np.broadcast_to(np.asarray(species, dtype=np.int64), (nat,))
+    tab = np.atleast_2d(np.asarray(tables, dtype=np.float64))
+    rgrid = np.arange(tab.shape[1]) * dr
+    i0, i1, i2 = np.meshgrid(np.arange(n[0]) / n[0], np.arange(n[1]) / n[1], np.arange(n[2]) / n[2], indexing="ij")
+    frac = np.stack([i0.ravel(), i1.ravel(), i2.ravel()], axis=1)
+    pts = frac @ lat
+    out = np.zeros(pts.shape[0])
+    rng = np.arange(-images, images + 1)
+    shifts = np.array(np.meshgrid(rng, rng, rng, indexing="ij")).reshape(3, -1).T @ lat
+    for a in range(nat):
+        for t in shifts:
+            d = np.linalg.norm(pts - pos[a] - t, axis=1)
+            m = d <= rad[a]
+            out[m] += np.interp(d[m], rgrid, tab[spc[a]])
+    return out
