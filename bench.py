@@ -19,6 +19,8 @@ Secondary lines in the same JSON object (same seeded inputs):
   contractions  overlap and rotate on the matrix cores at the Si64 and Si216 block
                 sizes, TFLOP/s and fraction of the dense MFMA peak; FP64 Cholesky and L^-1
   sd_si64       steepest_descent (5 preconditioned steps + orthogonalize) on the Si64 block
+  scf_steps     Si216 ionic density (216 atomic spheres) and Hartree solve at 96^3;
+                restart save/load of 32 Si64 states
 The CPU baseline is the FP64 oracle (a C/OpenMP port of the reference CPU data
 flow) on a bounded subset of the Si64 states.
 
@@ -339,6 +341,70 @@ def sd_si64(ctx, gh, b, c, nst, reps=5):
     return out
 
 
+def scf_steps(ctx, gh64, b64, c64, reps=5):
+    """The steps around H in a self-consistent run, on the device:
+    ionic density of Si216 (216 atoms, 96^3, atomic_potential::ionic_density,
+    atomic_potential.hpp:170-200, declared Gaussian of width 0.625) and its
+    Hartree potential and energy (self_consistency.hpp:115-120), plus a restart
+    round trip of 32 Si64 states (io::save / io::load, io.hpp:168-211: one
+    complex128 real-space file per state) through a temporary directory."""
+    import shutil
+    import tempfile
+    from inq_amd import DeviceArray, KSHamiltonian
+    from inq_amd import io as inqio
+    from inq_amd import potential as pot
+    from inq_amd import systems as S
+    out = {}
+    sysm = S.config(5)
+    gh = KSHamiltonian(ctx, sysm.lattice, sysm.n)
+    z = np.full(sysm.natoms, 4.0)
+    rho = pot.ionic_density(gh, sysm.positions, z)
+    vh, eh = gh.hartree(rho)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rho.upload(np.zeros(sysm.N))
+        pot.ionic_density(gh, sysm.positions, z, field=rho)
+    ctx.sync()
+    t_rho = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gh.hartree(rho, vh)
+    ctx.sync()
+    t_h = (time.perf_counter() - t0) / reps
+    dv = abs(np.linalg.det(sysm.lattice)) / sysm.N
+    out["si216_ionic_density_ms"] = t_rho * 1e3
+    out["si216_ionic_charge"] = float(rho.numpy().sum() * dv)
+    out["si216_hartree_ms"] = t_h * 1e3
+    out["si216_hartree_energy_Ha"] = eh
+    rho.free()
+    vh.free()
+    nst = 32
+    d = DeviceArray.from_numpy(ctx, c64[:nst].astype(np.complex64))
+    back = DeviceArray(ctx, d.shape, np.complex64)
+    tmp = tempfile.mkdtemp(prefix="inqx_restart_")
+    try:
+        t0 = time.perf_counter()
+        inqio.save_states(gh64, b64, d, os.path.join(tmp, "states"))
+        t_save = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        inqio.load_states(gh64, b64, back, os.path.join(tmp, "states"))
+        ctx.sync()
+        t_load = time.perf_counter() - t0
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    nbytes = 16 * nst * int(np.prod(gh64.n))
+    a, bb = d.numpy(), back.numpy()
+    out["restart_states"] = nst
+    out["restart_bytes"] = nbytes
+    out["restart_save_gbs"] = nbytes / t_save / 1e9
+    out["restart_load_gbs"] = nbytes / t_load / 1e9
+    out["restart_round_trip_rel"] = float(np.max(np.abs(bb - a)) / np.max(np.abs(a)))
+    d.free()
+    back.free()
+    return out
+
+
 def c5_ace(ctx, cpu, cpu_seconds=8.0):
     """ACE build on Si216 (96^3, 432 complex64 states, Gamma): nst^2 pair-density
     Poisson solves + W^H phi + FP64 Cholesky + xi = W L^-H, and one ACE apply."""
@@ -438,6 +504,7 @@ def main():
     ap.add_argument("--only-c5", action="store_true", help="run only the Si216 ACE line (profiling)")
     ap.add_argument("--only-contractions", action="store_true", help="run only the overlap / rotate timing")
     ap.add_argument("--only-sd", action="store_true", help="run only the Si64 steepest-descent line")
+    ap.add_argument("--only-scf-steps", action="store_true", help="run only the ionic density / Hartree / restart line")
     ap.add_argument("--c4-steps", type=int, default=2000, help="ETRS steps of the Si64 TDDFT line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -471,6 +538,9 @@ def main():
 
     if args.only_sd:
         print(json.dumps({"sd_si64": sd_si64(ctx, gh, b, c, sysm.nst)}), flush=True)
+        return
+    if args.only_scf_steps:
+        print(json.dumps({"scf_steps": scf_steps(ctx, gh, b, c)}), flush=True)
         return
 
     res, outs = {}, {}
@@ -559,6 +629,8 @@ def main():
         log(f"bench: contractions {out['contractions']}")
         out["sd_si64"] = sd_si64(ctx, gh, b, c, sysm.nst)
         log(f"bench: sd {out['sd_si64']}")
+        out["scf_steps"] = scf_steps(ctx, gh, b, c)
+        log(f"bench: scf steps {out['scf_steps']}")
 
     cpu = None
     if not args.no_cpu:
