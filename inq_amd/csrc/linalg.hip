@@ -72,13 +72,20 @@ constexpr int kOvChunkMax = 2048;  // real K per work-group (split K), upper bou
 constexpr int kOvChunkMin = 256;   // lower bound when few output tiles call for more chunks
 constexpr int kOvGridTarget = 1024;  // work-groups that keep every CU busy with 4 of them
 
+constexpr size_t kOvPartialMax = size_t(1) << 30;  // bytes of partial tiles at most
+
 // real K per work-group: the full 2048 when the output has enough tiles, else
-// shorter chunks (a multiple of the LDS stage) so that the grid fills the chip
-static int overlap_chunk(long K2, int tiles) {
+// shorter chunks (a multiple of the LDS stage) so that the grid fills the chip;
+// longer ones when the partial tiles (nchunk x tiles x 64 x 64 complex) would
+// pass kOvPartialMax (thousands of states over a long K)
+static long overlap_chunk(long K2, int tiles, size_t es) {
   const long want = std::max<long>(1, (kOvGridTarget + tiles - 1) / tiles);
   long c = (K2 + want - 1) / want;
-  c = (c + kOvKC - 1) / kOvKC * kOvKC;
-  return (int)std::min<long>(kOvChunkMax, std::max<long>(kOvChunkMin, c));
+  c = std::min<long>(kOvChunkMax, std::max<long>(kOvChunkMin, c));
+  const size_t tile_bytes = es * (size_t)tiles * kOvTile * kOvTile * 2;
+  const long nchunk_max = std::max<long>(1, (long)(kOvPartialMax / tile_bytes));
+  c = std::max<long>(c, (K2 + nchunk_max - 1) / nchunk_max);
+  return (c + kOvKC - 1) / kOvKC * kOvKC;
 }
 
 // partial[chunk][tile_i][tile_j] holds a 64x64 complex tile in T.  Each stage
@@ -418,6 +425,62 @@ __global__ void __launch_bounds__(256) k_chol_diag(double *S, int n, int c0, int
   }
 }
 
+// n <= kCholSmall: the whole factorisation in one work-group, the lower triangle
+// packed in LDS (row i at i(i+1)/2, n(n+1)/2 x 16 bytes <= 132 KB).  Right-looking
+// with one barrier per column: the lanes update the trailing lower triangle from
+// the unscaled column j, then, after the barrier, scale column j while the next
+// column's update (which reads column j + 1 and writes columns > j + 1) runs.
+constexpr int kCholSmall = 128;
+__global__ void __launch_bounds__(1024) k_chol_small(double *S, int n, int *info) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double2 *T = reinterpret_cast<double2 *>(smem_raw);
+  const int tid = threadIdx.x, ty = tid >> 5, tx = tid & 31;
+  for (int i = ty; i < n; i += 32)
+    for (int k = tx; k <= i; k += 32) T[i * (i + 1) / 2 + k] = double2{S[2 * ((long)i * n + k)], S[2 * ((long)i * n + k) + 1]};
+  __syncthreads();
+  int bad = 0;
+  for (int j = 0; j < n; ++j) {
+    const double d = T[j * (j + 1) / 2 + j].x;
+    if (!(d > 0.0)) {
+      bad = j + 1;
+      break;
+    }
+    const double rd = 1.0 / d;
+    for (int i = j + 1 + ty; i < n; i += 32) {
+      const double2 a = T[i * (i + 1) / 2 + j];
+      const double ar = a.x * rd, ai = a.y * rd;
+      for (int k = j + 1 + tx; k <= i; k += 32) {  // L[i][j] conj(L[k][j]) = D[i][j] conj(D[k][j]) / d
+        const double2 b = T[k * (k + 1) / 2 + j];
+        double2 &t = T[i * (i + 1) / 2 + k];
+        t.x -= ar * b.x + ai * b.y;
+        t.y -= ai * b.x - ar * b.y;
+      }
+    }
+    __syncthreads();
+    if (tid < n - j) {
+      const int i = j + tid;
+      double2 &c = T[i * (i + 1) / 2 + j];
+      const double ljj = sqrt(d);
+      if (tid == 0) {
+        c = double2{ljj, 0.0};
+      } else {
+        const double rl = 1.0 / ljj;
+        c.x *= rl;
+        c.y *= rl;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) info[0] = bad;
+  if (bad) return;
+  for (int e = tid; e < n * n; e += 1024) {
+    const int i = e / n, k = e - i * n;
+    const double2 v = k <= i ? T[i * (i + 1) / 2 + k] : double2{0.0, 0.0};
+    S[2 * (long)e] = v.x;
+    S[2 * (long)e + 1] = v.y;
+  }
+}
+
 // one lane per row i >= c0 + 32 (launched only when such rows exist, so the panel is 32 wide)
 __global__ void __launch_bounds__(256) k_chol_panel(double *S, int n, int c0, const int *info) {
   __shared__ double Lr[kChB][kChB + 1], Li[kChB][kChB + 1], rdiag[kChB];
@@ -644,9 +707,11 @@ void launch_overlap(Ctx &cx, int prec, int n1, int n2, long K, const void *A, co
                     DevMem &partial) {
   const long K2 = 2 * K;
   const int ti = (n1 + kOvTile - 1) / kOvTile, tj = (n2 + kOvTile - 1) / kOvTile;
-  const int kchunk = overlap_chunk(K2, ti * tj);
-  const int nchunk = (int)((K2 + kchunk - 1) / kchunk);
   const size_t es = prec == INQX_FP64 ? 8 : 4;
+  const long kc = overlap_chunk(K2, ti * tj, es);
+  INQX_CHECK(kc <= INT32_MAX, INQX_ERR_ARG, "overlap: K too long");
+  const int kchunk = (int)kc;
+  const int nchunk = (int)((K2 + kchunk - 1) / kchunk);
   partial.ensure(es * (size_t)nchunk * ti * tj * kOvTile * kOvTile * 2);
   const int grid = nchunk * ti * tj;
   const int rgrid = ti * tj * kOvTile;
@@ -865,6 +930,16 @@ void launch_fill_zero(Ctx &cx, void *p, size_t bytes) {
 
 void launch_cholesky(Ctx &cx, int n, double *S, int *info) {
   ProfScope ps(cx, "cholesky_f64");
+  if (n <= kCholSmall) {
+    const size_t lds = sizeof(double2) * (size_t)n * (n + 1) / 2;
+    static DeviceOnce attr;
+    if (!attr.test_and_set(cx.device))
+      INQX_HIP(hipFuncSetAttribute((const void *)k_chol_small, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(sizeof(double2) * kCholSmall * (kCholSmall + 1) / 2)));
+    hipLaunchKernelGGL(k_chol_small, dim3(1), dim3(1024), lds, cx.stream, S, n, info);
+    INQX_HIP(hipGetLastError());
+    return;
+  }
   for (int c0 = 0; c0 < n; c0 += kChB) {
     hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, cx.stream, S, n, c0, info);
     INQX_HIP(hipGetLastError());

@@ -67,10 +67,11 @@ def test_cholesky_not_positive_definite_is_a_status(ctx):
     assert "status -4" in str(e.value)
 
 
-@pytest.mark.parametrize("n", [32, 33, 100, 433])
+@pytest.mark.parametrize("n", [32, 33, 100, 128, 129, 433])
 def test_blocked_cholesky_and_inverse_vs_scipy(ctx, n):
-    """Sizes across several 32-wide panels, a ragged last panel included; the
-    strict upper triangle of the input holds garbage that must be ignored."""
+    """Sizes on both Cholesky paths (one work-group up to 128, 32-wide panels
+    above, a ragged last panel included) and across several inverse block rows;
+    the strict upper triangle of the input holds garbage that must be ignored."""
     say("case: blocked cholesky / invert_triangular", n)
     a = _rand((n, n), 10 + n)
     s = a @ a.conj().T + n * np.eye(n)
@@ -85,16 +86,17 @@ def test_blocked_cholesky_and_inverse_vs_scipy(ctx, n):
     assert np.all(X[np.triu_indices(n, 1)] == 0.0)
 
 
-def test_cholesky_not_positive_definite_in_a_later_panel(ctx):
+@pytest.mark.parametrize("n,bad", [(100, 70), (200, 150)])
+def test_cholesky_not_positive_definite_in_a_later_panel(ctx, n, bad):
     from inq_amd import InqxError
-    a = _rand((100, 100), 11)
-    s = a @ a.conj().T + 100 * np.eye(100)
-    s[70, 70] = -5.0
+    a = _rand((n, n), 11)
+    s = a @ a.conj().T + n * np.eye(n)
+    s[bad, bad] = -5.0
     with pytest.raises(InqxError) as e:
         ops.cholesky(ctx, DeviceArray.from_numpy(ctx, s))
-    assert "minor 71 " in str(e.value)
+    assert f"minor {bad + 1} " in str(e.value)
     # the context stays usable, and the next factorisation starts clean
-    s[70, 70] = s[69, 69]
+    s[bad, bad] = s[bad - 1, bad - 1]
     L = ops.cholesky(ctx, DeviceArray.from_numpy(ctx, s)).numpy()
     assert np.max(np.abs(L - sla.cholesky(s, lower=True))) < 1e-11
 
