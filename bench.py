@@ -509,7 +509,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding of the path is described in DESIGN.md and not measured.", file=sys.stderr)
+              "the multi-GPU sharding of the path is design only and not measured.", file=sys.stderr)
         sys.exit(2)
 
     from inq_amd import Context, DeviceArray

@@ -1,7 +1,7 @@
 /*
  * oracle.c -- FP64 CPU restatement of LLNL/inq's Kohn-Sham H|psi> hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker.  Only tests/,
+ * TEST INFRASTRUCTURE ONLY: it checks the GPU results.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it (through
  * oracle/oracle.py).  The product library inq_amd/libinqx.so never links,
  * loads or calls anything here; the product path fails loudly when its HIP

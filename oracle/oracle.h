@@ -1,7 +1,7 @@
 /*
  * oracle.h -- FP64 CPU restatement of LLNL/inq's Kohn-Sham H|psi> hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  The oracle is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  The oracle checks results: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
  * product library (inq_amd/libinqx.so) never links, loads or calls it.
  *

@@ -1,8 +1,8 @@
 """ctypes wrapper of the FP64 CPU oracle (oracle/liborc.so).
 
 TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and
-bench.py's ``cpu_baseline`` leg may import this module, and only as the
-checker (or, for the baseline, as the timed CPU port).  The product package
+bench.py's ``cpu_baseline`` leg may import this module, and only to check
+results (or, for the baseline, as the timed CPU port).  The product package
 ``inq_amd`` never imports it.
 """
 import ctypes as C

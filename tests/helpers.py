@@ -1,5 +1,5 @@
 """Shared builders: one synthetic system set up identically in the HIP
-library (through the C ABI) and in the FP64 oracle (the checker)."""
+library (through the C ABI) and in the FP64 oracle that checks it."""
 import sys
 
 import numpy as np
