@@ -262,6 +262,15 @@ int inqx_prop_create(inqx_ham ham, int nblocks, const inqx_basis *bases, const i
 int inqx_prop_step(inqx_prop prop, int nsteps, int use_graph);
 /* observables of the current states (no propagation), appended to the ring */
 int inqx_prop_energy(inqx_prop prop);
+/* energy::calculate for a ground-state or any other caller, without a propagator
+ * (src/hamiltonian/energy.hpp:61-113 with ks_hamiltonian::non_local_energy,
+ * ks_hamiltonian.hpp:124-133, and observables::dipole, dipole.hpp:24-59): one H
+ * application of the given band-major device states of every block, reduced in
+ * FP64 into out[8] (host) = {kinetic, sum occ eigenvalue, non-local, sum of
+ * norms, dipole x, y, z, 0}, the record layout of the propagator.  occ: host
+ * arrays [nst[i]].  Synchronises the context's stream. */
+int inqx_energy(inqx_ham ham, int nblocks, const inqx_basis *bases, const int *nst, int precision,
+                void *const *phi, const double *const *occ, double *out);
 /* number of records written so far and the most recent min(count, cap) records [k][8] */
 int inqx_prop_records(inqx_prop prop, int *count, double *records, int cap);
 /* a propagator that captured a graph pins its Hamiltonian and bases: they

@@ -92,6 +92,7 @@ SIGNATURES = {
     "inqx_prop_step": (C.c_int, [P, C.c_int, C.c_int]),
     "inqx_prop_energy": (C.c_int, [P]),
     "inqx_prop_records": (C.c_int, [P, IP, DP, C.c_int]),
+    "inqx_energy": (C.c_int, [P, C.c_int, PP, IP, C.c_int, PP, PP, DP]),
     "inqx_prop_destroy": (C.c_int, [P]),
 }
 

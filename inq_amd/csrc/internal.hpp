@@ -282,6 +282,8 @@ struct inqx_ham_s {
   inqx::DevMem sd_states, sd_scal;
   inqx::DevMem hartree_work, hartree_part;  // Hartree solve: complex128 [N] and the energy partials
   inqx::DevMem atom_work;                   // atomic-sphere fields: atom records and radial tables
+  inqx::DevMem en_ring, en_counter, en_occ, en_desc;  // inqx_energy: one observables record
+  std::vector<char> en_host;
   size_t exx_pair_bytes = 0;  // budget of the pair workspace (one j-chunk); 0: a quarter of free memory, <= 64 GiB
 };
 

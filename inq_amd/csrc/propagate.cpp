@@ -323,6 +323,45 @@ int inqx_prop_step(inqx_prop p, int nsteps, int use_graph) {
   API_END(ctx)
 }
 
+int inqx_energy(inqx_ham h, int nblocks, const inqx_basis *bases, const int *nst, int precision, void *const *phi,
+                const double *const *occ, double *out) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && nblocks > 0 && bases && nst && phi && occ && out, INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
+  std::vector<Basis *> bb(nblocks);
+  std::vector<double> occ_h;
+  for (int i = 0; i < nblocks; ++i) {
+    INQX_CHECK(bases[i] && bases[i]->b.ham == h, INQX_ERR_ARG, "basis belongs to another Hamiltonian");
+    INQX_CHECK(nst[i] > 0 && phi[i] && occ[i], INQX_ERR_ARG, "bad block");
+    bb[i] = &bases[i]->b;
+    occ_h.insert(occ_h.end(), occ[i], occ[i] + nst[i]);
+  }
+  h->en_ring.ensure(8 * sizeof(double));
+  h->en_counter.ensure(sizeof(int));
+  h->en_occ.ensure(sizeof(double) * occ_h.size());
+  const int zero = 0;
+  INQX_HIP(hipMemcpyAsync(h->en_occ.p, occ_h.data(), sizeof(double) * occ_h.size(), hipMemcpyHostToDevice,
+                          ctx->c.stream));
+  INQX_HIP(hipMemcpyAsync(h->en_counter.p, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->c.stream));
+  std::vector<const double *> occ_d(nblocks);
+  size_t off = 0;
+  for (int i = 0; i < nblocks; ++i) {
+    occ_d[i] = h->en_occ.as<double>() + off;
+    off += nst[i];
+  }
+  if (precision == INQX_FP64)
+    energy_record<double>(h, nblocks, bb.data(), nst, phi, occ_d.data(), h->en_ring.as<double>(), 1,
+                          h->en_counter.as<int>(), h->en_desc, h->en_host, false);
+  else
+    energy_record<float>(h, nblocks, bb.data(), nst, phi, occ_d.data(), h->en_ring.as<double>(), 1,
+                         h->en_counter.as<int>(), h->en_desc, h->en_host, false);
+  INQX_HIP(hipMemcpyAsync(out, h->en_ring.p, 8 * sizeof(double), hipMemcpyDeviceToHost, ctx->c.stream));
+  INQX_HIP(hipStreamSynchronize(ctx->c.stream));  // also keeps occ_h and zero alive until copied
+  API_END(ctx)
+}
+
 int inqx_prop_records(inqx_prop p, int *count, double *out, int cap) {
   inqx_ctx ctx = p ? p->ham->ctx : nullptr;
   API_BEGIN

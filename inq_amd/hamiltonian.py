@@ -272,6 +272,23 @@ class KSHamiltonian:
               "inqx_hartree")
         return vh, (e.value if energy else None)
 
+    def energy(self, bases, phis, occs):
+        """energy::calculate (energy.hpp:61-113) of the states of every block, with
+        the non-local energy (ks_hamiltonian.hpp:124-133) and the dipole
+        (dipole.hpp:24-59): one H application, FP64 reductions.  Returns
+        {kinetic, eigenvalues (sum occ eps), non_local, norm, dipole}."""
+        nb = len(bases)
+        occ = [np.ascontiguousarray(o, dtype=np.float64) for o in occs]
+        bb = (C.c_void_p * nb)(*[b.h.value for b in bases])
+        nst = (C.c_int * nb)(*[p.shape[0] for p in phis])
+        pp = (C.c_void_p * nb)(*[p.ptr.value for p in phis])
+        oc = (C.c_void_p * nb)(*[o.ctypes.data for o in occ])
+        out = np.zeros(8)
+        check(lib().inqx_energy(self.h, nb, bb, nst, precision_of(phis[0].dtype), pp, oc, _dp(out)), self.ctx.h,
+              "inqx_energy")
+        return {"kinetic": out[0], "eigenvalues": out[1], "non_local": out[2], "norm": out[3],
+                "dipole": out[4:7].copy()}
+
     def atomic_field_add(self, positions, radius, species, tables, dr, field):
         """field[p] += sum_a sum_T f_{species[a]}(|r_p - R_a - T|) within radius[a]
         (the sphere sums of atomic_potential::local_potential / ionic_density,
