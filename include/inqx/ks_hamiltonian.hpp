@@ -81,6 +81,13 @@ template <typename Complex> class orbital_block {
   }
 };
 
+// the terms of hamiltonian::energy::calculate (energy.hpp:61-113) for a set of
+// states, occupation-weighted and reduced in FP64, with the electronic dipole
+struct energy_terms {
+  double kinetic = 0.0, eigenvalues = 0.0, non_local = 0.0, norm = 0.0;
+  double dipole[3] = {0.0, 0.0, 0.0};
+};
+
 class ks_hamiltonian {
   inqx_ctx ctx_;
   inqx_ham h_ = nullptr;
@@ -123,6 +130,30 @@ class ks_hamiltonian {
   template <typename Complex> void apply(const orbital_block<Complex> &phi, orbital_block<Complex> &hphi) const {
     check(inqx_ham_apply(h_, phi.basis(), phi.set_size(), orbital_block<Complex>::precision, phi.data(), hphi.data()),
           ctx_, "apply");
+  }
+
+  // energy::calculate's sums for one block (one H application); occ has set_size() entries
+  template <typename Complex>
+  energy_terms energy(const orbital_block<Complex> &phi, const std::vector<double> &occ) const {
+    if ((int)occ.size() != phi.set_size()) throw error(INQX_ERR_ARG, "energy: one occupation per state");
+    inqx_basis b = phi.basis();
+    const int nst = phi.set_size();
+    void *p = const_cast<void *>(phi.data());
+    const double *o = occ.data();
+    double out[8];
+    check(inqx_energy(h_, 1, &b, &nst, orbital_block<Complex>::precision, &p, &o, out), ctx_, "energy");
+    energy_terms e;
+    e.kinetic = out[0];
+    e.eigenvalues = out[1];
+    e.non_local = out[2];
+    e.norm = out[3];
+    for (int d = 0; d < 3; ++d) e.dipole[d] = out[4 + d];
+    return e;
+  }
+  // non_local_energy(phi, occupations) (ks_hamiltonian.hpp:124-133)
+  template <typename Complex>
+  double non_local_energy(const orbital_block<Complex> &phi, const std::vector<double> &occ) const {
+    return energy(phi, occ).non_local;
   }
 };
 

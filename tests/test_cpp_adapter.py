@@ -29,6 +29,10 @@ int main(int argc, char **) {
     inqx::operations::preconditioner prec(ham);
     inqx::eigensolvers::steepest_descent(ham, prec, phi);
     inqx::eigensolvers::steepest_descent(ham, inqx::operations::no_preconditioner{}, phi);
+    std::vector<double> occ(4, 2.0);
+    inqx::energy_terms e = ham.energy(phi, occ);
+    double enl = ham.non_local_energy(phi, occ);
+    if (e.kinetic + enl < -1e300) return 2;
   }
   return inqx_version() >= 1 ? 0 : 1;
 }
