@@ -505,6 +505,7 @@ def main():
     ap.add_argument("--only-contractions", action="store_true", help="run only the overlap / rotate timing")
     ap.add_argument("--only-sd", action="store_true", help="run only the Si64 steepest-descent line")
     ap.add_argument("--only-scf-steps", action="store_true", help="run only the ionic density / Hartree / restart line")
+    ap.add_argument("--only-c4", action="store_true", help="run only the Si64 TDDFT line (profiling)")
     ap.add_argument("--c4-steps", type=int, default=2000, help="ETRS steps of the Si64 TDDFT line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -521,6 +522,9 @@ def main():
         return
     if args.only_contractions:
         print(json.dumps({"contractions": contractions(Context(0))}), flush=True)
+        return
+    if args.only_c4:
+        print(json.dumps({"c4_tddft": c4_tddft(Context(0), args.c4_steps)}), flush=True)
         return
 
     sysm = S.config(2)
