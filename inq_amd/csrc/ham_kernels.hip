@@ -866,6 +866,8 @@ __global__ void __launch_bounds__(256) k_obs_combine(const ObsBlock *ob, int nbl
     const ObsBlock B = ob[b];
     for (int s = tid; s < B.nst; s += 256) {
       double d[4] = {0, 0, 0, 0};
+      // unrolled so that several loads of each chain are in flight (one lane per state)
+#pragma unroll 8
       for (int ch = 0; ch < B.nch; ++ch)
         for (int c = 0; c < 4; ++c) d[c] += B.dots[((size_t)s * B.nch + ch) * 4 + c];
       const double occ = B.occ[s];
@@ -875,7 +877,9 @@ __global__ void __launch_bounds__(256) k_obs_combine(const ObsBlock *ob, int nbl
       double enl = 0.0;
       if (B.proj) {
         const C *pj = reinterpret_cast<const C *>(B.proj), *dq = reinterpret_cast<const C *>(B.dp);
+#pragma unroll 2
         for (int a = 0; a < natoms; ++a)
+#pragma unroll
           for (int l = 0; l < kMaxNlm; ++l) {
             const C p = pj[((size_t)a * kMaxNlm + l) * B.nst + s], q = dq[((size_t)a * kMaxNlm + l) * B.nst + s];
             enl += (double)p.x * q.x + (double)p.y * q.y;
@@ -884,6 +888,7 @@ __global__ void __launch_bounds__(256) k_obs_combine(const ObsBlock *ob, int nbl
       acc[2] += occ * enl;
       if (B.dip) {
         double dx = 0, dy = 0, dz = 0;
+#pragma unroll 8
         for (int t = 0; t < ntile; ++t) {
           const double *q = B.dip + ((size_t)t * B.nst + s) * 4;
           dx += q[1];
