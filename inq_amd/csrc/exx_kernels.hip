@@ -139,7 +139,11 @@ __global__ void __launch_bounds__(256) k_pair_plane_fwd(PairDev<T> d, int total)
   const int line = pair_line(tid);
   const bool upper = (tid & 63) >= 32;
   const int half = upper ? 1 : 0;
-  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+  // one work item per work-group, the grid being the work count: a grid-stride loop
+  // here made the complex128 96-point kernels spill to scratch
+  {
+    const int wg = blockIdx.x;
+    if (wg >= total) return;
     const int i = wg % d.nst;
     const int r = wg / d.nst;
     const int i0 = r % N0;
@@ -193,10 +197,14 @@ __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
   const bool upper = lane >= 32;
   const int half = upper ? 1 : 0;
-  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+  // one work item per work-group, the grid being the work count: a grid-stride loop
+  // here made the complex128 96-point kernels spill to scratch
+  {
+    const int wg = blockIdx.x;
+    if (wg >= total) return;
     const int pair = wg / NGRP;
     const int tile = (wg - pair * NGRP) * 4 + wave;
-    if (tile >= NTILE) continue;  // wave-uniform
+    if (tile >= NTILE) return;  // wave-uniform; no barrier follows
     const int L = tile * 32 + (lane & 31);
     const int Lc = L < NP ? L : NP - 1;
     const auto rs = make_rsrc(d.t + (size_t)pair * N, (unsigned)(N * sizeof(C)));
@@ -247,7 +255,11 @@ __global__ void __launch_bounds__(256) k_pair_plane_inv(PairDev<T> d, int total)
   const int line = pair_line(tid);
   const bool upper = (tid & 63) >= 32;
   const int half = upper ? 1 : 0;
-  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+  // one work item per work-group, the grid being the work count: a grid-stride loop
+  // here made the complex128 96-point kernels spill to scratch
+  {
+    const int wg = blockIdx.x;
+    if (wg >= total) return;
     const int i = wg % d.nst;
     const int r = wg / d.nst;
     const int i0 = r % N0;
