@@ -194,7 +194,9 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const bool upper = lane >= 32;
   const int half = upper ? 1 : 0;
-  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+  {  // one work item per work-group: the grid is the work count
+    const int wg = blockIdx.x;
+    if (wg >= total) return;
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
     const int local = wg - gp(prefix)[b];
@@ -274,7 +276,9 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const bool upper = lane >= 32;
   const int half = upper ? 1 : 0;
-  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+  {  // one work item per work-group: the grid is the work count
+    const int wg = blockIdx.x;
+    if (wg >= total) return;
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
     const int local = wg - gp(prefix)[b];
@@ -284,7 +288,7 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
     const int np = min(P, B.nst - s0);
     const V4<int> sg = ld4<int>(B.slot_seg, slot);
     const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
-    if (nseg == 0) continue;  // uniform: nothing of this plane is on the sphere
+    if (nseg == 0) return;  // uniform: nothing of this plane is on the sphere
     if constexpr (Cfg::SPLIT_F) {
       // columns along i1 from the plane slots, then the sphere rows along i2,
       // a lane pair per line; the np planes are slot `slot` of states s0..s0+np-1
@@ -538,7 +542,9 @@ __global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blo
   const bool upper = lane >= 32;
   C *pts = reinterpret_cast<C *>(smem_raw) + wave * lds_per_wave;
   C *acc = pts + hd.maxpts;
-  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+  {  // one work item per work-group: the grid is the work count
+    const int wg = blockIdx.x;
+    if (wg >= total) return;
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
     const int local = wg - gp(prefix)[b];
@@ -630,11 +636,13 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                    int total, HamDev hd) {
   using C = cplx<T>;
-  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+  {  // one work item per work-group: the grid is the work count
+    const int wg = blockIdx.x;
+    if (wg >= total) return;
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
     const int idx = (wg - gp(prefix)[b]) * blockDim.x + threadIdx.x;
-    if (idx >= hd.natoms * B.nst) continue;
+    if (idx >= hd.natoms * B.nst) return;
     const int a = idx / B.nst, s = idx - a * B.nst;
     double pr[kMaxNlm], pi[kMaxNlm];
 #pragma unroll
@@ -685,7 +693,9 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
   const bool upper = lane >= 32;
   C *pts = reinterpret_cast<C *>(smem_raw) + wave * lds_per_wave;
   C *dpl = pts + hd.maxpts;
-  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+  {  // one work item per work-group: the grid is the work count
+    const int wg = blockIdx.x;
+    if (wg >= total) return;
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
     const int local = wg - gp(prefix)[b];
@@ -1144,14 +1154,16 @@ __global__ void __launch_bounds__(256) k_line_to_real(const SphereBlock<T> *bloc
   constexpr int H = N0 / 2;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
   const bool upper = lane >= 32;
-  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+  {  // one work item per work-group: the grid is the work count
+    const int wg = blockIdx.x;
+    if (wg >= total) return;
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
     const int local = wg - gp(prefix)[b];
     const int ngrp = (B.nst + 3) >> 2;
     const int tile = local / ngrp;
     const int s = (local - tile * ngrp) * 4 + wave;
-    if (s >= B.nst) continue;  // wave-uniform
+    if (s >= B.nst) return;  // wave-uniform
     const int L = tile * 32 + (lane & 31);
     const int Lc = L < n12 ? L : n12 - 1;
     C v[H];
@@ -1182,14 +1194,16 @@ __global__ void __launch_bounds__(256) k_line_from_real(const SphereBlock<T> *bl
   constexpr int H = N0 / 2;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
   const bool upper = lane >= 32;
-  for (int wg = blockIdx.x; wg < total; wg += gridDim.x) {
+  {  // one work item per work-group: the grid is the work count
+    const int wg = blockIdx.x;
+    if (wg >= total) return;
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
     const int local = wg - gp(prefix)[b];
     const int ngrp = (B.nst + 3) >> 2;
     const int tile = local / ngrp;
     const int s = (local - tile * ngrp) * 4 + wave;
-    if (s >= B.nst) continue;
+    if (s >= B.nst) return;
     const int L = tile * 32 + (lane & 31);
     const int Lc = L < n12 ? L : n12 - 1;
     C v[H];
