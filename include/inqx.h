@@ -97,11 +97,15 @@ int inqx_ham_create(inqx_ctx ctx, const double lattice[9], const int n[3], inqx_
 int inqx_ham_destroy(inqx_ham ham);
 /* ks_hamiltonian::scalar_potential() (ks_hamiltonian.hpp:241-243): host array
  * [N][ncomp] in Hartree; spin index `spin` of a basis selects the column
- * (src/hamiltonian/scalar_potential.hpp:31-34). */
+ * (src/hamiltonian/scalar_potential.hpp:31-34).  INQX_ERR_ARG when a basis of
+ * this Hamiltonian has a spin index >= ncomp (an apply on such a basis is
+ * rejected the same way); INQX_ERR_STATE while a propagator with a captured
+ * graph uses the Hamiltonian. */
 int inqx_ham_set_potential(inqx_ham ham, const double *v, int ncomp);
 /* ks_hamiltonian::uniform_vector_potential() (ks_hamiltonian.hpp:257-262):
  * covariant A added to every block's k-point (velocity-gauge kick,
- * src/perturbations/kick.hpp:36-48). */
+ * src/perturbations/kick.hpp:36-48).  INQX_ERR_STATE while a propagator with
+ * a captured graph uses the Hamiltonian. */
 int inqx_ham_set_vector_potential(inqx_ham ham, const double a_cov[3]);
 /* one projector of projector_all (src/hamiltonian/projector.hpp:102-113,
  * projector_all.hpp:40-109).  The sphere is built as
@@ -323,7 +327,9 @@ int inqx_poisson(inqx_ham ham, int n, int precision, void *rho, const double gsh
  * at the Gamma point, in sphere form: exxphi += mask(FFT(sum_j f occ_j hf_j
  * Poisson(conj(hf_j) phi_i))), f = -exx_coeff scale / 2, phi_i = to_real of
  * the sphere states (j with |occ_j| < 1e-10 skipped).  phi [nst][npw], hf
- * [nhf][npw] and exxphi [nst][npw] on the basis' sphere; occ host [nhf]. */
+ * [nhf][npw] and exxphi [nst][npw] on the basis' sphere; occ host [nhf].
+ * complex128 (INQX_FP64) is offered up to a 64^3 grid; larger grids return
+ * INQX_ERR_UNSUPPORTED for it (the same holds for inqx_ace_build). */
 int inqx_exchange_direct(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *phi, int nhf,
                          const void *hf, const double *occ, double exx_coeff, double scale, double zeroterm,
                          void *exxphi);

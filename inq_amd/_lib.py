@@ -13,6 +13,12 @@ _LIB = None
 INQX_OK = 0
 INQX_FP64 = 0
 INQX_FP32 = 1
+# status codes (inqx_status in include/inqx.h)
+INQX_ERR_ARG = -1
+INQX_ERR_HIP = -2
+INQX_ERR_UNSUPPORTED = -3
+INQX_ERR_NOT_PD = -4
+INQX_ERR_STATE = -5
 
 # name -> (restype, argtypes); every symbol include/inqx.h declares
 P = C.c_void_p

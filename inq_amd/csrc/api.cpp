@@ -446,6 +446,12 @@ int inqx_ham_set_potential(inqx_ham h, const double *v, int ncomp) {
   inqx_ctx ctx = h ? h->ctx : nullptr;
   API_BEGIN
   INQX_CHECK(h && v && ncomp >= 1 && ncomp <= 4, INQX_ERR_ARG, "bad potential arguments");
+  // a captured propagation graph holds the device addresses of V and its component count
+  INQX_CHECK(h->pinned == 0, INQX_ERR_STATE, "potential pinned by a captured propagation graph");
+  for (inqx_basis bs : h->bases)
+    INQX_CHECK(bs->b.spin < ncomp, INQX_ERR_ARG,
+               "a basis of this Hamiltonian has spin index " + std::to_string(bs->b.spin) + ", beyond the " +
+                   std::to_string(ncomp) + " potential components");
   const long N = h->g.N;
   invalidate_launch_cache(h);
   h->ncomp = ncomp;
@@ -467,9 +473,14 @@ int inqx_ham_set_potential(inqx_ham h, const double *v, int ncomp) {
 }
 
 int inqx_ham_set_vector_potential(inqx_ham h, const double a_cov[3]) {
-  if (!h || !a_cov) return INQX_ERR_ARG;
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && a_cov, INQX_ERR_ARG, "null argument");
+  for (int d = 0; d < 3; ++d) INQX_CHECK(std::isfinite(a_cov[d]), INQX_ERR_ARG, "non-finite vector potential");
+  // a captured propagation graph holds the kinetic table and KB phases of the current k + A
+  INQX_CHECK(h->pinned == 0, INQX_ERR_STATE, "vector potential pinned by a captured propagation graph");
   for (int d = 0; d < 3; ++d) h->A[d] = a_cov[d];
-  return INQX_OK;
+  API_END(ctx)
 }
 
 int inqx_ham_add_projector(inqx_ham h, const double center[3], double radius, int nlm, const double *matrix,
@@ -1116,6 +1127,12 @@ static void exchange_direct_impl(inqx_ham h, Basis &B, int nst, int prec, const 
                                  bool accumulate) {
   Ctx &cx = h->ctx->c;
   INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "exchange needs a cubic grid of a built size");
+  // Repeated complex128 exchanges on identical inputs have differed in a few
+  // output states (about 1e-3 relative) at 72^3 and 96^3; no cause in the code
+  // has been shown, so the complex128 pair path with occupied states is not
+  // offered above 64^3.  Plain complex128 Poisson solves are not affected.
+  INQX_CHECK(!(prec == INQX_FP64 && h->g.n[0] > kExxFp64MaxGrid), INQX_ERR_UNSUPPORTED,
+             "complex128 exchange is not offered above 64^3 (use complex64, or the CPU oracle as the FP64 reference)");
   const size_t per = (prec == INQX_FP64 ? 16 : 8) * (size_t)h->g.N;
   const double factor = -0.5 * scale * coeff;
   std::vector<int> jl;

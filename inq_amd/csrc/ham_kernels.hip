@@ -1332,6 +1332,10 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   INQX_CHECK(grid_supported(g.n), INQX_ERR_UNSUPPORTED, "grid size not built for the sphere path");
   if (mode == 0) {
     INQX_CHECK(h->has_v, INQX_ERR_STATE, "potential not set");
+    for (int i = 0; i < nblocks; ++i)
+      INQX_CHECK(bases[i]->spin < h->ncomp, INQX_ERR_ARG,
+                 "basis spin index " + std::to_string(bases[i]->spin) + " has no potential component (ncomp " +
+                     std::to_string(h->ncomp) + ")");
     if (!h->kb.built) build_kb_tables(h);
     if (prec == INQX_FP32 && !h->v32.p) INQX_CHECK(false, INQX_ERR_STATE, "potential not uploaded");
   } else {

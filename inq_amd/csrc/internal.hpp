@@ -149,6 +149,7 @@ struct Projector {
 #define INQX_SIZES(X) X(12) X(16) X(18) X(20) X(24) X(28) X(30) X(32) X(36) X(40) X(48) X(54) X(60) X(64) X(72) X(80) X(96)
 
 constexpr int kMaxNlm = 4;   // s + 3 p per atom, padded
+constexpr int kExxFp64MaxGrid = 64;  // largest grid side of the complex128 exchange
 constexpr int kTile = 32;    // x-pencils (lines) per tile: one wavefront, a lane pair per line
 
 // tables of the real-space KB pass, shared by all blocks of a Hamiltonian

@@ -87,3 +87,21 @@ def test_gsphere_matches_oracle(cid):
         g = b.gindex()
         assert np.array_equal(g, oh.gsphere(k, ec))
         assert np.max(np.abs(b.kinetic() - oh.gsphere_kinetic(k, g))) < 1e-12
+
+
+def test_potential_components_cover_every_basis_spin():
+    """A basis with spin index s reads column s of the potential, so a potential
+    with ncomp <= s is rejected (INQX_ERR_ARG), and so is the apply on such a
+    basis; ncomp = s + 1 is accepted."""
+    ctx = Context(device=-1)
+    n = (12, 12, 12)
+    h = KSHamiltonian(ctx, np.eye(3) * 6.0, n)
+    b1 = h.basis((0.0, 0.0, 0.0), 4.0, spin=1)
+    N = int(np.prod(n))
+    v1 = np.zeros(N)
+    st = _lib.lib().inqx_ham_set_potential(h.h, v1.ctypes.data_as(C.POINTER(C.c_double)), 1)
+    assert st == _lib.INQX_ERR_ARG
+    assert b"spin index 1" in _lib.lib().inqx_last_error(ctx.h)
+    v2 = np.zeros((N, 2))
+    h.set_scalar_potential(v2, ncomp=2)
+    del b1

@@ -102,3 +102,65 @@ def test_velocity_gauge_kick(ctx):
     gh.uniform_vector_potential = np.zeros(3)
     out0 = gh(b, d).numpy()
     assert rel_err(out0, oh.apply_sphere(c, b.gindex(), (0, 0, 0))) < 1e-11
+
+
+_C4_REF = {}
+
+
+def _c4_reference():
+    """the Si64 TDDFT workload (72^3, E_cut 30 Ha, 64 atoms, velocity-gauge kick
+    A = -0.01 x): the first 8 seeded states propagated 2 ETRS steps by the
+    oracle, with the energy/dipole record of those 8 states before and after
+    every step.  H does not depend on the density, so every state evolves on
+    its own and 8 states of the 128-state block are a valid subset."""
+    if not _C4_REF:
+        sysm = S.config(4)
+        v = S.local_potential(sysm)
+        A = sysm.lattice @ np.array([-0.01, 0.0, 0.0])
+        from inq_amd import KSHamiltonian
+        gh0 = KSHamiltonian(Context(-1), sysm.lattice, sysm.n)
+        g = gh0.basis((0, 0, 0), sysm.ecut).gindex()
+        c = S.random_states(len(g), sysm.nst, sysm.seed, S.sphere_volume_element(sysm))
+        oh = oracle_ham(sysm, v)
+        occ = np.full(8, 2.0)
+        ref = c[:8].copy()
+        recs = []
+        for step in range(3):
+            if step > 0:
+                ref = oh.etrs_step(ref, g, 0.05, k=A)
+            e = oh.energy_sphere(ref, g, occ, k=A)
+            _, dip = oh.density_dipole_sphere(ref, g, occ)
+            recs.append(np.concatenate([e, dip]))
+        _C4_REF.update(sysm=sysm, v=v, A=A, c=c, states=ref, recs=np.array(recs))
+    return _C4_REF
+
+
+@pytest.mark.parametrize("dtype,tol,etol", [(np.complex64, 2e-5, 1e-5), (np.complex128, 1e-11, 1e-10)])
+def test_c4_si64_block_two_steps(ctx, dtype, tol, etol):
+    """two graph-captured ETRS steps of the whole 128-state Si64 block with the
+    kick; states 0..7 against the oracle, and the records of an 8-state block
+    against the oracle's energy and dipole of the same states"""
+    say("case: C4 Si64 72^3 128 states, 2 ETRS steps", np.dtype(dtype).name)
+    R = _c4_reference()
+    sysm = R["sysm"]
+    gh = gpu_ham(ctx, sysm, R["v"])
+    gh.uniform_vector_potential = R["A"]
+    b = gh.basis((0, 0, 0), sysm.ecut)
+    d = DeviceArray.from_numpy(ctx, R["c"].astype(dtype))
+    prop = rt.Propagator(gh, [b], [d], [np.full(sysm.nst, 2.0)], dt=0.05)
+    prop.step(1, graph=False)
+    prop.step(1, graph=True)
+    ctx.sync()
+    assert rel_err(d.numpy()[:8], R["states"]) < tol
+    prop.close()
+    d8 = DeviceArray.from_numpy(ctx, np.ascontiguousarray(R["c"][:8]).astype(dtype))
+    p8 = rt.Propagator(gh, [b], [d8], [np.full(8, 2.0)], dt=0.05)
+    p8.energy()
+    p8.step(2, graph=False)
+    recs = p8.records()
+    assert recs.shape == (3, 8)
+    want = R["recs"]
+    err = np.max(np.abs(recs[:, :7] - want) / np.maximum(np.abs(want), 1.0))
+    assert err < etol, (recs[:, :7], want)
+    assert rel_err(d8.numpy(), R["states"]) < tol
+    p8.close()
