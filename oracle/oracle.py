@@ -74,6 +74,14 @@ def lib():
             "orc_ace_build": (C.c_double, [P, C.c_int, cp, dp, C.c_double, cp]),
             "orc_exchange_direct": (None, [P, C.c_int, cp, C.c_int, cp, dp, C.c_double, C.c_double, cp]),
             "orc_ace_apply": (None, [P, C.c_int, cp, C.c_int, cp, cp]),
+            # refcpu.c: the reference CPU data flow on FFTW3/BLAS/LAPACK (CPU baseline)
+            "rc_init": (C.c_int, []),
+            "rc_backend": (C.c_int, []),
+            "rc_ham_create": (P, [dp, ip, C.c_int, dp, C.c_int, ip, ip, C.c_int, dp, dp, C.c_int]),
+            "rc_ham_destroy": (None, [P]),
+            "rc_ham_apply": (None, [P, cp, cp]),
+            "rc_overlap_cholesky": (C.c_int, [C.c_long, C.c_int, C.c_double, cp, cp]),
+            "rc_exchange_pairs": (None, [dp, ip, C.c_int, cp, C.c_int, cp, dp, cp, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -129,6 +137,7 @@ class Ham:
         self.N = self.n[0] * self.n[1] * self.n[2]
         self.h = lib().orc_ham_create(self.lattice.ravel(), np.asarray(self.n, dtype=np.int32))
         self.projs = []
+        self._proj_meta = []
 
     def __del__(self):
         if getattr(self, "h", None) and _LIB is not None:
@@ -157,11 +166,15 @@ class Ham:
         i = lib().orc_ham_add_projector_gauss(self.h, _f64(center), float(radius), int(nlm), float(width),
                                               _f64(coeff).ravel())
         self.projs.append(i)
+        self._proj_meta.append((int(nlm), _f64(coeff).ravel().copy()))
         return i
 
     def add_projector_matrix(self, center, radius, nlm, matrix, coeff):
-        return lib().orc_ham_add_projector_matrix(self.h, _f64(center), float(radius), int(nlm),
-                                                  _f64(matrix).ravel(), _f64(coeff).ravel())
+        i = lib().orc_ham_add_projector_matrix(self.h, _f64(center), float(radius), int(nlm),
+                                               _f64(matrix).ravel(), _f64(coeff).ravel())
+        self.projs.append(i)
+        self._proj_meta.append((int(nlm), _f64(coeff).ravel().copy()))
+        return i
 
     def projector_points(self, i, nlm):
         L = lib()
@@ -485,4 +498,69 @@ def atomic_field(lattice, n, positions, radius, species, tables, dr, images=2):
             d = np.linalg.norm(pts - pos[a] - t, axis=1)
             m = d <= rad[a]
             out[m] += np.interp(d[m], rgrid, tab[spc[a]])
+    return out
+
+
+class RefCPU:
+    """The reference's CPU data flow for H on full-cube fourier_space orbitals
+    [N][nst] (refcpu.c): batched FFTW3 transforms over the state axis, V psi,
+    KB projectors as gather / DGEMM / coefficients / DGEMM / scatter, to_fourier,
+    laplacian_add; OpenMP threads own chunks of states, the FFTW/BLAS library
+    runs sequentially in each.  Gamma point only.  ``backend`` is "mkl-fftw3"
+    when MKL's FFTW3/BLAS/LAPACK interfaces were loaded, else "own-fft"."""
+
+    def __init__(self, ham, nst, threads):
+        L = lib()
+        L.rc_init()
+        self.nst = int(nst)
+        self.N = ham.N
+        npts, pts, beta, coeff = [], [], [], []
+        self.nlm = 4
+        for i, (nlm, D) in enumerate(ham._proj_meta):
+            co, _, _, mat = ham.projector_points(ham.projs[i], self.nlm)
+            n = ham.n
+            pts.append(((co[:, 0] % n[0]) * n[1] + (co[:, 1] % n[1])) * n[2] + (co[:, 2] % n[2]))
+            npts.append(len(co))
+            beta.append(mat)
+            dd = np.zeros((self.nlm, self.nlm))
+            dd[:nlm, :nlm] = np.asarray(D).reshape(nlm, nlm)
+            coeff.append(dd)
+        self._keep = (np.asarray(npts, np.int32), np.ascontiguousarray(np.concatenate(pts) if pts else [0], np.int32),
+                      np.ascontiguousarray(np.concatenate([b.ravel() for b in beta]) if beta else [0.0]),
+                      np.ascontiguousarray(np.concatenate([c.ravel() for c in coeff]) if coeff else [0.0]))
+        self.h = L.rc_ham_create(ham.lattice.ravel(), np.asarray(ham.n, np.int32), self.nst, ham.v, len(npts),
+                                 self._keep[0] if npts else np.zeros(1, np.int32), self._keep[1], self.nlm,
+                                 self._keep[2], self._keep[3], int(threads))
+        self.backend = "mkl-fftw3" if L.rc_backend() == 1 else "own-fft"
+
+    def apply(self, psi_fs, out=None):
+        """H on full-cube Fourier coefficients [N][nst] (complex128)"""
+        psi_fs = _c128(psi_fs)
+        if out is None:
+            out = np.zeros_like(psi_fs)
+        lib().rc_ham_apply(self.h, psi_fs, out)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.rc_ham_destroy(self.h)
+            self.h = None
+
+
+def ref_overlap_cholesky(phi, vol_el):
+    """S = vol_el Phi^H Phi (ZGEMM over the full grid, Phi [N][nst]) and ZPOTRF."""
+    phi = _c128(phi)
+    N, nst = phi.shape
+    S = np.zeros((nst, nst), dtype=np.complex128)
+    info = lib().rc_overlap_cholesky(N, nst, float(vol_el), phi, S)
+    return S, info
+
+
+def ref_exchange_pairs(lattice, n, phi_rs, hf_rs, coef, threads):
+    """Gamma-point pair solves: kphi_i = sum_j coef_j hf_j Poisson(conj(hf_j) phi_i),
+    phi_rs [N][nst], hf_rs [N][nhf] (G = 0 dropped)."""
+    phi_rs, hf_rs = _c128(phi_rs), _c128(hf_rs)
+    out = np.zeros_like(phi_rs)
+    lib().rc_exchange_pairs(_f64(lattice).ravel(), np.asarray(n, np.int32), phi_rs.shape[1], phi_rs, hf_rs.shape[1],
+                            hf_rs, _f64(coef), out, int(threads))
     return out
