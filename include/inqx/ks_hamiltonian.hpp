@@ -44,6 +44,10 @@ class context {
   void sync() const { check(inqx_sync(h_), h_, "sync"); }
 };
 
+// tag of the orbital_block constructor that allocates the same shape without copying
+struct uninitialized_t {};
+inline constexpr uninitialized_t uninitialized{};
+
 // device array of complex states [nst][npw] (band-major G sphere)
 template <typename Complex> class orbital_block {
   inqx_ctx ctx_;
@@ -59,10 +63,17 @@ template <typename Complex> class orbital_block {
     check(inqx_basis_size(basis, &npw_), ctx_, "basis size");
     check(inqx_malloc(ctx_, sizeof(Complex) * (size_t)nst * npw_, &ptr_), ctx_, "malloc");
   }
-  // orbital_set copies are deep (the reference returns states by value)
+  // the same basis and number of states, contents undefined (no copy, no synchronisation)
+  orbital_block(const orbital_block &o, uninitialized_t) : orbital_block(o.ctx_, o.basis_, o.nst_) {}
+  // orbital_set copies are deep (the reference returns states by value); the
+  // copy is ordered on the context's stream like every other call, so no host
+  // synchronisation is needed
   orbital_block(const orbital_block &o) : orbital_block(o.ctx_, o.basis_, o.nst_) {
     check(inqx_memcpy_d2d(ctx_, ptr_, o.ptr_, bytes()), ctx_, "copy");
-    check(inqx_sync(ctx_), ctx_, "sync");
+  }
+  orbital_block(orbital_block &&o) noexcept
+      : ctx_(o.ctx_), basis_(o.basis_), nst_(o.nst_), npw_(o.npw_), ptr_(o.ptr_) {
+    o.ptr_ = nullptr;
   }
   orbital_block &operator=(const orbital_block &) = delete;
   ~orbital_block() { inqx_free(ctx_, ptr_); }
@@ -122,7 +133,7 @@ class ks_hamiltonian {
 
   // operator()(orbital_set<fourier_space>) (ks_hamiltonian.hpp:160-180)
   template <typename Complex> orbital_block<Complex> operator()(const orbital_block<Complex> &phi) const {
-    orbital_block<Complex> hphi(phi);
+    orbital_block<Complex> hphi(phi, uninitialized);  // every element is written by the apply
     check(inqx_ham_apply(h_, phi.basis(), phi.set_size(), orbital_block<Complex>::precision, phi.data(), hphi.data()),
           ctx_, "apply");
     return hphi;
