@@ -1014,6 +1014,54 @@ void orc_etrs_step(const orc_ham *h, int nst, const double kcov[3], long npw, co
   free(half);
 }
 
+/* update_hamiltonian (self_consistency.hpp:100-125) with the Hartree term only:
+ * V = vion + V_H, V_H = poisson::solve(density) (poisson.hpp:217-223, real part
+ * of the complex in-place solve, G = 0 dropped) */
+static void sc_update(orc_ham *h, const double *vion, const double *rho) {
+  const long N = h->g.N;
+  cplx *c = (cplx *)malloc(sizeof(cplx) * (size_t)N);
+  for (long ip = 0; ip < N; ++ip) c[ip] = rho[ip];
+  const double zero[3] = {0, 0, 0};
+  orc_poisson_in_place(h, 1, c, zero, 0.0);
+  for (long ip = 0; ip < N; ++ip) h->v[ip] = vion[ip] + creal(c[ip]);
+  free(c);
+}
+
+int orc_etrs_sc_step(orc_ham *h, int nst, const double kcov[3], long npw, const int *gidx, double dt,
+                     const double *vion, const double *occ, int nscf, double threshold, cplx *phi, double *rho_out,
+                     double *delta) {
+  const long tot = (long)nst * npw, N = h->g.N;
+  const double dv = h->g.vol / (double)N;
+  cplx *save = (cplx *)malloc(sizeof(cplx) * (size_t)tot);
+  double *old = (double *)malloc(sizeof(double) * (size_t)N);
+  double dip[3];
+  /* half step and full step with H(t): save = exp(-i dt/2 H) phi, phi = exp(-i dt H) phi */
+  orc_exponential_2_for_1(h, nst, kcov, npw, gidx, -I * dt / 2.0, -I * dt, 4, phi, save);
+  /* density of the full-step prediction, H(t + dt), restore the half step */
+  orc_density_dipole_sphere(h, nst, npw, gidx, phi, occ, rho_out, dip);
+  sc_update(h, vion, rho_out);
+  memcpy(phi, save, sizeof(cplx) * (size_t)tot);
+  int passes = 0;
+  double d = 0.0;
+  for (int iscf = 0; iscf < nscf; ++iscf) {
+    if (iscf != 0) memcpy(phi, save, sizeof(cplx) * (size_t)tot);
+    orc_exponential_in_place(h, nst, kcov, npw, gidx, -I * dt / 2.0, 4, phi);
+    memcpy(old, rho_out, sizeof(double) * (size_t)N);
+    orc_density_dipole_sphere(h, nst, npw, gidx, phi, occ, rho_out, dip);
+    d = 0.0; /* operations::integral_sum_absdiff */
+    for (long ip = 0; ip < N; ++ip) d += fabs(old[ip] - rho_out[ip]);
+    d *= dv;
+    const int done = (d < threshold) || (iscf == nscf - 1);
+    sc_update(h, vion, rho_out);
+    ++passes;
+    if (done) break;
+  }
+  *delta = d;
+  free(old);
+  free(save);
+  return passes;
+}
+
 /* energy::calculate pieces on the sphere (energy.hpp:61-113):
  * out[0] kinetic = sum occ <phi|T|phi> (laplacian_expectation_value, laplacian.hpp:85-117)
  * out[1] eigenvalue sum = sum occ Re<phi|H|phi>/<phi|phi> (overlap_diagonal_normalized)

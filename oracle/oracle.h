@@ -108,6 +108,13 @@ void orc_exponential_2_for_1(const orc_ham *h, int nst, const double kcov[3], lo
                              orc_cplx *phi, orc_cplx *expphi);
 void orc_etrs_step(const orc_ham *h, int nst, const double kcov[3], long npw, const int *gidx,
                    double dt, orc_cplx *phi);
+/* self-consistent ETRS step (etrs.hpp:22-79) with update_hamiltonian =
+ * vion + V_Hartree[density] (self_consistency.hpp:100-125, no XC): one block,
+ * one spin component.  rho_out [N] receives the last density; returns the
+ * number of SCF passes, *delta the last integral |drho|. */
+int orc_etrs_sc_step(orc_ham *h, int nst, const double kcov[3], long npw, const int *gidx, double dt,
+                     const double *vion, const double *occ, int nscf, double threshold, orc_cplx *phi,
+                     double *rho_out, double *delta);
 void orc_energy_sphere(const orc_ham *h, int nst, const double kcov[3], long npw,
                        const int *gidx, const orc_cplx *phi, const double *occ, double out[4]);
 void orc_density_dipole_sphere(const orc_ham *h, int nst, long npw, const int *gidx,

@@ -66,6 +66,8 @@ def lib():
             "orc_subspace_diag_sphere": (C.c_int, [P, C.c_int, dp, C.c_long, ip, cp, dp]),
             "orc_etrs_step": (None, [P, C.c_int, dp, C.c_long, ip, C.c_double, cp]),
             "orc_energy_sphere": (None, [P, C.c_int, dp, C.c_long, ip, cp, dp, dp]),
+            "orc_etrs_sc_step": (C.c_int, [P, C.c_int, dp, C.c_long, ip, C.c_double, dp, dp, C.c_int, C.c_double,
+                                           cp, dp, dp]),
             "orc_density_dipole_sphere": (None, [P, C.c_int, C.c_long, ip, cp, dp, P, dp]),
             "orc_poisson_in_place": (None, [P, C.c_int, cp, dp, C.c_double]),
             "orc_sing_auxiliary": (C.c_double, [dp, dp]),
@@ -291,6 +293,18 @@ class Ham:
         lib().orc_etrs_step(self.h, phi.size // npw, _f64(k), npw, np.ascontiguousarray(gidx, np.int32),
                             float(dt), phi)
         return phi
+
+    def etrs_sc_step(self, phi, gidx, dt, vion, occ, nscf=5, threshold=5e-5, k=(0, 0, 0)):
+        """self-consistent ETRS step with V = vion + V_H[rho] (the potential held
+        by this oracle Hamiltonian is replaced); returns (phi, rho, passes, delta)"""
+        phi = _c128(phi).copy()
+        npw = len(gidx)
+        rho = np.zeros(self.N)
+        delta = np.zeros(1)
+        vion = _f64(vion).ravel()
+        n = lib().orc_etrs_sc_step(self.h, phi.size // npw, _f64(k), npw, np.ascontiguousarray(gidx, np.int32),
+                                   float(dt), vion, _f64(occ), int(nscf), float(threshold), phi, rho, delta)
+        return phi, rho, n, float(delta[0])
 
     def energy_sphere(self, phi, gidx, occ, k=(0, 0, 0)):
         phi = _c128(phi)
