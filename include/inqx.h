@@ -205,6 +205,15 @@ int inqx_steepest_descent(inqx_ham ham, inqx_basis basis, int nst, int precision
  * order (i0, i1, i2); vh or energy may be NULL; energy is a host double (the
  * call then synchronises).  FP64 throughout. */
 int inqx_hartree(inqx_ham ham, const double *density, double *vh, double *energy);
+/* inqx_hartree without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as inqx_hartree (bit for bit); its work grid and
+ * partial sums live in a 16 MiB device array of the library that each GPU
+ * holds until the process ends, so grids are limited to 1 048 320 points
+ * (INQX_ERR_ARG beyond).  Every call on a GPU uses that GPU's one array: calls
+ * that overlap on one GPU (other threads or streams) race. */
+int inqx_hartree_noalloc(inqx_ham ham, const double *density, double *vh, double *energy);
 
 /* Atomic-sphere fields: the per-atom sphere sums of
  * atomic_potential::local_potential (short-range radial functions added on each

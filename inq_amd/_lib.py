@@ -83,6 +83,7 @@ SIGNATURES = {
     "inqx_subspace_diag": (C.c_int, [P, P, C.c_int, C.c_int, P, DP]),
     "inqx_precondition": (C.c_int, [P, P, C.c_int, C.c_int, P]),
     "inqx_hartree": (C.c_int, [P, P, P, DP]),
+    "inqx_hartree_noalloc": (C.c_int, [P, P, P, DP]),
     "inqx_atomic_field_add": (C.c_int, [P, C.c_int, DP, DP, IP, C.c_int, C.c_int, C.c_double, DP, P]),
     "inqx_states_save": (C.c_int, [P, P, C.c_int, C.c_int, P, C.c_char_p, C.c_int]),
     "inqx_states_load": (C.c_int, [P, P, C.c_int, C.c_int, C.c_char_p, C.c_int, P]),

@@ -307,6 +307,13 @@ int inqx_ctx_create(int device, void *stream, inqx_ctx *out) {
   hipDeviceProp_t prop;
   INQX_HIP(hipGetDeviceProperties(&prop, device));
   ctx->c.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  try {
+    noalloc_scratch_init(device);  // per-device address of the allocation-free scratch
+  } catch (...) {
+    delete ctx;
+    ctx = nullptr;
+    throw;
+  }
   *out = ctx;
   API_END(ctx)
 }
@@ -1075,6 +1082,25 @@ int inqx_hartree(inqx_ham h, const double *density, double *vh, double *energy) 
     double e = 0.0;
     for (int c = 0; c < nch; ++c) e += part[c];
     *energy = 0.5 * e * h->g.vol / (double)N;
+  }
+  API_END(ctx)
+}
+
+int inqx_hartree_noalloc(inqx_ham h, const double *density, double *vh, double *energy) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && density && (vh || energy), INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(h->g.N <= kNoAllocMaxPoints && hartree_noalloc_bytes(h->g.N) <= kNoAllocScratchBytes, INQX_ERR_ARG,
+             "grid of " + std::to_string(h->g.N) + " points exceeds the allocation-free Hartree limit of " +
+                 std::to_string(kNoAllocMaxPoints) + " points");
+  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "Poisson solves need a cubic grid of a built size");
+  const double *total = launch_hartree_noalloc(h, density, vh);
+  if (energy) {
+    double e = 0.0;
+    INQX_HIP(hipMemcpyAsync(&e, total, sizeof(double), hipMemcpyDeviceToHost, ctx->c.stream));
+    INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+    *energy = 0.5 * e * h->g.vol / (double)h->g.N;
   }
   API_END(ctx)
 }

@@ -437,6 +437,64 @@ __global__ void __launch_bounds__(256) k_hartree_out(const double2 *c, const dou
 
 int hartree_chunks(long N) { return (int)((N + kHChunk - 1) / kHChunk); }
 
+// ---------------------------------------------------------------- allocation-free Hartree
+// The scratch of inqx_hartree_noalloc: the complex128 work grid, the energy
+// partial sums and their total, in one file-scope device array that every
+// device holds from module load until the process ends.  Host code never
+// passes the array's name to a kernel (that is a host-side stand-in, not a
+// device address): it uses the per-device address from hipGetSymbolAddress.
+namespace {
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+void *g_scratch_addr[64] = {};
+DeviceOnce g_scratch_once;
+
+// sum of the partials in chunk order (the order of inqx_hartree's host sum)
+__global__ void __launch_bounds__(64) k_sum_partials(const double *part, int n, double *out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double e = 0.0;
+    for (int c = 0; c < n; ++c) e += xg(part)[c];
+    xw(out)[0] = e;
+  }
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+  INQX_CHECK(device >= 0 && device < 64, INQX_ERR_ARG, "device index beyond 63");
+  if (g_scratch_once.test_and_set(device)) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr) {
+    g_scratch_once.bits.fetch_and(~(1ull << device));  // retry on the next context of this device
+    throw Error{INQX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+  }
+  g_scratch_addr[device] = p;
+}
+
+size_t hartree_noalloc_bytes(long N) {
+  return align256(16 * (size_t)N) + align256(sizeof(double) * (size_t)hartree_chunks(N)) + 256;
+}
+
+double *launch_hartree_noalloc(inqx_ham_s *h, const double *rho, double *vh) {
+  Ctx &cx = h->ctx->c;
+  const long N = h->g.N;
+  INQX_CHECK(hartree_noalloc_bytes(N) <= kNoAllocScratchBytes, INQX_ERR_ARG,
+             "grid of " + std::to_string(N) + " points exceeds the allocation-free Hartree scratch (" +
+                 std::to_string(kNoAllocMaxPoints) + " points at most)");
+  void *base = g_scratch_addr[cx.device & 63];
+  INQX_CHECK(base != nullptr, INQX_ERR_STATE, "allocation-free scratch not set up on this device");
+  unsigned char *p = static_cast<unsigned char *>(base);
+  void *work = p;
+  double *part = reinterpret_cast<double *>(p + align256(16 * (size_t)N));
+  double *total = reinterpret_cast<double *>(p + align256(16 * (size_t)N) + align256(sizeof(double) * hartree_chunks(N)));
+  launch_hartree(h, rho, vh, work, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, cx.stream, part, hartree_chunks(N), total);
+  INQX_HIP(hipGetLastError());
+  return total;
+}
+
 void launch_hartree(inqx_ham_s *h, const double *rho, double *vh, void *work, double *part) {
   Ctx &cx = h->ctx->c;
   const long N = h->g.N;

@@ -358,4 +358,13 @@ constexpr int kAtomRec = 5;
 void launch_atomic_field_add(inqx_ham_s *h, int natoms, const double *atoms_dev, int nr, double dr,
                              const double *tables_dev, double *field);
 void launch_hartree(inqx_ham_s *h, const double *rho, double *vh, void *work, double *part);
+// allocation-free Hartree (inqx_hartree_noalloc): its scratch is a file-scope
+// device array of kNoAllocScratchBytes; noalloc_scratch_init looks up its
+// address once per device (at context creation, with the device current).
+// The launcher returns the device address of the energy sum (sum rho vh).
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+constexpr long kNoAllocMaxPoints = 1048320;  // largest grid that fits (2^20 - 256 points)
+void noalloc_scratch_init(int device);
+size_t hartree_noalloc_bytes(long N);
+double *launch_hartree_noalloc(inqx_ham_s *h, const double *rho, double *vh);
 }  // namespace inqx
