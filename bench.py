@@ -167,6 +167,16 @@ def hbm_copy(ctx, nbytes=1 << 30, reps=10):
     return 2 * nbytes / dt / 1e9
 
 
+def tune_cache():
+    """the autotuner's cache file: INQX_TUNE_CACHE, else ~/.cache/inqx/tune.txt"""
+    path = os.environ.get("INQX_TUNE_CACHE") or os.path.join(os.path.expanduser("~"), ".cache", "inqx", "tune.txt")
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+    except OSError:
+        return None
+    return path
+
+
 def c3_kpoints(ctx, steps):
     from inq_amd import DeviceArray
     from inq_amd import systems as S
@@ -181,6 +191,9 @@ def c3_kpoints(ctx, steps):
         ins.append(DeviceArray.from_numpy(ctx, c))
         outs.append(DeviceArray(ctx, c.shape, np.complex64))
     npws = [b.npw for b in bases]
+    t0 = time.perf_counter()
+    shape, tuned_ms = gh.autotune_batch(bases, [sysm.nst] * len(bases), np.complex64, tune_cache())
+    t_tune = time.perf_counter() - t0
     gh.apply_batch(bases, ins, outs)
     ctx.sync()
     t0 = time.perf_counter()
@@ -190,7 +203,8 @@ def c3_kpoints(ctx, steps):
     dt = (time.perf_counter() - t0) / steps
     nsa = sysm.nst * len(bases)
     return {"blocks": len(bases), "nst": sysm.nst, "npw_min": int(min(npws)), "npw_max": int(max(npws)),
-            "ms_per_batch": dt * 1e3, "state_applies_per_s": nsa / dt, "dtype": "c64"}
+            "ms_per_batch": dt * 1e3, "state_applies_per_s": nsa / dt, "dtype": "c64",
+            "launch_shape": {"xpass_states_per_wg": shape[0], "planes_per_wg": shape[1], "tune_s": t_tune}}
 
 
 def c4_tddft(ctx, nsteps):
@@ -201,7 +215,7 @@ def c4_tddft(ctx, nsteps):
     v = S.local_potential(sysm)
     out = {}
     recs = {}
-    for dtype, name in ((np.complex64, "fp32"), (np.complex128, "fp64")):
+    for dtype, name in ((np.complex64, "fp32"), (np.complex64, "fp32_fp64state"), (np.complex128, "fp64")):
         gh = build(ctx, sysm, v)
         # velocity-gauge kick 0.01 along x: A = -0.01 x (src/perturbations/kick.hpp:36-48), covariant
         gh.uniform_vector_potential = sysm.lattice @ np.array([-0.01, 0.0, 0.0])
@@ -209,6 +223,8 @@ def c4_tddft(ctx, nsteps):
         c = S.random_states(b.npw, sysm.nst, sysm.seed, S.sphere_volume_element(sysm))
         d = DeviceArray.from_numpy(ctx, c.astype(dtype))
         prop = rt.Propagator(gh, [b], [d], [np.full(sysm.nst, 2.0)], dt=0.05, ring_capacity=nsteps + 8)
+        if name == "fp32_fp64state":
+            prop.set_state_precision(np.complex128)  # FP32 applications, complex128 states
         prop.energy()
         prop.step(2, graph=True)  # eager first step + capture
         ctx.sync()
@@ -221,8 +237,11 @@ def c4_tddft(ctx, nsteps):
                      "state_applies_per_s": 9 * sysm.nst / dt}
         prop.close()
         del prop
-    r32, r64 = recs["fp32"], recs["fp64"]
-    m = min(len(r32), len(r64))
+    r32, r64, rw = recs["fp32"], recs["fp64"], recs["fp32_fp64state"]
+    m = min(len(r32), len(r64), len(rw))
+    out["fp32_fp64state_band_energy_err_per_atom_Ha"] = float(np.max(np.abs(rw[:m, 1] - r64[:m, 1])) / sysm.natoms)
+    out["fp32_fp64state_dipole_trace_max_abs_err"] = float(np.max(np.abs(rw[:m, 4:7] - r64[:m, 4:7])))
+    out["fp32_fp64state_norm_vs_fp64"] = float(np.max(np.abs(rw[:m, 3] - r64[:m, 3])) / sysm.nst)
     dip_err = float(np.max(np.abs(r32[:m, 4:7] - r64[:m, 4:7])))
     e32 = r32[:m, 1]
     e64 = r64[:m, 1]
@@ -607,6 +626,7 @@ def main():
         d_in = DeviceArray.from_numpy(ctx, c.astype(dtype))
         d_out = DeviceArray(ctx, c.shape, dtype)
         gh.prepare(b, sysm.nst, dtype)
+        shape, _ = gh.autotune_batch([b], [sysm.nst], dtype, tune_cache())
         dt = time_applies(ctx, gh, b, d_in, d_out, args.steps, args.warmup)
         ctx.profile(True)
         ctx.profile_reset()
@@ -615,7 +635,8 @@ def main():
         ktimes = ctx.kernel_times()
         ctx.profile(False)
         outs[name] = d_out.numpy().astype(np.complex128)
-        res[name] = {"ms": dt * 1e3, "rate": sysm.nst / dt, "kernels": ktimes}
+        res[name] = {"ms": dt * 1e3, "rate": sysm.nst / dt, "kernels": ktimes,
+                     "launch_shape": {"xpass_states_per_wg": shape[0], "planes_per_wg": shape[1]}}
         log(f"bench: {name} {dt * 1e3:.3f} ms per {sysm.nst}-state apply, {sysm.nst / dt:.4g} state-applies/s")
         for kname, (cnt, ms) in sorted(ktimes.items()):
             log(f"   {kname:24s} {ms / cnt * 1e3:9.1f} us/launch")
@@ -657,6 +678,7 @@ def main():
                      "traffic_source": "profiles/hpsi_si64_fp32_hbm_counters.txt (2 x FETCH_SIZE + WRITE_SIZE)",
                      "bytes_per_launch": kb, "avg_launch_us": avg_s * 1e6},
         "kernels_us": {k: vv[1] / vv[0] * 1e3 for k, vv in kt.items()},
+        "launch_shape": res[head]["launch_shape"],
     }
     if "fp32" in res and "fp64" in res:
         ci = c.astype(np.complex64).astype(np.complex128)

@@ -133,11 +133,21 @@ int inqx_basis_kinetic(inqx_basis basis, double *kin);
  * later calls allocate nothing (required before hipGraph capture) */
 int inqx_ham_prepare(inqx_ham ham, inqx_basis basis, int nst, int precision);
 
-/* launch-shape autotuner: times the H application of nst states on this basis
- * for each candidate x-pass shape (1, 2 or 4 states per work-group) and keeps
- * the fastest for this Hamiltonian and precision.  Allocates scratch states for
- * the timing; do not call during graph capture.  best_xwaves / best_ms may be
- * NULL. */
+/* launch-shape autotuner: times the H application of the given blocks (one
+ * launch sequence, as inqx_ham_apply_batch runs them) for every launch shape
+ * (x-pass states per work-group 1, 2 or 4, times plane-pass planes per
+ * work-group from the largest the grid's LDS budget allows down to 1) and
+ * keeps the fastest for this Hamiltonian and precision.  cache_path: a text
+ * file of earlier results, one line per (device, precision, grid, KB size,
+ * blocks) key; a hit applies the stored shape without timing, a miss times and
+ * appends (NULL: no cache).  shape[2] (out, may be NULL): x-pass states,
+ * plane batch.  Allocates scratch states; do not call during graph capture. */
+int inqx_ham_autotune_batch(inqx_ham ham, int nblocks, const inqx_basis *bases, const int *nst, int precision,
+                            const char *cache_path, int *shape, double *best_ms);
+/* sets the plane-pass batch directly (planes per work-group; 0: the largest the
+ * grid's LDS budget allows, larger values are capped to it) */
+int inqx_ham_set_plane_batch(inqx_ham ham, int precision, int planes);
+/* the single-block form (no cache); best_xwaves / best_ms may be NULL */
 int inqx_ham_autotune(inqx_ham ham, inqx_basis basis, int nst, int precision, int *best_xwaves, double *best_ms);
 /* sets the x-pass shape directly (1, 2 or 4 states per work-group) */
 int inqx_ham_set_launch_shape(inqx_ham ham, int precision, int xwaves);
@@ -175,12 +185,19 @@ int inqx_rotate(inqx_ctx ctx, int precision, int n, long npw, const void *rot, v
 /* operations::rotate, 5-argument form (rotate.hpp:84-140): rotphi <- alpha phi rot + beta rotphi */
 int inqx_rotate_add(inqx_ctx ctx, int precision, int n, long npw, const void *rot, const void *phi, void *rotphi,
                     double alpha_re, double alpha_im, double beta_re, double beta_im);
+/* matrix::diagonalize (src/matrix/diagonalize.hpp:167-189): matrix is a device
+ * complex128 [n][n] Hermitian matrix, replaced by its eigenvectors in the
+ * reference's orientation (row k = the conjugate of the k-th eigenvector);
+ * eigenvalues (host [n]) ascending.  FP64 on the device (Householder
+ * tridiagonalisation, bisection, inverse iteration, back-transformation);
+ * n <= 1024.  Synchronises the context's stream. */
+int inqx_diagonalize(inqx_ctx ctx, int n, void *matrix, double *eigenvalues);
 /* operations::orthogonalize (src/operations/orthogonalize.hpp:25-32): overlap ->
  * Cholesky -> rotate_trs, in place */
 int inqx_orthogonalize(inqx_ctx ctx, int precision, int n, long npw, double vol_el, void *phi);
 /* ground_state::subspace_diagonalization (src/ground_state/subspace_diagonalization.hpp:21-29):
- * H_sub = phi^H H phi, FP64 eigendecomposition (eigenvalues ascending, host
- * array), phi <- phi U */
+ * H_sub = phi^H H phi, FP64 eigendecomposition on the device (eigenvalues
+ * ascending, copied to the host array), phi <- phi U; nst <= 1024 */
 int inqx_subspace_diag(inqx_ham ham, inqx_basis basis, int nst, int precision, void *phi, double *eigenvalues);
 /* operations::preconditioner on Fourier-space states (src/operations/preconditioner.hpp:26-58),
  * Teter-Payne-Allan: <T>_i = Re(<(1/2)|G|^2 phi_i|phi_i> / <phi_i|phi_i>), then
@@ -286,6 +303,38 @@ int inqx_energy(inqx_ham ham, int nblocks, const inqx_basis *bases, const int *n
                 void *const *phi, const double *const *occ, double *out);
 /* number of records written so far and the most recent min(count, cap) records [k][8] */
 int inqx_prop_records(inqx_prop prop, int *count, double *records, int cap);
+/* Self-consistent steps: real_time::etrs for a density-dependent Hamiltonian
+ * (src/real_time/etrs.hpp:22-79).  Each step takes the half and full steps
+ * with H(t) (2-for-1), the spin density of the full-step prediction, updates
+ * the potential to vion + V_Hartree[density] (inqx_ks_potential), restores the
+ * half step, then runs up to max_passes (the reference: 5) second half steps
+ * with H(t + dt), each followed by the density, the change integral |drho| dV
+ * and the potential update, stopping once the change is below threshold (the
+ * reference: 5e-5).  vion: device double [N] (ionic plus external potential),
+ * kept by the propagator.  The caller first sets the potential of the initial
+ * density (inqx_ks_potential).  These steps run eagerly (use_graph is
+ * rejected with INQX_ERR_UNSUPPORTED): the pass count depends on the density
+ * change, read after each pass.  max_passes = 0 returns to fixed-H steps.
+ * Cubic grids of the Poisson sizes only. */
+int inqx_prop_set_scf(inqx_prop prop, const double *vion, int max_passes, double threshold);
+/* State precision of an FP32 propagator: INQX_FP64 keeps a complex128 copy of
+ * the states that the Taylor terms of the FP32 H applications accumulate into
+ * (the rounding of the states to complex64 between steps is what makes FP32
+ * propagation drift); the caller's complex64 arrays receive the state rounded
+ * after every half step and feed the first H application of the next one.
+ * INQX_FP32 (the default) accumulates in complex64.  Set before the first
+ * graph step; fixed-H steps only. */
+int inqx_prop_set_state_precision(inqx_prop prop, int precision);
+/* SCF passes and the last integral |drho| dV of the most recent self-consistent step */
+int inqx_prop_scf_info(inqx_prop prop, int *passes, double *delta);
+/* hamiltonian::self_consistency::update_hamiltonian at the Hartree level
+ * (src/hamiltonian/self_consistency.hpp:100-125, no XC): the Hamiltonian's
+ * potential becomes vion + V_H[total density] in every spin component, on the
+ * device, in place (vion: device double [N]; density: device double
+ * [N][ncomp] spin density, ncomp that of the Hamiltonian's potential, which
+ * must have been set once).  energies (host, optional) = {external = integral
+ * density vion, Hartree = 1/2 integral density V_H}; the call synchronises. */
+int inqx_ks_potential(inqx_ham ham, const double *vion, const double *density, double *energies);
 /* a propagator that captured a graph pins its Hamiltonian and bases: they
  * cannot be destroyed (INQX_ERR_STATE) or have their workspaces grown until
  * the propagator is destroyed */
@@ -352,6 +401,16 @@ int inqx_ace_build(inqx_ham ham, inqx_basis basis, int nst, int precision, const
  * exxphi <- exxphi - xi (vol_el xi^H psi), xi [nxi][npw], psi/exxphi [nst][npw] */
 int inqx_ace_apply(inqx_ctx ctx, int precision, int nxi, int nst, long npw, double vol_el, const void *xi,
                    const void *psi, void *exxphi);
+/* hamiltonian::singularity_correction (src/hamiltonian/singularity_correction.hpp:
+ * 19-103, Carrier et al. PRB 75, 205126): fk[ik] = 4 pi / V sum_{k' != k}
+ * w_k' f(k - k') over the k-points (covariant kcov [nk][3], weights
+ * kweight [nk]) and the integral fzero, with f the auxiliary function of Eq.
+ * 16.  The value an exchange Poisson solve uses at G + q = 0 for the hf states
+ * of k-point ik is zeroterm = -nk V (fk[ik] - fzero) (singularity_correction.hpp:
+ * 100-102); with the Gamma point alone, zeroterm = V fzero.  The fzero grid
+ * (7 x 61 x 121 x 121 points) is summed on the device. */
+int inqx_singularity_correction(inqx_ctx ctx, const double lattice[9], int nk, const double *kcov,
+                                const double *kweight, double *fk, double *fzero);
 /* upper bound on the pair workspace of the exchange entry points (bytes;
  * 0, the default, takes a quarter of the free device memory, at most 64 GiB;
  * at least one j times all states is always used).  Larger chunks amortise

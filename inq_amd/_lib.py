@@ -68,6 +68,8 @@ SIGNATURES = {
     "inqx_ace_apply": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_long, C.c_double, P, P, P]),
     "inqx_ham_set_exchange_workspace": (C.c_int, [P, C.c_size_t]),
     "inqx_ham_autotune": (C.c_int, [P, P, C.c_int, C.c_int, IP, DP]),
+    "inqx_ham_autotune_batch": (C.c_int, [P, C.c_int, P, IP, C.c_int, C.c_char_p, IP, DP]),
+    "inqx_ham_set_plane_batch": (C.c_int, [P, C.c_int, C.c_int]),
     "inqx_randomize": (C.c_int, [P, C.c_int, C.c_long, C.c_int, C.c_int, C.c_int, P]),
     "inqx_overlap_diagonal": (C.c_int, [P, C.c_int, C.c_int, C.c_long, C.c_double, P, P, C.c_int, P]),
     "inqx_ham_apply_real": (C.c_int, [P, P, C.c_int, C.c_int, P, C.c_int, P]),
@@ -81,6 +83,7 @@ SIGNATURES = {
                                   C.c_double]),
     "inqx_orthogonalize": (C.c_int, [P, C.c_int, C.c_int, C.c_long, C.c_double, P]),
     "inqx_subspace_diag": (C.c_int, [P, P, C.c_int, C.c_int, P, DP]),
+    "inqx_diagonalize": (C.c_int, [P, C.c_int, P, DP]),
     "inqx_precondition": (C.c_int, [P, P, C.c_int, C.c_int, P]),
     "inqx_hartree": (C.c_int, [P, P, P, DP]),
     "inqx_hartree_noalloc": (C.c_int, [P, P, P, DP]),
@@ -99,6 +102,11 @@ SIGNATURES = {
     "inqx_prop_step": (C.c_int, [P, C.c_int, C.c_int]),
     "inqx_prop_energy": (C.c_int, [P]),
     "inqx_prop_records": (C.c_int, [P, IP, DP, C.c_int]),
+    "inqx_prop_set_scf": (C.c_int, [P, P, C.c_int, C.c_double]),
+    "inqx_prop_scf_info": (C.c_int, [P, IP, DP]),
+    "inqx_prop_set_state_precision": (C.c_int, [P, C.c_int]),
+    "inqx_ks_potential": (C.c_int, [P, P, P, DP]),
+    "inqx_singularity_correction": (C.c_int, [P, DP, C.c_int, DP, DP, DP, DP]),
     "inqx_energy": (C.c_int, [P, C.c_int, PP, IP, C.c_int, PP, PP, DP]),
     "inqx_prop_destroy": (C.c_int, [P]),
 }

@@ -223,68 +223,6 @@ static void build_basis(const Grid &g, Basis &b) {
   }
 }
 
-// Hermitian eigensolver for the subspace step, FP64 on the host: cyclic
-// Jacobi sweeps with a threshold (skip rotations whose off-diagonal entry is
-// negligible against the diagonal pair), eigenvalues sorted ascending.  On
-// exit u[k*n + j] holds component j of the k-th eigenvector.
-static void hermitian_eig(int n, std::vector<std::complex<double>> a, std::vector<double> &w,
-                          std::vector<std::complex<double>> &u) {
-  using cd = std::complex<double>;
-  std::vector<cd> v((size_t)n * n, cd(0.0, 0.0));
-  for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
-  auto A = [&](int i, int j) -> cd & { return a[(size_t)i * n + j]; };
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0.0, tot = 0.0;
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) {
-        const double m = std::norm(A(i, j));
-        tot += m;
-        if (i != j) off += m;
-      }
-    if (off <= 1e-30 * tot) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const cd apq = A(p, q);
-        const double mag = std::abs(apq);
-        const double app = A(p, p).real(), aqq = A(q, q).real();
-        if (mag <= 1e-300 || mag < 1e-18 * (std::fabs(app) + std::fabs(aqq))) continue;
-        // real rotation on the phase-aligned pair: tan(2t) = 2|apq| / (aqq - app)
-        const double tau = (aqq - app) / (2.0 * mag);
-        const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
-        const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c;
-        const cd ph = apq / mag;
-        // columns p, q of J: p' = c e_p - sn conj(ph) e_q, q' = sn ph e_p + c e_q
-        const cd jqp = -sn * std::conj(ph), jpq = sn * ph;
-        for (int k = 0; k < n; ++k) {
-          const cd akp = A(k, p), akq = A(k, q);
-          A(k, p) = akp * c + akq * jqp;
-          A(k, q) = akp * jpq + akq * c;
-        }
-        for (int k = 0; k < n; ++k) {
-          const cd apk = A(p, k), aqk = A(q, k);
-          A(p, k) = c * apk + std::conj(jqp) * aqk;
-          A(q, k) = std::conj(jpq) * apk + c * aqk;
-        }
-        A(p, q) = 0.0;
-        A(q, p) = 0.0;
-        for (int k = 0; k < n; ++k) {
-          const cd vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
-          v[(size_t)k * n + p] = vkp * c + vkq * jqp;
-          v[(size_t)k * n + q] = vkp * jpq + vkq * c;
-        }
-      }
-  }
-  std::vector<int> ord(n);
-  for (int i = 0; i < n; ++i) ord[i] = i;
-  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return A(x, x).real() < A(y, y).real(); });
-  w.resize(n);
-  u.assign((size_t)n * n, cd(0.0, 0.0));
-  for (int k = 0; k < n; ++k) {
-    w[k] = A(ord[k], ord[k]).real();
-    for (int j = 0; j < n; ++j) u[(size_t)k * n + j] = v[(size_t)j * n + ord[k]];
-  }
-}
-
 }  // namespace inqx
 
 extern "C" {
@@ -444,6 +382,7 @@ int inqx_ham_destroy(inqx_ham h) {
     return INQX_ERR_STATE;
   }
   invalidate_launch_cache(h);
+  release_ks_work(h);
   for (auto b : h->bases) b->b.ham = nullptr;
   delete h;
   return INQX_OK;
@@ -648,52 +587,153 @@ int inqx_ham_set_launch_shape(inqx_ham h, int precision, int xwaves) {
   API_END(ctx)
 }
 
-int inqx_ham_autotune(inqx_ham h, inqx_basis bs, int nst, int precision, int *best_xwaves, double *best_ms) {
+// ---------------------------------------------------------------- launch-shape autotuner
+// The cache key: device, precision, grid, KB table size and every block's (nst,
+// npw), hashed (FNV-1a); one line per result: key, x-pass waves, plane batch, ms
+static std::string tune_key(inqx_ham h, int nblocks, Basis *const *bb, const int *nst, int precision) {
+  hipDeviceProp_t prop;
+  INQX_HIP(hipGetDeviceProperties(&prop, h->ctx->c.device));
+  unsigned long long x = 1469598103934665603ull;
+  auto mix = [&](unsigned long long v) {
+    for (int b = 0; b < 8; ++b) {
+      x ^= (v >> (8 * b)) & 0xffu;
+      x *= 1099511628211ull;
+    }
+  };
+  for (const char *c = prop.gcnArchName; *c; ++c) mix((unsigned char)*c);
+  mix((unsigned long long)prop.multiProcessorCount);
+  mix((unsigned long long)precision);
+  for (int d = 0; d < 3; ++d) mix((unsigned long long)h->g.n[d]);
+  mix((unsigned long long)h->kb.natoms);
+  mix((unsigned long long)h->kb.nent);
+  mix((unsigned long long)nblocks);
+  for (int i = 0; i < nblocks; ++i) {
+    mix((unsigned long long)nst[i]);
+    mix((unsigned long long)bb[i]->npw);
+  }
+  char buf[32];
+  std::snprintf(buf, sizeof buf, "%016llx", x);
+  return buf;
+}
+
+static bool tune_cache_read(const char *path, const std::string &key, int &xw, int &pb, double &ms) {
+  if (!path || !*path) return false;
+  FILE *f = std::fopen(path, "r");
+  if (!f) return false;
+  char k[64];
+  int a, b;
+  double t;
+  bool found = false;
+  while (std::fscanf(f, "%63s %d %d %lf", k, &a, &b, &t) == 4)
+    if (key == k) {  // the last entry of a key wins
+      xw = a;
+      pb = b;
+      ms = t;
+      found = true;
+    }
+  std::fclose(f);
+  return found;
+}
+
+static void tune_cache_write(const char *path, const std::string &key, int xw, int pb, double ms) {
+  if (!path || !*path) return;
+  FILE *f = std::fopen(path, "a");
+  if (!f) return;  // an unwritable cache only costs the next run a search
+  std::fprintf(f, "%s %d %d %.6f\n", key.c_str(), xw, pb, ms);
+  std::fclose(f);
+}
+
+static void set_shape(inqx_ham h, int precision, int xw, int pb) {
+  if (h->xwaves[precision] != xw || h->plane_batch[precision] != pb) {
+    h->xwaves[precision] = xw;
+    h->plane_batch[precision] = pb;
+    invalidate_launch_cache(h);
+  }
+}
+
+int inqx_ham_set_plane_batch(inqx_ham h, int precision, int planes) {
   inqx_ctx ctx = h ? h->ctx : nullptr;
   API_BEGIN
-  INQX_CHECK(h && bs && nst > 0, INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(h, INQX_ERR_ARG, "null Hamiltonian");
+  INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
+  INQX_CHECK(planes >= 0, INQX_ERR_ARG, "plane batch must be >= 0 (0: the largest the grid allows)");
+  INQX_CHECK(h->pinned == 0, INQX_ERR_STATE, "Hamiltonian pinned by a captured propagation graph");
+  if (h->plane_batch[precision] != planes) {
+    h->plane_batch[precision] = planes;
+    invalidate_launch_cache(h);
+  }
+  API_END(ctx)
+}
+
+int inqx_ham_autotune_batch(inqx_ham h, int nblocks, const inqx_basis *bases, const int *nst, int precision,
+                            const char *cache_path, int *shape, double *best_ms) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && bases && nst && nblocks > 0, INQX_ERR_ARG, "bad arguments");
   INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
-  INQX_CHECK(bs->b.ham == h, INQX_ERR_ARG, "basis belongs to another Hamiltonian");
   INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
   INQX_CHECK(h->pinned == 0, INQX_ERR_STATE, "Hamiltonian pinned by a captured propagation graph");
+  INQX_CHECK(h->has_v, INQX_ERR_STATE, "potential not set");
+  if (!h->kb.built) build_kb_tables(h);
+  std::vector<Basis *> bb(nblocks);
+  for (int i = 0; i < nblocks; ++i) {
+    INQX_CHECK(bases[i] && bases[i]->b.ham == h && nst[i] > 0, INQX_ERR_ARG, "bad block");
+    bb[i] = &bases[i]->b;
+  }
+  const std::string key = tune_key(h, nblocks, bb.data(), nst, precision);
+  int xw = 0, pb = 0;
+  double t = 0.0;
+  if (tune_cache_read(cache_path, key, xw, pb, t) && (xw == 1 || xw == 2 || xw == 4) && pb >= 1) {
+    set_shape(h, precision, xw, pb);
+    if (shape) {
+      shape[0] = xw;
+      shape[1] = pb;
+    }
+    if (best_ms) *best_ms = t;
+    return INQX_OK;
+  }
   Ctx &cx = ctx->c;
-  const size_t bytes = (precision == INQX_FP64 ? 16 : 8) * (size_t)nst * bs->b.npw;
-  DevMem a, o;
-  a.ensure(bytes);
-  o.ensure(bytes);
-  launch_fill_zero(cx, a.p, bytes);
-  Basis *bb = &bs->b;
-  const void *pin = a.p;
-  void *pout = o.p;
+  const size_t cs = precision == INQX_FP64 ? 16 : 8;
+  std::vector<DevMem> in(nblocks), out(nblocks);
+  std::vector<const void *> pin(nblocks);
+  std::vector<void *> pout(nblocks);
+  for (int i = 0; i < nblocks; ++i) {
+    const size_t bytes = cs * (size_t)nst[i] * bb[i]->npw;
+    in[i].ensure(bytes);
+    out[i].ensure(bytes);
+    launch_fill_zero(cx, in[i].p, bytes);
+    pin[i] = in[i].p;
+    pout[i] = out[i].p;
+  }
   hipEvent_t e0, e1;
   INQX_HIP(hipEventCreate(&e0));
   INQX_HIP(hipEventCreate(&e1));
   const bool prof = cx.prof;
   cx.prof = false;
-  int best = h->xwaves[precision];
+  const int pmax = max_plane_batch(h->g.n, precision);
+  int best_x = h->xwaves[precision], best_p = h->plane_batch[precision];
   double best_t = 1e300;
   try {
-    for (int w : {4, 2, 1}) {
-      if (h->xwaves[precision] != w) {
-        h->xwaves[precision] = w;
-        invalidate_launch_cache(h);
+    for (int w : {1, 2, 4})
+      for (int p = pmax; p >= 1; --p) {
+        set_shape(h, precision, w, p);
+        auto run = [&]() {
+          if (precision == INQX_FP64) launch_apply<double>(h, nblocks, bb.data(), nst, pin.data(), pout.data());
+          else launch_apply<float>(h, nblocks, bb.data(), nst, pin.data(), pout.data());
+        };
+        run();
+        INQX_HIP(hipEventRecord(e0, cx.stream));
+        for (int r = 0; r < 3; ++r) run();
+        INQX_HIP(hipEventRecord(e1, cx.stream));
+        INQX_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        INQX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms / 3.0 < best_t) {
+          best_t = ms / 3.0;
+          best_x = w;
+          best_p = p;
+        }
       }
-      auto run = [&]() {
-        if (precision == INQX_FP64) launch_apply<double>(h, 1, &bb, &nst, &pin, &pout);
-        else launch_apply<float>(h, 1, &bb, &nst, &pin, &pout);
-      };
-      run();
-      INQX_HIP(hipEventRecord(e0, cx.stream));
-      for (int r = 0; r < 3; ++r) run();
-      INQX_HIP(hipEventRecord(e1, cx.stream));
-      INQX_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      INQX_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms / 3.0 < best_t) {
-        best_t = ms / 3.0;
-        best = w;
-      }
-    }
   } catch (...) {
     cx.prof = prof;
     (void)hipEventDestroy(e0);
@@ -703,14 +743,22 @@ int inqx_ham_autotune(inqx_ham h, inqx_basis bs, int nst, int precision, int *be
   cx.prof = prof;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (h->xwaves[precision] != best) {
-    h->xwaves[precision] = best;
-    invalidate_launch_cache(h);
-  }
+  set_shape(h, precision, best_x, best_p);
   INQX_HIP(hipStreamSynchronize(cx.stream));
-  if (best_xwaves) *best_xwaves = best;
+  tune_cache_write(cache_path, key, best_x, best_p, best_t);
+  if (shape) {
+    shape[0] = best_x;
+    shape[1] = best_p;
+  }
   if (best_ms) *best_ms = best_t;
   API_END(ctx)
+}
+
+int inqx_ham_autotune(inqx_ham h, inqx_basis bs, int nst, int precision, int *best_xwaves, double *best_ms) {
+  int shape[2] = {0, 0};
+  const int st = inqx_ham_autotune_batch(h, 1, &bs, &nst, precision, nullptr, shape, best_ms);
+  if (st == INQX_OK && best_xwaves) *best_xwaves = shape[0];
+  return st;
 }
 
 int inqx_ham_apply(inqx_ham h, inqx_basis bs, int nst, int precision, const void *psi, void *hpsi) {
@@ -872,20 +920,35 @@ int inqx_subspace_diag(inqx_ham h, inqx_basis bs, int nst, int precision, void *
   else launch_apply<float>(h, 1, &bb, &nst, &pin, &pout);
   const double vol = h->g.vol / ((double)h->g.N * (double)h->g.N);
   launch_overlap(ctx->c, precision, nst, nst, npw, phi, hphi, vol, ctx->c.ws_S.as<double>(), ctx->c.ws_partial);
-  std::vector<std::complex<double>> m((size_t)nst * nst);
-  INQX_HIP(hipMemcpyAsync(m.data(), ctx->c.ws_S.p, 16 * (size_t)nst * nst, hipMemcpyDeviceToHost, ctx->c.stream));
-  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
-  std::vector<double> w;
-  std::vector<std::complex<double>> u;
-  hermitian_eig(nst, m, w, u);
+  // FP64 eigensolve on the device: rows of U are the eigenvectors (ascending eigenvalues)
+  INQX_CHECK(nst <= kMaxEigN, INQX_ERR_UNSUPPORTED, "subspace diagonalisation takes at most 1024 states");
+  ctx->c.ws_eig.ensure(eigh_workspace_bytes(nst));
+  ctx->c.ws_X.ensure(16 * (size_t)nst * nst);
+  ctx->c.ws_lam.ensure(sizeof(double) * nst);
+  launch_eigh(ctx->c, nst, ctx->c.ws_S.p, ctx->c.ws_lam.as<double>(), ctx->c.ws_X.p, ctx->c.ws_eig.p);
   // rotation rows: phi_k <- sum_j u_k[j] phi_j  (mode 2: M = R)
-  INQX_HIP(hipMemcpyAsync(ctx->c.ws_S.p, u.data(), 16 * (size_t)nst * nst, hipMemcpyHostToDevice, ctx->c.stream));
-  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
   ctx->c.ws_tmp.ensure(cs * (size_t)nst * npw);
-  launch_rotate(ctx->c, precision, nst, nst, npw, ctx->c.ws_S.as<double>(), 2, 1.0, 0.0, phi, ctx->c.ws_tmp.p, 0.0, 0.0,
+  launch_rotate(ctx->c, precision, nst, nst, npw, ctx->c.ws_X.as<double>(), 2, 1.0, 0.0, phi, ctx->c.ws_tmp.p, 0.0, 0.0,
                 ctx->c.ws_ops);
   INQX_HIP(hipMemcpyAsync(phi, ctx->c.ws_tmp.p, cs * (size_t)nst * npw, hipMemcpyDeviceToDevice, ctx->c.stream));
-  for (int k = 0; k < nst; ++k) eigenvalues[k] = w[k];
+  INQX_HIP(hipMemcpyAsync(eigenvalues, ctx->c.ws_lam.p, sizeof(double) * nst, hipMemcpyDeviceToHost, ctx->c.stream));
+  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END(ctx)
+}
+
+int inqx_diagonalize(inqx_ctx ctx, int n, void *matrix, double *eigenvalues) {
+  API_BEGIN
+  INQX_CHECK(ctx && matrix && eigenvalues && n > 0, INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(n <= kMaxEigN, INQX_ERR_UNSUPPORTED, "the device eigensolver takes n <= 1024");
+  ctx->c.ws_eig.ensure(eigh_workspace_bytes(n));
+  ctx->c.ws_lam.ensure(sizeof(double) * n);
+  ctx->c.ws_X.ensure(16 * (size_t)n * n);
+  INQX_HIP(hipMemcpyAsync(ctx->c.ws_X.p, matrix, 16 * (size_t)n * n, hipMemcpyDeviceToDevice, ctx->c.stream));
+  launch_eigh(ctx->c, n, ctx->c.ws_X.p, ctx->c.ws_lam.as<double>(), matrix, ctx->c.ws_eig.p);
+  launch_conj(ctx->c, matrix, (long)n * n);  // rows = conj(eigenvectors), the reference's orientation
+  INQX_HIP(hipMemcpyAsync(eigenvalues, ctx->c.ws_lam.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->c.stream));
+  INQX_HIP(hipStreamSynchronize(ctx->c.stream));
   API_END(ctx)
 }
 

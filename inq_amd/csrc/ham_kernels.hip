@@ -77,7 +77,6 @@ struct HamDev {
   const int *slot_atom;
   const int *ent_cs;
   const void *ent_b;
-  const float *ent_pos;
   const int *atom_slots, *atom_slot_list;
   const double *coeff;
   double kb_scale;  // dV / N
@@ -95,9 +94,6 @@ __device__ __forceinline__ int find_block(const int *prefix, int nblocks, int wg
   return lo;
 }
 
-template <typename T> __device__ __forceinline__ void sincos_t(T a, T *s, T *c);
-template <> __device__ __forceinline__ void sincos_t<float>(float a, float *s, float *c) { sincosf(a, s, c); }
-template <> __device__ __forceinline__ void sincos_t<double>(double a, double *s, double *c) { sincos(a, s, c); }
 
 // -------------------------------------------------------------------------------------------
 // plane passes
@@ -184,10 +180,10 @@ template <typename T, int N1, int N2> struct PlaneCfg {
 
 template <typename T, int N1, int N2>
 __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                       int total) {
+                                                       int total, int pb) {
   using C = cplx<T>;
   using Cfg = PlaneCfg<T, N1, N2>;
-  constexpr int P = Cfg::P;
+  const int P = pb;  // planes per work-group, 1 <= pb <= Cfg::P (launch shape)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -266,10 +262,10 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
 
 template <typename T, int N1, int N2>
 __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                       int total, T c1r, T c1i, T c2r, T c2i) {
+                                                       int total, int pb, T c1r, T c1i, T c2r, T c2i) {
   using C = cplx<T>;
   using Cfg = PlaneCfg<T, N1, N2>;
-  constexpr int P = Cfg::P;
+  const int P = pb;  // planes per work-group, 1 <= pb <= Cfg::P (launch shape)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -361,8 +357,13 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
     const unsigned nb = (unsigned)((size_t)np * B.npw * sizeof(C));
     const auto rpsi = make_rsrc(B.psi + base, nb);
     const auto rh = make_rsrc(B.hpsi + base, nb);
-    const auto ra1 = make_rsrc(B.acc1 ? B.acc1 + base : B.hpsi + base, B.acc1 ? nb : 0u);
-    const auto ra2 = make_rsrc(B.acc2 ? B.acc2 + base : B.hpsi + base, B.acc2 ? nb : 0u);
+    // accumulators in the working precision, or complex128 (acc_wide: an FP64
+    // state accumulating FP32 H applications; offsets then scale by 2)
+    const unsigned wsh = B.acc_wide ? 1u : 0u;
+    const auto ra1 = make_rsrc(B.acc1 ? (const void *)((const char *)B.acc1 + (base * sizeof(C) << wsh)) : (const void *)(B.hpsi + base),
+                               B.acc1 ? nb << wsh : 0u);
+    const auto ra2 = make_rsrc(B.acc2 ? (const void *)((const char *)B.acc2 + (base * sizeof(C) << wsh)) : (const void *)(B.hpsi + base),
+                               B.acc2 ? nb << wsh : 0u);
     const auto rk = make_rsrc(B.kin ? (const void *)B.kin : (const void *)B.psi,
                               B.kin ? (unsigned)(B.npw * sizeof(T)) : 0u);
     const GAS int *pwyz = gp(B.pw_yz);
@@ -405,18 +406,36 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
           }
         }
         if (B.acc1) {
-          C q;
-          bload(ra1, oo[u], 0u, q);
-          q.x += c1r * out.x - c1i * out.y;
-          q.y += c1r * out.y + c1i * out.x;
-          bstore(ra1, oo[u], 0u, q);
+          if (B.acc_wide) {
+            cplx<double> q;
+            const unsigned ow = oo[u] == kOOB ? kOOB : oo[u] << 1;
+            bload(ra1, ow, 0u, q);
+            q.x += (double)c1r * (double)out.x - (double)c1i * (double)out.y;
+            q.y += (double)c1r * (double)out.y + (double)c1i * (double)out.x;
+            bstore(ra1, ow, 0u, q);
+          } else {
+            C q;
+            bload(ra1, oo[u], 0u, q);
+            q.x += c1r * out.x - c1i * out.y;
+            q.y += c1r * out.y + c1i * out.x;
+            bstore(ra1, oo[u], 0u, q);
+          }
         }
         if (B.acc2) {
-          C q;
-          bload(ra2, oo[u], 0u, q);
-          q.x += c2r * out.x - c2i * out.y;
-          q.y += c2r * out.y + c2i * out.x;
-          bstore(ra2, oo[u], 0u, q);
+          if (B.acc_wide) {
+            cplx<double> q;
+            const unsigned ow = oo[u] == kOOB ? kOOB : oo[u] << 1;
+            bload(ra2, ow, 0u, q);
+            q.x += (double)c2r * (double)out.x - (double)c2i * (double)out.y;
+            q.y += (double)c2r * (double)out.y + (double)c2i * (double)out.x;
+            bstore(ra2, ow, 0u, q);
+          } else {
+            C q;
+            bload(ra2, oo[u], 0u, q);
+            q.x += c2r * out.x - c2i * out.y;
+            q.y += c2r * out.y + c2i * out.x;
+            bstore(ra2, oo[u], 0u, q);
+          }
         }
       }
     }
@@ -515,16 +534,6 @@ template <int N0> __device__ __forceinline__ int mask_count_low(const uint32_t (
 __device__ __forceinline__ void lds_add(float *p, float v) { atomicAdd(p, v); }
 __device__ __forceinline__ void lds_add(double *p, double v) { atomicAdd(p, v); }
 
-// phase exp(sgn i k.pos) with the float relative position the reference
-// keeps (projector_all.hpp:123,284)
-template <typename T>
-__device__ __forceinline__ cplx<T> kb_phase(const double *kcov, V4<float> pos, T sgn) {
-  const T arg = sgn * (T)(kcov[0] * (double)pos.x + kcov[1] * (double)pos.y + kcov[2] * (double)pos.z);
-  T sn, cs;
-  sincos_t<T>(arg, &sn, &cs);
-  return cplx<T>{cs, sn};
-}
-
 // Inverse FFT along i0 of the 32 lines of a tile (one wave per (tile,
 // state), a lane pair per line), then the KB projections P[slot][l] =
 // sum_entries beta_l exp(+ik.pos) psi.  Each lane walks a contiguous chunk of
@@ -587,13 +596,13 @@ __global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blo
         // four consecutive entries: loads first, then the in-order accumulation
         V2<int> cs[4];
         V4<T> bt[4];
-        V4<float> ps4[4];
+        V2<T> ph[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int eu = min(e + u, eend - 1);
           cs[u] = ld2<int>(hd.ent_cs, eu);
           bt[u] = ld4<T>(hd.ent_b, eu);
-          if (!B.gamma) ps4[u] = ld4<float>(hd.ent_pos, eu);
+          if (!B.gamma) ph[u] = ld2<T>(B.kbph, eu);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -608,7 +617,7 @@ __global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blo
               a0 = a1 = a2 = a3 = C{0, 0};
             }
             C v = pts[cs[u].x];
-            if (!B.gamma) v = cmul(v, kb_phase<T>(B.kcov, ps4[u], (T)1));
+            if (!B.gamma) v = cmul(v, C{ph[u].x, ph[u].y});
             a0.x += bt[u].x * v.x; a0.y += bt[u].x * v.y;
             a1.x += bt[u].y * v.x; a1.y += bt[u].y * v.y;
             a2.x += bt[u].z * v.x; a2.y += bt[u].z * v.y;
@@ -785,7 +794,10 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
           C t;
           t.x = bt.x * dd[0].x + bt.y * dd[1].x + bt.z * dd[2].x + bt.w * dd[3].x;
           t.y = bt.x * dd[0].y + bt.y * dd[1].y + bt.z * dd[2].y + bt.w * dd[3].y;
-          if (!B.gamma) t = cmul(t, kb_phase<T>(B.kcov, ld4<float>(hd.ent_pos, u ? e1 : e), (T)-1));
+          if (!B.gamma) {
+            const V2<T> ph = ld2<T>(B.kbph, u ? e1 : e);
+            t = cmul(t, C{ph.x, -ph.y});  // exp(-i (k+A).r)
+          }
           T *q = reinterpret_cast<T *>(pts + cs.x);
           lds_add(q + 0, t.x);
           lds_add(q + 1, t.y);
@@ -1032,6 +1044,8 @@ void build_kb_tables(inqx_ham_s *h) {
   }
   kb.nslot = (int)slot_atom.size();
   kb.nent = (long)mem.size();
+  ++kb.gen;
+  kb.pos_host = epos;
   std::vector<int> atom_slots(kb.natoms + 1, 0), atom_slot_list;
   for (int a = 0; a < kb.natoms; ++a) {
     atom_slots[a] = (int)atom_slot_list.size();
@@ -1054,7 +1068,6 @@ void build_kb_tables(inqx_ham_s *h) {
   upload(kb.ent_cs, ent_cs, st);
   upload(kb.ent_b32, eb32, st);
   upload(kb.ent_b64, eb64, st);
-  upload(kb.ent_pos, epos, st);
   upload(kb.atom_slots, atom_slots, st);
   upload(kb.atom_slot_list, atom_slot_list, st);
   upload(kb.coeff, coeff, st);
@@ -1092,6 +1105,34 @@ void ensure_basis_device(inqx_ham_s *h, Basis &b) {
   }
 }
 
+// exp(+i (k+A).r_e) of every KB entry e for this basis' k and the current A,
+// with r_e the entry's float contravariant position as the reference keeps it
+// (projector_all.hpp:123,284): the phase depends on (k + A, entry) only, so it
+// is tabulated once instead of evaluated per (entry, state) in the x passes
+static void ensure_phase_table(inqx_ham_s *h, Basis &b) {
+  const KBTables &kb = h->kb;
+  if (b.ph_gen == kb.gen && b.ph_A[0] == h->A[0] && b.ph_A[1] == h->A[1] && b.ph_A[2] == h->A[2]) return;
+  INQX_CHECK(b.pinned == 0, INQX_ERR_STATE, "basis tables are pinned by a captured propagation graph");
+  const long nent = kb.nent;
+  std::vector<double> p64(2 * (size_t)std::max(1L, nent));
+  std::vector<float> p32(2 * (size_t)std::max(1L, nent));
+  const double kk[3] = {b.kcov[0] + h->A[0], b.kcov[1] + h->A[1], b.kcov[2] + h->A[2]};
+  for (long e = 0; e < nent; ++e) {
+    const float *r = &kb.pos_host[4 * (size_t)e];
+    const double arg = kk[0] * (double)r[0] + kk[1] * (double)r[1] + kk[2] * (double)r[2];
+    p64[2 * e] = std::cos(arg);
+    p64[2 * e + 1] = std::sin(arg);
+    p32[2 * e] = (float)p64[2 * e];
+    p32[2 * e + 1] = (float)p64[2 * e + 1];
+  }
+  hipStream_t st = h->ctx->c.stream;
+  upload(b.d_ph64, p64, st);
+  upload(b.d_ph32, p32, st);
+  INQX_HIP(hipStreamSynchronize(st));
+  b.ph_gen = kb.gen;
+  for (int d = 0; d < 3; ++d) b.ph_A[d] = h->A[d];
+}
+
 void ensure_workspace(inqx_ham_s *h, Basis &b, int nst, int prec) {
   const size_t cs = prec == INQX_FP64 ? 16 : 8;
   if (b.ws_nst[prec] >= nst && b.t1[prec].p && b.ws_kb_nslot[prec] == h->kb.nslot &&
@@ -1122,8 +1163,27 @@ struct LaunchData {
   int lds_per_wave = 0;
   int mode = 0;                      // ApplyOpts::mode
   int total_line = 0, ntile_line = 0;  // line passes of the transform modes
+  unsigned long long last_use = 0;     // lookup counter at the most recent use (eviction order)
+  int plane_batch = 1;                 // planes (states) per plane-pass work-group
 };
 static std::map<std::pair<inqx_ham_s *, LaunchKey>, std::shared_ptr<LaunchData>> g_launch_cache;
+static unsigned long long g_launch_clock = 0;
+constexpr size_t kLaunchCacheCap = 512;
+
+// drops the least recently used half of the descriptors: those of the current
+// step were just used and stay (a graph capture that follows an eager step
+// finds every configuration the step needs)
+static void evict_launch_cache() {
+  std::vector<unsigned long long> uses;
+  uses.reserve(g_launch_cache.size());
+  for (auto &kv : g_launch_cache) uses.push_back(kv.second->last_use);
+  std::nth_element(uses.begin(), uses.begin() + uses.size() / 2, uses.end());
+  const unsigned long long cut = uses[uses.size() / 2];
+  for (auto it = g_launch_cache.begin(); it != g_launch_cache.end();) {
+    if (it->second->last_use < cut) it = g_launch_cache.erase(it);
+    else ++it;
+  }
+}
 
 // while a hipGraph is being captured, every descriptor it uses is kept alive
 // by the capturing object, whatever happens to the cache later
@@ -1257,6 +1317,7 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   set_plane_attr<T, N12, N12>(h->ctx->c.device);
   const int cu = h->ctx->c.num_cu;
   const int grid_plane = ld.total_plane;  // one item per work-group
+  const int plane_lds = ld.plane_batch * Cfg::PLANE_BYTES;
   Ctx &cx = h->ctx->c;
   const bool f64 = std::is_same<T, double>::value;
   if (ld.mode == 2) {  // real space -> sphere
@@ -1269,16 +1330,16 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
     INQX_HIP(hipGetLastError());
     {
       ProfScope ps(cx, f64 ? "plane_forward_f64" : "plane_forward_f32");
-      hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS_F), Cfg::LDS, st, blk,
-                         pref_plane, nblocks, ld.total_plane, (T)0, (T)0, (T)0, (T)0);
+      hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS_F), plane_lds, st, blk,
+                         pref_plane, nblocks, ld.total_plane, ld.plane_batch, (T)0, (T)0, (T)0, (T)0);
     }
     INQX_HIP(hipGetLastError());
     return;
   }
   {
   ProfScope ps(cx, f64 ? "plane_inverse_f64" : "plane_inverse_f32");
-  hipLaunchKernelGGL((k_plane_inverse<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), Cfg::LDS, st, blk,
-                     pref_plane, nblocks, ld.total_plane);
+  hipLaunchKernelGGL((k_plane_inverse<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), plane_lds, st, blk,
+                     pref_plane, nblocks, ld.total_plane, ld.plane_batch);
   }
   INQX_HIP(hipGetLastError());
   if (ld.mode == 1) {  // sphere -> real space
@@ -1315,8 +1376,8 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   INQX_HIP(hipGetLastError());
   {
     ProfScope ps(cx, f64 ? "plane_forward_f64" : "plane_forward_f32");
-    hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS_F), Cfg::LDS, st, blk,
-                       pref_plane, nblocks, ld.total_plane, (T)cf[0], (T)cf[1], (T)cf[2], (T)cf[3]);
+    hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS_F), plane_lds, st, blk,
+                       pref_plane, nblocks, ld.total_plane, ld.plane_batch, (T)cf[0], (T)cf[1], (T)cf[2], (T)cf[3]);
   }
   INQX_HIP(hipGetLastError());
 }
@@ -1348,11 +1409,15 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   key.k.push_back((uintptr_t)opts->rs_layout);
   key.k.push_back((uintptr_t)(opts->accumulate ? 1 : 0));
   key.k.push_back((uintptr_t)h->xwaves[prec]);
+  key.k.push_back((uintptr_t)h->plane_batch[prec]);
   if (mode != 0)
     for (int i = 0; i < nblocks; ++i) key.k.push_back((uintptr_t)opts->rs[i]);
   for (int i = 0; i < nblocks; ++i) {
     ensure_basis_device(h, *bases[i]);
     ensure_workspace(h, *bases[i], nst[i], prec);
+    const double *kc = bases[i]->kcov;
+    const bool gamma_i = kc[0] + h->A[0] == 0.0 && kc[1] + h->A[1] == 0.0 && kc[2] + h->A[2] == 0.0;
+    if (has_kb && !gamma_i) ensure_phase_table(h, *bases[i]);
     key.k.push_back((uintptr_t)bases[i]);
     key.k.push_back((uintptr_t)nst[i]);
     key.k.push_back((uintptr_t)psi[i]);
@@ -1360,6 +1425,8 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     key.k.push_back((uintptr_t)bases[i]->t1[prec].p);
     key.k.push_back((uintptr_t)(opts->acc1 ? opts->acc1[i] : nullptr));
     key.k.push_back((uintptr_t)(opts->acc2 ? opts->acc2[i] : nullptr));
+    key.k.push_back((uintptr_t)(opts->acc_wide ? 1 : 0));
+    key.k.push_back((uintptr_t)(prec == INQX_FP64 ? bases[i]->d_ph64.p : bases[i]->d_ph32.p));
   }
   key.k.push_back((uintptr_t)(opts->write_h ? 1 : 0));
   key.k.push_back((uintptr_t)(opts->observables ? 1 : 0));
@@ -1385,7 +1452,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   auto ck = std::make_pair(h, key);
   auto it = g_launch_cache.find(ck);
   if (it == g_launch_cache.end()) {
-    if (g_launch_cache.size() > 256) g_launch_cache.clear();
+    if (g_launch_cache.size() >= kLaunchCacheCap) evict_launch_cache();
     INQX_CHECK(!g_capture_keep, INQX_ERR_STATE, "launch configuration first seen during graph capture");
     auto ld = std::make_shared<LaunchData>();
     std::vector<SphereBlock<T>> blk(nblocks);
@@ -1402,6 +1469,9 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
 #undef CASE
       default: break;
     }
+    // the launch shape's plane batch (autotuned), at most what the LDS budget allows
+    if (h->plane_batch[prec] > 0) plane_p = std::min(plane_p, h->plane_batch[prec]);
+    ld->plane_batch = plane_p;
     // waves (states) per x-pass workgroup within the LDS budget
     const int cs = (int)sizeof(cplx<T>);
     int nw = std::max(1, std::min(4, h->xwaves[prec]));  // states (waves) per x-pass work-group
@@ -1431,6 +1501,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
       d.accumulate = opts->accumulate ? 1 : 0;
       d.acc1 = opts->acc1 ? reinterpret_cast<cplx<T> *>(opts->acc1[i]) : nullptr;
       d.acc2 = opts->acc2 ? reinterpret_cast<cplx<T> *>(opts->acc2[i]) : nullptr;
+      d.acc_wide = opts->acc_wide && !std::is_same<T, double>::value ? 1 : 0;
       d.proj = opts->observables ? B.proj[prec].as<cplx<T>>() : nullptr;
       d.dip = opts->observables ? B.dip.as<double>() : nullptr;
       d.write_h = (opts->write_h || mode == 2) ? 1 : 0;
@@ -1442,6 +1513,9 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
       d.m_lo = B.m_lo;
       for (int k = 0; k < 3; ++k) d.kcov[k] = B.kcov[k] + Afull[k];
       d.gamma = (d.kcov[0] == 0.0 && d.kcov[1] == 0.0 && d.kcov[2] == 0.0) ? 1 : 0;
+      d.kbph = (has_kb && !d.gamma)
+                   ? (prec == INQX_FP64 ? (const cplx<T> *)B.d_ph64.p : (const cplx<T> *)B.d_ph32.p)
+                   : nullptr;
       pp[i + 1] = pp[i] + d.nslot * ((nst[i] + plane_p - 1) / plane_p);
       if (mode == 0)
         px[i + 1] = px[i] + h->kb.ntile * ((nst[i] + nw - 1) / nw);
@@ -1470,6 +1544,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     it = g_launch_cache.emplace(ck, std::move(ld)).first;
   }
   LaunchData &ld = *it->second;
+  ld.last_use = ++g_launch_clock;
   if (g_capture_keep) g_capture_keep->push_back(it->second);
   HamDev hd;
   hd.n0 = g.n[0];
@@ -1487,7 +1562,6 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   hd.slot_atom = h->kb.slot_atom.as<int>();
   hd.ent_cs = h->kb.ent_cs.as<int>();
   hd.ent_b = prec == INQX_FP64 ? h->kb.ent_b64.p : h->kb.ent_b32.p;
-  hd.ent_pos = h->kb.ent_pos.as<float>();
   hd.atom_slots = h->kb.atom_slots.as<int>();
   hd.atom_slot_list = h->kb.atom_slot_list.as<int>();
   hd.coeff = h->kb.coeff.as<double>();
@@ -1508,6 +1582,18 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
 }
 
 int dots_chunks(int npw) { return (npw + kDotChunk - 1) / kDotChunk; }
+
+// largest plane batch the plane passes of this grid and precision take
+int max_plane_batch(const int *n, int prec) {
+  switch (n[1]) {
+#define CASE(S) \
+  case S:       \
+    return prec == INQX_FP64 ? PlaneCfg<double, S, S>::P : PlaneCfg<float, S, S>::P;
+    INQX_SIZES(CASE)
+#undef CASE
+    default: return 1;
+  }
+}
 
 void launch_sphere_dots(Ctx &cx, int prec, const void *psi, const void *hpsi, const void *kin, int nst, int npw,
                         double *partial) {

@@ -106,7 +106,7 @@ struct Ctx {
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, KStat> stats;
   // workspaces of the linear-algebra entry points
-  DevMem ws_partial, ws_ops, ws_tmp, ws_tmp2, ws_S, ws_X, ws_info;
+  DevMem ws_partial, ws_ops, ws_tmp, ws_tmp2, ws_S, ws_X, ws_info, ws_eig, ws_lam;
   hipEvent_t get_event();
   void flush_prof();
 };
@@ -160,6 +160,7 @@ constexpr int kTile = 32;    // x-pencils (lines) per tile: one wavefront, a lan
 // x-pass kernels read them with independent, coalesced loads.
 struct KBTables {
   bool built = false;
+  int gen = 0;  // incremented by every build: the per-basis phase tables follow it
   int natoms = 0, maxsph = 0, ntile = 0, nslot = 0, maxpts = 0, maxslot = 0;
   long nent = 0;
   // per line: bitmask of sphere points along i0 (3 words) and the line's
@@ -170,10 +171,10 @@ struct KBTables {
   DevMem ent_cs;      // [nent] int2: compact point index, tile-local slot
   DevMem ent_b32;     // [nent] float4: projector values l = 0..3
   DevMem ent_b64;     // [nent] double4
-  DevMem ent_pos;     // [nent] float4: contravariant relative position (float, as the reference stores it)
   DevMem atom_slots;  // [natoms+1] offsets into atom_slot_list
   DevMem atom_slot_list;
   DevMem coeff;       // [natoms][kMaxNlm][kMaxNlm] double
+  std::vector<float> pos_host;  // [nent][4] the entry positions, for the per-basis phase tables
 };
 
 // ------------------------------------------------------------------ device block descriptor
@@ -190,6 +191,7 @@ template <typename T> struct SphereBlock {
   const T *kin;        // [npw] 1/2 |G+k|^2
   const T *vloc;       // [N] V/N of the block's spin
   C *acc1, *acc2;      // Taylor accumulators acc += c (H psi), or null
+  int acc_wide;        // the accumulators hold complex128 values (T = float)
   C *proj;             // KB projections P [natoms][kMaxNlm][nst], or null
   double *dip;         // dipole partials [ntile][nst][4], or null
   int nst, npw;
@@ -198,6 +200,7 @@ template <typename T> struct SphereBlock {
   int write_h;         // store H psi into hpsi
   double kcov[3];      // k + A, covariant
   // sphere <-> real-space transforms (kin == nullptr in plane_forward: plain masked FFT)
+  const C *kbph;       // [nent] exp(+i (k+A).r) of every KB entry, or null at k + A = 0
   C *rs;               // real-space states, [nst][N] (rs_layout 0) or [N][nst] (1)
   int rs_layout;
   int accumulate;      // plane_forward adds into hpsi instead of storing
@@ -208,6 +211,7 @@ struct ApplyOpts {
   void *const *acc1 = nullptr;  // per block, acc1 += c1 (H psi)
   void *const *acc2 = nullptr;  // per block, acc2 += c2 (H psi)
   double c1r = 0.0, c1i = 0.0, c2r = 0.0, c2i = 0.0;
+  bool acc_wide = false;        // acc1/acc2 are complex128 arrays for FP32 applications
   bool write_h = true;
   bool observables = false;     // projections and dipole partials into the basis workspaces
   // mode 0: H application; 1: sphere -> real space (psi -> rs, x 1/N);
@@ -233,6 +237,11 @@ struct Basis {
   DevMem d_slot_seg, d_rows, d_pw_yz, d_kin64, d_kin32;
   DevMem d_g2h;  // (1/2)|G|^2 per sphere point, FP64, no k shift (preconditioner)
   double kin_A[3] = {1e300, 0, 0};  // A for which d_kin* were built
+  // KB phases exp(+i (k+A).r_e) per KB entry e (FP64 and FP32 copies), built for
+  // the vector potential ph_A and the KB table generation ph_gen
+  DevMem d_ph64, d_ph32;
+  double ph_A[3] = {1e300, 0, 0};
+  int ph_gen = -1;
   // workspaces per precision
   DevMem t1[2], partial[2], dp[2], proj[2], dip, dots, hn[2][2], save[2];
   int ws_nst[2] = {0, 0};
@@ -275,6 +284,8 @@ struct inqx_ham_s {
   int pinned = 0;  // > 0 while a captured graph uses the projector tables
   // launch shape of the x-pass kernels per precision: states (waves) per work-group
   int xwaves[2] = {1, 1};
+  // plane-pass planes (states) per work-group per precision; 0: the largest the grid allows
+  int plane_batch[2] = {0, 0};
   // exact-exchange workspaces: real-space states, hf states, accumulator, pair planes
   inqx::DevMem exx_phi, exx_hf, exx_w, exx_t, exx_jlist, exx_coef;
   // real-space apply and density workspaces (sphere arrays of a full-cube basis, occupations)
@@ -303,6 +314,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
 void ensure_basis_device(inqx_ham_s *h, Basis &b);
 void ensure_workspace(inqx_ham_s *h, Basis &b, int nst, int prec);
 bool grid_supported(const int *n);
+int max_plane_batch(const int *n, int prec);
 // linalg.hip
 void launch_overlap(Ctx &cx, int prec, int n1, int n2, long K, const void *A, const void *B, double vol, double *S,
                     DevMem &partial);
@@ -358,6 +370,14 @@ constexpr int kAtomRec = 5;
 void launch_atomic_field_add(inqx_ham_s *h, int natoms, const double *atoms_dev, int nr, double dr,
                              const double *tables_dev, double *field);
 void launch_hartree(inqx_ham_s *h, const double *rho, double *vh, void *work, double *part);
+// eigh.hip: FP64 Hermitian eigensolver on the device.  A [n][n] (destroyed)
+// -> lam [n] ascending, U [n][n] rows = eigenvectors; ws of eigh_workspace_bytes(n)
+constexpr int kMaxEigN = 1024;
+size_t eigh_workspace_bytes(int n);
+void launch_eigh(Ctx &cx, int n, void *A, double *lam, void *U, void *ws);
+void launch_conj(Ctx &cx, void *a, long n);
+// propagate.cpp: workspaces of inqx_ks_potential, dropped with the Hamiltonian
+void release_ks_work(inqx_ham_s *h);
 // allocation-free Hartree (inqx_hartree_noalloc): its scratch is a file-scope
 // device array of kNoAllocScratchBytes; noalloc_scratch_init looks up its
 // address once per device (at context creation, with the device current).

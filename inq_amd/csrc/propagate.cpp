@@ -16,9 +16,11 @@
 // orbitals; everything is reduced in FP64 into a device ring of records.
 #include "internal.hpp"
 
+#include <map>
 #include <memory>
 
 #include <algorithm>
+#include <cmath>
 #include <complex>
 #include <cstring>
 
@@ -50,7 +52,7 @@ static void ensure_ws(Basis &B, DevMem &m, size_t bytes) {
 
 template <typename T>
 static void exp_in_place(inqx_ham_s *h, int nb, Basis *const *bases, const int *nst, void *const *phi,
-                         std::complex<double> f, int order) {
+                         std::complex<double> f, int order, void *const *acc = nullptr) {
   const int prec = std::is_same<T, double>::value ? INQX_FP64 : INQX_FP32;
   const size_t cs = sizeof(cplx<T>);
   std::vector<void *> hA(nb), hB(nb);
@@ -71,12 +73,40 @@ static void exp_in_place(inqx_ham_s *h, int nb, Basis *const *bases, const int *
       out[i] = (it % 2 == 1) ? hA[i] : hB[i];
     }
     ApplyOpts o;
-    o.acc1 = phi;
+    // the Taylor terms accumulate into phi, or into the complex128 copy acc of
+    // an FP32 state (phi then holds the FP32 input of the first application)
+    o.acc1 = acc ? acc : phi;
+    o.acc_wide = acc != nullptr;
     o.c1r = coeff.real();
     o.c1i = coeff.imag();
     o.write_h = it < order;
     launch_apply<T>(h, nb, bases, nst, in.data(), out.data(), &o);
   }
+}
+
+// complex64 <-> complex128 copies of a state block (n complex values)
+__global__ void __launch_bounds__(256) k_widen(const float2 *a, double2 *b, long n) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    const float2 v = a[i];
+    b[i] = double2{(double)v.x, (double)v.y};
+  }
+}
+__global__ void __launch_bounds__(256) k_narrow(const double2 *a, float2 *b, long n) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    const double2 v = a[i];
+    b[i] = float2{(float)v.x, (float)v.y};
+  }
+}
+static void launch_widen(Ctx &cx, const void *a, void *b, long n) {
+  hipLaunchKernelGGL(k_widen, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, cx.stream, (const float2 *)a, (double2 *)b, n);
+  INQX_HIP(hipGetLastError());
+}
+static void launch_narrow(Ctx &cx, const void *a, void *b, long n) {
+  ProfScope ps(cx, "state_narrow");
+  hipLaunchKernelGGL(k_narrow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, cx.stream, (const double2 *)a, (float2 *)b, n);
+  INQX_HIP(hipGetLastError());
 }
 
 template <typename T>
@@ -157,6 +187,155 @@ static void energy_record(inqx_ham_s *h, int nb, Basis *const *bases, const int 
   launch_obs_combine(h->ctx->c, prec, obs_desc.p, nb, h->kb.natoms, h->kb.ntile, vol_g, dip_scale, ring, cap, counter);
 }
 
+// ---------------------------------------------------------------- Kohn-Sham potential (Hartree level)
+// update_hamiltonian (src/hamiltonian/self_consistency.hpp:100-125) without
+// XC: V_KS = vion + V_H[total density] for every spin component, written in
+// place into the Hamiltonian's device potential (V/N in FP64 and FP32), with
+// the external and Hartree energies reduced in a fixed order.
+constexpr int kScfChunk = 8192;
+
+__global__ void __launch_bounds__(256) k_spin_total(const double *den, long N, int ncomp, double *tot) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  if (p >= N) return;
+  double t = 0.0;
+  for (int c = 0; c < ncomp; ++c) t += den[p * ncomp + c];
+  tot[p] = t;
+}
+
+// v64/v32[c][p] = (vion + vh) / N; part[chunk] = {sum rho vion, sum rho vh}
+__global__ void __launch_bounds__(256) k_vks_assemble(const double *vion, const double *vh, const double *rho, long N,
+                                                     int ncomp, double invN, double *v64, float *v32, double *part) {
+  __shared__ double red[2][256];
+  const int tid = threadIdx.x;
+  const long p0 = (long)blockIdx.x * kScfChunk;
+  double ae = 0.0, ah = 0.0;
+  for (long p = p0 + tid; p < p0 + kScfChunk && p < N; p += 256) {
+    const double vi = vion[p], h = vh[p];
+    const double v = (vi + h) * invN;
+    for (int c = 0; c < ncomp; ++c) {
+      v64[(size_t)c * N + p] = v;
+      v32[(size_t)c * N + p] = (float)v;
+    }
+    ae += rho[p] * vi;
+    ah += rho[p] * h;
+  }
+  red[0][tid] = ae;
+  red[1][tid] = ah;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+      red[0][tid] += red[0][tid + w];
+      red[1][tid] += red[1][tid + w];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    part[2 * blockIdx.x] = red[0][0];
+    part[2 * blockIdx.x + 1] = red[1][0];
+  }
+}
+
+// part[chunk] = sum |a - b| over the chunk (operations::integral_sum_absdiff before dV)
+__global__ void __launch_bounds__(256) k_absdiff(const double *a, const double *b, long n, double *part) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  const long p0 = (long)blockIdx.x * kScfChunk;
+  double acc = 0.0;
+  for (long p = p0 + tid; p < p0 + kScfChunk && p < n; p += 256) acc += fabs(a[p] - b[p]);
+  red[tid] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) part[blockIdx.x] = red[0];
+}
+
+static int scf_chunks(long n) { return (int)((n + kScfChunk - 1) / kScfChunk); }
+
+// workspaces of the potential update: total density, V_H, Hartree work, partial sums
+struct KsWork {
+  DevMem tot, vh, part;
+};
+
+// V_KS from the spin density den [N][ncomp] (FP64, device); energies (external,
+// Hartree) on the host when requested (the call then synchronises)
+static void ks_update(inqx_ham_s *h, const double *vion, const double *den, KsWork &w, double *energies) {
+  Ctx &cx = h->ctx->c;
+  const long N = h->g.N;
+  const int ncomp = h->ncomp;
+  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "the Hartree solve needs a cubic grid of a built size");
+  INQX_CHECK(h->v64.p && h->v32.p && h->v64.bytes >= sizeof(double) * (size_t)N * ncomp, INQX_ERR_STATE,
+             "set a potential (inqx_ham_set_potential) before updating it");
+  const double *rho = den;
+  if (ncomp > 1) {
+    w.tot.ensure(sizeof(double) * (size_t)N);
+    hipLaunchKernelGGL(k_spin_total, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, cx.stream, den, N, ncomp,
+                       w.tot.as<double>());
+    INQX_HIP(hipGetLastError());
+    rho = w.tot.as<double>();
+  }
+  w.vh.ensure(sizeof(double) * (size_t)N);
+  h->hartree_work.ensure(16 * (size_t)N);
+  h->hartree_part.ensure(sizeof(double) * (size_t)hartree_chunks(N));
+  launch_hartree(h, rho, w.vh.as<double>(), h->hartree_work.p, h->hartree_part.as<double>());
+  const int nch = scf_chunks(N);
+  w.part.ensure(sizeof(double) * 2 * (size_t)nch);
+  {
+    ProfScope ps(cx, "vks_assemble");
+    hipLaunchKernelGGL(k_vks_assemble, dim3(nch), dim3(256), 0, cx.stream, vion, w.vh.as<double>(), rho, N, ncomp,
+                       1.0 / (double)N, h->v64.as<double>(), h->v32.as<float>(), w.part.as<double>());
+  }
+  INQX_HIP(hipGetLastError());
+  if (energies) {
+    std::vector<double> part(2 * (size_t)nch);
+    INQX_HIP(hipMemcpyAsync(part.data(), w.part.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, cx.stream));
+    INQX_HIP(hipStreamSynchronize(cx.stream));
+    double e = 0.0, eh = 0.0;
+    for (int c = 0; c < nch; ++c) {
+      e += part[2 * c];
+      eh += part[2 * c + 1];
+    }
+    const double dv = h->g.vol / (double)N;
+    energies[0] = e * dv;         // energy.external (self_consistency.hpp:115)
+    energies[1] = 0.5 * eh * dv;  // energy.hartree (self_consistency.hpp:120)
+  }
+}
+
+// workspaces of the standalone potential update, one set per Hamiltonian
+// (dropped with it: inqx_ham_destroy calls release_ks_work)
+static std::map<inqx_ham_s *, std::unique_ptr<KsWork>> g_ks_work;
+static KsWork &ks_work_of(inqx_ham_s *h) {
+  auto &w = g_ks_work[h];
+  if (!w) w.reset(new KsWork());
+  return *w;
+}
+void release_ks_work(inqx_ham_s *h) { g_ks_work.erase(h); }
+
+// spin density of every block's sphere states: den[N][ncomp] = sum occ |to_real(phi)|^2
+template <typename T>
+static void blocks_density(inqx_ham_s *h, int nb, Basis *const *bases, const int *nst, void *const *phi,
+                           const double *const *occ, DevMem &rs, double *den) {
+  Ctx &cx = h->ctx->c;
+  const long N = h->g.N;
+  const int prec = std::is_same<T, double>::value ? INQX_FP64 : INQX_FP32;
+  launch_fill_zero(cx, den, sizeof(double) * (size_t)N * h->ncomp);
+  for (int i = 0; i < nb; ++i) {
+    INQX_CHECK(bases[i]->spin < h->ncomp, INQX_ERR_ARG, "a block's spin index has no density component");
+    rs.ensure(sizeof(cplx<T>) * (size_t)nst[i] * N);
+    ApplyOpts o;
+    o.mode = 1;
+    void *rsp = rs.p;
+    o.rs = &rsp;
+    o.rs_layout = 0;
+    const void *pin = phi[i];
+    void *pout = nullptr;
+    Basis *b = bases[i];
+    launch_apply<T>(h, 1, &b, &nst[i], &pin, &pout, &o);
+    launch_density_add(cx, prec, rs.p, occ[i], nst[i], N, h->ncomp, bases[i]->spin, den);
+  }
+}
+
 }  // namespace inqx
 
 struct inqx_prop_s {
@@ -177,18 +356,157 @@ struct inqx_prop_s {
   hipGraphExec_t exec = nullptr;
   int steps_done = 0;
   std::vector<std::shared_ptr<void>> keep;  // launch descriptors the graph uses
+  // self-consistent steps (inqx_prop_set_scf): ionic potential, SCF limits,
+  // half-step states, densities and workspaces, statistics of the last step
+  const double *vion = nullptr;
+  int nscf = 0;
+  double scf_threshold = 5e-5;
+  std::vector<DevMem> save;
+  DevMem rho, rho_old, rs_work, absdiff_part;
+  KsWork ks;
+  int last_passes = 0;
+  double last_delta = 0.0;
+  // complex128 copies of FP32 states (inqx_prop_set_state_precision): the
+  // propagated state; the caller's arrays receive it rounded after each half step
+  bool wide = false;
+  std::vector<DevMem> phi64;
+  std::vector<void *> phi64p;
 };
 
 template <typename T> static void prop_step_eager(inqx_prop_s *p) {
   const std::complex<double> half(0.0, -p->dt / 2.0);
   const int nb = (int)p->bases.size();
-  exp_in_place<T>(p->ham, nb, p->bases.data(), p->nst.data(), p->phi.data(), half, 4);
-  exp_in_place<T>(p->ham, nb, p->bases.data(), p->nst.data(), p->phi.data(), half, 4);
+  if (p->wide) {
+    for (int hs = 0; hs < 2; ++hs) {
+      exp_in_place<T>(p->ham, nb, p->bases.data(), p->nst.data(), p->phi.data(), half, 4, p->phi64p.data());
+      for (int i = 0; i < nb; ++i)
+        launch_narrow(p->ham->ctx->c, p->phi64p[i], p->phi[i], (long)p->nst[i] * p->bases[i]->npw);
+    }
+  } else {
+    exp_in_place<T>(p->ham, nb, p->bases.data(), p->nst.data(), p->phi.data(), half, 4);
+    exp_in_place<T>(p->ham, nb, p->bases.data(), p->nst.data(), p->phi.data(), half, 4);
+  }
   energy_record<T>(p->ham, nb, p->bases.data(), p->nst.data(), p->phi.data(), p->occ.data(), p->ring.as<double>(),
                    p->cap, p->counter.as<int>(), p->obs_desc, p->obs_host, p->capturing);
 }
 
+// real_time::etrs with a density-dependent H (src/real_time/etrs.hpp:22-79):
+// 2-for-1 half and full steps with H(t), density of the full-step prediction,
+// update_hamiltonian, restore the half step, then up to nscf passes of the
+// second half step with H(t + dt), each followed by the density, its change
+// integral |drho| dV and update_hamiltonian, stopping when the change is below
+// the threshold.  The pass count depends on the density change, which the host
+// reads after each pass: these steps run eagerly.
+template <typename T> static void prop_step_scf(inqx_prop_s *p) {
+  inqx_ham_s *h = p->ham;
+  Ctx &cx = h->ctx->c;
+  const int nb = (int)p->bases.size();
+  const std::complex<double> fhalf(0.0, -p->dt / 2.0), ffull(0.0, -p->dt);
+  const long N = h->g.N;
+  const size_t dbytes = sizeof(double) * (size_t)N * h->ncomp;
+  p->rho.ensure(dbytes);
+  p->rho_old.ensure(dbytes);
+  std::vector<size_t> bytes(nb);
+  for (int i = 0; i < nb; ++i) {
+    bytes[i] = sizeof(cplx<T>) * (size_t)p->nst[i] * p->bases[i]->npw;
+    p->save[i].ensure(bytes[i]);
+    exp_2_for_1<T>(h, p->bases[i], p->nst[i], p->phi[i], p->save[i].p, fhalf, ffull, 4);
+  }
+  blocks_density<T>(h, nb, p->bases.data(), p->nst.data(), p->phi.data(), p->occ.data(), p->rs_work,
+                    p->rho.as<double>());
+  ks_update(h, p->vion, p->rho.as<double>(), p->ks, nullptr);
+  const int nch = scf_chunks((long)N * h->ncomp);
+  p->absdiff_part.ensure(sizeof(double) * (size_t)nch);
+  std::vector<double> part(nch);
+  int passes = 0;
+  double delta = 0.0;
+  for (int iscf = 0; iscf < p->nscf; ++iscf) {
+    for (int i = 0; i < nb; ++i)
+      INQX_HIP(hipMemcpyAsync(p->phi[i], p->save[i].p, bytes[i], hipMemcpyDeviceToDevice, cx.stream));
+    exp_in_place<T>(h, nb, p->bases.data(), p->nst.data(), p->phi.data(), fhalf, 4);
+    INQX_HIP(hipMemcpyAsync(p->rho_old.p, p->rho.p, dbytes, hipMemcpyDeviceToDevice, cx.stream));
+    blocks_density<T>(h, nb, p->bases.data(), p->nst.data(), p->phi.data(), p->occ.data(), p->rs_work,
+                      p->rho.as<double>());
+    hipLaunchKernelGGL(k_absdiff, dim3(nch), dim3(256), 0, cx.stream, p->rho_old.as<double>(), p->rho.as<double>(),
+                       (long)N * h->ncomp, p->absdiff_part.as<double>());
+    INQX_HIP(hipGetLastError());
+    INQX_HIP(hipMemcpyAsync(part.data(), p->absdiff_part.p, sizeof(double) * nch, hipMemcpyDeviceToHost, cx.stream));
+    INQX_HIP(hipStreamSynchronize(cx.stream));
+    delta = 0.0;
+    for (int c = 0; c < nch; ++c) delta += part[c];
+    delta *= h->g.vol / (double)N;
+    const bool done = delta < p->scf_threshold || iscf == p->nscf - 1;
+    ks_update(h, p->vion, p->rho.as<double>(), p->ks, nullptr);
+    ++passes;
+    if (done) break;
+  }
+  p->last_passes = passes;
+  p->last_delta = delta;
+  energy_record<T>(h, nb, p->bases.data(), p->nst.data(), p->phi.data(), p->occ.data(), p->ring.as<double>(), p->cap,
+                   p->counter.as<int>(), p->obs_desc, p->obs_host, false);
+}
+
 extern "C" {
+
+int inqx_ks_potential(inqx_ham h, const double *vion, const double *density, double *energies) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(h && vion && density, INQX_ERR_ARG, "null argument");
+  INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(h->pinned == 0, INQX_ERR_STATE, "potential pinned by a captured propagation graph");
+  ks_update(h, vion, density, ks_work_of(h), energies);
+  if (!energies) INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END(ctx)
+}
+
+int inqx_prop_set_scf(inqx_prop p, const double *vion, int max_passes, double threshold) {
+  inqx_ctx ctx = p ? p->ham->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(p && max_passes >= 0 && std::isfinite(threshold) && threshold >= 0.0, INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!(p->wide && max_passes > 0), INQX_ERR_UNSUPPORTED, "FP64 states with FP32 applications: fixed-H steps only");
+  INQX_CHECK(max_passes == 0 || vion, INQX_ERR_ARG, "the self-consistent step needs the ionic potential");
+  INQX_CHECK(!p->exec, INQX_ERR_STATE, "the propagator already holds a captured graph of the fixed-H step");
+  if (max_passes > 0) INQX_CHECK(pair_grid_supported(p->ham->g.n), INQX_ERR_UNSUPPORTED,
+                                 "the Hartree solve needs a cubic grid of a built size");
+  p->vion = max_passes > 0 ? vion : nullptr;
+  p->nscf = max_passes;
+  p->scf_threshold = threshold;
+  p->save.clear();
+  p->save = std::vector<DevMem>(p->bases.size());
+  API_END(ctx)
+}
+
+int inqx_prop_set_state_precision(inqx_prop p, int precision) {
+  inqx_ctx ctx = p ? p->ham->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(p && (precision == INQX_FP64 || precision == INQX_FP32), INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!p->exec, INQX_ERR_STATE, "the propagator already holds a captured graph");
+  const bool wide = precision == INQX_FP64 && p->prec == INQX_FP32;
+  INQX_CHECK(!(wide && p->nscf > 0), INQX_ERR_UNSUPPORTED, "FP64 states with FP32 applications: fixed-H steps only");
+  p->wide = wide;
+  p->phi64.clear();
+  p->phi64p.clear();
+  if (wide) {
+    p->phi64 = std::vector<DevMem>(p->bases.size());
+    for (size_t i = 0; i < p->bases.size(); ++i) {
+      const long n = (long)p->nst[i] * p->bases[i]->npw;
+      p->phi64[i].ensure(16 * (size_t)n);
+      launch_widen(ctx->c, p->phi[i], p->phi64[i].p, n);
+      p->phi64p.push_back(p->phi64[i].p);
+    }
+    INQX_HIP(hipStreamSynchronize(ctx->c.stream));
+  }
+  API_END(ctx)
+}
+
+int inqx_prop_scf_info(inqx_prop p, int *passes, double *delta) {
+  inqx_ctx ctx = p ? p->ham->ctx : nullptr;
+  API_BEGIN
+  INQX_CHECK(p, INQX_ERR_ARG, "null propagator");
+  if (passes) *passes = p->last_passes;
+  if (delta) *delta = p->last_delta;
+  API_END(ctx)
+}
 
 int inqx_exponential(inqx_ham h, inqx_basis bs, int nst, int precision, double f_re, double f_im, int order,
                      void *phi) {
@@ -273,7 +591,15 @@ int inqx_prop_step(inqx_prop p, int nsteps, int use_graph) {
   API_BEGIN
   INQX_CHECK(p && nsteps >= 0, INQX_ERR_ARG, "bad arguments");
   Ctx &cx = ctx->c;
+  INQX_CHECK(!(use_graph && p->nscf > 0), INQX_ERR_UNSUPPORTED,
+             "self-consistent steps run eagerly (the SCF pass count depends on the density change)");
   for (int s = 0; s < nsteps; ++s) {
+    if (p->nscf > 0) {
+      if (p->prec == INQX_FP64) prop_step_scf<double>(p);
+      else prop_step_scf<float>(p);
+      ++p->steps_done;
+      continue;
+    }
     if (!use_graph || p->steps_done == 0) {
       // eager step; the first one also creates every workspace and launch descriptor
       if (p->prec == INQX_FP64) prop_step_eager<double>(p);

@@ -15,11 +15,39 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def singularity_correction(ctx, lattice, kpoints=((0.0, 0.0, 0.0),), weights=(1.0,)):
+    """hamiltonian::singularity_correction (singularity_correction.hpp:19-103):
+    returns (fk [nk], fzero) for covariant k-points and their weights."""
+    lat = np.ascontiguousarray(lattice, dtype=np.float64).reshape(9)
+    k = np.ascontiguousarray(kpoints, dtype=np.float64).reshape(-1, 3)
+    w = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+    fk = np.zeros(len(k))
+    fz = C.c_double()
+    check(lib().inqx_singularity_correction(ctx.h, _dp(lat), len(k), _dp(k), _dp(w), _dp(fk), C.byref(fz)), ctx.h,
+          "inqx_singularity_correction")
+    return fk, fz.value
+
+
+def singularity_zeroterm(ctx, lattice, kpoints=((0.0, 0.0, 0.0),), weights=(1.0,), ik=0):
+    """the G + q = 0 value of the exchange Poisson solves for the hf states of
+    k-point ik: -nk V (fk[ik] - fzero) (singularity_correction.hpp:100-102)"""
+    fk, fz = singularity_correction(ctx, lattice, kpoints, weights)
+    vol = abs(np.linalg.det(np.asarray(lattice, dtype=np.float64).reshape(3, 3)))
+    return -len(fk) * vol * (fk[ik] - fz)
+
+
 class ExchangeOperator:
     def __init__(self, ham, basis, exchange_coefficient=1.0, zeroterm=0.0):
+        """zeroterm: the G = 0 value of the pair Poisson solves, or "singularity"
+        for the reference's Carrier et al. correction of a Gamma-only k-point set
+        (exchange_operator.hpp:151 passes sing_(idx) to poisson::in_place)"""
         self.ham = ham
         self.basis = basis
         self.coefficient = float(exchange_coefficient)
+        if isinstance(zeroterm, str):
+            if zeroterm != "singularity":
+                raise ValueError(f"unknown zeroterm {zeroterm!r}")
+            zeroterm = singularity_zeroterm(ham.ctx, ham.lattice)
         self.zeroterm = float(zeroterm)
         self.xi = None
         self.energy = None

@@ -272,6 +272,15 @@ class KSHamiltonian:
               "inqx_hartree")
         return vh, (e.value if energy else None)
 
+    def ks_potential(self, vion, density):
+        """update_hamiltonian at the Hartree level (self_consistency.hpp:100-125):
+        the potential becomes vion + V_H[total density] on the device; vion [N]
+        and density [N][ncomp] are device float64 arrays.  Returns the energies
+        (external, Hartree)."""
+        e = (C.c_double * 2)()
+        check(lib().inqx_ks_potential(self.h, vion.ptr, density.ptr, e), self.ctx.h, "inqx_ks_potential")
+        return e[0], e[1]
+
     def energy(self, bases, phis, occs):
         """energy::calculate (energy.hpp:61-113) of the states of every block, with
         the non-local energy (ks_hamiltonian.hpp:124-133) and the dipole
@@ -312,6 +321,24 @@ class KSHamiltonian:
         check(lib().inqx_ham_autotune(self.h, basis.h, int(nst), precision_of(dtype), C.byref(w), C.byref(ms)),
               self.ctx.h, "inqx_ham_autotune")
         return w.value, ms.value
+
+    def autotune_batch(self, bases, nsts, dtype, cache=None):
+        """launch-shape search over (x-pass states per work-group, plane-pass planes
+        per work-group) for these blocks, through the on-disk cache file `cache`
+        (None: no cache); returns ((xwaves, plane batch), ms per launch sequence)."""
+        nb = len(bases)
+        bb = (C.c_void_p * nb)(*[b.h.value for b in bases])
+        ns = (C.c_int * nb)(*[int(n) for n in nsts])
+        shape = (C.c_int * 2)()
+        ms = C.c_double()
+        check(lib().inqx_ham_autotune_batch(self.h, nb, bb, ns, precision_of(dtype),
+                                            None if cache is None else str(cache).encode(), shape, C.byref(ms)),
+              self.ctx.h, "inqx_ham_autotune_batch")
+        return (shape[0], shape[1]), ms.value
+
+    def set_plane_batch(self, dtype, planes):
+        check(lib().inqx_ham_set_plane_batch(self.h, precision_of(dtype), int(planes)), self.ctx.h,
+              "set_plane_batch")
 
     def set_launch_shape(self, dtype, xwaves):
         check(lib().inqx_ham_set_launch_shape(self.h, precision_of(dtype), int(xwaves)), self.ctx.h,

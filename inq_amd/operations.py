@@ -93,3 +93,14 @@ def subspace_diagonalization(ham, basis, phi):
     check(lib().inqx_subspace_diag(ham.h, basis.h, nst, precision_of(phi.dtype), phi.ptr,
                                    ev.ctypes.data_as(C.POINTER(C.c_double))), ham.ctx.h, "inqx_subspace_diag")
     return ev
+
+
+def diagonalize(ctx, matrix):
+    """matrix::diagonalize (diagonalize.hpp:167-189) of a device complex128 [n][n]
+    Hermitian matrix, in place: rows become the conjugated eigenvectors (the
+    reference's orientation); returns the eigenvalues, ascending."""
+    n = matrix.shape[0]
+    ev = np.zeros(n)
+    check(lib().inqx_diagonalize(ctx.h, n, matrix.ptr, ev.ctypes.data_as(C.POINTER(C.c_double))), ctx.h,
+          "inqx_diagonalize")
+    return ev

@@ -61,6 +61,28 @@ class Propagator:
         m = min(n.value, cap)
         return out[:m]
 
+    def set_scf(self, vion, max_passes=5, threshold=5e-5):
+        """self-consistent steps (etrs.hpp:22-79) with V = vion + V_H[density]:
+        vion is a device float64 array [N] kept alive by the propagator; the
+        Hamiltonian's potential must already be that of the initial density
+        (KSHamiltonian.ks_potential).  max_passes = 0 returns to fixed-H steps."""
+        self._vion = vion
+        check(lib().inqx_prop_set_scf(self.h, vion.ptr if vion is not None else None, int(max_passes),
+                                      float(threshold)), self.ham.ctx.h, "inqx_prop_set_scf")
+
+    def set_state_precision(self, dtype):
+        """np.complex128 on an FP32 propagator: FP64 states accumulate the FP32 H
+        applications (the caller's complex64 array gets the rounded state)"""
+        check(lib().inqx_prop_set_state_precision(self.h, precision_of(dtype)), self.ham.ctx.h,
+              "inqx_prop_set_state_precision")
+
+    def scf_info(self):
+        """(SCF passes, integral |drho| dV of the last pass) of the most recent step"""
+        n = C.c_int()
+        d = C.c_double()
+        check(lib().inqx_prop_scf_info(self.h, C.byref(n), C.byref(d)), self.ham.ctx.h, "inqx_prop_scf_info")
+        return n.value, d.value
+
     def close(self):
         if self.h.value:
             lib().inqx_prop_destroy(self.h)

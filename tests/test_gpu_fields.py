@@ -141,3 +141,39 @@ def test_autotune_keeps_results(ctx):
     o = DeviceArray(ctx, c.shape, np.complex64)
     gh.apply(b, d, o)
     assert np.array_equal(o.numpy(), outs[0])
+
+
+def test_autotune_batch_cache_and_results(ctx, tmp_path):
+    """the batch autotuner searches x-pass and plane-pass shapes on uneven Al4
+    k-point blocks, stores the winner in its cache file, applies it from the
+    cache on a second call (same shape, no search), and no shape changes a
+    result bit"""
+    sysm = S.config(3)
+    say("case: autotune batch C3 (8 blocks)")
+    gh = gpu_ham(ctx, sysm)
+    blocks = list(sysm.blocks())[:8]
+    bases = [gh.basis(k, ec, 0) for (k, spin, ec) in blocks]
+    vol = S.sphere_volume_element(sysm)
+    ins = [DeviceArray.from_numpy(ctx, S.random_states(b.npw, sysm.nst, 40 + i, vol).astype(np.complex64))
+           for i, b in enumerate(bases)]
+
+    def run():
+        outs = [DeviceArray(ctx, d.shape, np.complex64) for d in ins]
+        gh.apply_batch(bases, ins, outs)
+        return [o.numpy() for o in outs]
+
+    ref = run()
+    cache = tmp_path / "tune.txt"
+    shape, ms = gh.autotune_batch(bases, [sysm.nst] * len(bases), np.complex64, cache)
+    assert shape[0] in (1, 2, 4) and shape[1] >= 1 and ms > 0
+    lines = cache.read_text().split()
+    assert len(lines) == 4 and int(lines[1]) == shape[0] and int(lines[2]) == shape[1]
+    shape2, ms2 = gh.autotune_batch(bases, [sysm.nst] * len(bases), np.complex64, cache)
+    assert shape2 == shape and ms2 == pytest.approx(ms, rel=1e-5)
+    assert len(cache.read_text().split()) == 4  # read from the cache, nothing appended
+    got = run()
+    assert all(np.array_equal(a, b) for a, b in zip(got, ref))
+    # every plane batch gives the same bits
+    for pb in (1, 2, 3, 0):
+        gh.set_plane_batch(np.complex64, pb)
+        assert all(np.array_equal(a, b) for a, b in zip(run(), ref)), pb

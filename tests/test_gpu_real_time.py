@@ -164,3 +164,28 @@ def test_c4_si64_block_two_steps(ctx, dtype, tol, etol):
     assert err < etol, (recs[:, :7], want)
     assert rel_err(d8.numpy(), R["states"]) < tol
     p8.close()
+
+
+def test_fp64_state_accumulation_reduces_fp32_drift(ctx):
+    """FP32 H applications accumulating into complex128 states
+    (inqx_prop_set_state_precision): over 40 graph steps the states stay closer
+    to the FP64 propagation than with complex64 states, and match the FP64
+    propagation to the FP32 application error (2e-6)"""
+    sysm, gh, oh, b, c = _setup(ctx, seed=6)
+    say("case: FP64 state accumulation, 40 steps")
+    occ = np.full(sysm.nst, 2.0)
+    runs = {}
+    for name, dtype, wide in (("fp64", np.complex128, False), ("fp32", np.complex64, False),
+                              ("wide", np.complex64, True)):
+        d = DeviceArray.from_numpy(ctx, c.astype(dtype))
+        prop = rt.Propagator(gh, [b], [d], [occ], dt=0.05)
+        if wide:
+            prop.set_state_precision(np.complex128)
+        prop.step(40, graph=True)
+        ctx.sync()
+        runs[name] = d.numpy().astype(np.complex128)
+        prop.close()
+    e32 = rel_err(runs["fp32"], runs["fp64"])
+    ew = rel_err(runs["wide"], runs["fp64"])
+    assert ew < e32, (ew, e32)
+    assert ew < 2e-6, ew
