@@ -205,16 +205,23 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag_eig(const double *dd, c
   const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax2);
   const double tnorm = s_tnorm;
   const double eps = 2.220446049250313e-16;
-  // bisection: lv[i] = i-th smallest eigenvalue
+  // bisection: lv[i] = i-th smallest eigenvalue.  count(lo) <= i < count(hi)
+  // keeps lambda_i in [lo, hi); the bracket shrinks to adjacent doubles (or
+  // below pivmin), and lo is then the largest double not above lambda_i
+  // (a 1 x 1 or diagonal T gives its diagonal exactly)
   for (int i = tid; i < n; i += nt) {
     double lo = s_gl - 2.0 * eps * tnorm - pivmin, hi = s_gu + 2.0 * eps * tnorm + pivmin;
+    bool done = false;
     for (int it = 0; it < 200; ++it) {
       const double mid = 0.5 * (lo + hi);
-      if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin || mid == lo || mid == hi) break;
+      if (mid <= lo || mid >= hi || hi - lo <= pivmin) {
+        done = true;
+        break;
+      }
       if (sturm_count(d, er, n, mid, pivmin) > i) hi = mid;
       else lo = mid;
     }
-    lv[i] = 0.5 * (lo + hi);
+    lv[i] = done ? lo : 0.5 * (lo + hi);
     lam[i] = lv[i];
   }
   __syncthreads();

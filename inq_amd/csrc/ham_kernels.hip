@@ -1165,6 +1165,7 @@ struct LaunchData {
   int total_line = 0, ntile_line = 0;  // line passes of the transform modes
   unsigned long long last_use = 0;     // lookup counter at the most recent use (eviction order)
   int plane_batch = 1;                 // planes (states) per plane-pass work-group
+  bool observables = false;            // the blocks point at the bases' projection and dipole buffers
 };
 static std::map<std::pair<inqx_ham_s *, LaunchKey>, std::shared_ptr<LaunchData>> g_launch_cache;
 static unsigned long long g_launch_clock = 0;
@@ -1195,6 +1196,15 @@ void set_capture_keep(std::vector<std::shared_ptr<void>> *keep) { g_capture_keep
 void invalidate_launch_cache(inqx_ham_s *h) {
   for (auto it = g_launch_cache.begin(); it != g_launch_cache.end();) {
     if (it->first.first == h) it = g_launch_cache.erase(it);
+    else ++it;
+  }
+}
+// only the descriptors that point at the projection and dipole buffers: the
+// plain applications of the same step stay cached (a graph captured after an
+// eager step whose energy record grew those buffers still finds them)
+static void invalidate_observable_launches(inqx_ham_s *h) {
+  for (auto it = g_launch_cache.begin(); it != g_launch_cache.end();) {
+    if (it->first.first == h && it->second->observables) it = g_launch_cache.erase(it);
     else ++it;
   }
 }
@@ -1440,7 +1450,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
         INQX_CHECK(B.pinned == 0, INQX_ERR_STATE, "basis workspaces are pinned by a captured propagation graph");
         B.proj[prec].ensure(need_p);
         B.dip.ensure(need_d);
-        invalidate_launch_cache(h);
+        invalidate_observable_launches(h);
       }
     }
   }
@@ -1476,6 +1486,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     const int cs = (int)sizeof(cplx<T>);
     int nw = std::max(1, std::min(4, h->xwaves[prec]));  // states (waves) per x-pass work-group
     ld->mode = mode;
+    ld->observables = opts->observables;
     ld->ntile_line = (g.n[1] * g.n[2] + 31) / 32;
     if (mode == 0) {
       ld->lds_per_wave = h->kb.maxpts + kMaxNlm * h->kb.maxslot;
