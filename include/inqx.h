@@ -135,7 +135,7 @@ int inqx_ham_prepare(inqx_ham ham, inqx_basis basis, int nst, int precision);
 
 /* launch-shape autotuner: times the H application of the given blocks (one
  * launch sequence, as inqx_ham_apply_batch runs them) for every launch shape
- * (x-pass states per work-group 1, 2 or 4, times plane-pass planes per
+ * (x-pass states per work-group 1, 2, 4 or 8, times plane-pass planes per
  * work-group from the largest the grid's LDS budget allows down to 1) and
  * keeps the fastest for this Hamiltonian and precision.  cache_path: a text
  * file of earlier results, one line per (device, precision, grid, KB size,
@@ -149,7 +149,7 @@ int inqx_ham_autotune_batch(inqx_ham ham, int nblocks, const inqx_basis *bases, 
 int inqx_ham_set_plane_batch(inqx_ham ham, int precision, int planes);
 /* the single-block form (no cache); best_xwaves / best_ms may be NULL */
 int inqx_ham_autotune(inqx_ham ham, inqx_basis basis, int nst, int precision, int *best_xwaves, double *best_ms);
-/* sets the x-pass shape directly (1, 2 or 4 states per work-group) */
+/* sets the x-pass shape directly (1, 2, 4 or 8 states per work-group) */
 int inqx_ham_set_launch_shape(inqx_ham ham, int precision, int xwaves);
 
 /* ------------------------------------------------------------ H |psi> */

@@ -578,7 +578,8 @@ int inqx_ham_set_launch_shape(inqx_ham h, int precision, int xwaves) {
   API_BEGIN
   INQX_CHECK(h, INQX_ERR_ARG, "null Hamiltonian");
   INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
-  INQX_CHECK(xwaves == 1 || xwaves == 2 || xwaves == 4, INQX_ERR_ARG, "x-pass shape must be 1, 2 or 4");
+  INQX_CHECK(xwaves == 1 || xwaves == 2 || xwaves == 4 || xwaves == 8, INQX_ERR_ARG,
+             "x-pass shape must be 1, 2, 4 or 8");
   INQX_CHECK(h->pinned == 0, INQX_ERR_STATE, "Hamiltonian pinned by a captured propagation graph");
   if (h->xwaves[precision] != xwaves) {
     h->xwaves[precision] = xwaves;
@@ -603,6 +604,7 @@ static std::string tune_key(inqx_ham h, int nblocks, Basis *const *bb, const int
   for (const char *c = prop.gcnArchName; *c; ++c) mix((unsigned char)*c);
   mix((unsigned long long)prop.multiProcessorCount);
   mix((unsigned long long)precision);
+  mix(2ull);  // kernel generation: x passes with LDS-staged tables
   for (int d = 0; d < 3; ++d) mix((unsigned long long)h->g.n[d]);
   mix((unsigned long long)h->kb.natoms);
   mix((unsigned long long)h->kb.nent);
@@ -683,7 +685,7 @@ int inqx_ham_autotune_batch(inqx_ham h, int nblocks, const inqx_basis *bases, co
   const std::string key = tune_key(h, nblocks, bb.data(), nst, precision);
   int xw = 0, pb = 0;
   double t = 0.0;
-  if (tune_cache_read(cache_path, key, xw, pb, t) && (xw == 1 || xw == 2 || xw == 4) && pb >= 1) {
+  if (tune_cache_read(cache_path, key, xw, pb, t) && (xw == 1 || xw == 2 || xw == 4 || xw == 8) && pb >= 1) {
     set_shape(h, precision, xw, pb);
     if (shape) {
       shape[0] = xw;
@@ -714,7 +716,7 @@ int inqx_ham_autotune_batch(inqx_ham h, int nblocks, const inqx_basis *bases, co
   int best_x = h->xwaves[precision], best_p = h->plane_batch[precision];
   double best_t = 1e300;
   try {
-    for (int w : {1, 2, 4})
+    for (int w : {1, 2, 4, 8})
       for (int p = pmax; p >= 1; --p) {
         set_shape(h, precision, w, p);
         auto run = [&]() {

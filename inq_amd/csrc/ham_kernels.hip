@@ -68,10 +68,15 @@ template <typename S> __device__ __forceinline__ V2<S> ld2(const void *p, size_t
 
 void invalidate_launch_cache(inqx_ham_s *h);
 
+// x-pass work-groups per group of states: the tiles padded to a multiple of 8
+// (work-groups are dealt to the 8 XCDs round robin)
+static inline int x_tiles_padded(int ntile) { return (ntile + 7) / 8 * 8; }
+
 struct HamDev {
   int n0, n1, n2, n12;
   long N;
   int natoms, has_kb, ntile, maxpts, maxslot;
+  int ntg;  // x-pass work-groups per group of states: the tiles padded to a multiple of 8
   const uint32_t *line_mask;
   const int *tile_info;
   const int *slot_atom;
@@ -531,97 +536,157 @@ template <int N0> __device__ __forceinline__ int mask_count_low(const uint32_t (
   return c;
 }
 
+// the lane's half of a line's point mask as bits 0..N0/2-1 (bit k: i0 = k +
+// half N0/2 holds a sphere point), and the number of points of the line
+// before that half; m = the line's 96-bit mask, base = points of the tile's
+// earlier lines
+template <int N0>
+__device__ __forceinline__ unsigned long long half_mask(const uint32_t (&m)[3], bool upper, int base, int &cnt0) {
+  constexpr int H = N0 / 2;
+  static_assert(H <= 48, "lines up to 96 points");
+  const unsigned long long lo = (unsigned long long)m[0] | ((unsigned long long)m[1] << 32);  // bits 0..63
+  const unsigned long long keep = (1ull << H) - 1ull;
+  const unsigned long long ml = lo & keep;
+  const unsigned long long mu = ((lo >> H) | ((unsigned long long)m[2] << (64 - H))) & keep;
+  cnt0 = base + (upper ? __popcll(ml) : 0);
+  return upper ? mu : ml;
+}
+
 __device__ __forceinline__ void lds_add(float *p, float v) { atomicAdd(p, v); }
 __device__ __forceinline__ void lds_add(double *p, double v) { atomicAdd(p, v); }
 
-// Inverse FFT along i0 of the 32 lines of a tile (one wave per (tile,
-// state), a lane pair per line), then the KB projections P[slot][l] =
-// sum_entries beta_l exp(+ik.pos) psi.  Each lane walks a contiguous chunk of
-// the tile's entries (sorted by slot), accumulates in registers and flushes to
-// the wave's LDS slot accumulators only at slot changes.  Every LDS buffer is
-// private to one wave.
+// one 16-byte record per lane from global memory straight into LDS (no
+// registers): lane l writes dst + 16 l, dst uniform over the wave; the wait
+// for it is the vmcnt wait in front of the next barrier
+__device__ __forceinline__ void dma16(const void *src, void *dst) {
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+}
+// nbytes (a multiple of 16) of contiguous global memory into LDS, shared out
+// over the ns waves of the work-group
+__device__ __forceinline__ void dma_copy(const void *src, void *dst, int nbytes, int wave, int ns, int lane) {
+  const int nch = nbytes >> 4;
+  for (int i = wave; i * 64 < nch; i += ns) {
+    const int c = i * 64 + lane;
+    if (c < nch) dma16(static_cast<const char *>(src) + (size_t)c * 16, static_cast<char *>(dst) + i * 1024);
+  }
+}
+
+// LDS layout of the x passes (byte offsets, computed on the host): the
+// work-group's shared part (V tile of the apply pass, then the tile's KB
+// records: point/slot ints, projector values, phases), then one region per
+// wave (state): compact sphere points, then the slot values (projection
+// accumulators of the project pass, D P of the apply pass)
+struct XLds {
+  int v, cs, bt, ph, wave0, wave_bytes, aux;
+};
+
+// Inverse FFT along i0 of the 32 lines of a tile, then the KB projections
+// P[slot][l] = sum_entries beta_l exp(+ik.pos) psi.  A work-group takes ns
+// states (one per wave) of one tile and loads the tile's entry records into
+// LDS once, before the line loads; each lane walks a contiguous run of the
+// tile's entries (sorted by slot) from LDS, accumulates in registers and
+// flushes to its wave's slot accumulators at slot changes.  Every
+// accumulator belongs to one wave, so the sums do not depend on timing.
 template <typename T, int N0>
-__global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                       int total, HamDev hd, int lds_per_wave) {
+__global__ void __launch_bounds__(512) k_xpass_project(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
+                                                       int total, HamDev hd, XLds xl) {
   using C = cplx<T>;
   constexpr int H = N0 / 2;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
   const int ns = blockDim.x >> 6, ptid = lane;
   const bool upper = lane >= 32;
-  C *pts = reinterpret_cast<C *>(smem_raw) + wave * lds_per_wave;
-  C *acc = pts + hd.maxpts;
+  const int2 *scs = reinterpret_cast<const int2 *>(smem_raw + xl.cs);
+  const V4<T> *sbt = reinterpret_cast<const V4<T> *>(smem_raw + xl.bt);
+  const C *sph = reinterpret_cast<const C *>(smem_raw + xl.ph);
+  C *pts = reinterpret_cast<C *>(smem_raw + xl.wave0 + wave * xl.wave_bytes);
+  C *acc = reinterpret_cast<C *>(smem_raw + xl.wave0 + wave * xl.wave_bytes + xl.aux);
   {  // one work item per work-group: the grid is the work count
     const int wg = blockIdx.x;
     if (wg >= total) return;
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
+    // work order: consecutive work-groups take consecutive tiles of the same
+    // states; the tiles of one state group are padded to hd.ntg, a multiple of
+    // 8, so a tile's work-groups always go to the same XCD
     const int local = wg - gp(prefix)[b];
-    const int ngrp = (B.nst + ns - 1) / ns;
-    const int tile = local / ngrp;
-    const int s = (local - tile * ngrp) * ns + wave;
+    const int sgrp = local / hd.ntg;
+    const int tile = local - sgrp * hd.ntg;
+    if (tile >= hd.ntile) return;  // uniform: a padding work-group
+    const int s = sgrp * ns + wave;
     const int L = tile * kTile + (lane & 31);
     const V4<int> ti = ld4<int>(hd.tile_info, 2 * tile);
     const V4<int> tj = ld4<int>(hd.tile_info, 2 * tile + 1);
     const int npts = ti.x, nsl = ti.y, slot_base = ti.z, ent_base = ti.w, nent = tj.x;
-    const bool active = (s < B.nst) && npts > 0;
+    if (npts == 0) return;  // uniform: no sphere point in the tile
+    const int chunk = (nent + 63) >> 6;
+    dma_copy(hd.ent_cs + 2 * (size_t)ent_base, smem_raw + xl.cs, 64 * chunk * 8, wave, ns, lane);
+    dma_copy(static_cast<const char *>(hd.ent_b) + (size_t)ent_base * 4 * sizeof(T), smem_raw + xl.bt,
+             64 * chunk * 4 * (int)sizeof(T), wave, ns, lane);
+    if (!B.gamma) dma_copy(B.kbph + ent_base, smem_raw + xl.ph, 64 * chunk * (int)sizeof(C), wave, ns, lane);
+    const bool active = s < B.nst;
+    const bool valid = L < hd.n12;
+    C v[H];
+    V4<uint32_t> w{0u, 0u, 0u, 0u};
     if (active) {
-      const bool valid = L < hd.n12;
-      C v[H];
-      // line loads, then the line's point mask, then the transform: both loads in flight together
+      // line loads and the line's point mask in flight together
       load_line_slots_split<T, N0>(B, hd.n12, s, valid ? L : hd.n12 - 1, upper, v);
-      const V4<uint32_t> w = ld4<uint32_t>(hd.line_mask, valid ? L : hd.n12 - 1);
+      w = ld4<uint32_t>(hd.line_mask, valid ? L : hd.n12 - 1);
       for (int i = ptid; i < nsl * kMaxNlm; i += 64) acc[i] = C{0, 0};
       fft_split<N0, +1, C>::dit(v, upper);
-      if (valid) {
-        const uint32_t m[3] = {w.x, w.y, w.z};
-        int cnt = (int)w.w + (upper ? mask_count_low<N0>(m) : 0);
+      // every value goes out: the line's sphere points to their compact
+      // slots, the others (and lanes past the last line) to the trash slot
+      // pts[npts + 1], so the stores need no branches
+      const uint32_t m[3] = {w.x, w.y, w.z};
+      int cnt;
+      const unsigned long long mh = valid ? half_mask<N0>(m, upper, (int)w.w, cnt) : 0ull;
+      if (!valid) cnt = 0;
+      const int trash = npts + 1;
 #pragma unroll
-        for (int k = 0; k < H; ++k) {
-          if (mask_bit<N0>(m, k, upper)) {
-            pts[cnt] = v[k];
-            ++cnt;
-          }
-        }
+      for (int k = 0; k < H; ++k) {
+        const int bit = (int)((mh >> k) & 1ull);
+        pts[bit ? cnt : trash] = v[k];
+        cnt += bit;
       }
     }
-    __syncthreads();
+    __syncthreads();  // entry records in LDS, sphere points staged
     if (active) {
-      const int chunk = (nent + 63) >> 6;
-      int e = ent_base + ptid * chunk;
-      const int eend = min(e + chunk, ent_base + nent);
+      // the lane's run of the tile's entries: entry j of the run at 64 j + lane
+      const int cnt = max(0, min(chunk, nent - ptid * chunk));
       int cur = -1;
       C a0{0, 0}, a1{0, 0}, a2{0, 0}, a3{0, 0};
-      for (; e < eend; e += 4) {
-        // four consecutive entries: loads first, then the in-order accumulation
-        V2<int> cs[4];
-        V4<T> bt[4];
-        V2<T> ph[4];
+      for (int j = 0; j < cnt; j += 2) {
+        // two entries of the run: all their LDS reads before the accumulation
+        int2 cs[2];
+        V4<T> bt[2];
+        C u[2];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int eu = min(e + u, eend - 1);
-          cs[u] = ld2<int>(hd.ent_cs, eu);
-          bt[u] = ld4<T>(hd.ent_b, eu);
-          if (!B.gamma) ph[u] = ld2<T>(B.kbph, eu);
+        for (int q = 0; q < 2; ++q) cs[q] = scs[64 * min(j + q, cnt - 1) + ptid];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int e = 64 * min(j + q, cnt - 1) + ptid;
+          bt[q] = sbt[e];
+          u[q] = pts[cs[q].x];
+          if (!B.gamma) u[q] = cmul(u[q], sph[e]);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (e + u < eend) {
-            if (cs[u].y != cur) {
+        for (int q = 0; q < 2; ++q) {
+          if (j + q < cnt) {
+            if (cs[q].y != cur) {
               if (cur >= 0) {
-                T *q = reinterpret_cast<T *>(acc + cur * kMaxNlm);
-                lds_add(q + 0, a0.x); lds_add(q + 1, a0.y); lds_add(q + 2, a1.x); lds_add(q + 3, a1.y);
-                lds_add(q + 4, a2.x); lds_add(q + 5, a2.y); lds_add(q + 6, a3.x); lds_add(q + 7, a3.y);
+                T *r = reinterpret_cast<T *>(acc + cur * kMaxNlm);
+                lds_add(r + 0, a0.x); lds_add(r + 1, a0.y); lds_add(r + 2, a1.x); lds_add(r + 3, a1.y);
+                lds_add(r + 4, a2.x); lds_add(r + 5, a2.y); lds_add(r + 6, a3.x); lds_add(r + 7, a3.y);
               }
-              cur = cs[u].y;
+              cur = cs[q].y;
               a0 = a1 = a2 = a3 = C{0, 0};
             }
-            C v = pts[cs[u].x];
-            if (!B.gamma) v = cmul(v, C{ph[u].x, ph[u].y});
-            a0.x += bt[u].x * v.x; a0.y += bt[u].x * v.y;
-            a1.x += bt[u].y * v.x; a1.y += bt[u].y * v.y;
-            a2.x += bt[u].z * v.x; a2.y += bt[u].z * v.y;
-            a3.x += bt[u].w * v.x; a3.y += bt[u].w * v.y;
+            const V4<T> b = bt[q];
+            const C x = u[q];
+            a0.x += b.x * x.x; a0.y += b.x * x.y;
+            a1.x += b.y * x.x; a1.y += b.y * x.y;
+            a2.x += b.z * x.x; a2.y += b.z * x.y;
+            a3.x += b.w * x.x; a3.y += b.w * x.y;
           }
         }
       }
@@ -637,10 +702,12 @@ __global__ void __launch_bounds__(256) k_xpass_project(const SphereBlock<T> *blo
       for (int i = ptid; i < nsl * kMaxNlm; i += 64)
         pt[((size_t)(slot_base + i / kMaxNlm) * kMaxNlm + (i % kMaxNlm)) * B.nst + s] = acc[i];
     }
-    __syncthreads();
   }
 }
 
+// FP64 sum over an atom's slots, x dV, x D.  Writes D P per (atom, l, state)
+// for the energy record and per (slot, state, l) -- every slot of the atom --
+// so that the apply pass loads a tile's values as contiguous records.
 template <typename T>
 __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                    int total, HamDev hd) {
@@ -675,6 +742,7 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
       for (int l = 0; l < kMaxNlm; ++l)
         pj[((size_t)a * kMaxNlm + l) * B.nst + s] = C{(T)(pr[l] * hd.kb_scale), (T)(pi[l] * hd.kb_scale)};
     }
+    C r[kMaxNlm];
 #pragma unroll
     for (int l = 0; l < kMaxNlm; ++l) {
       double dr = 0.0, di = 0.0;
@@ -683,37 +751,55 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
         dr += D[l * kMaxNlm + m] * pr[m];
         di += D[l * kMaxNlm + m] * pi[m];
       }
-      dp[((size_t)a * kMaxNlm + l) * B.nst + s] = C{(T)(dr * hd.kb_scale), (T)(di * hd.kb_scale)};
+      r[l] = C{(T)(dr * hd.kb_scale), (T)(di * hd.kb_scale)};
+      dp[((size_t)a * kMaxNlm + l) * B.nst + s] = r[l];
+    }
+    GAS C *dps = gw(B.dps);
+    for (int q = q0; q < q1; ++q) {
+      GAS C *o = dps + ((size_t)gp(hd.atom_slot_list)[q] * B.nst + s) * kMaxNlm;
+#pragma unroll
+      for (int l = 0; l < kMaxNlm; ++l) o[l] = r[l];
     }
   }
 }
+
 // Inverse FFT along i0, times V/N, plus the KB term exp(-ik.r) beta D P at
 // the sphere points, forward FFT along i0, stored back into the sphere's
-// plane slots; optionally the dipole partial sums of the tile.  One wave per
-// (tile, state), a lane pair per line; LDS buffers private to the wave.
+// plane slots; optionally the dipole partial sums of the tile.  A work-group
+// takes ns states (one per wave) of one tile.  Before the line loads, the
+// work-group loads into LDS everything else it will read: the tile's V/N
+// (shared), its KB entry records (shared) and each state's D P of the tile's
+// slots, so no memory round trip sits between the two line transforms.
 template <typename T, int N0>
-__global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                     int total, HamDev hd, int lds_per_wave) {
+__global__ void __launch_bounds__(512) k_xpass_apply(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
+                                                     int total, HamDev hd, XLds xl) {
   using C = cplx<T>;
   constexpr int H = N0 / 2;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
   const int ns = blockDim.x >> 6, ptid = lane;
   const bool upper = lane >= 32;
-  C *pts = reinterpret_cast<C *>(smem_raw) + wave * lds_per_wave;
-  C *dpl = pts + hd.maxpts;
+  const T *sv = reinterpret_cast<const T *>(smem_raw + xl.v);  // [N0][32] V/N of the tile's lines
+  const int2 *scs = reinterpret_cast<const int2 *>(smem_raw + xl.cs);
+  const V4<T> *sbt = reinterpret_cast<const V4<T> *>(smem_raw + xl.bt);
+  const C *sph = reinterpret_cast<const C *>(smem_raw + xl.ph);
+  C *pts = reinterpret_cast<C *>(smem_raw + xl.wave0 + wave * xl.wave_bytes);
+  C *dpl = reinterpret_cast<C *>(smem_raw + xl.wave0 + wave * xl.wave_bytes + xl.aux);
   {  // one work item per work-group: the grid is the work count
     const int wg = blockIdx.x;
     if (wg >= total) return;
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
+    // work order as in k_xpass_project
     const int local = wg - gp(prefix)[b];
-    const int ngrp = (B.nst + ns - 1) / ns;
-    const int tile = local / ngrp;
-    const int s = (local - tile * ngrp) * ns + wave;
+    const int sgrp = local / hd.ntg;
+    const int tile = local - sgrp * hd.ntg;
+    if (tile >= hd.ntile) return;  // uniform: a padding work-group
+    const int s = sgrp * ns + wave;
     const int L = tile * kTile + (lane & 31);
     const bool valid = L < hd.n12;
     const int Lc = valid ? L : hd.n12 - 1;
+    const bool act = s < B.nst;
     int npts = 0, nsl = 0, slot_base = 0, ent_base = 0, nent = 0;
     if (hd.has_kb) {
       const V4<int> ti = ld4<int>(hd.tile_info, 2 * tile);
@@ -724,16 +810,50 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
       ent_base = ti.w;
       nent = tj.x;
     }
-    const bool act = s < B.nst;
+    const int ncap = 64 * ((nent + 63) >> 6);  // the tile's records including padding
+    {
+      // V/N rows of the tile: N0 rows of 32 lines; a 16-byte record past the
+      // last line (partial last tile) loads the row's first record instead
+      constexpr int RPR = 32 * (int)sizeof(T) / 16;  // records per row
+      constexpr int LPR = 16 / (int)sizeof(T);       // lines per record
+      const char *vg = reinterpret_cast<const char *>(B.vloc);
+      for (int i = wave; i * 64 < N0 * RPR; i += ns) {
+        const int c = i * 64 + lane;
+        if (c < N0 * RPR) {
+          const int r = c / RPR, cc = c - r * RPR;
+          const int line = tile * kTile + cc * LPR;
+          const int l0 = line < hd.n12 ? line : tile * kTile;
+          dma16(vg + ((size_t)r * hd.n12 + l0) * sizeof(T), smem_raw + xl.v + i * 1024);
+        }
+      }
+    }
+    if (npts > 0) {
+      dma_copy(hd.ent_cs + 2 * (size_t)ent_base, smem_raw + xl.cs, ncap * 8, wave, ns, lane);
+      dma_copy(static_cast<const char *>(hd.ent_b) + (size_t)ent_base * 4 * sizeof(T), smem_raw + xl.bt,
+               ncap * 4 * (int)sizeof(T), wave, ns, lane);
+      if (!B.gamma) dma_copy(B.kbph + ent_base, smem_raw + xl.ph, ncap * (int)sizeof(C), wave, ns, lane);
+      if (act) {
+        // this state's D P of the tile's slots: kMaxNlm values per slot
+        constexpr int RPS = kMaxNlm * (int)sizeof(C) / 16;  // records per slot
+        const char *dg = reinterpret_cast<const char *>(B.dps);
+        for (int i = 0; i * 64 < nsl * RPS; ++i) {
+          const int c = i * 64 + lane;
+          if (c < nsl * RPS) {
+            const int q = c / RPS, cc = c - q * RPS;
+            dma16(dg + ((size_t)(slot_base + q) * B.nst + s) * kMaxNlm * sizeof(C) + cc * 16,
+                  reinterpret_cast<char *>(dpl) + i * 1024);
+          }
+        }
+      }
+    }
     C v[H];
+    V4<uint32_t> w{0u, 0u, 0u, 0u};
     if (act) {
-      // the line's loads go out first; the projection staging below overlaps them
       load_line_slots_split<T, N0>(B, hd.n12, s, Lc, upper, v);
       if (npts > 0) {
-        const GAS C *dp = gp(B.dp);
-        for (int i = ptid; i < nsl * kMaxNlm; i += 64)
-          dpl[i] = dp[((size_t)gp(hd.slot_atom)[slot_base + i / kMaxNlm] * kMaxNlm + (i % kMaxNlm)) * B.nst + s];
-        for (int i = ptid; i < npts; i += 64) pts[i] = C{0, 0};
+        w = ld4<uint32_t>(hd.line_mask, Lc);
+        // the compact points, the zero slot pts[npts] and the trash slot pts[npts + 1]
+        for (int i = ptid; i < npts + 2; i += 64) pts[i] = C{0, 0};
       }
       fft_split<N0, +1, C>::dit(v, upper);
       if (B.dip) {
@@ -742,10 +862,10 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
         if (valid) {
 #pragma unroll
           for (int k = 0; k < H; ++k) {
-            const double w = (double)v[k].x * (double)v[k].x + (double)v[k].y * (double)v[k].y;
+            const double x2 = (double)v[k].x * (double)v[k].x + (double)v[k].y * (double)v[k].y;
             const double f0 = upper ? (double)sym_of<N0>(k + H) / (double)N0 : (double)sym_of<N0>(k) / (double)N0;
-            s0 += w;
-            s1 += w * f0;
+            s0 += x2;
+            s1 += x2 * f0;
           }
         }
         const int i1 = Lc / hd.n2, i2 = Lc - i1 * hd.n2;
@@ -765,63 +885,73 @@ __global__ void __launch_bounds__(256) k_xpass_apply(const SphereBlock<T> *block
           for (int c = 0; c < 4; ++c) dq[c] = d[c];
         }
       }
+    }
+    __syncthreads();  // V tile, entry records and D P in LDS
+    if (act) {
       // V/N at i0 = k + half N0/2
-      const unsigned ps = (unsigned)(hd.n12 * sizeof(T));
-      const auto rv = make_rsrc(B.vloc, (unsigned)N0 * ps);
-      const unsigned vo = (unsigned)(Lc * sizeof(T)) + (upper ? (unsigned)H * ps : 0u);
+      const T *vrow = sv + (upper ? H * kTile : 0) + (lane & 31);
 #pragma unroll
       for (int k = 0; k < H; ++k) {
-        T vv;
-        bload(rv, vo, (unsigned)k * ps, vv);
+        const T vv = vrow[k * kTile];
         v[k].x *= vv;
         v[k].y *= vv;
       }
-    }
-    __syncthreads();
-    if (act && npts > 0) {
-      // two entries per lane and step: both table loads are in flight together
-      const int eend = ent_base + nent;
-      for (int e = ent_base + ptid; e < eend; e += 128) {
-        const int e1 = e + 64 < eend ? e + 64 : e;  // clamped: the second add is skipped below
-        const V2<int> cs0 = ld2<int>(hd.ent_cs, e), cs1 = ld2<int>(hd.ent_cs, e1);
-        const V4<T> bt0 = ld4<T>(hd.ent_b, e), bt1 = ld4<T>(hd.ent_b, e1);
+      if (npts > 0) {
+        // the tile's records in LDS order, two rows per step with all reads
+        // first; padding records (point -1, zero projector values) add into the
+        // trash slot
+        const int trash = npts + 1;
+        for (int e0 = 0; e0 < ncap; e0 += 128) {
+          const int nr = min(2, (ncap - e0) >> 6);  // uniform
+          int2 cs[2];
+          V4<T> bt[2];
+          C dd[2][kMaxNlm];
+          C ph[2];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          if (u == 1 && e + 64 >= eend) break;
-          const V2<int> cs = u ? cs1 : cs0;
-          const V4<T> bt = u ? bt1 : bt0;
-          const C *dd = dpl + cs.y * kMaxNlm;
-          C t;
-          t.x = bt.x * dd[0].x + bt.y * dd[1].x + bt.z * dd[2].x + bt.w * dd[3].x;
-          t.y = bt.x * dd[0].y + bt.y * dd[1].y + bt.z * dd[2].y + bt.w * dd[3].y;
-          if (!B.gamma) {
-            const V2<T> ph = ld2<T>(B.kbph, u ? e1 : e);
-            t = cmul(t, C{ph.x, -ph.y});  // exp(-i (k+A).r)
+          for (int q = 0; q < 2; ++q) cs[q] = scs[e0 + 64 * min(q, nr - 1) + ptid];
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const int e = e0 + 64 * min(q, nr - 1) + ptid;
+            bt[q] = sbt[e];
+            const C *d = dpl + max(cs[q].y, 0) * kMaxNlm;
+#pragma unroll
+            for (int l = 0; l < kMaxNlm; ++l) dd[q][l] = d[l];
+            if (!B.gamma) ph[q] = sph[e];
           }
-          T *q = reinterpret_cast<T *>(pts + cs.x);
-          lds_add(q + 0, t.x);
-          lds_add(q + 1, t.y);
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            if (q < nr) {
+              C t;
+              t.x = bt[q].x * dd[q][0].x + bt[q].y * dd[q][1].x + bt[q].z * dd[q][2].x + bt[q].w * dd[q][3].x;
+              t.y = bt[q].x * dd[q][0].y + bt[q].y * dd[q][1].y + bt[q].z * dd[q][2].y + bt[q].w * dd[q][3].y;
+              if (!B.gamma) t = cmul(t, C{ph[q].x, -ph[q].y});  // exp(-i (k+A).r)
+              T *r = reinterpret_cast<T *>(pts + (cs[q].x < 0 ? trash : cs[q].x));
+              lds_add(r + 0, t.x);
+              lds_add(r + 1, t.y);
+            }
+          }
         }
       }
     }
     __syncthreads();
     if (act) {
-      if (npts > 0 && valid) {
-        const V4<uint32_t> w = ld4<uint32_t>(hd.line_mask, L);
+      if (npts > 0) {
+        // every value reads: its compact point, or the zero slot pts[npts]
+        // (no branches, so the reads are in flight together)
         const uint32_t m[3] = {w.x, w.y, w.z};
-        int cnt = (int)w.w + (upper ? mask_count_low<N0>(m) : 0);
+        int cnt;
+        const unsigned long long mh = valid ? half_mask<N0>(m, upper, (int)w.w, cnt) : 0ull;
+        if (!valid) cnt = 0;
 #pragma unroll
         for (int k = 0; k < H; ++k) {
-          if (mask_bit<N0>(m, k, upper)) {
-            v[k] = cadd(v[k], pts[cnt]);
-            ++cnt;
-          }
+          const int bit = (int)((mh >> k) & 1ull);
+          v[k] = cadd(v[k], pts[bit ? cnt : npts]);
+          cnt += bit;
         }
       }
       fft_split<N0, -1, C>::dif(v, upper);
       store_line_slots_split<T, N0>(B, hd.n12, s, L, upper, valid, v);
     }
-    __syncthreads();
   }
 }
 
@@ -969,6 +1099,7 @@ void build_kb_tables(inqx_ham_s *h) {
   const int n12 = g.n[1] * g.n[2];
   kb.natoms = (int)h->projs.size();
   kb.maxsph = 1;
+  kb.maxchunk = 0;
   for (auto &p : h->projs) kb.maxsph = std::max<int>(kb.maxsph, (int)p.npts());
   kb.ntile = (n12 + kTile - 1) / kTile;
   struct Mem {
@@ -1043,7 +1174,42 @@ void build_kb_tables(inqx_ham_s *h) {
     epos.push_back(0.0f);
   }
   kb.nslot = (int)slot_atom.size();
-  kb.nent = (long)mem.size();
+  // lane-interleaved order per tile: the x passes give lane l the contiguous
+  // run [l c, l c + c) of a tile's nent entries (c = ceil(nent / 64)), and
+  // entry (l, j) of that run is stored at tile base + j * 64 + l, so the 64
+  // lanes' j-th loads are consecutive records; runs are padded to c with
+  // empty entries (point and slot -1, zero projector values)
+  {
+    std::vector<int> cs2;
+    std::vector<float> b32, pos2;
+    std::vector<double> b64;
+    long base = 0;
+    for (int t = 0; t < kb.ntile; ++t) {
+      const int e0 = tile_info[8 * t + 3], ne = tile_info[8 * t + 4];
+      const int c = (ne + 63) / 64;
+      for (int j = 0; j < c; ++j)
+        for (int l = 0; l < 64; ++l) {
+          const int k = l * c + j;
+          const bool ok = k < ne;
+          const size_t o = (size_t)e0 + (size_t)(ok ? k : 0);
+          cs2.push_back(ok ? ent_cs[2 * o] : -1);
+          cs2.push_back(ok ? ent_cs[2 * o + 1] : -1);
+          for (int q = 0; q < kMaxNlm; ++q) {
+            b32.push_back(ok ? eb32[kMaxNlm * o + q] : 0.0f);
+            b64.push_back(ok ? eb64[kMaxNlm * o + q] : 0.0);
+          }
+          for (int q = 0; q < 4; ++q) pos2.push_back(ok ? epos[4 * o + q] : 0.0f);
+        }
+      tile_info[8 * t + 3] = (int)base;
+      base += 64L * c;
+      kb.maxchunk = std::max(kb.maxchunk, c);
+    }
+    ent_cs.swap(cs2);
+    eb32.swap(b32);
+    eb64.swap(b64);
+    epos.swap(pos2);
+  }
+  kb.nent = (long)ent_cs.size() / 2;
   ++kb.gen;
   kb.pos_host = epos;
   std::vector<int> atom_slots(kb.natoms + 1, 0), atom_slot_list;
@@ -1144,6 +1310,7 @@ void ensure_workspace(inqx_ham_s *h, Basis &b, int nst, int prec) {
   b.t1[prec].ensure(cs * (size_t)std::max(1, b.nslot) * ns * n12);
   b.partial[prec].ensure(cs * (size_t)std::max(1, h->kb.nslot) * kMaxNlm * ns);
   b.dp[prec].ensure(cs * (size_t)std::max(1, h->kb.natoms) * kMaxNlm * ns);
+  b.dps[prec].ensure(cs * (size_t)std::max(1, h->kb.nslot) * kMaxNlm * ns);
   b.ws_nst[prec] = ns;
   b.ws_kb_nslot[prec] = h->kb.nslot;
   b.ws_natoms[prec] = h->kb.natoms;
@@ -1160,12 +1327,13 @@ struct LaunchData {
   int total_plane = 0, total_x = 0, total_red = 0;
   size_t off_pref[3] = {0, 0, 0};
   int nwave = 4;
-  int lds_per_wave = 0;
   int mode = 0;                      // ApplyOpts::mode
   int total_line = 0, ntile_line = 0;  // line passes of the transform modes
   unsigned long long last_use = 0;     // lookup counter at the most recent use (eviction order)
   int plane_batch = 1;                 // planes (states) per plane-pass work-group
   bool observables = false;            // the blocks point at the bases' projection and dipole buffers
+  XLds xl_apply{}, xl_project{};       // x-pass LDS layouts (byte offsets)
+  int xlds_apply = 0, xlds_project = 0;  // x-pass LDS bytes per work-group
 };
 static std::map<std::pair<inqx_ham_s *, LaunchKey>, std::shared_ptr<LaunchData>> g_launch_cache;
 static unsigned long long g_launch_clock = 0;
@@ -1361,14 +1529,13 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
     return;
   }
   const int xthreads = 64 * ld.nwave;  // one wave per (tile, state)
-  const int xlds = ld.nwave * ld.lds_per_wave * (int)sizeof(cplx<T>);
-  set_x_attr<T, N0>(h->ctx->c.device, xlds);
+  set_x_attr<T, N0>(h->ctx->c.device, std::max(ld.xlds_apply, ld.xlds_project));
   const int grid_x = ld.total_x;
   if (hd.has_kb) {
     {
       ProfScope ps(cx, f64 ? "xpass_project_f64" : "xpass_project_f32");
-      hipLaunchKernelGGL((k_xpass_project<T, N0>), dim3(grid_x), dim3(xthreads), xlds, st, blk, pref_x, nblocks,
-                         ld.total_x, hd, ld.lds_per_wave);
+      hipLaunchKernelGGL((k_xpass_project<T, N0>), dim3(grid_x), dim3(xthreads), ld.xlds_project, st, blk, pref_x,
+                         nblocks, ld.total_x, hd, ld.xl_project);
     }
     INQX_HIP(hipGetLastError());
     {
@@ -1380,8 +1547,8 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   }
   {
     ProfScope ps(cx, f64 ? "xpass_apply_f64" : "xpass_apply_f32");
-    hipLaunchKernelGGL((k_xpass_apply<T, N0>), dim3(grid_x), dim3(xthreads), xlds, st, blk, pref_x, nblocks,
-                       ld.total_x, hd, ld.lds_per_wave);
+    hipLaunchKernelGGL((k_xpass_apply<T, N0>), dim3(grid_x), dim3(xthreads), ld.xlds_apply, st, blk, pref_x, nblocks,
+                       ld.total_x, hd, ld.xl_apply);
   }
   INQX_HIP(hipGetLastError());
   {
@@ -1482,16 +1649,41 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     // the launch shape's plane batch (autotuned), at most what the LDS budget allows
     if (h->plane_batch[prec] > 0) plane_p = std::min(plane_p, h->plane_batch[prec]);
     ld->plane_batch = plane_p;
-    // waves (states) per x-pass workgroup within the LDS budget
+    // waves (states) per x-pass work-group within the LDS budget
     const int cs = (int)sizeof(cplx<T>);
-    int nw = std::max(1, std::min(4, h->xwaves[prec]));  // states (waves) per x-pass work-group
+    int nw = std::max(1, std::min(8, h->xwaves[prec]));  // states (waves) per x-pass work-group
     ld->mode = mode;
     ld->observables = opts->observables;
     ld->ntile_line = (g.n[1] * g.n[2] + 31) / 32;
     if (mode == 0) {
-      ld->lds_per_wave = h->kb.maxpts + kMaxNlm * h->kb.maxslot;
-      while (nw > 1 && nw * ld->lds_per_wave * cs > 160 * 1024) --nw;
-      INQX_CHECK(ld->lds_per_wave * cs <= 160 * 1024, INQX_ERR_UNSUPPORTED, "KB tile too dense for LDS");
+      bool need_ph = false;  // a block with k + A != 0 reads the entry phases
+      for (int i = 0; i < nblocks && has_kb; ++i) {
+        const double *kc = bases[i]->kcov;
+        if (kc[0] + h->A[0] != 0.0 || kc[1] + h->A[1] != 0.0 || kc[2] + h->A[2] != 0.0) need_ph = true;
+      }
+      auto a16 = [](long x) { return (int)((x + 15) / 16 * 16); };
+      const long nrec = 64L * h->kb.maxchunk;  // entry records of the largest tile
+      XLds x;
+      x.v = 0;
+      x.cs = a16((long)g.n[0] * kTile * (long)sizeof(T));
+      x.bt = x.cs + a16(nrec * 8);
+      x.ph = x.bt + a16(nrec * 4 * (long)sizeof(T));
+      x.wave0 = x.ph + (need_ph ? a16(nrec * cs) : 0);
+      x.aux = a16((long)(h->kb.maxpts + 2) * cs);  // + the zero and trash slots
+      x.wave_bytes = x.aux + a16((long)kMaxNlm * h->kb.maxslot * cs);
+      XLds xp = x;  // the project pass has no V tile
+      xp.v = 0;
+      xp.cs = 0;
+      xp.bt = x.bt - x.cs;
+      xp.ph = x.ph - x.cs;
+      xp.wave0 = x.wave0 - x.cs;
+      constexpr int kLdsMax = 160 * 1024;
+      INQX_CHECK(x.wave0 + x.wave_bytes <= kLdsMax, INQX_ERR_UNSUPPORTED, "KB tile too dense for LDS");
+      while (nw > 1 && x.wave0 + nw * x.wave_bytes > kLdsMax) --nw;
+      ld->xl_apply = x;
+      ld->xl_project = xp;
+      ld->xlds_apply = x.wave0 + nw * x.wave_bytes;
+      ld->xlds_project = xp.wave0 + nw * x.wave_bytes;
     }
     ld->nwave = nw;
     const double *Afull = h->A;
@@ -1503,6 +1695,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
       d.t1 = B.t1[prec].as<cplx<T>>();
       d.partial = B.partial[prec].as<cplx<T>>();
       d.dp = B.dp[prec].as<cplx<T>>();
+      d.dps = B.dps[prec].as<cplx<T>>();
       d.slot_seg = B.d_slot_seg.as<int>();
       d.rows = B.d_rows.as<int>();
       d.pw_yz = B.d_pw_yz.as<int>();
@@ -1529,7 +1722,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
                    : nullptr;
       pp[i + 1] = pp[i] + d.nslot * ((nst[i] + plane_p - 1) / plane_p);
       if (mode == 0)
-        px[i + 1] = px[i] + h->kb.ntile * ((nst[i] + nw - 1) / nw);
+        px[i + 1] = px[i] + x_tiles_padded(h->kb.ntile) * ((nst[i] + nw - 1) / nw);
       else
         px[i + 1] = px[i] + ld->ntile_line * ((nst[i] + 3) / 4);
       pr[i + 1] = pr[i] + (std::max(1, h->kb.natoms) * nst[i] + 255) / 256;
@@ -1568,6 +1761,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   hd.ntile = h->kb.ntile;
   hd.maxpts = h->kb.maxpts;
   hd.maxslot = h->kb.maxslot;
+  hd.ntg = x_tiles_padded(h->kb.ntile);
   hd.line_mask = h->kb.line_mask.as<uint32_t>();
   hd.tile_info = h->kb.tile_info.as<int>();
   hd.slot_atom = h->kb.slot_atom.as<int>();

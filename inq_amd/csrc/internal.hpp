@@ -162,6 +162,7 @@ struct KBTables {
   bool built = false;
   int gen = 0;  // incremented by every build: the per-basis phase tables follow it
   int natoms = 0, maxsph = 0, ntile = 0, nslot = 0, maxpts = 0, maxslot = 0;
+  int maxchunk = 0;  // max over tiles of ceil(entries / 64): 64-entry rows of a tile's records
   long nent = 0;
   // per line: bitmask of sphere points along i0 (3 words) and the line's
   // first compact point index inside its tile
@@ -194,6 +195,7 @@ template <typename T> struct SphereBlock {
   int acc_wide;        // the accumulators hold complex128 values (T = float)
   C *proj;             // KB projections P [natoms][kMaxNlm][nst], or null
   double *dip;         // dipole partials [ntile][nst][4], or null
+  C *dps;              // D P per (KB slot, state): [kb nslot][nst][kMaxNlm]
   int nst, npw;
   int nslot, m_lo;     // slot j holds plane index m = m_lo + j (symmetric range)
   int gamma;           // k+A == 0: no KB phases
@@ -243,7 +245,7 @@ struct Basis {
   double ph_A[3] = {1e300, 0, 0};
   int ph_gen = -1;
   // workspaces per precision
-  DevMem t1[2], partial[2], dp[2], proj[2], dip, dots, hn[2][2], save[2];
+  DevMem t1[2], partial[2], dp[2], dps[2], proj[2], dip, dots, hn[2][2], save[2];
   int ws_nst[2] = {0, 0};
   int ws_kb_nslot[2] = {-1, -1}, ws_natoms[2] = {-1, -1};
   int pinned = 0;  // > 0 while a captured graph uses the workspaces
@@ -283,7 +285,7 @@ struct inqx_ham_s {
   std::vector<inqx_basis> bases;
   int pinned = 0;  // > 0 while a captured graph uses the projector tables
   // launch shape of the x-pass kernels per precision: states (waves) per work-group
-  int xwaves[2] = {1, 1};
+  int xwaves[2] = {4, 4};  // states per x-pass work-group (complex128, complex64)
   // plane-pass planes (states) per work-group per precision; 0: the largest the grid allows
   int plane_batch[2] = {0, 0};
   // exact-exchange workspaces: real-space states, hf states, accumulator, pair planes
