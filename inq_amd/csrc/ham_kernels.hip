@@ -852,8 +852,10 @@ __global__ void __launch_bounds__(512) k_xpass_apply(const SphereBlock<T> *block
       load_line_slots_split<T, N0>(B, hd.n12, s, Lc, upper, v);
       if (npts > 0) {
         w = ld4<uint32_t>(hd.line_mask, Lc);
-        // the compact points, the zero slot pts[npts] and the trash slot pts[npts + 1]
-        for (int i = ptid; i < npts + 2; i += 64) pts[i] = C{0, 0};
+        // the zero slot pts[npts] and the trash slot pts[npts + 1]; the compact
+        // points too unless every point has exactly one record (then the KB
+        // loop below stores each point once)
+        for (int i = (nent == npts ? npts : 0) + ptid; i < npts + 2; i += 64) pts[i] = C{0, 0};
       }
       fft_split<N0, +1, C>::dit(v, upper);
       if (B.dip) {
@@ -898,9 +900,12 @@ __global__ void __launch_bounds__(512) k_xpass_apply(const SphereBlock<T> *block
       }
       if (npts > 0) {
         // the tile's records in LDS order, two rows per step with all reads
-        // first; padding records (point -1, zero projector values) add into the
-        // trash slot
+        // first; padding records (point -1, zero projector values) go to the
+        // trash slot.  Where no two records share a point (no overlapping
+        // projector spheres in the tile: as many records as points) each
+        // point is stored once; otherwise the records add with LDS atomics.
         const int trash = npts + 1;
+        const bool once = nent == npts;  // uniform
         for (int e0 = 0; e0 < ncap; e0 += 128) {
           const int nr = min(2, (ncap - e0) >> 6);  // uniform
           int2 cs[2];
@@ -925,9 +930,13 @@ __global__ void __launch_bounds__(512) k_xpass_apply(const SphereBlock<T> *block
               t.x = bt[q].x * dd[q][0].x + bt[q].y * dd[q][1].x + bt[q].z * dd[q][2].x + bt[q].w * dd[q][3].x;
               t.y = bt[q].x * dd[q][0].y + bt[q].y * dd[q][1].y + bt[q].z * dd[q][2].y + bt[q].w * dd[q][3].y;
               if (!B.gamma) t = cmul(t, C{ph[q].x, -ph[q].y});  // exp(-i (k+A).r)
-              T *r = reinterpret_cast<T *>(pts + (cs[q].x < 0 ? trash : cs[q].x));
-              lds_add(r + 0, t.x);
-              lds_add(r + 1, t.y);
+              C *r = pts + (cs[q].x < 0 ? trash : cs[q].x);
+              if (once) {
+                *r = t;
+              } else {
+                lds_add(&r->x, t.x);
+                lds_add(&r->y, t.y);
+              }
             }
           }
         }
