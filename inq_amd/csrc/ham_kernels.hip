@@ -79,6 +79,7 @@ struct HamDev {
   int ntg;  // x-pass work-groups per group of states: the tiles padded to a multiple of 8
   const uint32_t *line_mask;
   const int *tile_info;
+  const int *lane_info;  // [ntile][64] project-pass runs (see build_kb_tables)
   const int *slot_atom;
   const int *ent_cs;
   const void *ent_b;
@@ -577,16 +578,16 @@ __device__ __forceinline__ void dma_copy(const void *src, void *dst, int nbytes,
 // wave (state): compact sphere points, then the slot values (projection
 // accumulators of the project pass, D P of the apply pass)
 struct XLds {
-  int v, cs, bt, ph, wave0, wave_bytes, aux;
+  int v, cs, bt, ph, li, wave0, wave_bytes, aux;
 };
 
 // Inverse FFT along i0 of the 32 lines of a tile, then the KB projections
 // P[slot][l] = sum_entries beta_l exp(+ik.pos) psi.  A work-group takes ns
-// states (one per wave) of one tile and loads the tile's entry records into
-// LDS once, before the line loads; each lane walks a contiguous run of the
-// tile's entries (sorted by slot) from LDS, accumulates in registers and
-// flushes to its wave's slot accumulators at slot changes.  Every
-// accumulator belongs to one wave, so the sums do not depend on timing.
+// states (one per wave) of one tile and loads the tile's entry records and
+// lane table into LDS once, before the line loads.  Each lane walks a run of
+// records of one slot from LDS and sums it in registers; the lanes' partial
+// sums of a slot are then added in lane order (no atomics, the sums do not
+// depend on timing).
 template <typename T, int N0>
 __global__ void __launch_bounds__(512) k_xpass_project(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                        int total, HamDev hd, XLds xl) {
@@ -599,8 +600,9 @@ __global__ void __launch_bounds__(512) k_xpass_project(const SphereBlock<T> *blo
   const int2 *scs = reinterpret_cast<const int2 *>(smem_raw + xl.cs);
   const V4<T> *sbt = reinterpret_cast<const V4<T> *>(smem_raw + xl.bt);
   const C *sph = reinterpret_cast<const C *>(smem_raw + xl.ph);
+  const int *sli = reinterpret_cast<const int *>(smem_raw + xl.li);
   C *pts = reinterpret_cast<C *>(smem_raw + xl.wave0 + wave * xl.wave_bytes);
-  C *acc = reinterpret_cast<C *>(smem_raw + xl.wave0 + wave * xl.wave_bytes + xl.aux);
+  C *part = reinterpret_cast<C *>(smem_raw + xl.wave0 + wave * xl.wave_bytes + xl.aux);  // [64 lanes][kMaxNlm]
   {  // one work item per work-group: the grid is the work count
     const int wg = blockIdx.x;
     if (wg >= total) return;
@@ -619,11 +621,12 @@ __global__ void __launch_bounds__(512) k_xpass_project(const SphereBlock<T> *blo
     const V4<int> tj = ld4<int>(hd.tile_info, 2 * tile + 1);
     const int npts = ti.x, nsl = ti.y, slot_base = ti.z, ent_base = ti.w, nent = tj.x;
     if (npts == 0) return;  // uniform: no sphere point in the tile
-    const int chunk = (nent + 63) >> 6;
-    dma_copy(hd.ent_cs + 2 * (size_t)ent_base, smem_raw + xl.cs, 64 * chunk * 8, wave, ns, lane);
+    const int rows = tj.y;
+    dma_copy(hd.ent_cs + 2 * (size_t)ent_base, smem_raw + xl.cs, 64 * rows * 8, wave, ns, lane);
     dma_copy(static_cast<const char *>(hd.ent_b) + (size_t)ent_base * 4 * sizeof(T), smem_raw + xl.bt,
-             64 * chunk * 4 * (int)sizeof(T), wave, ns, lane);
-    if (!B.gamma) dma_copy(B.kbph + ent_base, smem_raw + xl.ph, 64 * chunk * (int)sizeof(C), wave, ns, lane);
+             64 * rows * 4 * (int)sizeof(T), wave, ns, lane);
+    if (!B.gamma) dma_copy(B.kbph + ent_base, smem_raw + xl.ph, 64 * rows * (int)sizeof(C), wave, ns, lane);
+    dma_copy(hd.lane_info + (size_t)tile * 64, smem_raw + xl.li, 64 * 4, wave, ns, lane);
     const bool active = s < B.nst;
     const bool valid = L < hd.n12;
     C v[H];
@@ -632,7 +635,6 @@ __global__ void __launch_bounds__(512) k_xpass_project(const SphereBlock<T> *blo
       // line loads and the line's point mask in flight together
       load_line_slots_split<T, N0>(B, hd.n12, s, valid ? L : hd.n12 - 1, upper, v);
       w = ld4<uint32_t>(hd.line_mask, valid ? L : hd.n12 - 1);
-      for (int i = ptid; i < nsl * kMaxNlm; i += 64) acc[i] = C{0, 0};
       fft_split<N0, +1, C>::dit(v, upper);
       // every value goes out: the line's sphere points to their compact
       // slots, the others (and lanes past the last line) to the trash slot
@@ -651,12 +653,11 @@ __global__ void __launch_bounds__(512) k_xpass_project(const SphereBlock<T> *blo
     }
     __syncthreads();  // entry records in LDS, sphere points staged
     if (active) {
-      // the lane's run of the tile's entries: entry j of the run at 64 j + lane
-      const int cnt = max(0, min(chunk, nent - ptid * chunk));
-      int cur = -1;
+      // the lane's run (records of one slot): record j at 64 j + lane
+      const int cnt = sli[ptid] & 255;
       C a0{0, 0}, a1{0, 0}, a2{0, 0}, a3{0, 0};
       for (int j = 0; j < cnt; j += 2) {
-        // two entries of the run: all their LDS reads before the accumulation
+        // two records: all their LDS reads before the accumulation
         int2 cs[2];
         V4<T> bt[2];
         C u[2];
@@ -672,15 +673,6 @@ __global__ void __launch_bounds__(512) k_xpass_project(const SphereBlock<T> *blo
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           if (j + q < cnt) {
-            if (cs[q].y != cur) {
-              if (cur >= 0) {
-                T *r = reinterpret_cast<T *>(acc + cur * kMaxNlm);
-                lds_add(r + 0, a0.x); lds_add(r + 1, a0.y); lds_add(r + 2, a1.x); lds_add(r + 3, a1.y);
-                lds_add(r + 4, a2.x); lds_add(r + 5, a2.y); lds_add(r + 6, a3.x); lds_add(r + 7, a3.y);
-              }
-              cur = cs[q].y;
-              a0 = a1 = a2 = a3 = C{0, 0};
-            }
             const V4<T> b = bt[q];
             const C x = u[q];
             a0.x += b.x * x.x; a0.y += b.x * x.y;
@@ -690,17 +682,22 @@ __global__ void __launch_bounds__(512) k_xpass_project(const SphereBlock<T> *blo
           }
         }
       }
-      if (cur >= 0) {
-        T *q = reinterpret_cast<T *>(acc + cur * kMaxNlm);
-        lds_add(q + 0, a0.x); lds_add(q + 1, a0.y); lds_add(q + 2, a1.x); lds_add(q + 3, a1.y);
-        lds_add(q + 4, a2.x); lds_add(q + 5, a2.y); lds_add(q + 6, a3.x); lds_add(q + 7, a3.y);
-      }
+      part[ptid * kMaxNlm + 0] = a0;
+      part[ptid * kMaxNlm + 1] = a1;
+      part[ptid * kMaxNlm + 2] = a2;
+      part[ptid * kMaxNlm + 3] = a3;
     }
     __syncthreads();
     if (active) {
+      // slot q's sum over its lanes [first(q), first(q + 1)), in lane order
       GAS C *pt = gw(B.partial);
-      for (int i = ptid; i < nsl * kMaxNlm; i += 64)
-        pt[((size_t)(slot_base + i / kMaxNlm) * kMaxNlm + (i % kMaxNlm)) * B.nst + s] = acc[i];
+      for (int i = ptid; i < nsl * kMaxNlm; i += 64) {
+        const int q = i / kMaxNlm, l = i - q * kMaxNlm;
+        const int f0 = sli[q] >> 16, f1 = sli[q + 1] >> 16;
+        C sum = part[f0 * kMaxNlm + l];
+        for (int k = f0 + 1; k < f1; ++k) sum = cadd(sum, part[k * kMaxNlm + l]);
+        pt[((size_t)(slot_base + q) * kMaxNlm + l) * B.nst + s] = sum;
+      }
     }
   }
 }
@@ -810,7 +807,9 @@ __global__ void __launch_bounds__(512) k_xpass_apply(const SphereBlock<T> *block
       ent_base = ti.w;
       nent = tj.x;
     }
-    const int ncap = 64 * ((nent + 63) >> 6);  // the tile's records including padding
+    int rows = 0;
+    if (hd.has_kb) rows = ld4<int>(hd.tile_info, 2 * tile + 1).y;
+    const int ncap = 64 * rows;  // the tile's records including padding
     {
       // V/N rows of the tile: N0 rows of 32 lines; a 16-byte record past the
       // last line (partial last tile) loads the row's first record instead
@@ -1183,24 +1182,56 @@ void build_kb_tables(inqx_ham_s *h) {
     epos.push_back(0.0f);
   }
   kb.nslot = (int)slot_atom.size();
-  // lane-interleaved order per tile: the x passes give lane l the contiguous
-  // run [l c, l c + c) of a tile's nent entries (c = ceil(nent / 64)), and
-  // entry (l, j) of that run is stored at tile base + j * 64 + l, so the 64
-  // lanes' j-th loads are consecutive records; runs are padded to c with
-  // empty entries (point and slot -1, zero projector values)
+  // Record order per tile, for the project pass: every lane of a wave walks a
+  // run of records of ONE slot, so it sums in registers and its partial
+  // needs no atomics.  A slot's records are split into contiguous runs over
+  // a contiguous range of lanes, the lanes dealt out so that the longest run
+  // (the tile's rows) is as short as possible; record j of lane l's run is
+  // stored at the tile base + 64 j + l (the 64 lanes' j-th loads are
+  // consecutive), runs shorter than the rows padded with empty records
+  // (point and slot -1, zero projector values).  lane_info[tile][l] = run
+  // length | (slot + 1) << 8 | (first lane of slot l, l <= slots) << 16.
+  std::vector<int> lane_info((size_t)kb.ntile * 64, 0);
   {
     std::vector<int> cs2;
     std::vector<float> b32, pos2;
     std::vector<double> b64;
     long base = 0;
     for (int t = 0; t < kb.ntile; ++t) {
-      const int e0 = tile_info[8 * t + 3], ne = tile_info[8 * t + 4];
-      const int c = (ne + 63) / 64;
-      for (int j = 0; j < c; ++j)
-        for (int l = 0; l < 64; ++l) {
-          const int k = l * c + j;
-          const bool ok = k < ne;
-          const size_t o = (size_t)e0 + (size_t)(ok ? k : 0);
+      const int e0 = tile_info[8 * t + 3], ne = tile_info[8 * t + 4], nsl_t = tile_info[8 * t + 1];
+      INQX_CHECK(nsl_t <= 63, INQX_ERR_UNSUPPORTED, "more than 63 projector atoms meet one 32-line tile");
+      std::vector<int> off(nsl_t + 1, 0), cnt(nsl_t, 0), nl(nsl_t, 1);
+      for (int k = 0; k < ne; ++k) ++cnt[ent_cs[2 * ((size_t)e0 + k) + 1]];
+      for (int q = 0; q < nsl_t; ++q) off[q + 1] = off[q] + cnt[q];
+      auto rows_of = [&](int q) { return (cnt[q] + nl[q] - 1) / nl[q]; };
+      for (int used = nsl_t; used < 64; ++used) {  // one more lane to the slot with the longest run
+        int best = -1;
+        for (int q = 0; q < nsl_t; ++q)
+          if (rows_of(q) > 1 && (best < 0 || rows_of(q) > rows_of(best))) best = q;
+        if (best < 0) break;
+        ++nl[best];
+      }
+      int rows = 0;
+      for (int q = 0; q < nsl_t; ++q) rows = std::max(rows, rows_of(q));
+      INQX_CHECK(rows <= 255, INQX_ERR_UNSUPPORTED, "KB tile too dense (a lane run above 255 records)");
+      std::vector<int> lane_slot(64, -1), lane_first(64, 0), lane_len(64, 0);
+      int l = 0;
+      for (int q = 0; q < nsl_t; ++q) {
+        const int c = rows_of(q);
+        lane_info[(size_t)t * 64 + q] |= l << 16;  // first lane of slot q
+        for (int part = 0; part < nl[q]; ++part, ++l) {
+          const int b0 = std::min(cnt[q], part * c), b1 = std::min(cnt[q], (part + 1) * c);
+          lane_slot[l] = q;
+          lane_first[l] = off[q] + b0;
+          lane_len[l] = b1 - b0;
+        }
+      }
+      lane_info[(size_t)t * 64 + nsl_t] |= l << 16;  // one past the last lane used
+      for (int k = 0; k < 64; ++k) lane_info[(size_t)t * 64 + k] |= lane_len[k] | ((lane_slot[k] + 1) << 8);
+      for (int j = 0; j < rows; ++j)
+        for (int k = 0; k < 64; ++k) {
+          const bool ok = j < lane_len[k];
+          const size_t o = (size_t)e0 + (size_t)(ok ? lane_first[k] + j : 0);
           cs2.push_back(ok ? ent_cs[2 * o] : -1);
           cs2.push_back(ok ? ent_cs[2 * o + 1] : -1);
           for (int q = 0; q < kMaxNlm; ++q) {
@@ -1210,8 +1241,9 @@ void build_kb_tables(inqx_ham_s *h) {
           for (int q = 0; q < 4; ++q) pos2.push_back(ok ? epos[4 * o + q] : 0.0f);
         }
       tile_info[8 * t + 3] = (int)base;
-      base += 64L * c;
-      kb.maxchunk = std::max(kb.maxchunk, c);
+      tile_info[8 * t + 5] = rows;
+      base += 64L * rows;
+      kb.maxchunk = std::max(kb.maxchunk, rows);
     }
     ent_cs.swap(cs2);
     eb32.swap(b32);
@@ -1239,6 +1271,7 @@ void build_kb_tables(inqx_ham_s *h) {
   }
   upload(kb.line_mask, mask, st);
   upload(kb.tile_info, tile_info, st);
+  upload(kb.lane_info, lane_info, st);
   upload(kb.slot_atom, slot_atom, st);
   upload(kb.ent_cs, ent_cs, st);
   upload(kb.ent_b32, eb32, st);
@@ -1677,22 +1710,26 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
       x.cs = a16((long)g.n[0] * kTile * (long)sizeof(T));
       x.bt = x.cs + a16(nrec * 8);
       x.ph = x.bt + a16(nrec * 4 * (long)sizeof(T));
-      x.wave0 = x.ph + (need_ph ? a16(nrec * cs) : 0);
+      x.li = x.ph + (need_ph ? a16(nrec * cs) : 0);
+      x.wave0 = x.li + 64 * 4;
       x.aux = a16((long)(h->kb.maxpts + 2) * cs);  // + the zero and trash slots
       x.wave_bytes = x.aux + a16((long)kMaxNlm * h->kb.maxslot * cs);
-      XLds xp = x;  // the project pass has no V tile
+      XLds xp = x;  // the project pass has no V tile, and 64 lanes' partial sums per wave
       xp.v = 0;
       xp.cs = 0;
       xp.bt = x.bt - x.cs;
       xp.ph = x.ph - x.cs;
+      xp.li = x.li - x.cs;
       xp.wave0 = x.wave0 - x.cs;
+      xp.wave_bytes = xp.aux + 64 * kMaxNlm * cs;
       constexpr int kLdsMax = 160 * 1024;
-      INQX_CHECK(x.wave0 + x.wave_bytes <= kLdsMax, INQX_ERR_UNSUPPORTED, "KB tile too dense for LDS");
-      while (nw > 1 && x.wave0 + nw * x.wave_bytes > kLdsMax) --nw;
+      INQX_CHECK(x.wave0 + x.wave_bytes <= kLdsMax && xp.wave0 + xp.wave_bytes <= kLdsMax, INQX_ERR_UNSUPPORTED,
+                 "KB tile too dense for LDS");
+      while (nw > 1 && (x.wave0 + nw * x.wave_bytes > kLdsMax || xp.wave0 + nw * xp.wave_bytes > kLdsMax)) --nw;
       ld->xl_apply = x;
       ld->xl_project = xp;
       ld->xlds_apply = x.wave0 + nw * x.wave_bytes;
-      ld->xlds_project = xp.wave0 + nw * x.wave_bytes;
+      ld->xlds_project = xp.wave0 + nw * xp.wave_bytes;
     }
     ld->nwave = nw;
     const double *Afull = h->A;
@@ -1773,6 +1810,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   hd.ntg = x_tiles_padded(h->kb.ntile);
   hd.line_mask = h->kb.line_mask.as<uint32_t>();
   hd.tile_info = h->kb.tile_info.as<int>();
+  hd.lane_info = h->kb.lane_info.as<int>();
   hd.slot_atom = h->kb.slot_atom.as<int>();
   hd.ent_cs = h->kb.ent_cs.as<int>();
   hd.ent_b = prec == INQX_FP64 ? h->kb.ent_b64.p : h->kb.ent_b32.p;

@@ -168,6 +168,7 @@ struct KBTables {
   // first compact point index inside its tile
   DevMem line_mask;   // [n12] uint4: m0, m1, m2, base
   DevMem tile_info;   // [ntile] int4: npts, nslot_t, slot_base, ent_base ; [ntile] ent count in a 2nd int4
+  DevMem lane_info;   // [ntile][64] int: project-pass run length | (slot + 1) << 8 | (first lane of slot) << 16
   DevMem slot_atom;   // [nslot] atom of each global slot
   DevMem ent_cs;      // [nent] int2: compact point index, tile-local slot
   DevMem ent_b32;     // [nent] float4: projector values l = 0..3
