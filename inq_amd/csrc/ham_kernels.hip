@@ -184,6 +184,36 @@ template <typename T, int N1, int N2> struct PlaneCfg {
   static constexpr int LDS = P * PLANE_BYTES;
 };
 
+// sphere points per thread in one round of the plane passes' scatter and
+// gather: all their loads are in flight together (fewer for complex128, whose
+// transforms leave no registers for more without spilling)
+template <typename T> constexpr int gather_round() { return sizeof(T) == 4 ? 16 : 8; }
+
+// x-pass work-group size limit: up to 8 states for complex64; complex128
+// needs a wave per SIMD (VGPRs plus AGPRs) and takes at most 4
+template <typename T> constexpr int xpass_threads() { return sizeof(T) == 4 ? 512 : 256; }
+
+// loads of one scatter round: KG sphere points per thread from point index
+// i0 in steps of nth (past the end: the last point again, not stored later)
+// (real and imaginary parts in separate arrays: an array of 16-byte complex
+// structs here was kept in scratch memory instead of registers)
+template <typename T, int KG, int N2, int PLANE>
+__device__ __forceinline__ void scatter_round(const GAS cplx<T> *psi, const GAS int *pwyz, long npw, int s0, int seg0,
+                                              int nseg, int tot, int i0, int nth, int (&ldx)[KG], T (&vr)[KG],
+                                              T (&vi)[KG]) {
+#pragma unroll
+  for (int u = 0; u < KG; ++u) {
+    const int i = min(i0 + u * nth, tot - 1);
+    const int p = i / nseg, j = i - p * nseg;
+    const int yz = pwyz[seg0 + j];
+    const int y = yz / N2, z = yz - y * N2;
+    ldx[u] = p * PLANE + y * (N2 + 1) + z;
+    const cplx<T> v = psi[(size_t)(s0 + p) * npw + seg0 + j];
+    vr[u] = v.x;
+    vi[u] = v.y;
+  }
+}
+
 template <typename T, int N1, int N2>
 __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                        int total, int pb) {
@@ -208,27 +238,23 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
     const int np = min(P, B.nst - s0);
     const V4<int> sg = ld4<int>(B.slot_seg, slot);
     const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
-    for (int i = tid; i < np * Cfg::PLANE; i += nth) lds[i] = C{0, 0};
-    __syncthreads();
     const GAS C *psi = gp(B.psi);
     const GAS int *pwyz = gp(B.pw_yz);
-    // four sphere points per thread and step: their loads are in flight together
+    constexpr int KG = gather_round<T>();
+    // the planes' sphere points, KG per thread in one round (a whole
+    // plane at the configurations' sizes): the first round's loads go out
+    // before the plane is cleared, so their latency overlaps the clearing
     const int tot = np * nseg;
-    for (int i0 = tid; i0 < tot; i0 += 4 * nth) {
-      int ldx[4];
-      C val[4];
+    int ldx[KG];
+    T vr[KG], vi[KG];
+    if (tot > 0) scatter_round<T, KG, N2, Cfg::PLANE>(psi, pwyz, B.npw, s0, seg0, nseg, tot, tid, nth, ldx, vr, vi);
+    for (int i = tid; i < np * Cfg::PLANE; i += nth) lds[i] = C{0, 0};
+    __syncthreads();
+    for (int i0 = tid; i0 < tot; i0 += KG * nth) {
+      if (i0 != tid) scatter_round<T, KG, N2, Cfg::PLANE>(psi, pwyz, B.npw, s0, seg0, nseg, tot, i0, nth, ldx, vr, vi);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = min(i0 + u * nth, tot - 1);
-        const int p = i / nseg, j = i - p * nseg;
-        const int yz = pwyz[seg0 + j];
-        const int y = yz / N2, z = yz - y * N2;
-        ldx[u] = p * Cfg::PLANE + y * (N2 + 1) + z;
-        val[u] = psi[(size_t)(s0 + p) * B.npw + seg0 + j];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (i0 + u * nth < tot) lds[ldx[u]] = val[u];
+      for (int u = 0; u < KG; ++u)
+        if (i0 + u * nth < tot) lds[ldx[u]] = C{vr[u], vi[u]};
     }
     __syncthreads();
     // rows along i2 (sphere rows only), a lane pair per row
@@ -291,6 +317,21 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
     const V4<int> sg = ld4<int>(B.slot_seg, slot);
     const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
     if (nseg == 0) return;  // uniform: nothing of this plane is on the sphere
+    const GAS int *pwyz = gp(B.pw_yz);
+    const int tot = np * nseg;
+    constexpr int KG = gather_round<T>();
+    // complex64: the first gather round's point indices are loaded now, so the
+    // gather after the transforms has no dependent load (complex128 has no
+    // registers to spare and loads them in the gather)
+    constexpr bool kEarlyYz = sizeof(T) == 4;
+    int yz0[KG];
+    if constexpr (kEarlyYz) {
+#pragma unroll
+      for (int u = 0; u < KG; ++u) {
+        const int i = min(tid + u * nth, tot - 1);
+        yz0[u] = pwyz[seg0 + i % nseg];
+      }
+    }
     if constexpr (Cfg::SPLIT_F) {
       // columns along i1 from the plane slots, then the sphere rows along i2,
       // a lane pair per line; the np planes are slot `slot` of states s0..s0+np-1
@@ -372,19 +413,19 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
                                B.acc2 ? nb << wsh : 0u);
     const auto rk = make_rsrc(B.kin ? (const void *)B.kin : (const void *)B.psi,
                               B.kin ? (unsigned)(B.npw * sizeof(T)) : 0u);
-    const GAS int *pwyz = gp(B.pw_yz);
-    const int tot = np * nseg;
-    for (int i0 = tid; i0 < tot; i0 += 4 * nth) {
-      // four sphere points per thread and step: loads first, then the updates
-      unsigned oo[4];
-      C outv[4], cv[4];
-      T kv[4];
+    for (int i0 = tid; i0 < tot; i0 += KG * nth) {
+      // KG sphere points per thread and round: loads first, then the updates
+      unsigned oo[KG];
+      C outv[KG], cv[KG];
+      T kv[KG];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < KG; ++u) {
         const int i = i0 + u * nth;
         const int ic = i < tot ? i : tot - 1;
         const int p = ic / nseg, j = ic - p * nseg;
-        const int yz = pwyz[seg0 + j];
+        int yz;
+        if constexpr (kEarlyYz) yz = i0 == tid ? yz0[u] : pwyz[seg0 + j];
+        else yz = pwyz[seg0 + j];
         const int y = yz / N2, z = yz - y * N2;
         oo[u] = i < tot ? (unsigned)(((size_t)p * B.npw + seg0 + j) * sizeof(C)) : kOOB;
         outv[u] = lds[p * Cfg::PLANE + y * (N2 + 1) + z];
@@ -394,7 +435,7 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
         }
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < KG; ++u) {
         C out = outv[u];
         if (B.kin) {  // H application: + 1/2|G+k|^2 psi; without it a plain masked forward FFT
           out.x += kv[u] * cv[u].x;
@@ -589,7 +630,7 @@ struct XLds {
 // sums of a slot are then added in lane order (no atomics, the sums do not
 // depend on timing).
 template <typename T, int N0>
-__global__ void __launch_bounds__(512) k_xpass_project(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
+__global__ void __launch_bounds__(xpass_threads<T>()) k_xpass_project(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                        int total, HamDev hd, XLds xl) {
   using C = cplx<T>;
   constexpr int H = N0 / 2;
@@ -768,7 +809,7 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
 // (shared), its KB entry records (shared) and each state's D P of the tile's
 // slots, so no memory round trip sits between the two line transforms.
 template <typename T, int N0>
-__global__ void __launch_bounds__(512) k_xpass_apply(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
+__global__ void __launch_bounds__(xpass_threads<T>()) k_xpass_apply(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                      int total, HamDev hd, XLds xl) {
   using C = cplx<T>;
   constexpr int H = N0 / 2;
@@ -1693,7 +1734,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     ld->plane_batch = plane_p;
     // waves (states) per x-pass work-group within the LDS budget
     const int cs = (int)sizeof(cplx<T>);
-    int nw = std::max(1, std::min(8, h->xwaves[prec]));  // states (waves) per x-pass work-group
+    int nw = std::max(1, std::min(xpass_threads<T>() / 64, h->xwaves[prec]));  // states (waves) per x-pass work-group
     ld->mode = mode;
     ld->observables = opts->observables;
     ld->ntile_line = (g.n[1] * g.n[2] + 31) / 32;
