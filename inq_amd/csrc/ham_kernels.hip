@@ -231,10 +231,12 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
     if (wg >= total) return;
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
+    // work order: consecutive work-groups take consecutive slots (adjacent
+    // planes in memory) of the same states
     const int local = wg - gp(prefix)[b];
-    const int ngrp = (B.nst + P - 1) / P;
-    const int slot = local / ngrp;
-    const int s0 = (local - slot * ngrp) * P;
+    const int sgrp = local / B.nslot;
+    const int slot = local - sgrp * B.nslot;
+    const int s0 = sgrp * P;
     const int np = min(P, B.nst - s0);
     const V4<int> sg = ld4<int>(B.slot_seg, slot);
     const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
@@ -309,10 +311,12 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
     if (wg >= total) return;
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
+    // work order: consecutive work-groups take consecutive slots (adjacent
+    // planes in memory) of the same states
     const int local = wg - gp(prefix)[b];
-    const int ngrp = (B.nst + P - 1) / P;
-    const int slot = local / ngrp;
-    const int s0 = (local - slot * ngrp) * P;
+    const int sgrp = local / B.nslot;
+    const int slot = local - sgrp * B.nslot;
+    const int s0 = sgrp * P;
     const int np = min(P, B.nst - s0);
     const V4<int> sg = ld4<int>(B.slot_seg, slot);
     const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
@@ -763,14 +767,25 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
     for (int l = 0; l < kMaxNlm; ++l) pr[l] = pi[l] = 0.0;
     const int q0 = gp(hd.atom_slots)[a], q1 = gp(hd.atom_slots)[a + 1];
     const GAS C *part = gp(B.partial);
-    for (int q = q0; q < q1; ++q) {
-      const int sl = gp(hd.atom_slot_list)[q];
+    // four slots per step: their slot indices, then their partial sums, all in
+    // flight together; added in slot order
+    for (int q = q0; q < q1; q += 4) {
+      int sl[4];
 #pragma unroll
-      for (int l = 0; l < kMaxNlm; ++l) {
-        const C v = part[((size_t)sl * kMaxNlm + l) * B.nst + s];
-        pr[l] += (double)v.x;
-        pi[l] += (double)v.y;
-      }
+      for (int u = 0; u < 4; ++u) sl[u] = gp(hd.atom_slot_list)[min(q + u, q1 - 1)];
+      C v[4][kMaxNlm];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int l = 0; l < kMaxNlm; ++l) v[u][l] = part[((size_t)sl[u] * kMaxNlm + l) * B.nst + s];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (q + u < q1)
+#pragma unroll
+          for (int l = 0; l < kMaxNlm; ++l) {
+            pr[l] += (double)v[u][l].x;
+            pi[l] += (double)v[u][l].y;
+          }
     }
     const GAS double *D = gp(hd.coeff) + (size_t)a * kMaxNlm * kMaxNlm;
     GAS C *dp = gw(B.dp);
