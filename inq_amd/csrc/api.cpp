@@ -196,8 +196,9 @@ static void build_basis(const Grid &g, Basis &b) {
   }
   b.m_lo = b.npw ? mlo : 0;
   b.nslot = b.npw ? mhi - mlo + 1 : 0;
-  b.slot_seg.assign((size_t)b.nslot * 4, 0);
+  b.slot_seg.assign((size_t)b.nslot * 8, 0);
   b.rows.clear();
+  b.nwin_max = 0;
   // segments are contiguous in the FFT-index order of the sphere
   long i = 0;
   std::vector<long> seg_start(g.n[0], -1), seg_len(g.n[0], 0);
@@ -212,14 +213,24 @@ static void build_basis(const Grid &g, Basis &b) {
   for (int sl = 0; sl < b.nslot; ++sl) {
     const int m = b.m_lo + sl;
     const int i0 = (m % g.n[0] + g.n[0]) % g.n[0];
-    b.slot_seg[4 * sl + 0] = seg_start[i0] < 0 ? 0 : (int)seg_start[i0];
-    b.slot_seg[4 * sl + 1] = (int)seg_len[i0];
-    b.slot_seg[4 * sl + 2] = (int)b.rows.size();
+    int *ss = &b.slot_seg[8 * sl];
+    ss[0] = seg_start[i0] < 0 ? 0 : (int)seg_start[i0];
+    ss[1] = (int)seg_len[i0];
+    ss[2] = (int)b.rows.size();
+    int wlo = 1 << 30, whi = -(1 << 30);
     for (long q = seg_start[i0]; q >= 0 && q < seg_start[i0] + seg_len[i0]; ++q) {
       const int i1 = b.pw_yz[q] / g.n[2];
-      if (b.rows.size() == (size_t)b.slot_seg[4 * sl + 2] || b.rows.back() != i1) b.rows.push_back(i1);
+      if (b.rows.size() == (size_t)ss[2] || b.rows.back() != i1) {
+        b.rows.push_back(i1);
+        wlo = std::min(wlo, Grid::sym(i1, g.n[1]));
+        whi = std::max(whi, Grid::sym(i1, g.n[1]));
+      }
     }
-    b.slot_seg[4 * sl + 3] = (int)b.rows.size() - b.slot_seg[4 * sl + 2];
+    ss[3] = (int)b.rows.size() - ss[2];
+    // the plane's row window: the symmetric range of i1 that holds its rows
+    ss[4] = ss[3] > 0 ? wlo : 0;
+    ss[5] = ss[3] > 0 ? whi - wlo + 1 : 0;
+    b.nwin_max = std::max(b.nwin_max, ss[5]);
   }
 }
 

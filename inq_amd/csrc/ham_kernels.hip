@@ -146,6 +146,32 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, 
 // a per-lane buffer offset beyond every record: loads return 0, stores are dropped
 constexpr unsigned kOOB = 0x80000000u;
 
+// Plane layout in LDS.  complex64 planes hold only their row window (the
+// symmetric range of i1 that holds sphere rows, nwin rows from wlo) at LDS
+// rows 1 .. nwin, so more work-groups fit per CU; complex128 plane passes are
+// limited by registers to one wave per SIMD, and their planes hold all N1
+// rows at LDS row i1.  plane_row(y) is the LDS row of a row y that holds
+// sphere points; row_in_plane(y) whether row y has an LDS row at all.
+template <typename T> constexpr bool plane_window() { return sizeof(T) == 4; }
+template <typename T, int N> __device__ __forceinline__ int plane_row(int y, int wlo) {
+  if constexpr (plane_window<T>()) return (y >= (N + 1) / 2 ? y - N : y) - wlo + 1;
+  else return y;
+}
+template <typename T, int N> __device__ __forceinline__ bool row_in_plane(int y, int wlo, int nwin) {
+  if constexpr (plane_window<T>()) return (unsigned)((y >= (N + 1) / 2 ? y - N : y) - wlo) < (unsigned)nwin;
+  else return true;
+}
+
+// i / d for 0 <= i < 2^24 and d >= 1 from the float reciprocal rd = 1/d: the
+// float quotient is off by at most one, corrected from the remainder
+__device__ __forceinline__ int div_rcp(int i, int d, float rd) {
+  int q = (int)((float)i * rd);
+  const int r = i - q * d;
+  q += r >= d ? 1 : 0;
+  q -= r < 0 ? 1 : 0;
+  return q;
+}
+
 // FFT of one LDS row of N values split over the lane pair; lanes with ok ==
 // false transform a copy of a valid row and store nothing
 template <typename C, int N, int SIGN>
@@ -162,25 +188,22 @@ __device__ __forceinline__ void plane_row_split(C *row, bool upper, bool ok) {
   }
 }
 
-// In the inverse pass every row and column of a plane is split over a lane
+// In both plane passes every row and column of a plane is split over a lane
 // pair (lane, lane + 32): a wave transforms 32 lines; P planes share a
-// work-group.  The forward pass does the same for long lines (SPLIT_F) and
-// gives each line one lane otherwise.
+// work-group.  A plane occupies LDS only for its row window (the rows that hold
+// sphere points, 51 of 72 at the 72^3 sphere), so more work-groups fit per CU.
 template <typename T, int N1, int N2> struct PlaneCfg {
   static_assert(N1 % 2 == 0 && N2 % 2 == 0, "split lines need even lengths");
   static constexpr int NL = (N1 > N2) ? N1 : N2;
-  static constexpr int PLANE = N1 * (N2 + 1);  // padded row stride N2+1
+  // a plane in LDS (padded row stride N2+1): complex64 the rows of its row
+  // window (at most N1) and two more, complex128 all N1 rows
+  static constexpr int PLANE = (plane_window<T>() ? N1 + 2 : N1) * (N2 + 1);
   static constexpr int PLANE_BYTES = PLANE * (int)sizeof(cplx<T>);
   static constexpr int P0 = 128 / NL > 0 ? 128 / NL : 1;  // 128 lines per 256 threads
   static constexpr int PL = (65536 / PLANE_BYTES) > 0 ? (65536 / PLANE_BYTES) : 1;
   static constexpr int P = (P0 < PL ? P0 : PL) > 0 ? (P0 < PL ? P0 : PL) : 1;
   static constexpr int WAVES = (P * NL + 31) / 32 < 4 ? (P * NL + 31) / 32 : 4;
   static constexpr int THREADS = 64 * WAVES;  // split lines
-  // forward pass: lines split when a whole line would hold more than 72
-  // complex64 values (complex128 from 40 points, complex64 above 72); measured
-  // on the 72^3 sphere: complex128 522 -> 417 us split, complex64 186 -> 192 us
-  static constexpr bool SPLIT_F = (int)sizeof(T) * NL > 4 * 72;
-  static constexpr int THREADS_F = SPLIT_F ? THREADS : (P * NL + 63) / 64 * 64;
   static constexpr int LDS = P * PLANE_BYTES;
 };
 
@@ -194,32 +217,50 @@ template <typename T> constexpr int gather_round() { return sizeof(T) == 4 ? 16 
 template <typename T> constexpr int xpass_threads() { return sizeof(T) == 4 ? 512 : 256; }
 
 // loads of one scatter round: KG sphere points per thread from point index
-// i0 in steps of nth (past the end: the last point again, not stored later)
-// (real and imaginary parts in separate arrays: an array of 16-byte complex
-// structs here was kept in scratch memory instead of registers)
-template <typename T, int KG, int N2, int PLANE>
-__device__ __forceinline__ void scatter_round(const GAS cplx<T> *psi, const GAS int *pwyz, long npw, int s0, int seg0,
-                                              int nseg, int tot, int i0, int nth, int (&ldx)[KG], T (&vr)[KG],
-                                              T (&vi)[KG]) {
+// i0 in steps of nth (past the end: the last point again, not stored later).
+// Point i is point j of plane p (i = p nseg + j); pwl[j] is its LDS offset in
+// the plane.  (Real and imaginary parts in separate arrays: an array of
+// 16-byte complex structs here was kept in scratch memory instead of registers.)
+template <typename T, int KG>
+__device__ __forceinline__ void scatter_round(__amdgpu_buffer_rsrc_t rpsi, const GAS int *pwl, int npw, int nseg,
+                                              float rseg, int tot, int i0, int nth, int pstr, int (&ldx)[KG],
+                                              T (&vr)[KG], T (&vi)[KG]) {
 #pragma unroll
   for (int u = 0; u < KG; ++u) {
     const int i = min(i0 + u * nth, tot - 1);
-    const int p = i / nseg, j = i - p * nseg;
-    const int yz = pwyz[seg0 + j];
-    const int y = yz / N2, z = yz - y * N2;
-    ldx[u] = p * PLANE + y * (N2 + 1) + z;
-    const cplx<T> v = psi[(size_t)(s0 + p) * npw + seg0 + j];
+    const int p = div_rcp(i, nseg, rseg), j = i - p * nseg;
+    ldx[u] = p * pstr + pwl[j];
+    cplx<T> v;
+    bload(rpsi, (unsigned)((p * npw + j) * (int)sizeof(cplx<T>)), 0u, v);
     vr[u] = v.x;
     vi[u] = v.y;
   }
 }
 
+// work item of a plane-pass work-group: block, slot and first state (the
+// block's work-groups take consecutive slots of the same states)
+struct PlaneItem {
+  int b, slot, s0, np;
+};
+template <typename T>
+__device__ __forceinline__ PlaneItem plane_item(const SphereBlock<T> *blocks, const int *prefix, int nblocks, int wg,
+                                                int P, SphereBlock<T> &B) {
+  PlaneItem it;
+  it.b = find_block(prefix, nblocks, wg);
+  B = blocks[it.b];
+  const int local = wg - gp(prefix)[it.b];
+  const int sgrp = local / B.nslot;
+  it.slot = local - sgrp * B.nslot;
+  it.s0 = sgrp * P;
+  it.np = min(P, B.nst - it.s0);
+  return it;
+}
+
 template <typename T, int N1, int N2>
 __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                       int total, int pb) {
+                                                       int total, int pb, int pstr) {
   using C = cplx<T>;
   using Cfg = PlaneCfg<T, N1, N2>;
-  const int P = pb;  // planes per work-group, 1 <= pb <= Cfg::P (launch shape)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -229,45 +270,49 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
   {  // one work item per work-group: the grid is the work count
     const int wg = blockIdx.x;
     if (wg >= total) return;
-    const int b = find_block(prefix, nblocks, wg);
-    const SphereBlock<T> B = blocks[b];
-    // work order: consecutive work-groups take consecutive slots (adjacent
-    // planes in memory) of the same states
-    const int local = wg - gp(prefix)[b];
-    const int sgrp = local / B.nslot;
-    const int slot = local - sgrp * B.nslot;
-    const int s0 = sgrp * P;
-    const int np = min(P, B.nst - s0);
-    const V4<int> sg = ld4<int>(B.slot_seg, slot);
+    SphereBlock<T> B;
+    const PlaneItem it = plane_item(blocks, prefix, nblocks, wg, pb, B);
+    const int slot = it.slot, s0 = it.s0, np = it.np;
+    const V4<int> sg = ld4<int>(B.slot_seg, 2 * slot);
     const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
-    const GAS C *psi = gp(B.psi);
-    const GAS int *pwyz = gp(B.pw_yz);
+    const V4<int> sw = ld4<int>(B.slot_seg, 2 * slot + 1);
+    const int wlo = sw.x, nwin = sw.y;  // row window (complex64 layout)
+    const float rseg = 1.0f / (float)max(nseg, 1), rrow = 1.0f / (float)max(nrow, 1);
+    const GAS int *pwl = gp(B.pw_lds) + seg0;
+    const GAS int *rows = gp(B.rows) + r0;
     constexpr int KG = gather_round<T>();
+    // the first row-pass line's row index, loaded now so that its latency
+    // overlaps the scatter
+    const int nr = np * nrow;
+    const int lr0 = min(wave * 32 + (lane & 31), max(nr - 1, 0));
+    const int pr0 = div_rcp(lr0, nrow, rrow);
+    const int yr0 = nr > 0 ? rows[lr0 - pr0 * nrow] : 0;
     // the planes' sphere points, KG per thread in one round (a whole
     // plane at the configurations' sizes): the first round's loads go out
     // before the plane is cleared, so their latency overlaps the clearing
     const int tot = np * nseg;
+    const auto rpsi = make_rsrc(B.psi + (size_t)s0 * B.npw + seg0,
+                                (unsigned)(((size_t)(np - 1) * B.npw + nseg) * sizeof(C)));
     int ldx[KG];
     T vr[KG], vi[KG];
-    if (tot > 0) scatter_round<T, KG, N2, Cfg::PLANE>(psi, pwyz, B.npw, s0, seg0, nseg, tot, tid, nth, ldx, vr, vi);
-    for (int i = tid; i < np * Cfg::PLANE; i += nth) lds[i] = C{0, 0};
+    if (tot > 0) scatter_round<T, KG>(rpsi, pwl, B.npw, nseg, rseg, tot, tid, nth, pstr, ldx, vr, vi);
+    for (int i = tid; i < np * pstr; i += nth) lds[i] = C{0, 0};
     __syncthreads();
     for (int i0 = tid; i0 < tot; i0 += KG * nth) {
-      if (i0 != tid) scatter_round<T, KG, N2, Cfg::PLANE>(psi, pwyz, B.npw, s0, seg0, nseg, tot, i0, nth, ldx, vr, vi);
+      if (i0 != tid) scatter_round<T, KG>(rpsi, pwl, B.npw, nseg, rseg, tot, i0, nth, pstr, ldx, vr, vi);
 #pragma unroll
       for (int u = 0; u < KG; ++u)
         if (i0 + u * nth < tot) lds[ldx[u]] = C{vr[u], vi[u]};
     }
     __syncthreads();
     // rows along i2 (sphere rows only), a lane pair per row
-    const GAS int *rows = gp(B.rows);
-    for (int g = wave; g * 32 < np * nrow; g += Cfg::WAVES) {
+    for (int g = wave; g * 32 < nr; g += Cfg::WAVES) {
       const int li = g * 32 + (lane & 31);
-      const bool ok = li < np * nrow;
-      const int lc = ok ? li : np * nrow - 1;
-      const int p = lc / nrow;
-      const int y = rows[r0 + lc - p * nrow];
-      plane_row_split<C, N2, +1>(lds + p * Cfg::PLANE + y * (N2 + 1), upper, ok);
+      const bool ok = li < nr;
+      const int lc = ok ? li : nr - 1;
+      const int p = div_rcp(lc, nrow, rrow);
+      const int y = g == wave ? yr0 : rows[lc - p * nrow];
+      plane_row_split<C, N2, +1>(lds + p * pstr + plane_row<T, N1>(y, wlo) * (N2 + 1), upper, ok);
     }
     __syncthreads();
     // columns along i1 to the plane slots, a lane pair per column; the np
@@ -280,10 +325,16 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
       const bool ok = li < np * N2;
       const int lc = ok ? li : np * N2 - 1;
       const int p = lc / N2, z = lc - p * N2;
-      const C *col = lds + p * Cfg::PLANE + z;
+      // rows y = 2n + half; in the window layout rows outside the window are zero
+      // (row(y) = y + const within each half of the FFT order: one base per half)
+      const int bA = p * pstr + z + plane_row<T, N1>(half, wlo) * (N2 + 1);  // rows y < N1/2
+      const int bB = bA - (plane_window<T>() ? N1 * (N2 + 1) : 0);          // rows y >= N1/2
       C v[N1 / 2];
 #pragma unroll
-      for (int n = 0; n < N1 / 2; ++n) v[n] = col[(2 * n + half) * (N2 + 1)];
+      for (int n = 0; n < N1 / 2; ++n) {
+        const int y = 2 * n + half;
+        v[n] = row_in_plane<T, N1>(y, wlo, nwin) ? lds[(y < N1 / 2 ? bA : bB) + 2 * n * (N2 + 1)] : C{0, 0};
+      }
       fft_split<N1, +1, C>::dit(v, upper);
       // lanes past the last column store out of range (dropped), no branch
       const unsigned vo = ok ? (unsigned)(((size_t)p * pstride + z + half * (N1 / 2) * N2) * sizeof(C)) : kOOB;
@@ -294,12 +345,14 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
   }
 }
 
+// complex64 up to 72-point lines: at most 168 VGPRs, three waves per SIMD
+// (longer lines would spill)
 template <typename T, int N1, int N2>
-__global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                       int total, int pb, T c1r, T c1i, T c2r, T c2i) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && N1 <= 72 ? 3 : 1)))
+k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, int total, int pb, int pstr, T c1r,
+                T c1i, T c2r, T c2i) {
   using C = cplx<T>;
   using Cfg = PlaneCfg<T, N1, N2>;
-  const int P = pb;  // planes per work-group, 1 <= pb <= Cfg::P (launch shape)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -309,103 +362,67 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
   {  // one work item per work-group: the grid is the work count
     const int wg = blockIdx.x;
     if (wg >= total) return;
-    const int b = find_block(prefix, nblocks, wg);
-    const SphereBlock<T> B = blocks[b];
-    // work order: consecutive work-groups take consecutive slots (adjacent
-    // planes in memory) of the same states
-    const int local = wg - gp(prefix)[b];
-    const int sgrp = local / B.nslot;
-    const int slot = local - sgrp * B.nslot;
-    const int s0 = sgrp * P;
-    const int np = min(P, B.nst - s0);
-    const V4<int> sg = ld4<int>(B.slot_seg, slot);
+    SphereBlock<T> B;
+    const PlaneItem it = plane_item(blocks, prefix, nblocks, wg, pb, B);
+    const int slot = it.slot, s0 = it.s0, np = it.np;
+    const V4<int> sg = ld4<int>(B.slot_seg, 2 * slot);
     const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
     if (nseg == 0) return;  // uniform: nothing of this plane is on the sphere
-    const GAS int *pwyz = gp(B.pw_yz);
-    const int tot = np * nseg;
+    const V4<int> sw = ld4<int>(B.slot_seg, 2 * slot + 1);
+    const int wlo = sw.x, nwin = sw.y;  // row window (complex64 layout)
+    const float rseg = 1.0f / (float)nseg, rrow = 1.0f / (float)nrow;
+    const GAS int *pwl = gp(B.pw_lds) + seg0;
+    const GAS int *rows = gp(B.rows) + r0;
+    const int tot = np * nseg, nr = np * nrow;
     constexpr int KG = gather_round<T>();
-    // complex64: the first gather round's point indices are loaded now, so the
-    // gather after the transforms has no dependent load (complex128 has no
-    // registers to spare and loads them in the gather)
-    constexpr bool kEarlyYz = sizeof(T) == 4;
-    int yz0[KG];
-    if constexpr (kEarlyYz) {
+    // the first row-pass line's row index, loaded now
+    const int lr0 = min(wave * 32 + (lane & 31), nr - 1);
+    const int pr0 = div_rcp(lr0, nrow, rrow);
+    const int yr0 = rows[lr0 - pr0 * nrow];
+    // columns along i1 from the plane slots, then the sphere rows along i2,
+    // a lane pair per line; the np planes are slot `slot` of states s0..s0+np-1
+    const size_t pstride = (size_t)B.nslot * (N1 * N2);
+    const auto rt = make_rsrc(B.t1 + (size_t)s0 * pstride + (size_t)slot * (N1 * N2),
+                              (unsigned)(((size_t)(np - 1) * pstride + N1 * N2) * sizeof(C)));
+    for (int g = wave; g * 32 < np * N2; g += Cfg::WAVES) {
+      const int li = g * 32 + (lane & 31);
+      const bool ok = li < np * N2;
+      const int lc = ok ? li : np * N2 - 1;
+      const int p = lc / N2, z = lc - p * N2;
+      const unsigned vo = (unsigned)(((size_t)p * pstride + z + half * N2) * sizeof(C));
+      C v[N1 / 2];
 #pragma unroll
-      for (int u = 0; u < KG; ++u) {
-        const int i = min(tid + u * nth, tot - 1);
-        yz0[u] = pwyz[seg0 + i % nseg];
-      }
-    }
-    if constexpr (Cfg::SPLIT_F) {
-      // columns along i1 from the plane slots, then the sphere rows along i2,
-      // a lane pair per line; the np planes are slot `slot` of states s0..s0+np-1
-      const size_t pstride = (size_t)B.nslot * (N1 * N2);
-      const auto rt = make_rsrc(B.t1 + (size_t)s0 * pstride + (size_t)slot * (N1 * N2),
-                                (unsigned)(((size_t)(np - 1) * pstride + N1 * N2) * sizeof(C)));
-      for (int g = wave; g * 32 < np * N2; g += Cfg::WAVES) {
-        const int li = g * 32 + (lane & 31);
-        const bool ok = li < np * N2;
-        const int lc = ok ? li : np * N2 - 1;
-        const int p = lc / N2, z = lc - p * N2;
-        const unsigned vo = (unsigned)(((size_t)p * pstride + z + half * N2) * sizeof(C));
-        C v[N1 / 2];
+      for (int n = 0; n < N1 / 2; ++n) bload(rt, vo, (unsigned)(2 * n * N2 * sizeof(C)), v[n]);
+      fft_split<N1, -1, C>::dit(v, upper);
+      // v[k] is row y = k + half N1/2: the lower lanes hold y < N1/2, the upper
+      // lanes y >= N1/2; in the window layout rows outside the window are not stored
+      const int sb = p * pstr + z + plane_row<T, N1>(half * (N1 / 2), wlo) * (N2 + 1);
+      if (ok) {
 #pragma unroll
-        for (int n = 0; n < N1 / 2; ++n) bload(rt, vo, (unsigned)(2 * n * N2 * sizeof(C)), v[n]);
-        fft_split<N1, -1, C>::dit(v, upper);
-        C *col = lds + p * Cfg::PLANE + z;
-        if (ok) {
-#pragma unroll
-          for (int k = 0; k < N1 / 2; ++k) col[(k + half * (N1 / 2)) * (N2 + 1)] = v[k];
+        for (int k = 0; k < N1 / 2; ++k) {
+          const int y = k + half * (N1 / 2);
+          if (row_in_plane<T, N1>(y, wlo, nwin)) lds[sb + k * (N2 + 1)] = v[k];
         }
       }
-      __syncthreads();
-      const GAS int *rows = gp(B.rows);
-      for (int g = wave; g * 32 < np * nrow; g += Cfg::WAVES) {
-        const int li = g * 32 + (lane & 31);
-        const bool ok = li < np * nrow;
-        const int lc = ok ? li : np * nrow - 1;
-        const int p = lc / nrow;
-        const int y = rows[r0 + lc - p * nrow];
-        plane_row_split<C, N2, -1>(lds + p * Cfg::PLANE + y * (N2 + 1), upper, ok);
-      }
-      __syncthreads();
-    } else {
-      // the same with one lane per line
-      const GAS C *t1 = gp(B.t1);
-      for (int i = tid; i < np * N2; i += nth) {
-        const int p = i / N2, z = i - p * N2;
-        const GAS C *in = t1 + ((size_t)(s0 + p) * B.nslot + slot) * (N1 * N2) + z;
-        C x[N1];
-#pragma unroll
-        for (int y = 0; y < N1; ++y) x[y] = in[y * N2];
-        fft_line<N1, -1, C>::run(x);
-        C *col = lds + p * Cfg::PLANE + z;
-#pragma unroll
-        for (int y = 0; y < N1; ++y) col[y * (N2 + 1)] = x[y];
-      }
-      __syncthreads();
-      const GAS int *rows = gp(B.rows);
-      for (int i = tid; i < np * nrow; i += nth) {
-        const int p = i / nrow;
-        const int y = rows[r0 + i - p * nrow];
-        C *row = lds + p * Cfg::PLANE + y * (N2 + 1);
-        C x[N2];
-#pragma unroll
-        for (int z = 0; z < N2; ++z) x[z] = row[z];
-        fft_line<N2, -1, C>::run(x);
-#pragma unroll
-        for (int z = 0; z < N2; ++z) row[z] = x[z];
-      }
-      __syncthreads();
     }
+    __syncthreads();
+    for (int g = wave; g * 32 < nr; g += Cfg::WAVES) {
+      const int li = g * 32 + (lane & 31);
+      const bool ok = li < nr;
+      const int lc = ok ? li : nr - 1;
+      const int p = div_rcp(lc, nrow, rrow);
+      const int y = g == wave ? yr0 : rows[lc - p * nrow];
+      plane_row_split<C, N2, -1>(lds + p * pstr + plane_row<T, N1>(y, wlo) * (N2 + 1), upper, ok);
+    }
+    __syncthreads();
     // keep the gather's index arithmetic out of the transform phases' live range
     __builtin_amdgcn_sched_barrier(0);
     // the sphere values of this work-group's states s0 .. s0+np-1 through
-    // buffer resources based at state s0: one 32-bit offset per point, and
-    // points past the end use an offset beyond the buffer (loads give zero,
-    // stores are dropped) instead of a branch
-    const size_t base = (size_t)s0 * B.npw;
-    const unsigned nb = (unsigned)((size_t)np * B.npw * sizeof(C));
+    // buffer resources based at point seg0 of state s0: one 32-bit offset per
+    // point, and points past the end use an offset beyond the buffer (loads
+    // give zero, stores are dropped) instead of a branch
+    const size_t base = (size_t)s0 * B.npw + seg0;
+    const unsigned nb = (unsigned)(((size_t)(np - 1) * B.npw + nseg) * sizeof(C));
     const auto rpsi = make_rsrc(B.psi + base, nb);
     const auto rh = make_rsrc(B.hpsi + base, nb);
     // accumulators in the working precision, or complex128 (acc_wide: an FP64
@@ -415,8 +432,8 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
                                B.acc1 ? nb << wsh : 0u);
     const auto ra2 = make_rsrc(B.acc2 ? (const void *)((const char *)B.acc2 + (base * sizeof(C) << wsh)) : (const void *)(B.hpsi + base),
                                B.acc2 ? nb << wsh : 0u);
-    const auto rk = make_rsrc(B.kin ? (const void *)B.kin : (const void *)B.psi,
-                              B.kin ? (unsigned)(B.npw * sizeof(T)) : 0u);
+    const auto rk = make_rsrc(B.kin ? (const void *)(B.kin + seg0) : (const void *)B.psi,
+                              B.kin ? (unsigned)(nseg * sizeof(T)) : 0u);
     for (int i0 = tid; i0 < tot; i0 += KG * nth) {
       // KG sphere points per thread and round: loads first, then the updates
       unsigned oo[KG];
@@ -426,16 +443,12 @@ __global__ void __launch_bounds__(256) k_plane_forward(const SphereBlock<T> *blo
       for (int u = 0; u < KG; ++u) {
         const int i = i0 + u * nth;
         const int ic = i < tot ? i : tot - 1;
-        const int p = ic / nseg, j = ic - p * nseg;
-        int yz;
-        if constexpr (kEarlyYz) yz = i0 == tid ? yz0[u] : pwyz[seg0 + j];
-        else yz = pwyz[seg0 + j];
-        const int y = yz / N2, z = yz - y * N2;
-        oo[u] = i < tot ? (unsigned)(((size_t)p * B.npw + seg0 + j) * sizeof(C)) : kOOB;
-        outv[u] = lds[p * Cfg::PLANE + y * (N2 + 1) + z];
+        const int p = div_rcp(ic, nseg, rseg), j = ic - p * nseg;
+        oo[u] = i < tot ? (unsigned)((p * B.npw + j) * (int)sizeof(C)) : kOOB;
+        outv[u] = lds[p * pstr + pwl[j]];
         if (B.kin) {
           bload(rpsi, oo[u], 0u, cv[u]);
-          bload(rk, (unsigned)((seg0 + j) * sizeof(T)), 0u, kv[u]);
+          bload(rk, (unsigned)(j * (int)sizeof(T)), 0u, kv[u]);
         }
       }
 #pragma unroll
@@ -1341,10 +1354,25 @@ void build_kb_tables(inqx_ham_s *h) {
 
 void ensure_basis_device(inqx_ham_s *h, Basis &b) {
   hipStream_t st = h->ctx->c.stream;
-  if (!b.d_pw_yz.p) {
+  if (!b.d_pw_lw.p) {
+    // LDS offset of every sphere point in its plane: row window layout
+    // (complex64) and whole-plane layout (complex128)
+    const Grid &g = h->g;
+    const int n2p = g.n[2] + 1;
+    std::vector<int> lw(b.npw), lf(b.npw);
+    for (int sl = 0; sl < b.nslot; ++sl) {
+      const int *ss = &b.slot_seg[8 * sl];
+      for (int j = 0; j < ss[1]; ++j) {
+        const long q = ss[0] + j;
+        const int y = b.pw_yz[q] / g.n[2], z = b.pw_yz[q] - y * g.n[2];
+        lw[q] = (Grid::sym(y, g.n[1]) - ss[4] + 1) * n2p + z;
+        lf[q] = y * n2p + z;
+      }
+    }
     upload(b.d_slot_seg, b.slot_seg, st);
     upload(b.d_rows, b.rows, st);
-    upload(b.d_pw_yz, b.pw_yz, st);
+    upload(b.d_pw_lw, lw, st);
+    upload(b.d_pw_lf, lf, st);
     INQX_HIP(hipStreamSynchronize(st));
   }
   if (b.kin_A[0] != h->A[0] || b.kin_A[1] != h->A[1] || b.kin_A[2] != h->A[2]) {
@@ -1429,6 +1457,7 @@ struct LaunchData {
   int total_line = 0, ntile_line = 0;  // line passes of the transform modes
   unsigned long long last_use = 0;     // lookup counter at the most recent use (eviction order)
   int plane_batch = 1;                 // planes (states) per plane-pass work-group
+  int plane_pstr = 0;                  // LDS elements per plane: rows (the longest row window + 2, or n1) x (n2 + 1)
   bool observables = false;            // the blocks point at the bases' projection and dipole buffers
   XLds xl_apply{}, xl_project{};       // x-pass LDS layouts (byte offsets)
   int xlds_apply = 0, xlds_project = 0;  // x-pass LDS bytes per work-group
@@ -1593,7 +1622,7 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   set_plane_attr<T, N12, N12>(h->ctx->c.device);
   const int cu = h->ctx->c.num_cu;
   const int grid_plane = ld.total_plane;  // one item per work-group
-  const int plane_lds = ld.plane_batch * Cfg::PLANE_BYTES;
+  const int plane_lds = ld.plane_batch * ld.plane_pstr * (int)sizeof(cplx<T>);
   Ctx &cx = h->ctx->c;
   const bool f64 = std::is_same<T, double>::value;
   if (ld.mode == 2) {  // real space -> sphere
@@ -1606,8 +1635,8 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
     INQX_HIP(hipGetLastError());
     {
       ProfScope ps(cx, f64 ? "plane_forward_f64" : "plane_forward_f32");
-      hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS_F), plane_lds, st, blk,
-                         pref_plane, nblocks, ld.total_plane, ld.plane_batch, (T)0, (T)0, (T)0, (T)0);
+      hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), plane_lds, st, blk,
+                         pref_plane, nblocks, ld.total_plane, ld.plane_batch, ld.plane_pstr, (T)0, (T)0, (T)0, (T)0);
     }
     INQX_HIP(hipGetLastError());
     return;
@@ -1615,7 +1644,7 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   {
   ProfScope ps(cx, f64 ? "plane_inverse_f64" : "plane_inverse_f32");
   hipLaunchKernelGGL((k_plane_inverse<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), plane_lds, st, blk,
-                     pref_plane, nblocks, ld.total_plane, ld.plane_batch);
+                     pref_plane, nblocks, ld.total_plane, ld.plane_batch, ld.plane_pstr);
   }
   INQX_HIP(hipGetLastError());
   if (ld.mode == 1) {  // sphere -> real space
@@ -1651,8 +1680,8 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   INQX_HIP(hipGetLastError());
   {
     ProfScope ps(cx, f64 ? "plane_forward_f64" : "plane_forward_f32");
-    hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS_F), plane_lds, st, blk,
-                       pref_plane, nblocks, ld.total_plane, ld.plane_batch, (T)cf[0], (T)cf[1], (T)cf[2], (T)cf[3]);
+    hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), plane_lds, st, blk,
+                       pref_plane, nblocks, ld.total_plane, ld.plane_batch, ld.plane_pstr, (T)cf[0], (T)cf[1], (T)cf[2], (T)cf[3]);
   }
   INQX_HIP(hipGetLastError());
 }
@@ -1800,7 +1829,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
       d.dps = B.dps[prec].as<cplx<T>>();
       d.slot_seg = B.d_slot_seg.as<int>();
       d.rows = B.d_rows.as<int>();
-      d.pw_yz = B.d_pw_yz.as<int>();
+      d.pw_lds = plane_window<T>() ? B.d_pw_lw.as<int>() : B.d_pw_lf.as<int>();
       d.kin = mode == 2 ? nullptr : prec == INQX_FP64 ? (const T *)B.d_kin64.p : (const T *)B.d_kin32.p;
       d.rs = mode != 0 ? reinterpret_cast<cplx<T> *>(opts->rs[i]) : nullptr;
       d.rs_layout = opts->rs_layout;
@@ -1829,6 +1858,9 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
         px[i + 1] = px[i] + ld->ntile_line * ((nst[i] + 3) / 4);
       pr[i + 1] = pr[i] + (std::max(1, h->kb.natoms) * nst[i] + 255) / 256;
     }
+    int nwin = 1;
+    for (int i = 0; i < nblocks; ++i) nwin = std::max(nwin, bases[i]->nwin_max);
+    ld->plane_pstr = (plane_window<T>() ? nwin + 2 : g.n[1]) * (g.n[2] + 1);
     ld->total_plane = pp[nblocks];
     ld->total_x = px[nblocks];
     ld->total_line = px[nblocks];

@@ -187,9 +187,10 @@ template <typename T> struct SphereBlock {
   C *t1;               // [nslot][nst][n1*n2] plane intermediates (in place for the x pass)
   C *partial;          // [kb nslot][kMaxNlm][nst]
   C *dp;               // [natoms][kMaxNlm][nst]
-  const int *slot_seg; // [nslot] int4: sphere segment start, length, row list start, row count
+  const int *slot_seg; // [nslot] 2 x int4: sphere segment start, length, row list start, row count;
+                       // row window start (symmetric i1), window length, 0, 0
   const int *rows;     // rows (i1) holding sphere points, per slot
-  const int *pw_yz;    // [npw] i1*n2 + i2
+  const int *pw_lds;   // [npw] LDS offset of each sphere point in its plane (plane-pass layout)
   const T *kin;        // [npw] 1/2 |G+k|^2
   const T *vloc;       // [N] V/N of the block's spin
   C *acc1, *acc2;      // Taylor accumulators acc += c (H psi), or null
@@ -235,9 +236,11 @@ struct Basis {
   std::vector<int> gidx;      // flattened FFT index
   // plane slots: slot j <-> symmetric plane index m_lo + j, contiguous range
   int m_lo = 0, nslot = 0;
-  std::vector<int> slot_seg;  // [nslot][4] segment start, length, row start, row count
+  std::vector<int> slot_seg;  // [nslot][8] segment start, length, row start, row count, row window start
+                              // (symmetric i1 of the plane's first row), window length, 0, 0
+  int nwin_max = 0;           // longest row window of a plane
   std::vector<int> rows, pw_yz;
-  DevMem d_slot_seg, d_rows, d_pw_yz, d_kin64, d_kin32;
+  DevMem d_slot_seg, d_rows, d_pw_lw, d_pw_lf, d_kin64, d_kin32;
   DevMem d_g2h;  // (1/2)|G|^2 per sphere point, FP64, no k shift (preconditioner)
   double kin_A[3] = {1e300, 0, 0};  // A for which d_kin* were built
   // KB phases exp(+i (k+A).r_e) per KB entry e (FP64 and FP32 copies), built for
