@@ -150,7 +150,7 @@ def time_applies(ctx, gh, b, d_in, d_out, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def hbm_copy(ctx, nbytes=1 << 30, reps=10):
+def hbm_copy(ctx, nbytes=1 << 31, reps=10):
     from inq_amd import DeviceArray, lib
     a = DeviceArray(ctx, (nbytes // 8,), np.complex64)
     b = DeviceArray(ctx, (nbytes // 8,), np.complex64)
