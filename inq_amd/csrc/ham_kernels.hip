@@ -508,7 +508,7 @@ k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, in
 }
 
 // -------------------------------------------------------------------------------------------
-// x passes: one lane per line (i1,i2), 64 consecutive lines per wave, one state per wave
+// x passes: a lane pair per line (i1,i2), 32 consecutive lines (a tile) per wave, one state per wave
 // -------------------------------------------------------------------------------------------
 
 // symmetric plane index of FFT index i (compile time)
