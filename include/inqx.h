@@ -382,21 +382,41 @@ int inqx_density_add(inqx_ham ham, inqx_basis basis, int nst, int precision, con
  * gshift_cart = q (cartesian). */
 int inqx_poisson(inqx_ham ham, int n, int precision, void *rho, const double gshift_cart[3], double zeroterm);
 /* exchange_operator::direct (src/hamiltonian/exchange_operator.hpp:124-186)
- * at the Gamma point, in sphere form: exxphi += mask(FFT(sum_j f occ_j hf_j
- * Poisson(conj(hf_j) phi_i))), f = -exx_coeff scale / 2, phi_i = to_real of
- * the sphere states (j with |occ_j| < 1e-10 skipped).  phi [nst][npw], hf
- * [nhf][npw] and exxphi [nst][npw] on the basis' sphere; occ host [nhf].
- * complex128 (INQX_FP64) is offered up to a 64^3 grid; larger grids return
- * INQX_ERR_UNSUPPORTED for it (the same holds for inqx_ace_build). */
+ * in sphere form, the hf states at the basis' own k-point: exxphi +=
+ * mask(FFT(sum_j f occ_j hf_j Poisson(conj(hf_j) phi_i))), f = -exx_coeff
+ * scale / 2, phi_i = to_real of the sphere states (the periodic part on the
+ * grid; j with |occ_j| < 1e-10 skipped).  phi [nst][npw], hf [nhf][npw] and
+ * exxphi [nst][npw] on the basis' sphere; occ host [nhf].  complex128
+ * (INQX_FP64) is offered up to a 64^3 grid; larger grids return
+ * INQX_ERR_UNSUPPORTED for it (the same holds for every exchange entry point). */
 int inqx_exchange_direct(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *phi, int nhf,
                          const void *hf, const double *occ, double exx_coeff, double scale, double zeroterm,
                          void *exxphi);
+/* the same with the occupied states of several k-points, as exchange_operator
+ * keeps them (exchange_operator.hpp:76-91): block b holds nhf[b] states hf[b]
+ * on the sphere of hf_bases[b] (k-point q_b, a basis of the same
+ * Hamiltonian), occupations occ[b] (host) and the singularity value of its
+ * k-point zeroterms[b]; each pair solve is poisson::in_place(rho_ij,
+ * -k + q_b, zeroterms[b]) (exchange_operator.hpp:151) with k the k-point of
+ * basis. */
+int inqx_exchange_direct_kpts(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *phi, int nblocks,
+                              const inqx_basis *hf_bases, const int *nhf, const void *const *hf,
+                              const double *const *occ, const double *zeroterms, double exx_coeff, double scale,
+                              void *exxphi);
 /* exchange_operator::update's ACE build (exchange_operator.hpp:93-117):
  * W = direct(phi, -1), M = vol_el W^H phi (FP64), E_x = -1/2 sum occ_i
  * Re M_ii, M = L L^H, xi = W L^{-H}.  xi [nst][npw] on the sphere (the
  * sphere projection of the reference's real-space xi); *energy = E_x. */
 int inqx_ace_build(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *phi, const double *occ,
                    double exx_coeff, double zeroterm, void *xi, double *energy);
+/* the ACE build of one (k-point, spin) block against the occupied states of
+ * every k-point (exchange_operator.hpp:97-114: W = direct(phi, -1) over all
+ * stored orbitals), hf blocks as in inqx_exchange_direct_kpts; occ: host
+ * [nst], the block's own occupations for E_x */
+int inqx_ace_build_kpts(inqx_ham ham, inqx_basis basis, int nst, int precision, const void *phi, const double *occ,
+                        int nblocks, const inqx_basis *hf_bases, const int *nhf, const void *const *hf,
+                        const double *const *hf_occ, const double *zeroterms, double exx_coeff, void *xi,
+                        double *energy);
 /* exchange_operator::ace (exchange_operator.hpp:217-231):
  * exxphi <- exxphi - xi (vol_el xi^H psi), xi [nxi][npw], psi/exxphi [nst][npw] */
 int inqx_ace_apply(inqx_ctx ctx, int precision, int nxi, int nst, long npw, double vol_el, const void *xi,

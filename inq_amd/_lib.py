@@ -65,6 +65,10 @@ SIGNATURES = {
     "inqx_exchange_direct": (C.c_int, [P, P, C.c_int, C.c_int, P, C.c_int, P, DP, C.c_double, C.c_double,
                                        C.c_double, P]),
     "inqx_ace_build": (C.c_int, [P, P, C.c_int, C.c_int, P, DP, C.c_double, C.c_double, P, DP]),
+    "inqx_exchange_direct_kpts": (C.c_int, [P, P, C.c_int, C.c_int, P, C.c_int, PP, IP, PP, PP, DP, C.c_double,
+                                            C.c_double, P]),
+    "inqx_ace_build_kpts": (C.c_int, [P, P, C.c_int, C.c_int, P, DP, C.c_int, PP, IP, PP, PP, DP, C.c_double, P,
+                                      DP]),
     "inqx_ace_apply": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_long, C.c_double, P, P, P]),
     "inqx_ham_set_exchange_workspace": (C.c_int, [P, C.c_size_t]),
     "inqx_ham_autotune": (C.c_int, [P, P, C.c_int, C.c_int, IP, DP]),

@@ -1223,9 +1223,22 @@ int inqx_atomic_field_add(inqx_ham h, int natoms, const double *positions, const
   API_END(ctx)
 }
 
-// exchange_operator::direct in sphere form: out (+)= mask(FFT(sum_j f occ_j hf_j Poisson(conj(hf_j) phi_i)))
-static void exchange_direct_impl(inqx_ham h, Basis &B, int nst, int prec, const void *phi, int nhf, const void *hf,
-                                 const double *occ, double coeff, double scale, double zeroterm, void *out,
+// the occupied (hf) states of one k-point: exchange_operator keeps every
+// k-point's orbitals with their k-point and its singularity value
+// (exchange_operator.hpp:76-91,151)
+struct HfBlock {
+  Basis *B;            // the hf states' sphere (their k-point)
+  int nhf;
+  const void *hf;      // [nhf][npw of B]
+  const double *occ;   // host [nhf]
+  double zeroterm;     // Poisson value at G + q = 0 for this k-point
+};
+
+// exchange_operator::direct in sphere form (exchange_operator.hpp:135-186):
+// out (+)= mask(FFT(sum_blocks sum_j f occ_j hf_j Poisson_{-k+q}(conj(hf_j) phi_i)))
+// with phi on basis B (k-point k) and each hf block at its own k-point q
+static void exchange_direct_impl(inqx_ham h, Basis &B, int nst, int prec, const void *phi,
+                                 const std::vector<HfBlock> &blocks, double coeff, double scale, void *out,
                                  bool accumulate) {
   Ctx &cx = h->ctx->c;
   INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "exchange needs a cubic grid of a built size");
@@ -1237,34 +1250,51 @@ static void exchange_direct_impl(inqx_ham h, Basis &B, int nst, int prec, const 
              "complex128 exchange is not offered above 64^3 (use complex64, or the CPU oracle as the FP64 reference)");
   const size_t per = (prec == INQX_FP64 ? 16 : 8) * (size_t)h->g.N;
   const double factor = -0.5 * scale * coeff;
-  std::vector<int> jl;
-  std::vector<double> cf(nhf);
-  for (int j = 0; j < nhf; ++j) {
-    cf[j] = factor * occ[j];
-    if (std::fabs(occ[j]) >= 1e-10) jl.push_back(j);  // exchange_operator.hpp:144
-  }
   h->exx_phi.ensure(per * nst);
   transform_impl(h, B, nst, prec, 1, phi, h->exx_phi.p, INQX_STATE_MAJOR, false, nullptr);
-  const void *hf_rs = h->exx_phi.p;
-  if (!(hf == phi && nhf == nst)) {
-    h->exx_hf.ensure(per * nhf);
-    transform_impl(h, B, nhf, prec, 1, hf, h->exx_hf.p, INQX_STATE_MAJOR, false, nullptr);
-    hf_rs = h->exx_hf.p;
-  }
   h->exx_w.ensure(per * nst);
   launch_fill_zero(cx, h->exx_w.p, per * nst);
-  if (!jl.empty()) {
-    upload(h->exx_jlist, jl, cx.stream);
-    upload(h->exx_coef, cf, cx.stream);
-    INQX_HIP(hipStreamSynchronize(cx.stream));
-    size_t budget = h->exx_pair_bytes;
-    if (budget == 0) {  // a quarter of what is free, at most 64 GiB
-      size_t fr = 0, tot = 0;
-      INQX_HIP(hipMemGetInfo(&fr, &tot));
-      budget = std::min<size_t>(fr / 4 + h->exx_t.bytes, size_t(64) << 30);
+  // workspaces sized for the largest block before any pair launch: a buffer
+  // that grew while earlier launches still read it would be freed under them
+  int maxnhf = 1, maxhf_rs = 0;
+  for (const HfBlock &hb : blocks) {
+    maxnhf = std::max(maxnhf, hb.nhf);
+    if (!(hb.B == &B && hb.hf == phi && hb.nhf == nst)) maxhf_rs = std::max(maxhf_rs, hb.nhf);
+  }
+  if (maxhf_rs > 0) h->exx_hf.ensure(per * maxhf_rs);
+  h->exx_jlist.ensure(sizeof(int) * maxnhf);
+  h->exx_coef.ensure(sizeof(double) * maxnhf);
+  size_t budget = h->exx_pair_bytes;
+  if (budget == 0) {  // a quarter of what is free, at most 64 GiB
+    size_t fr = 0, tot = 0;
+    INQX_HIP(hipMemGetInfo(&fr, &tot));
+    budget = std::min<size_t>(fr / 4 + h->exx_t.bytes, size_t(64) << 30);
+  }
+  const size_t njbudget = std::max<size_t>(1, std::min<size_t>(maxnhf, budget / (per * nst)));
+  h->exx_t.ensure(per * nst * njbudget);
+  for (const HfBlock &hb : blocks) {
+    std::vector<int> jl;
+    std::vector<double> cf(hb.nhf);
+    for (int j = 0; j < hb.nhf; ++j) {
+      cf[j] = factor * hb.occ[j];
+      if (std::fabs(hb.occ[j]) >= 1e-10) jl.push_back(j);  // exchange_operator.hpp:144
     }
-    const size_t njmax = std::max<size_t>(1, std::min<size_t>(jl.size(), budget / (per * nst)));
-    h->exx_t.ensure(per * nst * njmax);
+    if (jl.empty()) continue;
+    const void *hf_rs = h->exx_phi.p;
+    if (!(hb.B == &B && hb.hf == phi && hb.nhf == nst)) {
+      transform_impl(h, *hb.B, hb.nhf, prec, 1, hb.hf, h->exx_hf.p, INQX_STATE_MAJOR, false, nullptr);
+      hf_rs = h->exx_hf.p;
+    }
+    // same stream: the copies wait for the previous block's pair launches
+    INQX_HIP(hipMemcpyAsync(h->exx_jlist.p, jl.data(), sizeof(int) * jl.size(), hipMemcpyHostToDevice, cx.stream));
+    INQX_HIP(hipMemcpyAsync(h->exx_coef.p, cf.data(), sizeof(double) * cf.size(), hipMemcpyHostToDevice, cx.stream));
+    INQX_HIP(hipStreamSynchronize(cx.stream));  // jl and cf are released at the end of the iteration
+    const size_t njmax = std::min<size_t>(njbudget, jl.size());
+    // poisson::in_place(rhoij, -phi.kpoint() + kpt[jj], ...) (exchange_operator.hpp:151):
+    // the covariant shift q - k in cartesian components
+    double dq[3], gs[3];
+    for (int d = 0; d < 3; ++d) dq[d] = hb.B->kcov[d] - B.kcov[d];
+    h->g.cov_to_cart(dq, gs);
     for (size_t c0 = 0; c0 < jl.size(); c0 += njmax) {
       PairArgs a;
       a.phi = h->exx_phi.p;
@@ -1275,7 +1305,8 @@ static void exchange_direct_impl(inqx_ham h, Basis &B, int nst, int prec, const 
       a.coef = h->exx_coef.as<double>();
       a.nst = nst;
       a.nj = (int)std::min(njmax, jl.size() - c0);
-      a.zeroterm = zeroterm;
+      for (int d = 0; d < 3; ++d) a.gshift[d] = gs[d];
+      a.zeroterm = hb.zeroterm;
       launch_pairs(h, prec, a);
     }
   }
@@ -1287,32 +1318,30 @@ static void check_exchange(inqx_ham h, inqx_basis bs, int nst, int precision) {
   INQX_CHECK(!h->ctx->c.host_only, INQX_ERR_STATE, "host-only context");
   INQX_CHECK(bs->b.ham == h, INQX_ERR_ARG, "basis belongs to another Hamiltonian");
   INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
-  const double *k = bs->b.kcov;
-  INQX_CHECK(k[0] == 0.0 && k[1] == 0.0 && k[2] == 0.0 && h->A[0] == 0.0 && h->A[1] == 0.0 && h->A[2] == 0.0,
-             INQX_ERR_UNSUPPORTED, "exchange is built for the Gamma point only");
 }
 
-int inqx_exchange_direct(inqx_ham h, inqx_basis bs, int nst, int precision, const void *phi, int nhf, const void *hf,
-                         const double *occ, double exx_coeff, double scale, double zeroterm, void *exxphi) {
-  inqx_ctx ctx = h ? h->ctx : nullptr;
-  API_BEGIN
-  check_exchange(h, bs, nst, precision);
-  INQX_CHECK(phi && hf && occ && exxphi && nhf > 0, INQX_ERR_ARG, "null argument");
-  INQX_CHECK(exxphi != phi && exxphi != hf, INQX_ERR_ARG, "exxphi must not alias the inputs");
-  exchange_direct_impl(h, bs->b, nst, precision, phi, nhf, hf, occ, exx_coeff, scale, zeroterm, exxphi, true);
-  API_END(ctx)
+// the hf blocks of the k-point forms: bases of the same Hamiltonian
+static std::vector<HfBlock> hf_blocks(inqx_ham h, int nblocks, const inqx_basis *bases, const int *nhf,
+                                      const void *const *hf, const double *const *occ, const double *zeroterms) {
+  INQX_CHECK(nblocks > 0 && bases && nhf && hf && occ && zeroterms, INQX_ERR_ARG, "null hf block arrays");
+  std::vector<HfBlock> v(nblocks);
+  for (int b = 0; b < nblocks; ++b) {
+    INQX_CHECK(bases[b] && bases[b]->b.ham == h, INQX_ERR_ARG, "hf basis belongs to another Hamiltonian");
+    INQX_CHECK(nhf[b] > 0 && hf[b] && occ[b], INQX_ERR_ARG, "bad hf block");
+    INQX_CHECK(std::isfinite(zeroterms[b]), INQX_ERR_ARG, "non-finite zeroterm");
+    v[b] = HfBlock{&bases[b]->b, nhf[b], hf[b], occ[b], zeroterms[b]};
+  }
+  return v;
 }
 
-int inqx_ace_build(inqx_ham h, inqx_basis bs, int nst, int precision, const void *phi, const double *occ,
-                   double exx_coeff, double zeroterm, void *xi, double *energy) {
-  inqx_ctx ctx = h ? h->ctx : nullptr;
-  API_BEGIN
-  check_exchange(h, bs, nst, precision);
-  INQX_CHECK(phi && occ && xi && energy && xi != phi, INQX_ERR_ARG, "bad arguments");
-  Basis &B = bs->b;
+// exchange_operator::update's ACE build (exchange_operator.hpp:98-114) from
+// the hf blocks: W = direct(phi, -1), M = vol_el W^H phi, E_x = -1/2 sum occ_i
+// Re M_ii, M = L L^H, xi = W L^{-H}
+static double ace_build_impl(inqx_ham h, Basis &B, int nst, int precision, const void *phi, const double *occ,
+                             const std::vector<HfBlock> &blocks, double exx_coeff, void *xi) {
+  inqx_ctx ctx = h->ctx;
   const long npw = B.npw;
-  // W = direct(phi, -1) on the sphere (exchange_operator.hpp:105)
-  exchange_direct_impl(h, B, nst, precision, phi, nst, phi, occ, exx_coeff, -1.0, zeroterm, xi, false);
+  exchange_direct_impl(h, B, nst, precision, phi, blocks, exx_coeff, -1.0, xi, false);
   // M = vol_el W^H phi; phi is band limited, so the sphere sum equals the real-space one
   const double vol = h->g.vol / ((double)h->g.N * (double)h->g.N);
   ctx->c.ws_S.ensure(16 * (size_t)nst * nst);
@@ -1328,7 +1357,59 @@ int inqx_ace_build(inqx_ham h, inqx_basis bs, int nst, int precision, const void
   INQX_CHECK(info == 0, INQX_ERR_NOT_PD, "ACE: exchange matrix not positive definite (minor " + std::to_string(info) + ")");
   launch_trinv(ctx->c, nst, S, X);
   rotate_inplace(ctx, precision, nst, npw, X, 0, xi);  // xi <- W L^{-H} (rotate_trs)
-  *energy = en;
+  return en;
+}
+
+int inqx_exchange_direct(inqx_ham h, inqx_basis bs, int nst, int precision, const void *phi, int nhf, const void *hf,
+                         const double *occ, double exx_coeff, double scale, double zeroterm, void *exxphi) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_exchange(h, bs, nst, precision);
+  INQX_CHECK(phi && hf && occ && exxphi && nhf > 0, INQX_ERR_ARG, "null argument");
+  INQX_CHECK(exxphi != phi && exxphi != hf, INQX_ERR_ARG, "exxphi must not alias the inputs");
+  INQX_CHECK(std::isfinite(zeroterm), INQX_ERR_ARG, "non-finite zeroterm");
+  const std::vector<HfBlock> blocks{HfBlock{&bs->b, nhf, hf, occ, zeroterm}};
+  exchange_direct_impl(h, bs->b, nst, precision, phi, blocks, exx_coeff, scale, exxphi, true);
+  API_END(ctx)
+}
+
+int inqx_exchange_direct_kpts(inqx_ham h, inqx_basis bs, int nst, int precision, const void *phi, int nblocks,
+                              const inqx_basis *hf_bases, const int *nhf, const void *const *hf,
+                              const double *const *occ, const double *zeroterms, double exx_coeff, double scale,
+                              void *exxphi) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_exchange(h, bs, nst, precision);
+  INQX_CHECK(phi && exxphi && exxphi != phi, INQX_ERR_ARG, "bad state arrays");
+  const std::vector<HfBlock> blocks = hf_blocks(h, nblocks, hf_bases, nhf, hf, occ, zeroterms);
+  for (const HfBlock &b : blocks) INQX_CHECK(b.hf != exxphi, INQX_ERR_ARG, "exxphi must not alias the inputs");
+  exchange_direct_impl(h, bs->b, nst, precision, phi, blocks, exx_coeff, scale, exxphi, true);
+  API_END(ctx)
+}
+
+int inqx_ace_build(inqx_ham h, inqx_basis bs, int nst, int precision, const void *phi, const double *occ,
+                   double exx_coeff, double zeroterm, void *xi, double *energy) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_exchange(h, bs, nst, precision);
+  INQX_CHECK(phi && occ && xi && energy && xi != phi, INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(std::isfinite(zeroterm), INQX_ERR_ARG, "non-finite zeroterm");
+  const std::vector<HfBlock> blocks{HfBlock{&bs->b, nst, phi, occ, zeroterm}};
+  *energy = ace_build_impl(h, bs->b, nst, precision, phi, occ, blocks, exx_coeff, xi);
+  API_END(ctx)
+}
+
+int inqx_ace_build_kpts(inqx_ham h, inqx_basis bs, int nst, int precision, const void *phi, const double *occ,
+                        int nblocks, const inqx_basis *hf_bases, const int *nhf, const void *const *hf,
+                        const double *const *hf_occ, const double *zeroterms, double exx_coeff, void *xi,
+                        double *energy) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_exchange(h, bs, nst, precision);
+  INQX_CHECK(phi && occ && xi && energy && xi != phi, INQX_ERR_ARG, "bad arguments");
+  const std::vector<HfBlock> blocks = hf_blocks(h, nblocks, hf_bases, nhf, hf, hf_occ, zeroterms);
+  for (const HfBlock &b : blocks) INQX_CHECK(b.hf != xi, INQX_ERR_ARG, "xi must not alias the inputs");
+  *energy = ace_build_impl(h, bs->b, nst, precision, phi, occ, blocks, exx_coeff, xi);
   API_END(ctx)
 }
 

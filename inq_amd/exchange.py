@@ -1,8 +1,10 @@
 """Exact exchange over the inqx C ABI: hamiltonian::exchange_operator
-(src/hamiltonian/exchange_operator.hpp) at the Gamma point, with states on
-the G sphere.  ``update`` builds the ACE projectors xi (exchange_operator.hpp:
-52-120), ``ace`` applies K psi = -xi (xi^H psi) (:217-231) and ``direct``
-runs the pair-density Poisson solves (:124-186)."""
+(src/hamiltonian/exchange_operator.hpp) with states on the G sphere of a
+(k-point, spin) block.  ``update`` builds the ACE projectors xi
+(exchange_operator.hpp:52-120), ``ace`` applies K psi = -xi (xi^H psi)
+(:217-231) and ``direct`` runs the pair-density Poisson solves (:124-186).
+The ``*_kpts`` forms take the occupied states of several k-points, each pair
+solve shifted by -k + q (exchange_operator.hpp:151)."""
 import ctypes as C
 
 import numpy as np
@@ -67,6 +69,47 @@ class ExchangeOperator:
                                           hf.shape[0], hf.ptr, _dp(occ), self.coefficient, float(scale),
                                           self.zeroterm, exxphi.ptr), self.ham.ctx.h, "inqx_exchange_direct")
         return exxphi
+
+    @staticmethod
+    def _hf_arrays(hf_blocks):
+        """hf_blocks: [(basis, hf DeviceArray, occupations, zeroterm)] -> ctypes arrays"""
+        n = len(hf_blocks)
+        bases = (C.c_void_p * n)(*[b.h for b, _, _, _ in hf_blocks])
+        nhf = np.ascontiguousarray([h.shape[0] for _, h, _, _ in hf_blocks], dtype=np.int32)
+        ptrs = (C.c_void_p * n)(*[h.ptr for _, h, _, _ in hf_blocks])
+        occs = [np.ascontiguousarray(o, dtype=np.float64) for _, _, o, _ in hf_blocks]
+        optr = (C.c_void_p * n)(*[o.ctypes.data for o in occs])
+        zt = np.ascontiguousarray([z for _, _, _, z in hf_blocks], dtype=np.float64)
+        return n, bases, nhf, ptrs, occs, optr, zt
+
+    def direct_kpts(self, phi, hf_blocks, scale=1.0, exxphi=None):
+        """exchange with the occupied states of several k-points:
+        hf_blocks = [(basis, hf, occ, zeroterm)], each pair solve shifted by
+        -k + q of its block"""
+        if exxphi is None:
+            exxphi = DeviceArray(self.ham.ctx, phi.shape, phi.dtype)
+            exxphi.upload(np.zeros(phi.shape, phi.dtype))
+        n, bases, nhf, ptrs, occs, optr, zt = self._hf_arrays(hf_blocks)
+        check(lib().inqx_exchange_direct_kpts(self.ham.h, self.basis.h, phi.shape[0], precision_of(phi.dtype),
+                                              phi.ptr, n, bases, nhf.ctypes.data_as(C.POINTER(C.c_int)), ptrs, optr,
+                                              _dp(zt), self.coefficient, float(scale), exxphi.ptr), self.ham.ctx.h,
+              "inqx_exchange_direct_kpts")
+        return exxphi
+
+    def update_kpts(self, phi, occ, hf_blocks):
+        """ACE build of this block against the occupied states of every k-point
+        (hf_blocks as in direct_kpts); returns this block's exchange energy"""
+        occ = np.ascontiguousarray(occ, dtype=np.float64)
+        if self.xi is None or self.xi.shape != phi.shape or self.xi.dtype != phi.dtype:
+            self.xi = DeviceArray(self.ham.ctx, phi.shape, phi.dtype)
+        n, bases, nhf, ptrs, occs, optr, zt = self._hf_arrays(hf_blocks)
+        en = C.c_double()
+        check(lib().inqx_ace_build_kpts(self.ham.h, self.basis.h, phi.shape[0], precision_of(phi.dtype), phi.ptr,
+                                        _dp(occ), n, bases, nhf.ctypes.data_as(C.POINTER(C.c_int)), ptrs, optr,
+                                        _dp(zt), self.coefficient, self.xi.ptr, C.byref(en)), self.ham.ctx.h,
+              "inqx_ace_build_kpts")
+        self.energy = en.value
+        return self.energy
 
     def update(self, phi, occ):
         """ACE build from the states phi; returns the exchange energy."""

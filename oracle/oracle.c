@@ -1230,25 +1230,36 @@ void orc_sing_correction(const double lattice[9], int nk, const double *kcov, co
 
 /* direct: exxphi += -0.5*scale*alpha sum_j occ_j hf_j v_ij,
  * v_ij = Poisson(conj(hf_j) phi_i) (exchange_operator.hpp:124-161,165-186) */
-void orc_exchange_direct(const orc_ham *h, int nst, const cplx *phi, int nhf, const cplx *hf, const double *occ,
-                         double exx_coeff, double scale, cplx *exxphi) {
+/* exchange_operator::block_exchange (exchange_operator.hpp:135-161) for the hf
+ * states of one k-point: rho_ij = conj(hf_j) phi_i, poisson::in_place with the
+ * shift -k + kpt[jj] (here gshift_cart, already cartesian) and the singularity
+ * value of that k-point (zeroterm), exxphi_i += factor occ_j hf_j rho_ij. */
+void orc_exchange_direct_shift(const orc_ham *h, int nst, const cplx *phi, int nhf, const cplx *hf,
+                               const double *occ, double exx_coeff, double scale, const double gshift_cart[3],
+                               double zeroterm, cplx *exxphi) {
   const grid_t *g = &h->g;
   double factor = -0.5 * scale * exx_coeff;
   long N = g->N;
   cplx *rho = (cplx *)malloc(sizeof(cplx) * (size_t)N * nst);
-  double zero[3] = {0, 0, 0};
   for (int j = 0; j < nhf; ++j) {
     if (fabs(occ[j]) < 1e-10) continue;
 #pragma omp parallel for schedule(static)
     for (long ip = 0; ip < N; ++ip)
       for (int s = 0; s < nst; ++s) rho[ip * nst + s] = conj(hf[ip * nhf + j]) * phi[ip * nst + s];
-    orc_poisson_in_place(h, nst, rho, zero, 0.0);
+    orc_poisson_in_place(h, nst, rho, gshift_cart, zeroterm);
     double sc = factor * occ[j];
 #pragma omp parallel for schedule(static)
     for (long ip = 0; ip < N; ++ip)
       for (int s = 0; s < nst; ++s) exxphi[ip * nst + s] += sc * hf[ip * nhf + j] * rho[ip * nst + s];
   }
   free(rho);
+}
+
+/* the Gamma point: no shift, zero at G = 0 */
+void orc_exchange_direct(const orc_ham *h, int nst, const cplx *phi, int nhf, const cplx *hf, const double *occ,
+                         double exx_coeff, double scale, cplx *exxphi) {
+  const double zero[3] = {0, 0, 0};
+  orc_exchange_direct_shift(h, nst, phi, nhf, hf, occ, exx_coeff, scale, zero, 0.0, exxphi);
 }
 
 /* ACE build (exchange_operator.hpp:98-114): W = direct(phi, -1),

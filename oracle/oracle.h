@@ -132,6 +132,10 @@ double orc_ace_build(const orc_ham *h, int nst, const orc_cplx *phi_rs, const do
 void   orc_exchange_direct(const orc_ham *h, int nst, const orc_cplx *phi_rs, int nhf,
                            const orc_cplx *hf_rs, const double *occ, double exx_coeff,
                            double scale, orc_cplx *exxphi);
+void   orc_exchange_direct_shift(const orc_ham *h, int nst, const orc_cplx *phi_rs, int nhf,
+                                 const orc_cplx *hf_rs, const double *occ, double exx_coeff,
+                                 double scale, const double gshift_cart[3], double zeroterm,
+                                 orc_cplx *exxphi);
 void   orc_ace_apply(const orc_ham *h, int nxi, const orc_cplx *xi, int nst, const orc_cplx *phi_rs,
                      orc_cplx *exxphi);
 

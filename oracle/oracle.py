@@ -75,6 +75,8 @@ def lib():
             "orc_sing_correction": (None, [dp, C.c_int, dp, dp, dp, dp]),
             "orc_ace_build": (C.c_double, [P, C.c_int, cp, dp, C.c_double, cp]),
             "orc_exchange_direct": (None, [P, C.c_int, cp, C.c_int, cp, dp, C.c_double, C.c_double, cp]),
+            "orc_exchange_direct_shift": (None, [P, C.c_int, cp, C.c_int, cp, dp, C.c_double, C.c_double, dp,
+                                                 C.c_double, cp]),
             "orc_ace_apply": (None, [P, C.c_int, cp, C.c_int, cp, cp]),
             # refcpu.c: the reference CPU data flow on FFTW3/BLAS/LAPACK (CPU baseline)
             "rc_init": (C.c_int, []),
@@ -335,13 +337,21 @@ class Ham:
         ex = lib().orc_ace_build(self.h, nst, phi_rs, _f64(occ), float(exx_coeff), xi)
         return xi, ex
 
-    def exchange_direct(self, phi_rs, hf_rs, occ, exx_coeff=1.0, scale=1.0):
+    def exchange_direct(self, phi_rs, hf_rs, occ, exx_coeff=1.0, scale=1.0, gshift=None, zeroterm=0.0):
+        """exchange of phi (real space [N][nst]) with the hf states of one
+        k-point; gshift: cartesian -k + q of the pair Poisson solves (None: Gamma)"""
         phi_rs = _c128(phi_rs)
         hf_rs = _c128(hf_rs)
         nst = phi_rs.size // self.N
         nhf = hf_rs.size // self.N
         out = np.zeros_like(phi_rs)
-        lib().orc_exchange_direct(self.h, nst, phi_rs, nhf, hf_rs, _f64(occ), float(exx_coeff), float(scale), out)
+        if gshift is None and zeroterm == 0.0:
+            lib().orc_exchange_direct(self.h, nst, phi_rs, nhf, hf_rs, _f64(occ), float(exx_coeff), float(scale),
+                                      out)
+        else:
+            gs = _f64(gshift if gshift is not None else (0.0, 0.0, 0.0))
+            lib().orc_exchange_direct_shift(self.h, nst, phi_rs, nhf, hf_rs, _f64(occ), float(exx_coeff),
+                                            float(scale), gs, float(zeroterm), out)
         return out
 
     def ace_apply(self, xi, phi_rs):

@@ -157,6 +157,68 @@ def test_ace_build_and_apply_vs_oracle(ctx, dtype):
     assert rel_err(kpsi, ref) < TOL_X[dtype]
 
 
+def _cart(kcov):
+    """covariant k (the units of 2 pi m) in cartesian components"""
+    return np.asarray(kcov, dtype=np.float64) @ np.linalg.inv(LAT).T
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_exchange_kpoints_vs_oracle(ctx, dtype):
+    """states at k against occupied states at k and at q (two hf blocks, each
+    with its own singularity value): every pair solve of block q shifted by
+    -k + q (exchange_operator.hpp:151), against the oracle; the ACE built from
+    the same blocks reproduces the direct exchange on its own states and its
+    energy equals -1/2 sum occ Re <phi_i|W_i> from the oracle's W"""
+    say("case: exchange at two k-points 16^3", np.dtype(dtype).name)
+    gh = _ham(ctx)
+    k, q = (0.4, -0.3, 0.2), (-0.5, 0.6, 0.1)
+    bk, bq = gh.basis(k, 6.0), gh.basis(q, 6.0)
+    gk, gq = bk.gindex(), bq.gindex()
+    phi = _states(bk, 5, 41).astype(dtype)
+    hk = _states(bk, 3, 42).astype(dtype)
+    hq = _states(bq, 4, 43).astype(dtype)
+    ok, oq = np.array([2.0, 1.0, 0.5]), np.array([2.0, 0.0, 1.5, 0.3])
+    zk, zq = 1.7, 0.9
+    alpha = 0.25
+    ex = ExchangeOperator(gh, bk, exchange_coefficient=alpha)
+    init = _states(bk, 5, 44).astype(dtype)
+    out = DeviceArray.from_numpy(ctx, init)
+    dphi = DeviceArray.from_numpy(ctx, phi)
+    blocks = [(bk, DeviceArray.from_numpy(ctx, hk), ok, zk), (bq, DeviceArray.from_numpy(ctx, hq), oq, zq)]
+    ex.direct_kpts(dphi, blocks, scale=1.0, exxphi=out)
+    oh = orc.Ham(LAT, NG)
+    rs = lambda c, g: np.ascontiguousarray(_to_real(c.astype(np.complex128), g).T)
+    p_rs = rs(phi, gk)
+    w_rs = (oh.exchange_direct(p_rs, rs(hk, gk), ok, exx_coeff=alpha, gshift=(0.0, 0.0, 0.0), zeroterm=zk) +
+            oh.exchange_direct(p_rs, rs(hq, gq), oq, exx_coeff=alpha, gshift=_cart(q) - _cart(k), zeroterm=zq))
+    ref = init.astype(np.complex128) + _to_sphere(np.ascontiguousarray(w_rs.T), gk)
+    assert rel_err(out.numpy() - init, ref - init) < TOL_X[dtype]
+    # ACE of the block phi (occupations of its own states for the energy)
+    occ_phi = np.array([2.0, 2.0, 1.0, 1.0, 0.5])
+    e = ex.update_kpts(dphi, occ_phi, blocks)
+    dv = V / N
+    m_ii = dv * np.sum(np.conj(-w_rs) * p_rs, axis=0)  # W = direct(phi, -1) = -K phi
+    assert e == pytest.approx(-0.5 * float(np.sum(occ_phi * m_ii.real)), rel=TOL_E[dtype])
+    kace = ex(dphi).numpy()
+    kdir = ex.direct_kpts(dphi, blocks, scale=1.0).numpy()
+    assert rel_err(kace, kdir) < TOL_X[dtype]
+
+
+def test_exchange_at_one_kpoint_equals_kpts_form(ctx):
+    """hf states at the block's own k-point: inqx_exchange_direct and the
+    k-point form with one block give the same result bit for bit"""
+    say("case: exchange one k-point, both entry points 16^3")
+    gh = _ham(ctx)
+    b = gh.basis((0.25, 0.5, -0.75), 6.0)
+    phi = DeviceArray.from_numpy(ctx, _states(b, 4, 51).astype(np.complex64))
+    hf = DeviceArray.from_numpy(ctx, _states(b, 3, 52).astype(np.complex64))
+    occ = np.array([2.0, 1.0, 0.25])
+    ex = ExchangeOperator(gh, b, exchange_coefficient=1.0, zeroterm=0.6)
+    a = ex.direct(phi, hf, occ).numpy()
+    c = ex.direct_kpts(phi, [(b, hf, occ, 0.6)]).numpy()
+    assert np.array_equal(a, c)
+
+
 def test_two_plane_waves_closed_form(ctx):
     say("case: exchange two plane waves 16^3")
     gh = _ham(ctx)

@@ -6,6 +6,11 @@ oracle/oracle.c) by closed forms and by the defining property of ACE.
     K phi_a = -(2 pi alpha occ_b / (V |dG|^2)) phi_a (rho_aa has only G = 0,
     which is excluded); the ACE energy is -2 pi alpha occ_a occ_b/(V |dG|^2).
     A dropped 1/2, 4 pi, occupation or volume factor fails this.
+  * the same with the hf state at another k-point: the pair solve is
+    shifted by s = -k + q (exchange_operator.hpp:151), so K phi_a =
+    -(2 pi alpha occ_b / (V |G_a - G_b + s|^2)) phi_a; with G_b = G_a the
+    density is uniform and the shift alone sets the value; with s = 0 the
+    uniform density takes the zeroterm.  A shift of the wrong sign fails it.
   * ACE is exact on the states it is built from: -xi xi^H phi dV == K phi.
   * K is Hermitian: <a|K b> = conj(<b|K a>).
 """
@@ -89,3 +94,25 @@ def test_ace_exact_on_building_set_and_hermitian():
     e2 = 0.5 * sum(occ[i] * dv * np.vdot(phi[:, i], kphi[:, i]).real for i in range(4))
     assert ex == pytest.approx(e2, rel=1e-12)
     assert ex < 0
+
+
+def test_shifted_pair_solves_closed_form():
+    b = 2 * np.pi / np.diag(LAT)
+    Ga = np.array([b[0], 0, 0])
+    Gb = np.array([0, -b[1], 2 * b[2]])
+    phi, V = _plane_waves([Ga, Gb])
+    alpha, occ_b = 0.25, 0.7
+    s = np.array([0.31, -0.22, 0.17])  # cartesian -k + q, not orthogonal to G_a - G_b
+    h = orc.Ham(LAT, NG)
+    ka = h.exchange_direct(phi[:, :1], phi[:, 1:], [occ_b], exx_coeff=alpha, gshift=s)
+    lam = -2 * np.pi * alpha * occ_b / (V * float(np.sum((Ga - Gb + s) ** 2)))
+    assert np.max(np.abs(ka[:, 0] - lam * phi[:, 0])) < 1e-13 * np.max(np.abs(phi))
+    wrong = -2 * np.pi * alpha * occ_b / (V * float(np.sum((Ga - Gb - s) ** 2)))
+    assert abs(wrong - lam) > 1e-3 * abs(lam)
+    # the same plane wave at another k-point: uniform density, shifted
+    kaa = h.exchange_direct(phi[:, :1], phi[:, :1], [occ_b], exx_coeff=alpha, gshift=s)
+    lam0 = -2 * np.pi * alpha * occ_b / (V * float(np.sum(s * s)))
+    assert np.max(np.abs(kaa[:, 0] - lam0 * phi[:, 0])) < 1e-13 * np.max(np.abs(phi))
+    # no shift: the uniform density takes the zeroterm, -alpha/2 occ zeroterm / V
+    kz = h.exchange_direct(phi[:, :1], phi[:, :1], [occ_b], exx_coeff=alpha, gshift=(0, 0, 0), zeroterm=3.5)
+    assert np.max(np.abs(kz[:, 0] + 0.5 * alpha * occ_b * 3.5 / V * phi[:, 0])) < 1e-13 * np.max(np.abs(phi))
