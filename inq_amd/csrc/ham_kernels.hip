@@ -1686,6 +1686,37 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   INQX_HIP(hipGetLastError());
 }
 
+// the pass templates for a grid n0 x n1 x n1: x-pass line length N0 and
+// plane size N12 x N12 are independent compile-time sizes (host code only:
+// every kernel is instantiated once per size)
+template <typename T, int N0>
+static void dispatch_n12(int n12, inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev &hd, const double *cf) {
+  switch (n12) {
+#define CASE(S)                                     \
+  case S:                                           \
+    run_passes<T, N0, S>(h, ld, nblocks, hd, cf);   \
+    break;
+    INQX_SIZES(CASE)
+#undef CASE
+    default:
+      INQX_CHECK(false, INQX_ERR_UNSUPPORTED, "grid size not built for the sphere path");
+  }
+}
+template <typename T>
+static void dispatch_passes(int n0, int n12, inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev &hd,
+                            const double *cf) {
+  switch (n0) {
+#define CASE(S)                                         \
+  case S:                                               \
+    dispatch_n12<T, S>(n12, h, ld, nblocks, hd, cf);    \
+    break;
+    INQX_SIZES(CASE)
+#undef CASE
+    default:
+      INQX_CHECK(false, INQX_ERR_UNSUPPORTED, "grid size not built for the sphere path");
+  }
+}
+
 template <typename T>
 void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *nst, const void *const *psi,
                   void *const *hpsi, const ApplyOpts *opts) {
@@ -1909,16 +1940,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) hd.lat[3 * r + c] = g.a[r][c];
   const double cf[4] = {opts->c1r, opts->c1i, opts->c2r, opts->c2i};
-  switch (g.n[0] * 1000 + g.n[1]) {
-#define CASE(S)                                            \
-  case S * 1000 + S:                                       \
-    run_passes<T, S, S>(h, ld, nblocks, hd, cf);           \
-    break;
-    INQX_SIZES(CASE)
-#undef CASE
-    default:
-      INQX_CHECK(false, INQX_ERR_UNSUPPORTED, "only cubic grid shapes are built for the sphere path");
-  }
+  dispatch_passes<T>(g.n[0], g.n[1], h, ld, nblocks, hd, cf);
 }
 
 int dots_chunks(int npw) { return (npw + kDotChunk - 1) / kDotChunk; }

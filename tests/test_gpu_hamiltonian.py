@@ -47,6 +47,24 @@ def test_small_random_system(ctx, dtype, k):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,lat", [((32, 24, 24), (8.0, 6.0, 6.0)), ((20, 30, 30), (5.0, 7.5, 7.5))])
+def test_grid_n0_differs_from_n1(ctx, dtype, n, lat):
+    """a grid whose first axis differs from the other two (orthorhombic cell,
+    k != 0, three atoms): x-pass line length and plane size are independent"""
+    rng = np.random.default_rng(17)
+    L = np.diag(lat)
+    sysm = S.System(f"ortho{n[0]}x{n[1]}", 190, L, n, rng.uniform(-0.5, 0.5, (3, 3)) @ L, 8.0, 5,
+                    v_sigma=0.6, kb_width=0.7, kb_radius=1.6)
+    k = (0.35, -0.25, 0.15)
+    say("case: grid", n, np.dtype(dtype).name)
+    gh = gpu_ham(ctx, sysm)
+    oh = oracle_ham(sysm)
+    b, c, out = _run(ctx, gh, sysm, k, sysm.ecut, sysm.nst, dtype, 19)
+    ref = oh.apply_sphere(c.astype(dtype).astype(np.complex128), b.gindex(), k)
+    assert rel_err(out, ref) < TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 def test_c1_diamond(ctx, dtype):
     sysm = S.config(1)
     say("case: C1 diamond 32^3", np.dtype(dtype).name)
