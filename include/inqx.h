@@ -160,6 +160,27 @@ int inqx_ham_apply(inqx_ham ham, inqx_basis basis, int nst, int precision, const
 int inqx_ham_apply_batch(inqx_ham ham, int nblocks, const inqx_basis *bases, const int *nst, int precision,
                          const void *const *psi, void *const *hpsi);
 
+/* Fourier-space layouts of a block of sphere states */
+enum { INQX_SPHERE_BAND = 0, INQX_SPHERE_POINT = 1, INQX_CUBE_POINT = 2 };
+/* states of basis in a layout -> band-major sphere [nst][npw] (device arrays).
+ * INQX_SPHERE_POINT: src [npw][nst] (the reference's point-major [ipw][ist]);
+ * INQX_CUBE_POINT: src [N][nst], the full Fourier cube [Gx][Gy][Gz][ist] of an
+ * orbital_set<fourier_space> (states/orbital_set.hpp:19-161; points off the
+ * sphere are dropped); INQX_SPHERE_BAND: a copy. */
+int inqx_states_to_sphere(inqx_ham ham, inqx_basis basis, int nst, int precision, int layout, const void *src,
+                          void *sphere);
+/* band-major sphere -> layout (INQX_CUBE_POINT: points off the sphere are set to zero) */
+int inqx_states_from_sphere(inqx_ham ham, inqx_basis basis, int nst, int precision, int layout, const void *sphere,
+                            void *dst);
+/* H|psi> with psi and hpsi in one layout.  INQX_CUBE_POINT on a basis that
+ * holds the whole cube (an ecut above every |G+k|^2/2) is the reference's
+ * Fourier-space operator on its own layout, ks_hamiltonian::operator()
+ * (orbital_set<fourier_space>) (ks_hamiltonian.hpp:160-180); on a sphere basis
+ * it is mask H mask.  Converts through a band-major workspace of the
+ * Hamiltonian; stream-ordered, no host synchronisation. */
+int inqx_ham_apply_layout(inqx_ham ham, inqx_basis basis, int nst, int precision, int layout, const void *psi,
+                          void *hpsi);
+
 /* ------------------------------------------------------------ overlap / orthogonalisation */
 /* All matrices are device arrays of complex128, row-major.  States are
  * band-major device arrays [n][npw] of the given precision.  vol_el is the

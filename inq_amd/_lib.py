@@ -59,6 +59,9 @@ SIGNATURES = {
     "inqx_ham_prepare": (C.c_int, [P, P, C.c_int, C.c_int]),
     "inqx_ham_apply": (C.c_int, [P, P, C.c_int, C.c_int, P, P]),
     "inqx_ham_apply_batch": (C.c_int, [P, C.c_int, PP, IP, C.c_int, PP, PP]),
+    "inqx_ham_apply_layout": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, P]),
+    "inqx_states_to_sphere": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, P]),
+    "inqx_states_from_sphere": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, P]),
     "inqx_sphere_to_real": (C.c_int, [P, P, C.c_int, C.c_int, P, C.c_int, P]),
     "inqx_real_to_sphere": (C.c_int, [P, P, C.c_int, C.c_int, P, C.c_int, C.c_int, P]),
     "inqx_poisson": (C.c_int, [P, C.c_int, C.c_int, P, DP, C.c_double]),

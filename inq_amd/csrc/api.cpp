@@ -1102,6 +1102,91 @@ int inqx_ham_apply_real(inqx_ham h, inqx_basis bs, int nst, int precision, const
   API_END(ctx)
 }
 
+// ---------------------------------------------------------------- reference layouts of Fourier-space states
+static void check_layout(inqx_ham h, inqx_basis bs, int nst, int precision, int layout) {
+  INQX_CHECK(h && bs && nst > 0, INQX_ERR_ARG, "bad arguments");
+  INQX_CHECK(!h->ctx->c.host_only, INQX_ERR_STATE, "host-only context");
+  INQX_CHECK(bs->b.ham == h, INQX_ERR_ARG, "basis belongs to another Hamiltonian");
+  INQX_CHECK(precision == INQX_FP64 || precision == INQX_FP32, INQX_ERR_ARG, "bad precision");
+  INQX_CHECK(layout == INQX_SPHERE_BAND || layout == INQX_SPHERE_POINT || layout == INQX_CUBE_POINT, INQX_ERR_ARG,
+             "unknown layout");
+}
+
+static const int *basis_gidx(inqx_ham h, Basis &B) {
+  if (!B.d_gidx.p) {
+    upload(B.d_gidx, B.gidx, h->ctx->c.stream);
+    INQX_HIP(hipStreamSynchronize(h->ctx->c.stream));
+  }
+  return B.d_gidx.as<int>();
+}
+
+static void to_sphere_impl(inqx_ham h, Basis &B, int nst, int prec, int layout, const void *src, void *sph) {
+  Ctx &cx = h->ctx->c;
+  const size_t cs = prec == INQX_FP64 ? 16 : 8;
+  if (layout == INQX_SPHERE_BAND)
+    INQX_HIP(hipMemcpyAsync(sph, src, cs * (size_t)nst * B.npw, hipMemcpyDeviceToDevice, cx.stream));
+  else
+    launch_point_to_band(cx, prec, src, layout == INQX_CUBE_POINT ? basis_gidx(h, B) : nullptr, B.npw, nst, sph);
+}
+
+static void from_sphere_impl(inqx_ham h, Basis &B, int nst, int prec, int layout, const void *sph, void *dst) {
+  Ctx &cx = h->ctx->c;
+  const size_t cs = prec == INQX_FP64 ? 16 : 8;
+  if (layout == INQX_SPHERE_BAND) {
+    INQX_HIP(hipMemcpyAsync(dst, sph, cs * (size_t)nst * B.npw, hipMemcpyDeviceToDevice, cx.stream));
+  } else if (layout == INQX_SPHERE_POINT) {
+    launch_band_to_point(cx, prec, sph, nullptr, B.npw, nst, dst);
+  } else {
+    launch_fill_zero(cx, dst, cs * (size_t)nst * h->g.N);
+    launch_band_to_point(cx, prec, sph, basis_gidx(h, B), B.npw, nst, dst);
+  }
+}
+
+int inqx_states_to_sphere(inqx_ham h, inqx_basis bs, int nst, int precision, int layout, const void *src,
+                          void *sphere) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_layout(h, bs, nst, precision, layout);
+  INQX_CHECK(src && sphere && src != sphere, INQX_ERR_ARG, "bad state pointers");
+  to_sphere_impl(h, bs->b, nst, precision, layout, src, sphere);
+  API_END(ctx)
+}
+
+int inqx_states_from_sphere(inqx_ham h, inqx_basis bs, int nst, int precision, int layout, const void *sphere,
+                            void *dst) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_layout(h, bs, nst, precision, layout);
+  INQX_CHECK(sphere && dst && sphere != dst, INQX_ERR_ARG, "bad state pointers");
+  from_sphere_impl(h, bs->b, nst, precision, layout, sphere, dst);
+  API_END(ctx)
+}
+
+int inqx_ham_apply_layout(inqx_ham h, inqx_basis bs, int nst, int precision, int layout, const void *psi,
+                          void *hpsi) {
+  inqx_ctx ctx = h ? h->ctx : nullptr;
+  API_BEGIN
+  check_layout(h, bs, nst, precision, layout);
+  INQX_CHECK(psi && hpsi && psi != hpsi, INQX_ERR_ARG, "bad state pointers");
+  Basis &B = bs->b;
+  Basis *bb = &B;
+  if (layout == INQX_SPHERE_BAND) {
+    if (precision == INQX_FP64) launch_apply<double>(h, 1, &bb, &nst, &psi, &hpsi);
+    else launch_apply<float>(h, 1, &bb, &nst, &psi, &hpsi);
+  } else {
+    const size_t bytes = (precision == INQX_FP64 ? 16 : 8) * (size_t)nst * B.npw;
+    h->lay_a.ensure(bytes);
+    h->lay_b.ensure(bytes);
+    to_sphere_impl(h, B, nst, precision, layout, psi, h->lay_a.p);
+    const void *pin = h->lay_a.p;
+    void *pout = h->lay_b.p;
+    if (precision == INQX_FP64) launch_apply<double>(h, 1, &bb, &nst, &pin, &pout);
+    else launch_apply<float>(h, 1, &bb, &nst, &pin, &pout);
+    from_sphere_impl(h, B, nst, precision, layout, h->lay_b.p, hpsi);
+  }
+  API_END(ctx)
+}
+
 int inqx_density_add(inqx_ham h, inqx_basis bs, int nst, int precision, const void *phi, const double *occ,
                      int ncomp, int ispin, void *density) {
   inqx_ctx ctx = h ? h->ctx : nullptr;

@@ -242,6 +242,7 @@ struct Basis {
   std::vector<int> rows, pw_yz;
   DevMem d_slot_seg, d_rows, d_pw_lw, d_pw_lf, d_kin64, d_kin32;
   DevMem d_g2h;  // (1/2)|G|^2 per sphere point, FP64, no k shift (preconditioner)
+  DevMem d_gidx; // flattened FFT index per sphere point (layout conversions), on first use
   double kin_A[3] = {1e300, 0, 0};  // A for which d_kin* were built
   // KB phases exp(+i (k+A).r_e) per KB entry e (FP64 and FP32 copies), built for
   // the vector potential ph_A and the KB table generation ph_gen
@@ -296,6 +297,7 @@ struct inqx_ham_s {
   inqx::DevMem exx_phi, exx_hf, exx_w, exx_t, exx_jlist, exx_coef;
   // real-space apply and density workspaces (sphere arrays of a full-cube basis, occupations)
   inqx::DevMem rs_a, rs_b, dens_occ;
+  inqx::DevMem lay_a, lay_b;  // band-major copies for inqx_ham_apply_layout
   // steepest-descent workspace: H phi, residual, H residual, dot partials and results
   inqx::DevMem sd_states, sd_scal;
   inqx::DevMem hartree_work, hartree_part;  // Hartree solve: complex128 [N] and the energy partials
@@ -382,6 +384,9 @@ constexpr int kMaxEigN = 1024;
 size_t eigh_workspace_bytes(int n);
 void launch_eigh(Ctx &cx, int n, void *A, double *lam, void *U, void *ws);
 void launch_conj(Ctx &cx, void *a, long n);
+// layout.hip: band[s][i] <-> point-major rows (rows[i], or i when rows is null) of [row][nst]
+void launch_point_to_band(Ctx &cx, int prec, const void *src, const int *rows, long npw, int nst, void *band);
+void launch_band_to_point(Ctx &cx, int prec, const void *band, const int *rows, long npw, int nst, void *dst);
 // propagate.cpp: workspaces of inqx_ks_potential, dropped with the Hamiltonian
 void release_ks_work(inqx_ham_s *h);
 // allocation-free Hartree (inqx_hartree_noalloc): its scratch is a file-scope

@@ -209,6 +209,39 @@ class KSHamiltonian:
         check(lib().inqx_ham_apply(self.h, basis.h, nst, precision_of(psi.dtype), psi.ptr, hpsi.ptr), self.ctx.h,
               "inqx_ham_apply")
 
+    # Fourier-space layouts (inqx.h): band-major sphere [nst][npw], the
+    # reference's point-major sphere [npw][nst], the full cube [N][nst]
+    SPHERE_BAND, SPHERE_POINT, CUBE_POINT = 0, 1, 2
+
+    def _rows(self, basis, layout):
+        return {0: basis.npw, 1: basis.npw, 2: int(np.prod(self.n))}[layout]
+
+    def apply_layout(self, basis, psi, hpsi, layout):
+        """hpsi = H psi with both in `layout`: [nst][npw], [npw][nst] or the
+        full Fourier cube [N][nst] ([Gx][Gy][Gz][ist])"""
+        nst = psi.shape[0] if layout == 0 else psi.shape[1]
+        check(lib().inqx_ham_apply_layout(self.h, basis.h, nst, precision_of(psi.dtype), layout, psi.ptr, hpsi.ptr),
+              self.ctx.h, "inqx_ham_apply_layout")
+
+    def states_to_sphere(self, basis, src, layout, out=None):
+        """states in `layout` -> band-major sphere [nst][npw]"""
+        nst = src.shape[0] if layout == 0 else src.shape[1]
+        if out is None:
+            out = DeviceArray(self.ctx, (nst, basis.npw), src.dtype)
+        check(lib().inqx_states_to_sphere(self.h, basis.h, nst, precision_of(src.dtype), layout, src.ptr, out.ptr),
+              self.ctx.h, "inqx_states_to_sphere")
+        return out
+
+    def states_from_sphere(self, basis, sph, layout, out=None):
+        """band-major sphere [nst][npw] -> `layout`"""
+        nst = sph.shape[0]
+        if out is None:
+            shape = (nst, basis.npw) if layout == 0 else (self._rows(basis, layout), nst)
+            out = DeviceArray(self.ctx, shape, sph.dtype)
+        check(lib().inqx_states_from_sphere(self.h, basis.h, nst, precision_of(sph.dtype), layout, sph.ptr,
+                                            out.ptr), self.ctx.h, "inqx_states_from_sphere")
+        return out
+
     def apply_batch(self, bases, psis, hpsis):
         nb = len(bases)
         prec = precision_of(psis[0].dtype)

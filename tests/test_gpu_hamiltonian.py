@@ -139,3 +139,53 @@ def test_c2_si64_subset_and_hermiticity(ctx, dtype):
     oh = oracle_ham(sysm)
     ref = oh.apply_sphere(ci[:8], b.gindex(), (0, 0, 0))
     assert rel_err(out[:8], ref) < TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_reference_layouts(ctx, dtype):
+    """the reference's Fourier-space layouts: point-major sphere [npw][nst] and
+    the full cube [Gx][Gy][Gz][ist].  Conversions move values bit for bit (the
+    cube gets zeros off the sphere); H in the point-major layouts equals the
+    band-major H bit for bit; on a basis holding the whole cube, H on the
+    cube layout is the reference's full-cube operator (oracle a2)"""
+    sysm = S.small_config(seed=9, n=(16, 16, 16), natoms=2, nst=5, ecut=6.0)
+    say("case: layouts 16^3", np.dtype(dtype).name)
+    gh = gpu_ham(ctx, sysm)
+    k = (0.2, -0.4, 0.1)
+    b = gh.basis(k, sysm.ecut)
+    g = b.gindex()
+    c = S.random_states(b.npw, sysm.nst, 12, S.sphere_volume_element(sysm)).astype(dtype)
+    N = sysm.N
+    cube = np.zeros((N, sysm.nst), dtype=dtype)
+    cube[g, :] = c.T
+    rng = np.random.default_rng(3)
+    off = np.ones(N, bool)
+    off[g] = False
+    noisy = cube.copy()
+    noisy[off, :] = (rng.standard_normal((int(off.sum()), sysm.nst)) * 7).astype(dtype)  # dropped on input
+    sph = gh.states_to_sphere(b, DeviceArray.from_numpy(ctx, noisy), gh.CUBE_POINT).numpy()
+    assert np.array_equal(sph, c)
+    pm = np.ascontiguousarray(c.T)
+    assert np.array_equal(gh.states_to_sphere(b, DeviceArray.from_numpy(ctx, pm), gh.SPHERE_POINT).numpy(), c)
+    d = DeviceArray.from_numpy(ctx, c)
+    assert np.array_equal(gh.states_from_sphere(b, d, gh.CUBE_POINT).numpy(), cube)
+    assert np.array_equal(gh.states_from_sphere(b, d, gh.SPHERE_POINT).numpy(), pm)
+    # H in the point-major layouts equals the band-major H
+    hb = gh(b, d).numpy()
+    out_pm = DeviceArray(ctx, pm.shape, dtype)
+    gh.apply_layout(b, DeviceArray.from_numpy(ctx, pm), out_pm, gh.SPHERE_POINT)
+    assert np.array_equal(out_pm.numpy(), np.ascontiguousarray(hb.T))
+    out_cube = DeviceArray(ctx, cube.shape, dtype)
+    gh.apply_layout(b, DeviceArray.from_numpy(ctx, noisy), out_cube, gh.CUBE_POINT)
+    want = np.zeros_like(cube)
+    want[g, :] = hb.T
+    assert np.array_equal(out_cube.numpy(), want)
+    # whole-cube basis: the reference's Fourier-space operator on [Gx][Gy][Gz][ist]
+    bw = gh.basis(k, 1e6)
+    assert bw.npw == N
+    full = (rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))).astype(dtype)
+    out_full = DeviceArray(ctx, full.shape, dtype)
+    gh.apply_layout(bw, DeviceArray.from_numpy(ctx, full), out_full, gh.CUBE_POINT)
+    oh = oracle_ham(sysm)
+    ref = oh.apply_sphere(np.ascontiguousarray(full.T).astype(np.complex128), bw.gindex(), k)
+    assert rel_err(out_full.numpy(), ref.T) < TOL[dtype]
