@@ -110,10 +110,13 @@ int inqx_ham_set_vector_potential(inqx_ham ham, const double a_cov[3]);
 /* one projector of projector_all (src/hamiltonian/projector.hpp:102-113,
  * projector_all.hpp:40-109).  The sphere is built as
  * basis::spherical_grid(grid, center, radius) (src/basis/spherical_grid.hpp:59-158).
- * matrix: host [nlm][npts] projector values on that sphere, or NULL for the
- * Gaussian s+p form exp(-r^2/(2 width^2)) Y_lm (l=0, then l=1 with m = -1,0,1
- * mapped to y, z, x).  kb_coeff: host [nlm][nlm] (projector.hpp:85-93).
- * Returns the projector index (>= 0) or a negative status. */
+ * nlm = 1..9 channels (l up to 2).  matrix: host [nlm][npts] projector
+ * values on that sphere, or NULL for the Gaussian form exp(-r^2/(2 width^2))
+ * Y_lm in the channel order of projector.hpp:63 (l = 0; l = 1 with m = -1, 0,
+ * 1 = y, z, x; l = 2 with m = -2..2 = xy, yz, 3z^2 - r^2, xz, x^2 - y^2, the
+ * standard real harmonics).  kb_coeff: host [nlm][nlm] (projector.hpp:85-93),
+ * any coupling between channels.  Returns the projector index (>= 0) or a
+ * negative status. */
 int inqx_ham_add_projector(inqx_ham ham, const double center[3], double radius, int nlm,
                            const double *matrix, const double *kb_coeff, double width);
 int inqx_ham_projector_size(inqx_ham ham, int iproj, long *npts);

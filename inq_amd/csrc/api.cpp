@@ -145,27 +145,43 @@ static void build_sphere(const Grid &g, const double *center, double radius, Pro
   }
 }
 
-// Gaussian s + p projector values on the float sphere tables (the
+// real spherical harmonic of channel ilm (l = 0: ilm 0; l = 1: ilm 1..3,
+// m = -1, 0, 1; l = 2: ilm 4..8, m = -2..2) at the cartesian vector v: the
+// standard real harmonics, Y_1m = sqrt(3/4pi) (y, z, x)/r and
+// Y_2m = sqrt(15/4pi) (xy, yz, (3z^2 - r^2)/(2 sqrt 3), xz, (x^2 - y^2)/2)/r^2,
+// zero at r = 0 for l > 0 (the order of projector.hpp:63, m - l for m in [0, 2l])
+static double real_ylm(int ilm, const double *v) {
+  if (ilm == 0) return 0.5 / std::sqrt(M_PI);
+  const double r2 = dot3(v, v);
+  if (r2 == 0.0) return 0.0;
+  const double x = v[0], y = v[1], z = v[2];
+  if (ilm <= 3) {
+    const double c = std::sqrt(3.0 / (4.0 * M_PI)) / std::sqrt(r2);
+    return c * (ilm == 1 ? y : (ilm == 2 ? z : x));
+  }
+  const double c = std::sqrt(15.0 / (4.0 * M_PI)) / r2;
+  switch (ilm) {
+    case 4: return c * x * y;
+    case 5: return c * y * z;
+    case 6: return c * (3.0 * z * z - r2) / (2.0 * std::sqrt(3.0));
+    case 7: return c * x * z;
+    default: return c * 0.5 * (x * x - y * y);
+  }
+}
+
+// Gaussian s, p (and d) projector values on the float sphere tables (the
 // reference's projector::build, src/hamiltonian/projector.hpp:38-66, with an
 // analytic radial function in place of the pseudopotential spline)
 static void gauss_matrix(const Grid &g, Projector &P, double width) {
   const long np = P.npts();
   P.matrix.assign((size_t)P.nlm * np, 0.0);
-  const double y00 = 0.5 / std::sqrt(M_PI), y1 = std::sqrt(3.0 / (4.0 * M_PI));
   for (long ip = 0; ip < np; ++ip) {
     const double rr = (double)P.dist[ip];
     const double radial = std::exp(-rr * rr / (2.0 * width * width));
     double c[3] = {(double)P.relpos[3 * ip], (double)P.relpos[3 * ip + 1], (double)P.relpos[3 * ip + 2]};
     double v[3];
     g.contra_to_cart(c, v);
-    const double r = std::sqrt(dot3(v, v));
-    for (int l = 0; l < P.nlm; ++l) {
-      double y;
-      if (l == 0) y = y00;
-      else if (r == 0.0) y = 0.0;
-      else y = y1 * (l == 1 ? v[1] : (l == 2 ? v[2] : v[0])) / r;
-      P.matrix[(size_t)l * np + ip] = radial * y;
-    }
+    for (int l = 0; l < P.nlm; ++l) P.matrix[(size_t)l * np + ip] = radial * real_ylm(l, v);
   }
 }
 
@@ -446,7 +462,7 @@ int inqx_ham_add_projector(inqx_ham h, const double center[3], double radius, in
   int idx = -1;
   API_BEGIN
   INQX_CHECK(h && center && kb_coeff, INQX_ERR_ARG, "null argument");
-  INQX_CHECK(nlm >= 1 && nlm <= kMaxNlm, INQX_ERR_UNSUPPORTED, "nlm must be 1..4");
+  INQX_CHECK(nlm >= 1 && nlm <= kMaxNlmAtom, INQX_ERR_UNSUPPORTED, "nlm must be 1..9 (channels up to l = 2)");
   INQX_CHECK(radius > 0.0, INQX_ERR_ARG, "radius must be positive");
   Projector P;
   for (int d = 0; d < 3; ++d) P.center[d] = center[d];

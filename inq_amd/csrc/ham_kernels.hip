@@ -84,7 +84,8 @@ struct HamDev {
   const int *ent_cs;
   const void *ent_b;
   const int *atom_slots, *atom_slot_list;
-  const double *coeff;
+  const double *coeff;    // [natoms][kMaxNlm][kCoeffCols]
+  const int *group_sib;   // [natoms] first sibling group, sibling count
   double kb_scale;  // dV / N
   double lat[9];    // lattice vectors (rows), for positions r of grid points
 };
@@ -760,9 +761,13 @@ __global__ void __launch_bounds__(xpass_threads<T>()) k_xpass_project(const Sphe
   }
 }
 
-// FP64 sum over an atom's slots, x dV, x D.  Writes D P per (atom, l, state)
-// for the energy record and per (slot, state, l) -- every slot of the atom --
-// so that the apply pass loads a tile's values as contiguous records.
+// FP64 sum over a KB group's slots, x dV, x D.  A projector with more than
+// four channels spans several groups: every group's thread sums the slots of
+// all groups of its projector and multiplies by its rows of the projector's
+// coefficient matrix (projector_all.hpp:136-189, multiply_by_coefficients
+// over all lm of an atom).  Writes D P per (group, l, state) for the energy
+// record and per (slot, state, l) -- every slot of the group -- so that the
+// apply pass loads a tile's values as contiguous records.
 template <typename T>
 __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                    int total, HamDev hd) {
@@ -775,51 +780,73 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
     const int idx = (wg - gp(prefix)[b]) * blockDim.x + threadIdx.x;
     if (idx >= hd.natoms * B.nst) return;
     const int a = idx / B.nst, s = idx - a * B.nst;
-    double pr[kMaxNlm], pi[kMaxNlm];
+    // the projections of every group of this group's projector (siblings),
+    // each the FP64 sum over its slots
+    const V2<int> sb = ld2<int>(hd.group_sib, a);
+    const int g0 = sb.x, ng = sb.y;
+    double pr[kMaxGroups][kMaxNlm], pi[kMaxGroups][kMaxNlm];
 #pragma unroll
-    for (int l = 0; l < kMaxNlm; ++l) pr[l] = pi[l] = 0.0;
-    const int q0 = gp(hd.atom_slots)[a], q1 = gp(hd.atom_slots)[a + 1];
+    for (int q = 0; q < kMaxGroups; ++q)
+#pragma unroll
+      for (int l = 0; l < kMaxNlm; ++l) pr[q][l] = pi[q][l] = 0.0;
     const GAS C *part = gp(B.partial);
-    // four slots per step: their slot indices, then their partial sums, all in
-    // flight together; added in slot order
-    for (int q = q0; q < q1; q += 4) {
-      int sl[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) sl[u] = gp(hd.atom_slot_list)[min(q + u, q1 - 1)];
-      C v[4][kMaxNlm];
+    for (int gq = 0; gq < kMaxGroups; ++gq) {
+      if (gq >= ng) break;
+      const int q0 = gp(hd.atom_slots)[g0 + gq], q1 = gp(hd.atom_slots)[g0 + gq + 1];
+      // four slots per step: their slot indices, then their partial sums, all in
+      // flight together; added in slot order
+      for (int q = q0; q < q1; q += 4) {
+        int sl[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 4; ++u) sl[u] = gp(hd.atom_slot_list)[min(q + u, q1 - 1)];
+        C v[4][kMaxNlm];
 #pragma unroll
-        for (int l = 0; l < kMaxNlm; ++l) v[u][l] = part[((size_t)sl[u] * kMaxNlm + l) * B.nst + s];
+        for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (q + u < q1)
+          for (int l = 0; l < kMaxNlm; ++l) v[u][l] = part[((size_t)sl[u] * kMaxNlm + l) * B.nst + s];
 #pragma unroll
-          for (int l = 0; l < kMaxNlm; ++l) {
-            pr[l] += (double)v[u][l].x;
-            pi[l] += (double)v[u][l].y;
-          }
+        for (int u = 0; u < 4; ++u)
+          if (q + u < q1)
+#pragma unroll
+            for (int l = 0; l < kMaxNlm; ++l) {
+              pr[gq][l] += (double)v[u][l].x;
+              pi[gq][l] += (double)v[u][l].y;
+            }
+      }
     }
-    const GAS double *D = gp(hd.coeff) + (size_t)a * kMaxNlm * kMaxNlm;
+    const int me = a - g0;  // this group among its siblings
+    const GAS double *D = gp(hd.coeff) + (size_t)a * kMaxNlm * kCoeffCols;
     GAS C *dp = gw(B.dp);
     if (B.proj) {
       GAS C *pj = gw(B.proj);
 #pragma unroll
-      for (int l = 0; l < kMaxNlm; ++l)
-        pj[((size_t)a * kMaxNlm + l) * B.nst + s] = C{(T)(pr[l] * hd.kb_scale), (T)(pi[l] * hd.kb_scale)};
+      for (int l = 0; l < kMaxNlm; ++l) {
+        double xr = 0.0, xi = 0.0;
+#pragma unroll
+        for (int q = 0; q < kMaxGroups; ++q)
+          if (q == me) {
+            xr = pr[q][l];
+            xi = pi[q][l];
+          }
+        pj[((size_t)a * kMaxNlm + l) * B.nst + s] = C{(T)(xr * hd.kb_scale), (T)(xi * hd.kb_scale)};
+      }
     }
     C r[kMaxNlm];
 #pragma unroll
     for (int l = 0; l < kMaxNlm; ++l) {
       double dr = 0.0, di = 0.0;
 #pragma unroll
-      for (int m = 0; m < kMaxNlm; ++m) {
-        dr += D[l * kMaxNlm + m] * pr[m];
-        di += D[l * kMaxNlm + m] * pi[m];
-      }
+      for (int q = 0; q < kMaxGroups; ++q)
+#pragma unroll
+        for (int m = 0; m < kMaxNlm; ++m) {
+          dr += D[l * kCoeffCols + q * kMaxNlm + m] * pr[q][m];
+          di += D[l * kCoeffCols + q * kMaxNlm + m] * pi[q][m];
+        }
       r[l] = C{(T)(dr * hd.kb_scale), (T)(di * hd.kb_scale)};
       dp[((size_t)a * kMaxNlm + l) * B.nst + s] = r[l];
     }
+    const int q0 = gp(hd.atom_slots)[a], q1 = gp(hd.atom_slots)[a + 1];
     GAS C *dps = gw(B.dps);
     for (int q = q0; q < q1; ++q) {
       GAS C *o = dps + ((size_t)gp(hd.atom_slot_list)[q] * B.nst + s) * kMaxNlm;
@@ -1174,7 +1201,19 @@ void build_kb_tables(inqx_ham_s *h) {
   hipStream_t st = h->ctx->c.stream;
   const bool dev = !h->ctx->c.host_only;
   const int n12 = g.n[1] * g.n[2];
-  kb.natoms = (int)h->projs.size();
+  // KB groups: projector j of nlm channels -> groups gfirst[j] .. + ng(j) - 1
+  std::vector<int> gproj, gindex, sib;  // per group: projector, group index in it; sibling (first, count)
+  for (int j = 0; j < (int)h->projs.size(); ++j) {
+    const int ng = (h->projs[j].nlm + kMaxNlm - 1) / kMaxNlm;
+    const int first = (int)gproj.size();
+    for (int q = 0; q < ng; ++q) {
+      gproj.push_back(j);
+      gindex.push_back(q);
+      sib.push_back(first);
+      sib.push_back(ng);
+    }
+  }
+  kb.natoms = (int)gproj.size();
   kb.maxsph = 1;
   kb.maxchunk = 0;
   for (auto &p : h->projs) kb.maxsph = std::max<int>(kb.maxsph, (int)p.npts());
@@ -1184,7 +1223,7 @@ void build_kb_tables(inqx_ham_s *h) {
   };
   std::vector<Mem> mem;
   for (int a = 0; a < kb.natoms; ++a) {
-    const Projector &P = h->projs[a];
+    const Projector &P = h->projs[gproj[a]];
     for (long ip = 0; ip < P.npts(); ++ip) {
       const int *c = &P.coords[3 * ip];
       const int L = c[1] * g.n[2] + c[2];
@@ -1241,9 +1280,10 @@ void build_kb_tables(inqx_ham_s *h) {
     kb.maxslot = std::max(kb.maxslot, tile_info[8 * m.tile + 1]);
     ent_cs.push_back(compact(m.tile, m.lane, m.i0));
     ent_cs.push_back((int)slot_atom.size() - 1 - tile_info[8 * m.tile + 2]);
-    const Projector &P = h->projs[m.a];
+    const Projector &P = h->projs[gproj[m.a]];
     for (int l = 0; l < kMaxNlm; ++l) {
-      const double v = l < P.nlm ? P.matrix[(size_t)l * P.npts() + m.p] : 0.0;
+      const int ch = kMaxNlm * gindex[m.a] + l;  // channel of the projector
+      const double v = ch < P.nlm ? P.matrix[(size_t)ch * P.npts() + m.p] : 0.0;
       eb64.push_back(v);
       eb32.push_back((float)v);
     }
@@ -1328,12 +1368,18 @@ void build_kb_tables(inqx_ham_s *h) {
     for (int x : atom_slot[a]) atom_slot_list.push_back(x);
   }
   atom_slots[kb.natoms] = (int)atom_slot_list.size();
-  std::vector<double> coeff((size_t)std::max(1, kb.natoms) * kMaxNlm * kMaxNlm, 0.0);
+  // coefficient rows of each group: rows 4q .. 4q+3 of its projector's D,
+  // columns over all channels of the projector (group by group)
+  std::vector<double> coeff((size_t)std::max(1, kb.natoms) * kMaxNlm * kCoeffCols, 0.0);
   for (int a = 0; a < kb.natoms; ++a) {
-    const Projector &P = h->projs[a];
-    for (int l = 0; l < P.nlm; ++l)
-      for (int m = 0; m < P.nlm; ++m) coeff[((size_t)a * kMaxNlm + l) * kMaxNlm + m] = P.coeff[l * P.nlm + m];
+    const Projector &P = h->projs[gproj[a]];
+    for (int l = 0; l < kMaxNlm; ++l) {
+      const int row = kMaxNlm * gindex[a] + l;
+      if (row >= P.nlm) continue;
+      for (int m = 0; m < P.nlm; ++m) coeff[((size_t)a * kMaxNlm + l) * kCoeffCols + m] = P.coeff[row * P.nlm + m];
+    }
   }
+  if (sib.empty()) sib.assign(2, 0);
   if (!dev) {
     kb.built = true;
     return;
@@ -1348,6 +1394,7 @@ void build_kb_tables(inqx_ham_s *h) {
   upload(kb.atom_slots, atom_slots, st);
   upload(kb.atom_slot_list, atom_slot_list, st);
   upload(kb.coeff, coeff, st);
+  upload(kb.group_sib, sib, st);
   INQX_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope
   kb.built = true;
 }
@@ -1936,6 +1983,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   hd.atom_slots = h->kb.atom_slots.as<int>();
   hd.atom_slot_list = h->kb.atom_slot_list.as<int>();
   hd.coeff = h->kb.coeff.as<double>();
+  hd.group_sib = h->kb.group_sib.as<int>();
   hd.kb_scale = (g.vol / (double)g.N) / (double)g.N;
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) hd.lat[3 * r + c] = g.a[r][c];

@@ -148,7 +148,14 @@ struct Projector {
 // line lengths with compiled register FFTs (radix-4/2/3/5/7 stages)
 #define INQX_SIZES(X) X(12) X(16) X(18) X(20) X(24) X(28) X(30) X(32) X(36) X(40) X(48) X(54) X(60) X(64) X(72) X(80) X(96)
 
-constexpr int kMaxNlm = 4;   // s + 3 p per atom, padded
+constexpr int kMaxNlm = 4;   // channels of one KB group (s + 3 p, padded)
+// A projector of nlm channels (up to l = 2: s + 3 p + 5 d = 9) is split into
+// ceil(nlm / 4) KB groups of four channels over the same sphere; the x passes
+// treat every group as an atom, and kb_reduce couples a projector's groups
+// through its full nlm x nlm coefficient matrix.
+constexpr int kMaxNlmAtom = 9;
+constexpr int kMaxGroups = (kMaxNlmAtom + kMaxNlm - 1) / kMaxNlm;  // groups of one projector
+constexpr int kCoeffCols = kMaxGroups * kMaxNlm;                   // coefficient row length per group
 constexpr int kExxFp64MaxGrid = 64;  // largest grid side of the complex128 exchange
 constexpr int kTile = 32;    // x-pencils (lines) per tile: one wavefront, a lane pair per line
 
@@ -161,7 +168,8 @@ constexpr int kTile = 32;    // x-pencils (lines) per tile: one wavefront, a lan
 struct KBTables {
   bool built = false;
   int gen = 0;  // incremented by every build: the per-basis phase tables follow it
-  int natoms = 0, maxsph = 0, ntile = 0, nslot = 0, maxpts = 0, maxslot = 0;
+  int natoms = 0;  // KB groups (see kMaxGroups): the x passes' atoms
+  int maxsph = 0, ntile = 0, nslot = 0, maxpts = 0, maxslot = 0;
   int maxchunk = 0;  // max over tiles of ceil(entries / 64): 64-entry rows of a tile's records
   long nent = 0;
   // per line: bitmask of sphere points along i0 (3 words) and the line's
@@ -175,7 +183,8 @@ struct KBTables {
   DevMem ent_b64;     // [nent] double4
   DevMem atom_slots;  // [natoms+1] offsets into atom_slot_list
   DevMem atom_slot_list;
-  DevMem coeff;       // [natoms][kMaxNlm][kMaxNlm] double
+  DevMem coeff;       // [natoms][kMaxNlm][kCoeffCols] double: group rows of the projector's D
+  DevMem group_sib;   // [natoms] int2: first group of the group's projector, number of groups
   std::vector<float> pos_host;  // [nent][4] the entry positions, for the per-basis phase tables
 };
 

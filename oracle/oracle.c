@@ -406,15 +406,31 @@ long orc_sphere_build(const orc_ham *h, const double center[3], double radius, i
 /* real spherical harmonics, builder-declared convention (the reference's
  * sharmonic library is not vendored): m-index 0..2l of l=1 maps to
  * (y, z, x); Y00 = 1/(2 sqrt(pi)); Y1m = sqrt(3/(4 pi)) u/r; Y1m(0) = 0. */
+/* real spherical harmonics up to l = 2 (declared convention: the standard
+ * real harmonics with m = -l..l, Y_1 = (y, z, x), Y_2 = (xy, yz, 3z^2 - r^2,
+ * xz, x^2 - y^2) with their normalisations; the reference takes them from the
+ * un-vendored sharmonic library, projector.hpp:63) */
 static double ylm_real(int l, int m, const double *v) {
   if (l == 0) return 0.5 / sqrt(M_PI);
-  double r = sqrt(dot3(v, v));
-  if (r == 0.0) return 0.0;
-  double c = sqrt(3.0 / (4.0 * M_PI));
-  if (m == -1) return c * v[1] / r;
-  if (m == 0) return c * v[2] / r;
-  return c * v[0] / r;
+  double r2 = dot3(v, v);
+  if (r2 == 0.0) return 0.0;
+  if (l == 1) {
+    double c = sqrt(3.0 / (4.0 * M_PI)) / sqrt(r2);
+    if (m == -1) return c * v[1];
+    if (m == 0) return c * v[2];
+    return c * v[0];
+  }
+  double c = sqrt(15.0 / (4.0 * M_PI)) / r2, x = v[0], y = v[1], z = v[2];
+  switch (m) {
+    case -2: return c * x * y;
+    case -1: return c * y * z;
+    case 0: return 0.5 * sqrt(5.0 / (4.0 * M_PI)) * (3.0 * z * z - r2) / r2;
+    case 1: return c * x * z;
+    default: return c * 0.5 * (x * x - y * y);
+  }
 }
+
+double orc_ylm_real(int l, int m, const double v[3]) { return ylm_real(l, m, v); }
 
 static proj_t *proj_push(orc_ham *h) {
   if (h->nproj == h->cap) {
@@ -451,8 +467,8 @@ int orc_ham_add_projector_gauss(orc_ham *h, const double center[3], double radiu
     double cart[3];
     contra_to_cart(&h->g, c, cart);
     for (int ilm = 0; ilm < nlm; ++ilm) {
-      int l = (ilm == 0) ? 0 : 1;
-      int m = (ilm == 0) ? 0 : ilm - 2; /* projector.hpp:63: m - l for m in [0, 2l] */
+      int l = (ilm == 0) ? 0 : (ilm < 4 ? 1 : 2);
+      int m = ilm - l * l - l; /* projector.hpp:63: m - l for m in [0, 2l] */
       p->matrix[(long)ilm * p->npts + ip] = radial * ylm_real(l, m, cart);
     }
   }

@@ -121,6 +121,9 @@ void orc_density_dipole_sphere(const orc_ham *h, int nst, long npw, const int *g
                                const orc_cplx *phi, const double *occ, double *density,
                                double dipole[3]);
 
+/* real spherical harmonic Y_lm(v / |v|), l <= 2, m = -l..l (projector channels) */
+double orc_ylm_real(int l, int m, const double v[3]);
+
 /* ---- Poisson / exchange ---- */
 void orc_poisson_in_place(const orc_ham *h, int nst, orc_cplx *rho, const double gshift_cart[3],
                           double zeroterm);

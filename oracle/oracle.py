@@ -71,6 +71,7 @@ def lib():
             "orc_density_dipole_sphere": (None, [P, C.c_int, C.c_long, ip, cp, dp, P, dp]),
             "orc_poisson_in_place": (None, [P, C.c_int, cp, dp, C.c_double]),
             "orc_sing_auxiliary": (C.c_double, [dp, dp]),
+            "orc_ylm_real": (C.c_double, [C.c_int, C.c_int, dp]),
             "orc_randomize": (None, [C.c_long, C.c_int, C.c_int, C.c_int, dp]),
             "orc_sing_correction": (None, [dp, C.c_int, dp, dp, dp, dp]),
             "orc_ace_build": (C.c_double, [P, C.c_int, cp, dp, C.c_double, cp]),
@@ -588,3 +589,8 @@ def ref_exchange_pairs(lattice, n, phi_rs, hf_rs, coef, threads):
     lib().rc_exchange_pairs(_f64(lattice).ravel(), np.asarray(n, np.int32), phi_rs.shape[1], phi_rs, hf_rs.shape[1],
                             hf_rs, _f64(coef), out, int(threads))
     return out
+
+
+def ylm_real(l, m, v):
+    """the oracle's real spherical harmonic Y_lm(v/|v|) (projector channels)"""
+    return float(lib().orc_ylm_real(int(l), int(m), _f64(v)))

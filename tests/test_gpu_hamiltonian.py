@@ -189,3 +189,36 @@ def test_reference_layouts(ctx, dtype):
     oh = oracle_ham(sysm)
     ref = oh.apply_sphere(np.ascontiguousarray(full.T).astype(np.complex128), bw.gindex(), k)
     assert rel_err(out_full.numpy(), ref.T) < TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_d_channel_projectors(ctx, dtype):
+    """projectors with 9 (s + p + d), 4 and 1 channels and full coefficient
+    matrices coupling every channel, at k != 0: H|psi> and the non-local
+    energy against the oracle (a 9-channel projector runs as three KB groups
+    coupled in kb_reduce)"""
+    from inq_amd import KSHamiltonian
+    from oracle import oracle as orc
+    sysm = S.small_config(seed=11, n=(20, 20, 20), natoms=3, nst=5, ecut=7.0)
+    say("case: d-channel projectors 20^3", np.dtype(dtype).name)
+    rng = np.random.default_rng(23)
+    v = S.local_potential(sysm)
+    gh = KSHamiltonian(ctx, sysm.lattice, sysm.n)
+    gh.set_scalar_potential(v)
+    oh = orc.Ham(sysm.lattice, sysm.n)
+    oh.set_potential(v)
+    for p, nlm in zip(sysm.positions, [9, 4, 1]):
+        A = rng.standard_normal((nlm, nlm))
+        D = 0.5 * (A + A.T)
+        gh.add_projector(p, sysm.kb_radius, nlm, D, width=sysm.kb_width)
+        oh.add_projector_gauss(p, sysm.kb_radius, nlm, sysm.kb_width, D)
+    k = (0.3, -0.15, 0.2)
+    b, c, out = _run(ctx, gh, sysm, k, sysm.ecut, sysm.nst, dtype, 29)
+    ci = c.astype(dtype).astype(np.complex128)
+    ref = oh.apply_sphere(ci, b.gindex(), k)
+    assert rel_err(out, ref) < TOL[dtype]
+    occ = np.linspace(2.0, 0.5, sysm.nst)
+    want = oh.energy_sphere(ci, b.gindex(), occ, k=b.kpoint)
+    got = gh.energy([b], [DeviceArray.from_numpy(ctx, c.astype(dtype))], [occ])
+    etol = 1e-10 if dtype == np.complex128 else 1e-5
+    assert abs(got["non_local"] - want[2]) < etol * max(1.0, abs(want[2])), (got, want)
