@@ -339,7 +339,7 @@ int inqx_prop_records(inqx_prop prop, int *count, double *records, int cap);
  * density (inqx_ks_potential).  These steps run eagerly (use_graph is
  * rejected with INQX_ERR_UNSUPPORTED): the pass count depends on the density
  * change, read after each pass.  max_passes = 0 returns to fixed-H steps.
- * Cubic grids of the Poisson sizes only. */
+ * Grids of the Poisson sizes only (every axis a built size, n1 = n2). */
 int inqx_prop_set_scf(inqx_prop prop, const double *vion, int max_passes, double threshold);
 /* State precision of an FP32 propagator: INQX_FP64 keeps a complex128 copy of
  * the states that the Taylor terms of the FP32 H applications accumulate into

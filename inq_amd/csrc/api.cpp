@@ -1245,7 +1245,7 @@ int inqx_hartree(inqx_ham h, const double *density, double *vh, double *energy) 
   API_BEGIN
   INQX_CHECK(h && density && (vh || energy), INQX_ERR_ARG, "bad arguments");
   INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
-  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "Poisson solves need a cubic grid of a built size");
+  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "Poisson solves need built grid sizes with n1 = n2");
   const long N = h->g.N;
   const int nch = hartree_chunks(N);
   h->hartree_work.ensure(16 * (size_t)N);
@@ -1271,7 +1271,7 @@ int inqx_hartree_noalloc(inqx_ham h, const double *density, double *vh, double *
   INQX_CHECK(h->g.N <= kNoAllocMaxPoints && hartree_noalloc_bytes(h->g.N) <= kNoAllocScratchBytes, INQX_ERR_ARG,
              "grid of " + std::to_string(h->g.N) + " points exceeds the allocation-free Hartree limit of " +
                  std::to_string(kNoAllocMaxPoints) + " points");
-  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "Poisson solves need a cubic grid of a built size");
+  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "Poisson solves need built grid sizes with n1 = n2");
   const double *total = launch_hartree_noalloc(h, density, vh);
   if (energy) {
     double e = 0.0;
@@ -1342,12 +1342,13 @@ static void exchange_direct_impl(inqx_ham h, Basis &B, int nst, int prec, const 
                                  const std::vector<HfBlock> &blocks, double coeff, double scale, void *out,
                                  bool accumulate) {
   Ctx &cx = h->ctx->c;
-  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "exchange needs a cubic grid of a built size");
+  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "exchange needs built grid sizes with n1 = n2");
   // Repeated complex128 exchanges on identical inputs have differed in a few
   // output states (about 1e-3 relative) at 72^3 and 96^3; no cause in the code
   // has been shown, so the complex128 pair path with occupied states is not
   // offered above 64^3.  Plain complex128 Poisson solves are not affected.
-  INQX_CHECK(!(prec == INQX_FP64 && h->g.n[0] > kExxFp64MaxGrid), INQX_ERR_UNSUPPORTED,
+  INQX_CHECK(!(prec == INQX_FP64 && std::max({h->g.n[0], h->g.n[1], h->g.n[2]}) > kExxFp64MaxGrid),
+             INQX_ERR_UNSUPPORTED,
              "complex128 exchange is not offered above 64^3 (use complex64, or the CPU oracle as the FP64 reference)");
   const size_t per = (prec == INQX_FP64 ? 16 : 8) * (size_t)h->g.N;
   const double factor = -0.5 * scale * coeff;

@@ -82,6 +82,7 @@ template <typename T> struct PairDev {
   const int *jlist;    // [nj]
   const double *coef;  // [nhf]
   int nst, nj;
+  int n0, n1, n2;      // grid (the plane passes take n0 at run time, the line pass n1 and n2)
   T b[3][3];           // reciprocal lattice vectors (rows, cartesian, 2 pi included)
   T q[3];              // cartesian shift added to G
   T scal;              // 4 pi / N
@@ -126,13 +127,13 @@ template <int N> __host__ __device__ constexpr int symi(int i) { return i >= (N 
 __device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
 
-template <typename T, int N0, int N1, int N2>
+template <typename T, int N1, int N2>
 __global__ void __launch_bounds__(256) k_pair_plane_fwd(PairDev<T> d, int total) {
   using C = cplx<T>;
   using Cfg = PairPlaneCfg<T, N1, N2>;
   constexpr int NP = N1 * N2;
   constexpr int H1 = N1 / 2;
-  constexpr long N = (long)N0 * NP;
+  const long N = (long)d.n0 * NP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x;
@@ -146,8 +147,8 @@ __global__ void __launch_bounds__(256) k_pair_plane_fwd(PairDev<T> d, int total)
     if (wg >= total) return;
     const int i = wg % d.nst;
     const int r = wg / d.nst;
-    const int i0 = r % N0;
-    const int jj = r / N0;
+    const int i0 = r % d.n0;
+    const int jj = r / d.n0;
     const size_t plane = (size_t)i0 * NP;
     const GAS C *ph = xg(d.phi) + (size_t)i * N + plane;
     const GAS C *hf = nullptr;
@@ -186,14 +187,18 @@ __global__ void __launch_bounds__(256) k_pair_plane_fwd(PairDev<T> d, int total)
 }
 
 // each line is split over a lane pair (lane, lane + 32): 32 lines per wave
-template <typename T, int N0, int N1, int N2>
+// CUBE: n1 = n2 = N0 known at compile time (the row strides of the loads
+// become constants; 143 instead of 198 VGPRs at 96^3 complex64), otherwise
+// n1 and n2 are read at run time
+template <typename T, int N0, bool CUBE>
 __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
   using C = cplx<T>;
-  constexpr int NP = N1 * N2;
+  const int N1 = CUBE ? N0 : d.n1, N2 = CUBE ? N0 : d.n2;
+  const int NP = N1 * N2;
   constexpr int H = N0 / 2;
-  constexpr int NTILE = (NP + 31) / 32;
-  constexpr int NGRP = (NTILE + 3) / 4;
-  constexpr long N = (long)N0 * NP;
+  const int NTILE = (NP + 31) / 32;
+  const int NGRP = (NTILE + 3) / 4;
+  const long N = (long)N0 * NP;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
   const bool upper = lane >= 32;
   const int half = upper ? 1 : 0;
@@ -242,13 +247,13 @@ __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
 // inverse 2D transform of one pair plane in place, the same decomposition as
 // the forward pass: the column transform reads t straight from memory, the
 // row transform runs from LDS.  k_pair_reduce then forms sum_j c_j hf_j t_j.
-template <typename T, int N0, int N1, int N2>
+template <typename T, int N1, int N2>
 __global__ void __launch_bounds__(256) k_pair_plane_inv(PairDev<T> d, int total) {
   using C = cplx<T>;
   using Cfg = PairPlaneCfg<T, N1, N2>;
   constexpr int NP = N1 * N2;
   constexpr int H1 = N1 / 2;
-  constexpr long N = (long)N0 * NP;
+  const long N = (long)d.n0 * NP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x;
@@ -262,8 +267,8 @@ __global__ void __launch_bounds__(256) k_pair_plane_inv(PairDev<T> d, int total)
     if (wg >= total) return;
     const int i = wg % d.nst;
     const int r = wg / d.nst;
-    const int i0 = r % N0;
-    const int jj = r / N0;
+    const int i0 = r % d.n0;
+    const int jj = r / d.n0;
     const size_t plane = (size_t)i0 * NP;
     GAS C *tp = xw(d.t) + ((size_t)jj * d.nst + i) * N + plane;
     {
@@ -345,7 +350,8 @@ __global__ void __launch_bounds__(256) k_pair_reduce(PairDev<T> d, long nvec) {
   }
 }
 
-template <typename T, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
+// S0: line length along i0; S: the plane size (n1 = n2)
+template <typename T, int S0, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
   using Cfg = PairPlaneCfg<T, S, S>;
   Ctx &cx = h->ctx->c;
   const Grid &g = h->g;
@@ -358,6 +364,9 @@ template <typename T, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
   d.coef = a.coef;
   d.nst = a.nst;
   d.nj = a.hf ? a.nj : 1;
+  d.n0 = g.n[0];
+  d.n1 = g.n[1];
+  d.n2 = g.n[2];
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) d.b[r][c] = (T)g.b[r][c];
   for (int c = 0; c < 3; ++c) d.q[c] = (T)a.gshift[c];
@@ -365,16 +374,16 @@ template <typename T, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
   d.zero = (T)(a.zeroterm / (double)g.N);
   static DeviceOnce attr;
   if (Cfg::LDS > 65536 && !attr.test_and_set(cx.device)) {
-    INQX_HIP(hipFuncSetAttribute((const void *)k_pair_plane_fwd<T, S, S, S>,
+    INQX_HIP(hipFuncSetAttribute((const void *)k_pair_plane_fwd<T, S, S>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
-    INQX_HIP(hipFuncSetAttribute((const void *)k_pair_plane_inv<T, S, S, S>,
+    INQX_HIP(hipFuncSetAttribute((const void *)k_pair_plane_inv<T, S, S>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
   }
   const bool f64 = std::is_same<T, double>::value;
-  const int tot1 = d.nj * S * d.nst;
+  const int tot1 = d.nj * S0 * d.nst;
   {
     ProfScope ps(cx, f64 ? "pair_plane_fwd_f64" : "pair_plane_fwd_f32");
-    hipLaunchKernelGGL((k_pair_plane_fwd<T, S, S, S>), dim3(tot1), dim3(Cfg::THREADS), Cfg::LDS,
+    hipLaunchKernelGGL((k_pair_plane_fwd<T, S, S>), dim3(tot1), dim3(Cfg::THREADS), Cfg::LDS,
                        cx.stream, d, tot1);
   }
   INQX_HIP(hipGetLastError());
@@ -382,12 +391,13 @@ template <typename T, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
   const int tot2 = d.nj * d.nst * ngrp;
   {
     ProfScope ps(cx, f64 ? "pair_xline_f64" : "pair_xline_f32");
-    hipLaunchKernelGGL((k_pair_xline<T, S, S, S>), dim3(tot2), dim3(256), 0, cx.stream, d, tot2);
+    if (S0 == S) hipLaunchKernelGGL((k_pair_xline<T, S0, true>), dim3(tot2), dim3(256), 0, cx.stream, d, tot2);
+    else hipLaunchKernelGGL((k_pair_xline<T, S0, false>), dim3(tot2), dim3(256), 0, cx.stream, d, tot2);
   }
   INQX_HIP(hipGetLastError());
   {
     ProfScope ps(cx, f64 ? "pair_plane_inv_f64" : "pair_plane_inv_f32");
-    hipLaunchKernelGGL((k_pair_plane_inv<T, S, S, S>), dim3(tot1), dim3(Cfg::THREADS), Cfg::LDS,
+    hipLaunchKernelGGL((k_pair_plane_inv<T, S, S>), dim3(tot1), dim3(Cfg::THREADS), Cfg::LDS,
                        cx.stream, d, tot1);
   }
   INQX_HIP(hipGetLastError());
@@ -520,27 +530,45 @@ void launch_hartree(inqx_ham_s *h, const double *rho, double *vh, void *work, do
 }
 
 bool pair_grid_supported(const int *n) {
-  if (n[0] != n[1] || n[1] != n[2]) return false;
-  switch (n[0]) {
+  if (n[1] != n[2]) return false;
+  auto ok = [](int v) {
+    switch (v) {
 #define CASE(S) case S:
+      INQX_SIZES(CASE)
+#undef CASE
+      return true;
+      default: return false;
+    }
+  };
+  return ok(n[0]) && ok(n[1]);
+}
+
+namespace {
+template <typename T, int S0> void pairs_n12(inqx_ham_s *h, const PairArgs &a) {
+  switch (h->g.n[1]) {
+#define CASE(S)                   \
+  case S:                         \
+    run_pairs<T, S0, S>(h, a);    \
+    break;
     INQX_SIZES(CASE)
 #undef CASE
-    return true;
-    default: return false;
+    default: break;
   }
 }
+}  // namespace
 
 void launch_pairs(inqx_ham_s *h, int prec, const PairArgs &a) {
   const int *n = h->g.n;
-  INQX_CHECK(pair_grid_supported(n), INQX_ERR_UNSUPPORTED, "pair Poisson solves need a cubic grid of a built size");
+  INQX_CHECK(pair_grid_supported(n), INQX_ERR_UNSUPPORTED,
+             "pair Poisson solves need built sizes with n1 = n2");
   INQX_CHECK(a.nst > 0 && a.nj > 0 && a.phi && a.t && a.w, INQX_ERR_ARG, "bad pair arguments");
   INQX_CHECK(!a.hf || (a.jlist && a.coef), INQX_ERR_ARG, "pair chunk without j list");
   INQX_CHECK(a.hf || a.t == a.w, INQX_ERR_ARG, "a plain Poisson solve works in place");
   switch (n[0]) {
-#define CASE(S)                                    \
-  case S:                                          \
-    if (prec == INQX_FP64) run_pairs<double, S>(h, a); \
-    else run_pairs<float, S>(h, a);                \
+#define CASE(S)                                       \
+  case S:                                             \
+    if (prec == INQX_FP64) pairs_n12<double, S>(h, a); \
+    else pairs_n12<float, S>(h, a);                   \
     break;
     INQX_SIZES(CASE)
 #undef CASE

@@ -385,3 +385,48 @@ def test_c5_si216_exchange_subset(ctx):
     kace = ex(sub).numpy()
     kdir = ex.direct(sub, sub, occb, scale=1.0).numpy()
     assert rel_err(kace, kdir) < TOL_X[np.complex64]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_noncubic_grid_poisson_exchange_hartree(ctx, dtype):
+    """a 24 x 16 x 16 grid (first axis longer): batched Poisson with shift and
+    zeroterm, exchange of states at k with occupied states at k and q, and the
+    Hartree potential, against the oracle"""
+    say("case: non-cubic 24x16x16 Poisson / exchange / Hartree", np.dtype(dtype).name)
+    lat2, ng2 = np.diag([9.0, 6.0, 6.0]), (24, 16, 16)
+    n2 = int(np.prod(ng2))
+    gh = KSHamiltonian(ctx, lat2, ng2)
+    oh = orc.Ham(lat2, ng2)
+    rng = np.random.default_rng(12)
+    rho = (rng.standard_normal((2, n2)) + 1j * rng.standard_normal((2, n2))).astype(dtype)
+    for q, z in (((0.0, 0.0, 0.0), 0.4), ((0.1, -0.2, 0.05), 0.0)):
+        got = gh.poisson(DeviceArray.from_numpy(ctx, rho), q, z).numpy()
+        ref = oh.poisson_in_place(np.ascontiguousarray(rho.astype(np.complex128).T), gshift=q, zeroterm=z).T
+        assert rel_err(got, ref) < TOL_T[dtype], (q, z)
+    # exchange at two k-points
+    k, q = (0.3, -0.2, 0.1), (-0.4, 0.5, 0.2)
+    bk, bq = gh.basis(k, 5.0), gh.basis(q, 5.0)
+    vel = 54.0 * 6.0 / float(n2) ** 2
+    phi = S.random_states(bk.npw, 3, 61, vel).astype(dtype)
+    hq = S.random_states(bq.npw, 2, 62, vel).astype(dtype)
+    ex = ExchangeOperator(gh, bk, exchange_coefficient=0.5)
+    got = ex.direct_kpts(DeviceArray.from_numpy(ctx, phi),
+                         [(bq, DeviceArray.from_numpy(ctx, hq), np.array([2.0, 1.0]), 0.0)]).numpy()
+
+    def rs(c, g):
+        cube = np.zeros((c.shape[0], n2), dtype=np.complex128)
+        cube[:, g] = c
+        return np.ascontiguousarray(np.fft.ifftn(cube.reshape((c.shape[0],) + ng2), axes=(1, 2, 3))
+                                    .reshape(c.shape[0], n2).T)
+    gk, gq = bk.gindex(), bq.gindex()
+    cart = lambda v: np.asarray(v, dtype=np.float64) @ np.linalg.inv(lat2).T
+    w = oh.exchange_direct(rs(phi.astype(np.complex128), gk), rs(hq.astype(np.complex128), gq),
+                           np.array([2.0, 1.0]), exx_coeff=0.5, gshift=cart(q) - cart(k), zeroterm=0.0)
+    ref = np.fft.fftn(np.ascontiguousarray(w.T).reshape((3,) + ng2), axes=(1, 2, 3)).reshape(3, n2)[:, gk]
+    assert rel_err(got, ref) < TOL_X[dtype]
+    # Hartree potential of a real density (complex128 path)
+    dens = 0.01 + rng.random(n2) * 0.02
+    vh, eh = gh.hartree(DeviceArray.from_numpy(ctx, dens))
+    vref = oh.poisson_in_place(dens.astype(np.complex128)[:, None], zeroterm=0.0)[:, 0].real
+    assert rel_err(vh.numpy(), vref) < 1e-11
+    assert eh == pytest.approx(0.5 * float(np.sum(dens * vref)) * 324.0 / n2, rel=1e-11)
