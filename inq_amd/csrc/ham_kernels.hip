@@ -375,7 +375,6 @@ k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, in
     const GAS int *pwl = gp(B.pw_lds) + seg0;
     const GAS int *rows = gp(B.rows) + r0;
     const int tot = np * nseg, nr = np * nrow;
-    constexpr int KG = gather_round<T>();
     // the first row-pass line's row index, loaded now
     const int lr0 = min(wave * 32 + (lane & 31), nr - 1);
     const int pr0 = div_rcp(lr0, nrow, rrow);
@@ -407,6 +406,32 @@ k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, in
       }
     }
     __syncthreads();
+    // the sphere values of this work-group's states s0 .. s0+np-1 through
+    // buffer resources based at point seg0 of state s0: one 32-bit offset per
+    // point, and points past the end use an offset beyond the buffer (loads
+    // give zero, stores are dropped) instead of a branch
+    const size_t base = (size_t)s0 * B.npw + seg0;
+    const unsigned nb = (unsigned)(((size_t)(np - 1) * B.npw + nseg) * sizeof(C));
+    const auto rpsi = make_rsrc(B.psi + base, nb);
+    const auto rk = make_rsrc(B.kin ? (const void *)(B.kin + seg0) : (const void *)B.psi,
+                              B.kin ? (unsigned)(nseg * sizeof(T)) : 0u);
+    // complex128: the LDS offsets of the gather's first round are loaded before
+    // the row transforms, so that their latency overlaps them (420 -> 358 us at
+    // Si64; complex64 has no registers to spare under its 168-VGPR limit and
+    // loads them in the gather)
+    constexpr int KG = gather_round<T>();
+    constexpr bool kEarly = sizeof(T) == 8;
+    int lx[KG];
+    auto gather_offsets = [&](int i0) {
+#pragma unroll
+      for (int u = 0; u < KG; ++u) {
+        const int i = i0 + u * nth;
+        const int ic = i < tot ? i : tot - 1;
+        const int p = div_rcp(ic, nseg, rseg), j = ic - p * nseg;
+        lx[u] = p * pstr + pwl[j];
+      }
+    };
+    if constexpr (kEarly) gather_offsets(tid);
     for (int g = wave; g * 32 < nr; g += Cfg::WAVES) {
       const int li = g * 32 + (lane & 31);
       const bool ok = li < nr;
@@ -416,15 +441,6 @@ k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, in
       plane_row_split<C, N2, -1>(lds + p * pstr + plane_row<T, N1>(y, wlo) * (N2 + 1), upper, ok);
     }
     __syncthreads();
-    // keep the gather's index arithmetic out of the transform phases' live range
-    __builtin_amdgcn_sched_barrier(0);
-    // the sphere values of this work-group's states s0 .. s0+np-1 through
-    // buffer resources based at point seg0 of state s0: one 32-bit offset per
-    // point, and points past the end use an offset beyond the buffer (loads
-    // give zero, stores are dropped) instead of a branch
-    const size_t base = (size_t)s0 * B.npw + seg0;
-    const unsigned nb = (unsigned)(((size_t)(np - 1) * B.npw + nseg) * sizeof(C));
-    const auto rpsi = make_rsrc(B.psi + base, nb);
     const auto rh = make_rsrc(B.hpsi + base, nb);
     // accumulators in the working precision, or complex128 (acc_wide: an FP64
     // state accumulating FP32 H applications; offsets then scale by 2)
@@ -433,20 +449,19 @@ k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, in
                                B.acc1 ? nb << wsh : 0u);
     const auto ra2 = make_rsrc(B.acc2 ? (const void *)((const char *)B.acc2 + (base * sizeof(C) << wsh)) : (const void *)(B.hpsi + base),
                                B.acc2 ? nb << wsh : 0u);
-    const auto rk = make_rsrc(B.kin ? (const void *)(B.kin + seg0) : (const void *)B.psi,
-                              B.kin ? (unsigned)(nseg * sizeof(T)) : 0u);
     for (int i0 = tid; i0 < tot; i0 += KG * nth) {
       // KG sphere points per thread and round: loads first, then the updates
       unsigned oo[KG];
       C outv[KG], cv[KG];
       T kv[KG];
+      if (!kEarly || i0 != tid) gather_offsets(i0);
 #pragma unroll
       for (int u = 0; u < KG; ++u) {
         const int i = i0 + u * nth;
         const int ic = i < tot ? i : tot - 1;
         const int p = div_rcp(ic, nseg, rseg), j = ic - p * nseg;
         oo[u] = i < tot ? (unsigned)((p * B.npw + j) * (int)sizeof(C)) : kOOB;
-        outv[u] = lds[p * pstr + pwl[j]];
+        outv[u] = lds[lx[u]];
         if (B.kin) {
           bload(rpsi, oo[u], 0u, cv[u]);
           bload(rk, (unsigned)(j * (int)sizeof(T)), 0u, kv[u]);
