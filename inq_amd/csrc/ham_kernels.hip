@@ -224,13 +224,13 @@ template <typename T> constexpr int xpass_threads() { return sizeof(T) == 4 ? 51
 // 16-byte complex structs here was kept in scratch memory instead of registers.)
 template <typename T, int KG>
 __device__ __forceinline__ void scatter_round(__amdgpu_buffer_rsrc_t rpsi, const GAS int *pwl, int npw, int nseg,
-                                              float rseg, int tot, int i0, int nth, int pstr, int (&ldx)[KG],
+                                              float rseg, int tot, int i0, int nth, int (&ldx)[KG],
                                               T (&vr)[KG], T (&vi)[KG]) {
 #pragma unroll
   for (int u = 0; u < KG; ++u) {
     const int i = min(i0 + u * nth, tot - 1);
     const int p = div_rcp(i, nseg, rseg), j = i - p * nseg;
-    ldx[u] = p * pstr + pwl[j];
+    ldx[u] = pwl[j];  // the plane's offset p pstr is added when the value is stored
     cplx<T> v;
     bload(rpsi, (unsigned)((p * npw + j) * (int)sizeof(cplx<T>)), 0u, v);
     vr[u] = v.x;
@@ -296,14 +296,16 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
                                 (unsigned)(((size_t)(np - 1) * B.npw + nseg) * sizeof(C)));
     int ldx[KG];
     T vr[KG], vi[KG];
-    if (tot > 0) scatter_round<T, KG>(rpsi, pwl, B.npw, nseg, rseg, tot, tid, nth, pstr, ldx, vr, vi);
+    if (tot > 0) scatter_round<T, KG>(rpsi, pwl, B.npw, nseg, rseg, tot, tid, nth, ldx, vr, vi);
     for (int i = tid; i < np * pstr; i += nth) lds[i] = C{0, 0};
     __syncthreads();
     for (int i0 = tid; i0 < tot; i0 += KG * nth) {
-      if (i0 != tid) scatter_round<T, KG>(rpsi, pwl, B.npw, nseg, rseg, tot, i0, nth, pstr, ldx, vr, vi);
+      if (i0 != tid) scatter_round<T, KG>(rpsi, pwl, B.npw, nseg, rseg, tot, i0, nth, ldx, vr, vi);
 #pragma unroll
-      for (int u = 0; u < KG; ++u)
-        if (i0 + u * nth < tot) lds[ldx[u]] = C{vr[u], vi[u]};
+      for (int u = 0; u < KG; ++u) {
+        const int i = i0 + u * nth;
+        if (i < tot) lds[div_rcp(i, nseg, rseg) * pstr + ldx[u]] = C{vr[u], vi[u]};
+      }
     }
     __syncthreads();
     // rows along i2 (sphere rows only), a lane pair per row
@@ -454,6 +456,8 @@ k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, in
       unsigned oo[KG];
       C outv[KG], cv[KG];
       T kv[KG];
+      // LDS offsets, then the sphere values and kinetic factors, then the LDS
+      // reads, so that no memory load waits behind another
       if (!kEarly || i0 != tid) gather_offsets(i0);
 #pragma unroll
       for (int u = 0; u < KG; ++u) {
@@ -461,12 +465,13 @@ k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, in
         const int ic = i < tot ? i : tot - 1;
         const int p = div_rcp(ic, nseg, rseg), j = ic - p * nseg;
         oo[u] = i < tot ? (unsigned)((p * B.npw + j) * (int)sizeof(C)) : kOOB;
-        outv[u] = lds[lx[u]];
         if (B.kin) {
           bload(rpsi, oo[u], 0u, cv[u]);
           bload(rk, (unsigned)(j * (int)sizeof(T)), 0u, kv[u]);
         }
       }
+#pragma unroll
+      for (int u = 0; u < KG; ++u) outv[u] = lds[lx[u]];
 #pragma unroll
       for (int u = 0; u < KG; ++u) {
         C out = outv[u];
