@@ -1374,6 +1374,19 @@ static void exchange_direct_impl(inqx_ham h, Basis &B, int nst, int prec, const 
   }
   const size_t njbudget = std::max<size_t>(1, std::min<size_t>(maxnhf, budget / (per * nst)));
   h->exx_t.ensure(per * nst * njbudget);
+  // hf = phi (the ACE build, or a direct exchange of a block with its own
+  // states): the pair potentials of (j, i) and (i, j) are complex conjugates
+  // (a real Coulomb kernel, no shift), so each unordered pair of occupied states
+  // is solved once; the skipped pairs' terms collect in exx_wb
+  const bool sym = blocks.size() == 1 && blocks[0].B == &B && blocks[0].hf == phi && blocks[0].nhf == nst;
+  if (sym) {
+    std::vector<int> occf(nst);
+    for (int s = 0; s < nst; ++s) occf[s] = std::fabs(blocks[0].occ[s]) >= 1e-10 ? 1 : 0;
+    upload(h->exx_occ, occf, cx.stream);
+    h->exx_wb.ensure(per * nst);
+    launch_fill_zero(cx, h->exx_wb.p, per * nst);
+    INQX_HIP(hipStreamSynchronize(cx.stream));  // occf is released on return
+  }
   for (const HfBlock &hb : blocks) {
     std::vector<int> jl;
     std::vector<double> cf(hb.nhf);
@@ -1409,9 +1422,14 @@ static void exchange_direct_impl(inqx_ham h, Basis &B, int nst, int prec, const 
       a.nj = (int)std::min(njmax, jl.size() - c0);
       for (int d = 0; d < 3; ++d) a.gshift[d] = gs[d];
       a.zeroterm = hb.zeroterm;
+      if (sym) {
+        a.sym_occ = h->exx_occ.as<int>();
+        a.wb = h->exx_wb.p;
+      }
       launch_pairs(h, prec, a);
     }
   }
+  if (sym) launch_add_inplace(cx, prec, h->exx_w.p, h->exx_wb.p, per * nst);
   transform_impl(h, B, nst, prec, 2, nullptr, h->exx_w.p, INQX_STATE_MAJOR, accumulate, out);
 }
 

@@ -83,6 +83,11 @@ template <typename T> struct PairDev {
   const double *coef;  // [nhf]
   int nst, nj;
   int n0, n1, n2;      // grid (the plane passes take n0 at run time, the line pass n1 and n2)
+  // symmetric pairs (hf = phi, no shift): pair (j, i) with i occupied and
+  // i > j is not solved; its potential is conj of pair (i, j)'s, added into
+  // wb by k_pair_reduce_sym.  occ: [nst] 1 for an occupied state, or null.
+  const int *occ;
+  cplx<T> *wb;         // [nst][N]
   T b[3][3];           // reciprocal lattice vectors (rows, cartesian, 2 pi included)
   T q[3];              // cartesian shift added to G
   T scal;              // 4 pi / N
@@ -102,6 +107,12 @@ template <typename T, int N1, int N2> struct PairPlaneCfg {
 
 // line index of the lane pair and the pair half of this thread
 __device__ __forceinline__ int pair_line(int tid) { return (tid >> 6) * 32 + (tid & 31); }
+
+// symmetric mode: the pair (hf index j of chunk entry jj, state i) is skipped
+// when both are occupied and i > j (the potential of pair (i, j) stands for it)
+template <typename T> __device__ __forceinline__ bool pair_skipped(const PairDev<T> &d, int jj, int i) {
+  return d.occ && i > xg(d.jlist)[jj] && xg(d.occ)[i] != 0;
+}
 
 // FFT along i2 of row r of the LDS plane, split over the lane pair; rows >= N1 run on a clamped row and do not store
 template <typename C, int N1, int N2, int SIGN>
@@ -149,6 +160,7 @@ __global__ void __launch_bounds__(256) k_pair_plane_fwd(PairDev<T> d, int total)
     const int r = wg / d.nst;
     const int i0 = r % d.n0;
     const int jj = r / d.n0;
+    if (pair_skipped(d, jj, i)) return;  // uniform over the work-group
     const size_t plane = (size_t)i0 * NP;
     const GAS C *ph = xg(d.phi) + (size_t)i * N + plane;
     const GAS C *hf = nullptr;
@@ -210,6 +222,7 @@ __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
     const int pair = wg / NGRP;
     const int tile = (wg - pair * NGRP) * 4 + wave;
     if (tile >= NTILE) return;  // wave-uniform; no barrier follows
+    if (pair_skipped(d, pair / d.nst, pair % d.nst)) return;
     const int L = tile * 32 + (lane & 31);
     const int Lc = L < NP ? L : NP - 1;
     const auto rs = make_rsrc(d.t + (size_t)pair * N, (unsigned)(N * sizeof(C)));
@@ -269,6 +282,7 @@ __global__ void __launch_bounds__(256) k_pair_plane_inv(PairDev<T> d, int total)
     const int r = wg / d.nst;
     const int i0 = r % d.n0;
     const int jj = r / d.n0;
+    if (pair_skipped(d, jj, i)) return;  // uniform over the work-group
     const size_t plane = (size_t)i0 * NP;
     GAS C *tp = xw(d.t) + ((size_t)jj * d.nst + i) * N + plane;
     {
@@ -330,6 +344,7 @@ __global__ void __launch_bounds__(256) k_pair_reduce(PairDev<T> d, long nvec) {
   }
   for (int jj = 0; jj < d.nj; ++jj) {
     const int j = xg(d.jlist)[jj];
+    if (d.occ && i > j && xg(d.occ)[i] != 0) continue;  // pair not solved: k_pair_reduce_sym adds it
     const T c = (T)xg(d.coef)[j];
     V x[U], h[U];
 #pragma unroll
@@ -350,6 +365,58 @@ __global__ void __launch_bounds__(256) k_pair_reduce(PairDev<T> d, long nvec) {
   }
 }
 
+// acc += c h conj(t)
+__device__ __forceinline__ void cmac_conj(float4 &a, float c, const float4 &h, const float4 &t) {
+  a.x += c * (h.x * t.x + h.y * t.y);
+  a.y += c * (h.y * t.x - h.x * t.y);
+  a.z += c * (h.z * t.z + h.w * t.w);
+  a.w += c * (h.w * t.z - h.z * t.w);
+}
+__device__ __forceinline__ void cmac_conj(double2 &a, double c, const double2 &h, const double2 &t) {
+  a.x += c * (h.x * t.x + h.y * t.y);
+  a.y += c * (h.y * t.x - h.x * t.y);
+}
+
+// symmetric mode: for every chunk entry jj (an occupied state j),
+// wb_j = sum over occupied states s < j, in s order, of c_s phi_s conj(t[jj][s])
+// -- the terms of w_j whose pair (j, s) was not solved (its potential is the
+// conjugate of pair (s, j)'s, exchange_operator.hpp:144-160 with real kernels).
+// Every w_j's terms land in one chunk, so wb needs no accumulation across chunks.
+template <typename T>
+__global__ void __launch_bounds__(256) k_pair_reduce_sym(PairDev<T> d, long nvec) {
+  using V = typename Vec16<T>::type;
+  constexpr int U = 4;
+  const int jj = (int)(blockIdx.x % (unsigned)d.nj);
+  const long v0 = (long)(blockIdx.x / (unsigned)d.nj) * 256 * U + threadIdx.x;
+  const int j = xg(d.jlist)[jj];
+  const GAS V *tv = xg(reinterpret_cast<const V *>(d.t)) + (size_t)jj * d.nst * nvec;
+  const GAS V *hv = xg(reinterpret_cast<const V *>(d.hf));
+  V acc[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) acc[u] = V{};
+  for (int s = 0; s < j; ++s) {
+    if (xg(d.occ)[s] == 0) continue;
+    const T c = (T)xg(d.coef)[s];
+    V x[U], h[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long v = v0 + u * 256;
+      if (v < nvec) {
+        x[u] = tv[(size_t)s * nvec + v];
+        h[u] = hv[(size_t)s * nvec + v];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) cmac_conj(acc[u], c, h[u], x[u]);
+  }
+  GAS V *bv = xw(reinterpret_cast<V *>(d.wb)) + (size_t)j * nvec;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const long v = v0 + u * 256;
+    if (v < nvec) bv[v] = acc[u];
+  }
+}
+
 // S0: line length along i0; S: the plane size (n1 = n2)
 template <typename T, int S0, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
   using Cfg = PairPlaneCfg<T, S, S>;
@@ -364,6 +431,8 @@ template <typename T, int S0, int S> void run_pairs(inqx_ham_s *h, const PairArg
   d.coef = a.coef;
   d.nst = a.nst;
   d.nj = a.hf ? a.nj : 1;
+  d.occ = a.hf ? a.sym_occ : nullptr;
+  d.wb = reinterpret_cast<cplx<T> *>(a.wb);
   d.n0 = g.n[0];
   d.n1 = g.n[1];
   d.n2 = g.n[2];
@@ -406,9 +475,17 @@ template <typename T, int S0, int S> void run_pairs(inqx_ham_s *h, const PairArg
     const long nvec = (long)g.N * 2 * (long)sizeof(T) / 16;
     const long per = (nvec + 1023) / 1024;
     const long tot4 = per * d.nst;
-    ProfScope ps(cx, f64 ? "pair_reduce_f64" : "pair_reduce_f32");
-    hipLaunchKernelGGL((k_pair_reduce<T>), dim3((unsigned)tot4), dim3(256), 0, cx.stream, d, nvec);
+    {
+      ProfScope ps(cx, f64 ? "pair_reduce_f64" : "pair_reduce_f32");
+      hipLaunchKernelGGL((k_pair_reduce<T>), dim3((unsigned)tot4), dim3(256), 0, cx.stream, d, nvec);
+    }
     INQX_HIP(hipGetLastError());
+    if (d.occ) {
+      const long tot5 = per * d.nj;
+      ProfScope ps(cx, f64 ? "pair_reduce_sym_f64" : "pair_reduce_sym_f32");
+      hipLaunchKernelGGL((k_pair_reduce_sym<T>), dim3((unsigned)tot5), dim3(256), 0, cx.stream, d, nvec);
+      INQX_HIP(hipGetLastError());
+    }
   }
 }
 
@@ -526,6 +603,46 @@ void launch_hartree(inqx_ham_s *h, const double *rho, double *vh, void *work, do
     hipLaunchKernelGGL(k_hartree_out, dim3((unsigned)hartree_chunks(N)), dim3(256), 0, cx.stream,
                        reinterpret_cast<const double2 *>(work), rho, vh, N, part);
   }
+  INQX_HIP(hipGetLastError());
+}
+
+namespace {
+__global__ void __launch_bounds__(256) k_add_vec4(float4 *w, const float4 *b, long n4) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n4) {
+    float4 x = xg(w)[i];
+    const float4 y = xg(b)[i];
+    x.x += y.x;
+    x.y += y.y;
+    x.z += y.z;
+    x.w += y.w;
+    xw(w)[i] = x;
+  }
+}
+__global__ void __launch_bounds__(256) k_add_dvec2(double2 *w, const double2 *b, long n2) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n2) {
+    double2 x = xg(w)[i];
+    const double2 y = xg(b)[i];
+    x.x += y.x;
+    x.y += y.y;
+    xw(w)[i] = x;
+  }
+}
+}  // namespace
+
+// w += b over bytes (a multiple of 16) of complex values of the precision
+void launch_add_inplace(Ctx &cx, int prec, void *w, const void *b, size_t bytes) {
+  const long n = (long)(bytes / 16);
+  if (n == 0) return;
+  INQX_CHECK((n + 255) / 256 <= INT32_MAX, INQX_ERR_ARG, "add: array too long");
+  ProfScope ps(cx, "pair_add_sym");
+  if (prec == INQX_FP64)
+    hipLaunchKernelGGL(k_add_dvec2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, cx.stream, (double2 *)w,
+                       (const double2 *)b, n);
+  else
+    hipLaunchKernelGGL(k_add_vec4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, cx.stream, (float4 *)w,
+                       (const float4 *)b, n);
   INQX_HIP(hipGetLastError());
 }
 

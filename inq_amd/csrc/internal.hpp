@@ -304,6 +304,7 @@ struct inqx_ham_s {
   int plane_batch[2] = {0, 0};
   // exact-exchange workspaces: real-space states, hf states, accumulator, pair planes
   inqx::DevMem exx_phi, exx_hf, exx_w, exx_t, exx_jlist, exx_coef;
+  inqx::DevMem exx_wb, exx_occ;  // symmetric pairs: the conjugate-pair accumulator and occupied flags
   // real-space apply and density workspaces (sphere arrays of a full-cube basis, occupations)
   inqx::DevMem rs_a, rs_b, dens_occ;
   inqx::DevMem lay_a, lay_b;  // band-major copies for inqx_ham_apply_layout
@@ -375,8 +376,15 @@ struct PairArgs {
   int nst = 0, nj = 1;
   double gshift[3] = {0, 0, 0};  // cartesian, added to G
   double zeroterm = 0.0;         // kernel value at |G + q|^2 < 1e-6 (times -4 pi)
+  // symmetric pairs (hf = phi, gshift = 0): device [nst] occupied flags; pairs
+  // (j, i) with both occupied and i > j are skipped and their terms of w_i land
+  // in wb [nst][N] (every occupied i whose index is in this chunk's j list)
+  const int *sym_occ = nullptr;
+  void *wb = nullptr;
 };
 void launch_pairs(inqx_ham_s *h, int prec, const PairArgs &a);
+// w += b, bytes a multiple of 16 (complex values of the precision)
+void launch_add_inplace(Ctx &cx, int prec, void *w, const void *b, size_t bytes);
 bool pair_grid_supported(const int *n);
 // Hartree potential of a real density [N] (FP64): work holds N complex128,
 // part hartree_chunks(N) doubles of sum rho vh partials (fixed order)

@@ -258,6 +258,34 @@ def test_chunked_pairs_are_bitwise_identical(ctx):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+def test_symmetric_pairs_match_full_pairs(ctx, dtype):
+    """hf = phi (the ACE build's case) solves each unordered pair of occupied
+    states once and takes the other's potential as its conjugate; with hf a copy
+    of phi (another array) every ordered pair is solved.  Both agree with each
+    other and with the oracle, with unoccupied states among them, and the
+    symmetric form does not depend on the chunking (bit for bit)"""
+    say("case: symmetric pairs 16^3", np.dtype(dtype).name)
+    gh = _ham(ctx)
+    b = gh.basis((0, 0, 0), 6.0)
+    g = b.gindex()
+    phi = _states(b, 7, 71).astype(dtype)
+    occ = np.array([2.0, 0.0, 1.0, 2.0, 0.0, 0.5, 2.0])
+    ex = ExchangeOperator(gh, b, exchange_coefficient=0.7, zeroterm=0.3)
+    d = DeviceArray.from_numpy(ctx, phi)
+    sym = ex.direct(d, d, occ).numpy()
+    full = ex.direct(d, DeviceArray.from_numpy(ctx, phi.copy()), occ).numpy()
+    assert rel_err(sym, full) < TOL_X[dtype]
+    oh = orc.Ham(LAT, NG)
+    p_rs = np.ascontiguousarray(_to_real(phi.astype(np.complex128), g).T)
+    k_rs = oh.exchange_direct(p_rs, p_rs, occ, exx_coeff=0.7, gshift=(0.0, 0.0, 0.0), zeroterm=0.3)
+    assert rel_err(sym, _to_sphere(np.ascontiguousarray(k_rs.T), g)) < TOL_X[dtype]
+    gh.set_exchange_workspace(1)  # one j per chunk
+    sym1 = ex.direct(d, d, occ).numpy()
+    gh.set_exchange_workspace(0)
+    assert np.array_equal(sym, sym1)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("n", [72, 96])
 def test_large_grid_paths(ctx, dtype, n):
     """the largest line lengths (their kernels run close to the register limit):
