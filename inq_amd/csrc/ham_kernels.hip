@@ -797,9 +797,15 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
     if (wg >= total) return;
     const int b = find_block(prefix, nblocks, wg);
     const SphereBlock<T> B = blocks[b];
-    const int idx = (wg - gp(prefix)[b]) * blockDim.x + threadIdx.x;
-    if (idx >= hd.natoms * B.nst) return;
-    const int a = idx / B.nst, s = idx - a * B.nst;
+    // four lanes per (group, state): lane quarter qr sums the slots qr, qr + 4,
+    // ... of every sibling group; the quarters are combined by two butterfly
+    // exchanges (every lane ends with the same sum, in a fixed order), and lane
+    // qr then writes channel qr
+    const int item = (wg - gp(prefix)[b]) * (blockDim.x >> 2) + (threadIdx.x >> 2);
+    const int qr = threadIdx.x & 3;
+    const bool live = item < hd.natoms * B.nst;  // uniform over the four lanes of an item
+    const int ic = live ? item : 0;
+    const int a = ic / B.nst, s = ic - a * B.nst;
     // the projections of every group of this group's projector (siblings),
     // each the FP64 sum over its slots
     const V2<int> sb = ld2<int>(hd.group_sib, a);
@@ -812,67 +818,72 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
     const GAS C *part = gp(B.partial);
 #pragma unroll
     for (int gq = 0; gq < kMaxGroups; ++gq) {
-      if (gq >= ng) break;
-      const int q0 = gp(hd.atom_slots)[g0 + gq], q1 = gp(hd.atom_slots)[g0 + gq + 1];
-      // four slots per step: their slot indices, then their partial sums, all in
-      // flight together; added in slot order
-      for (int q = q0; q < q1; q += 4) {
-        int sl[4];
+      if (gq < ng) {
+        const int q0 = gp(hd.atom_slots)[g0 + gq], q1 = gp(hd.atom_slots)[g0 + gq + 1];
+        // two of this lane's slots per step, their loads in flight together
+        for (int q = q0 + qr; q < q1; q += 8) {
+          const int sl0 = gp(hd.atom_slot_list)[q];
+          const int sl1 = gp(hd.atom_slot_list)[min(q + 4, q1 - 1)];
+          C v0[kMaxNlm], v1[kMaxNlm];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) sl[u] = gp(hd.atom_slot_list)[min(q + u, q1 - 1)];
-        C v[4][kMaxNlm];
+          for (int l = 0; l < kMaxNlm; ++l) {
+            v0[l] = part[((size_t)sl0 * kMaxNlm + l) * B.nst + s];
+            v1[l] = part[((size_t)sl1 * kMaxNlm + l) * B.nst + s];
+          }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int l = 0; l < kMaxNlm; ++l) v[u][l] = part[((size_t)sl[u] * kMaxNlm + l) * B.nst + s];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (q + u < q1)
+          for (int l = 0; l < kMaxNlm; ++l) {
+            pr[gq][l] += (double)v0[l].x;
+            pi[gq][l] += (double)v0[l].y;
+          }
+          if (q + 4 < q1) {
 #pragma unroll
             for (int l = 0; l < kMaxNlm; ++l) {
-              pr[gq][l] += (double)v[u][l].x;
-              pi[gq][l] += (double)v[u][l].y;
+              pr[gq][l] += (double)v1[l].x;
+              pi[gq][l] += (double)v1[l].y;
             }
+          }
+        }
       }
     }
-    const int me = a - g0;  // this group among its siblings
-    const GAS double *D = gp(hd.coeff) + (size_t)a * kMaxNlm * kCoeffCols;
-    GAS C *dp = gw(B.dp);
-    if (B.proj) {
-      GAS C *pj = gw(B.proj);
+    // (q0 + q1) + (q2 + q3) on every lane of the item
+#pragma unroll
+    for (int q = 0; q < kMaxGroups; ++q)
 #pragma unroll
       for (int l = 0; l < kMaxNlm; ++l) {
-        double xr = 0.0, xi = 0.0;
-#pragma unroll
-        for (int q = 0; q < kMaxGroups; ++q)
-          if (q == me) {
-            xr = pr[q][l];
-            xi = pi[q][l];
-          }
-        pj[((size_t)a * kMaxNlm + l) * B.nst + s] = C{(T)(xr * hd.kb_scale), (T)(xi * hd.kb_scale)};
+        pr[q][l] += __shfl_xor(pr[q][l], 1, 64);
+        pi[q][l] += __shfl_xor(pi[q][l], 1, 64);
+        pr[q][l] += __shfl_xor(pr[q][l], 2, 64);
+        pi[q][l] += __shfl_xor(pi[q][l], 2, 64);
       }
-    }
-    C r[kMaxNlm];
-#pragma unroll
-    for (int l = 0; l < kMaxNlm; ++l) {
-      double dr = 0.0, di = 0.0;
+    if (!live) return;
+    const int me = a - g0;  // this group among its siblings
+    const int l = qr;       // the channel this lane writes
+    if (B.proj) {
+      double xr = 0.0, xi = 0.0;
 #pragma unroll
       for (int q = 0; q < kMaxGroups; ++q)
 #pragma unroll
-        for (int m = 0; m < kMaxNlm; ++m) {
-          dr += D[l * kCoeffCols + q * kMaxNlm + m] * pr[q][m];
-          di += D[l * kCoeffCols + q * kMaxNlm + m] * pi[q][m];
-        }
-      r[l] = C{(T)(dr * hd.kb_scale), (T)(di * hd.kb_scale)};
-      dp[((size_t)a * kMaxNlm + l) * B.nst + s] = r[l];
+        for (int m = 0; m < kMaxNlm; ++m)
+          if (q == me && m == l) {
+            xr = pr[q][m];
+            xi = pi[q][m];
+          }
+      gw(B.proj)[((size_t)a * kMaxNlm + l) * B.nst + s] = C{(T)(xr * hd.kb_scale), (T)(xi * hd.kb_scale)};
     }
-    const int q0 = gp(hd.atom_slots)[a], q1 = gp(hd.atom_slots)[a + 1];
-    GAS C *dps = gw(B.dps);
-    for (int q = q0; q < q1; ++q) {
-      GAS C *o = dps + ((size_t)gp(hd.atom_slot_list)[q] * B.nst + s) * kMaxNlm;
+    const GAS double *D = gp(hd.coeff) + ((size_t)a * kMaxNlm + l) * kCoeffCols;
+    double dr = 0.0, di = 0.0;
 #pragma unroll
-      for (int l = 0; l < kMaxNlm; ++l) o[l] = r[l];
-    }
+    for (int q = 0; q < kMaxGroups; ++q)
+#pragma unroll
+      for (int m = 0; m < kMaxNlm; ++m) {
+        dr += D[q * kMaxNlm + m] * pr[q][m];
+        di += D[q * kMaxNlm + m] * pi[q][m];
+      }
+    const C r{(T)(dr * hd.kb_scale), (T)(di * hd.kb_scale)};
+    gw(B.dp)[((size_t)a * kMaxNlm + l) * B.nst + s] = r;
+    GAS C *dps = gw(B.dps);
+    const int q0 = gp(hd.atom_slots)[a], q1 = gp(hd.atom_slots)[a + 1];
+    for (int q = q0; q < q1; ++q) dps[((size_t)gp(hd.atom_slot_list)[q] * B.nst + s) * kMaxNlm + l] = r;
   }
 }
 
@@ -1954,7 +1965,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
         px[i + 1] = px[i] + x_tiles_padded(h->kb.ntile) * ((nst[i] + nw - 1) / nw);
       else
         px[i + 1] = px[i] + ld->ntile_line * ((nst[i] + 3) / 4);
-      pr[i + 1] = pr[i] + (std::max(1, h->kb.natoms) * nst[i] + 255) / 256;
+      pr[i + 1] = pr[i] + (std::max(1, h->kb.natoms) * nst[i] + 63) / 64;  // 64 (group, state) items per work-group
     }
     int nwin = 1;
     for (int i = 0; i < nblocks; ++i) nwin = std::max(nwin, bases[i]->nwin_max);
