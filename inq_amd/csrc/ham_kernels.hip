@@ -31,40 +31,13 @@
 // item and every array access goes through an explicit global-address-space
 // pointer, so the compiler emits scalar loads for uniform data and
 // global_load/global_store (not flat) for the streams.
-#include "internal.hpp"
+#include "plane_common.hpp"
 
 #include <algorithm>
 #include <cstring>
 #include <numeric>
 
-// global address space in the device pass; the host pass of the same
-// templates sees plain pointers
-#if defined(__HIP_DEVICE_COMPILE__)
-#define GAS __attribute__((address_space(1)))
-#else
-#define GAS
-#endif
-
 namespace inqx {
-
-template <typename T> __device__ __forceinline__ const GAS T *gp(const T *p) { return (const GAS T *)p; }
-template <typename T> __device__ __forceinline__ GAS T *gw(T *p) { return (GAS T *)p; }
-
-// 4- and 2-wide loads of plain records from global memory
-template <typename S> struct V4 {
-  S x, y, z, w;
-};
-template <typename S> struct V2 {
-  S x, y;
-};
-template <typename S> __device__ __forceinline__ V4<S> ld4(const void *p, size_t i) {
-  const GAS S *q = (const GAS S *)__builtin_assume_aligned(p, 4 * sizeof(S)) + 4 * i;
-  return V4<S>{q[0], q[1], q[2], q[3]};
-}
-template <typename S> __device__ __forceinline__ V2<S> ld2(const void *p, size_t i) {
-  const GAS S *q = (const GAS S *)__builtin_assume_aligned(p, 2 * sizeof(S)) + 2 * i;
-  return V2<S>{q[0], q[1]};
-}
 
 void invalidate_launch_cache(inqx_ham_s *h);
 
@@ -90,173 +63,13 @@ struct HamDev {
   double lat[9];    // lattice vectors (rows), for positions r of grid points
 };
 
-__device__ __forceinline__ int find_block(const int *prefix, int nblocks, int wg) {
-  const GAS int *pf = gp(prefix);
-  int lo = 0, hi = nblocks;  // prefix[lo] <= wg < prefix[hi]
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (pf[mid] <= wg) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
-
-// -------------------------------------------------------------------------------------------
-// plane passes
-// -------------------------------------------------------------------------------------------
-// ---- raw buffer access: per-lane byte offset in a VGPR, wave-uniform part in an SGPR
-// the base and size of a buffer resource are wave-uniform by construction;
-// readfirstlane makes that explicit so the resource stays in SGPRs
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, unsigned bytes) {
-  const unsigned long long a = (unsigned long long)base;
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
-  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
-  void *p = (void *)(((unsigned long long)hi << 32) | lo);
-  return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)__builtin_amdgcn_readfirstlane(bytes), 0x00020000);
-}
-__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<float> &v) {
-  const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-  v.x = __int_as_float(q[0]);
-  v.y = __int_as_float(q[1]);
-}
-__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<double> &v) {
-  const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
-  v.x = __longlong_as_double(((long long)(unsigned)q[1] << 32) | (unsigned)q[0]);
-  v.y = __longlong_as_double(((long long)(unsigned)q[3] << 32) | (unsigned)q[2]);
-}
-__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float &v) {
-  v = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double &v) {
-  const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-  v = __longlong_as_double(((long long)(unsigned)q[1] << 32) | (unsigned)q[0]);
-}
-__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<float> v) {
-  typedef int i2 __attribute__((ext_vector_type(2)));
-  i2 q = {__float_as_int(v.x), __float_as_int(v.y)};
-  __builtin_amdgcn_raw_buffer_store_b64(q, r, voff, soff, 0);
-}
-__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<double> v) {
-  typedef int i4 __attribute__((ext_vector_type(4)));
-  const long long a = __double_as_longlong(v.x), b = __double_as_longlong(v.y);
-  i4 q = {(int)a, (int)(a >> 32), (int)b, (int)(b >> 32)};
-  __builtin_amdgcn_raw_buffer_store_b128(q, r, voff, soff, 0);
-}
-
-// a per-lane buffer offset beyond every record: loads return 0, stores are dropped
-constexpr unsigned kOOB = 0x80000000u;
-
-// Plane layout in LDS.  complex64 planes hold only their row window (the
-// symmetric range of i1 that holds sphere rows, nwin rows from wlo) at LDS
-// rows 1 .. nwin, so more work-groups fit per CU; complex128 plane passes are
-// limited by registers to one wave per SIMD, and their planes hold all N1
-// rows at LDS row i1.  plane_row(y) is the LDS row of a row y that holds
-// sphere points; row_in_plane(y) whether row y has an LDS row at all.
-template <typename T> constexpr bool plane_window() { return sizeof(T) == 4; }
-template <typename T, int N> __device__ __forceinline__ int plane_row(int y, int wlo) {
-  if constexpr (plane_window<T>()) return (y >= (N + 1) / 2 ? y - N : y) - wlo + 1;
-  else return y;
-}
-template <typename T, int N> __device__ __forceinline__ bool row_in_plane(int y, int wlo, int nwin) {
-  if constexpr (plane_window<T>()) return (unsigned)((y >= (N + 1) / 2 ? y - N : y) - wlo) < (unsigned)nwin;
-  else return true;
-}
-
-// i / d for 0 <= i < 2^24 and d >= 1 from the float reciprocal rd = 1/d: the
-// float quotient is off by at most one, corrected from the remainder
-__device__ __forceinline__ int div_rcp(int i, int d, float rd) {
-  int q = (int)((float)i * rd);
-  const int r = i - q * d;
-  q += r >= d ? 1 : 0;
-  q -= r < 0 ? 1 : 0;
-  return q;
-}
-
-// FFT of one LDS row of N values split over the lane pair; lanes with ok ==
-// false transform a copy of a valid row and store nothing
-template <typename C, int N, int SIGN>
-__device__ __forceinline__ void plane_row_split(C *row, bool upper, bool ok) {
-  constexpr int H = N / 2;
-  C v[H];
-#pragma unroll
-  for (int n = 0; n < H; ++n) v[n] = row[2 * n + (upper ? 1 : 0)];
-  fft_split<N, SIGN, C>::dit(v, upper);
-  __builtin_amdgcn_wave_barrier();
-  if (ok) {
-#pragma unroll
-    for (int k = 0; k < H; ++k) row[k + (upper ? H : 0)] = v[k];
-  }
-}
-
-// In both plane passes every row and column of a plane is split over a lane
-// pair (lane, lane + 32): a wave transforms 32 lines; P planes share a
-// work-group.  A plane occupies LDS only for its row window (the rows that hold
-// sphere points, 51 of 72 at the 72^3 sphere), so more work-groups fit per CU.
-template <typename T, int N1, int N2> struct PlaneCfg {
-  static_assert(N1 % 2 == 0 && N2 % 2 == 0, "split lines need even lengths");
-  static constexpr int NL = (N1 > N2) ? N1 : N2;
-  // a plane in LDS (padded row stride N2+1): complex64 the rows of its row
-  // window (at most N1) and two more, complex128 all N1 rows
-  static constexpr int PLANE = (plane_window<T>() ? N1 + 2 : N1) * (N2 + 1);
-  static constexpr int PLANE_BYTES = PLANE * (int)sizeof(cplx<T>);
-  static constexpr int P0 = 128 / NL > 0 ? 128 / NL : 1;  // 128 lines per 256 threads
-  static constexpr int PL = (65536 / PLANE_BYTES) > 0 ? (65536 / PLANE_BYTES) : 1;
-  static constexpr int P = (P0 < PL ? P0 : PL) > 0 ? (P0 < PL ? P0 : PL) : 1;
-  static constexpr int WAVES = (P * NL + 31) / 32 < 4 ? (P * NL + 31) / 32 : 4;
-  static constexpr int THREADS = 64 * WAVES;  // split lines
-  static constexpr int LDS = P * PLANE_BYTES;
-};
-
-// sphere points per thread in one round of the plane passes' scatter and
-// gather: all their loads are in flight together (fewer for complex128, whose
-// transforms leave no registers for more without spilling)
-template <typename T> constexpr int gather_round() { return sizeof(T) == 4 ? 16 : 8; }
-
 // x-pass work-group size limit: up to 8 states for complex64; complex128
 // needs a wave per SIMD (VGPRs plus AGPRs) and takes at most 4
 template <typename T> constexpr int xpass_threads() { return sizeof(T) == 4 ? 512 : 256; }
 
-// loads of one scatter round: KG sphere points per thread from point index
-// i0 in steps of nth (past the end: the last point again, not stored later).
-// Point i is point j of plane p (i = p nseg + j); pwl[j] is its LDS offset in
-// the plane.  (Real and imaginary parts in separate arrays: an array of
-// 16-byte complex structs here was kept in scratch memory instead of registers.)
-template <typename T, int KG>
-__device__ __forceinline__ void scatter_round(__amdgpu_buffer_rsrc_t rpsi, const GAS int *pwl, int npw, int nseg,
-                                              float rseg, int tot, int i0, int nth, int (&ldx)[KG],
-                                              T (&vr)[KG], T (&vi)[KG]) {
-#pragma unroll
-  for (int u = 0; u < KG; ++u) {
-    const int i = min(i0 + u * nth, tot - 1);
-    const int p = div_rcp(i, nseg, rseg), j = i - p * nseg;
-    ldx[u] = pwl[j];  // the plane's offset p pstr is added when the value is stored
-    cplx<T> v;
-    bload(rpsi, (unsigned)((p * npw + j) * (int)sizeof(cplx<T>)), 0u, v);
-    vr[u] = v.x;
-    vi[u] = v.y;
-  }
-}
-
-// work item of a plane-pass work-group: block, slot and first state (the
-// block's work-groups take consecutive slots of the same states)
-struct PlaneItem {
-  int b, slot, s0, np;
-};
-template <typename T>
-__device__ __forceinline__ PlaneItem plane_item(const SphereBlock<T> *blocks, const int *prefix, int nblocks, int wg,
-                                                int P, SphereBlock<T> &B) {
-  PlaneItem it;
-  it.b = find_block(prefix, nblocks, wg);
-  B = blocks[it.b];
-  const int local = wg - gp(prefix)[it.b];
-  const int sgrp = local / B.nslot;
-  it.slot = local - sgrp * B.nslot;
-  it.s0 = sgrp * P;
-  it.np = min(P, B.nst - it.s0);
-  return it;
-}
-
+// -------------------------------------------------------------------------------------------
+// plane passes (shared helpers in plane_common.hpp)
+// -------------------------------------------------------------------------------------------
 template <typename T, int N1, int N2>
 __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                        int total, int pb, int pstr) {
@@ -409,14 +222,9 @@ k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, in
     }
     __syncthreads();
     // the sphere values of this work-group's states s0 .. s0+np-1 through
-    // buffer resources based at point seg0 of state s0: one 32-bit offset per
-    // point, and points past the end use an offset beyond the buffer (loads
-    // give zero, stores are dropped) instead of a branch
-    const size_t base = (size_t)s0 * B.npw + seg0;
-    const unsigned nb = (unsigned)(((size_t)(np - 1) * B.npw + nseg) * sizeof(C));
-    const auto rpsi = make_rsrc(B.psi + base, nb);
-    const auto rk = make_rsrc(B.kin ? (const void *)(B.kin + seg0) : (const void *)B.psi,
-                              B.kin ? (unsigned)(nseg * sizeof(T)) : 0u);
+    // buffer resources based at point seg0 of state s0 (past the end: an
+    // offset beyond the buffer, loads give zero and stores are dropped)
+    const GatherBufs R = gather_bufs(B, s0, seg0, np, nseg);
     // complex128: the LDS offsets of the gather's first round are loaded before
     // the row transforms, so that their latency overlaps them (420 -> 358 us at
     // Si64; complex64 has no registers to spare under its 168-VGPR limit and
@@ -424,16 +232,7 @@ k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, in
     constexpr int KG = gather_round<T>();
     constexpr bool kEarly = sizeof(T) == 8;
     int lx[KG];
-    auto gather_offsets = [&](int i0) {
-#pragma unroll
-      for (int u = 0; u < KG; ++u) {
-        const int i = i0 + u * nth;
-        const int ic = i < tot ? i : tot - 1;
-        const int p = div_rcp(ic, nseg, rseg), j = ic - p * nseg;
-        lx[u] = p * pstr + pwl[j];
-      }
-    };
-    if constexpr (kEarly) gather_offsets(tid);
+    if constexpr (kEarly) gather_offsets<KG>(lx, pwl, tid, nth, tot, nseg, rseg, pstr);
     for (int g = wave; g * 32 < nr; g += Cfg::WAVES) {
       const int li = g * 32 + (lane & 31);
       const bool ok = li < nr;
@@ -443,86 +242,9 @@ k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, in
       plane_row_split<C, N2, -1>(lds + p * pstr + plane_row<T, N1>(y, wlo) * (N2 + 1), upper, ok);
     }
     __syncthreads();
-    const auto rh = make_rsrc(B.hpsi + base, nb);
-    // accumulators in the working precision, or complex128 (acc_wide: an FP64
-    // state accumulating FP32 H applications; offsets then scale by 2)
-    const unsigned wsh = B.acc_wide ? 1u : 0u;
-    const auto ra1 = make_rsrc(B.acc1 ? (const void *)((const char *)B.acc1 + (base * sizeof(C) << wsh)) : (const void *)(B.hpsi + base),
-                               B.acc1 ? nb << wsh : 0u);
-    const auto ra2 = make_rsrc(B.acc2 ? (const void *)((const char *)B.acc2 + (base * sizeof(C) << wsh)) : (const void *)(B.hpsi + base),
-                               B.acc2 ? nb << wsh : 0u);
     for (int i0 = tid; i0 < tot; i0 += KG * nth) {
-      // KG sphere points per thread and round: loads first, then the updates
-      unsigned oo[KG];
-      C outv[KG], cv[KG];
-      T kv[KG];
-      // LDS offsets, then the sphere values and kinetic factors, then the LDS
-      // reads, so that no memory load waits behind another
-      if (!kEarly || i0 != tid) gather_offsets(i0);
-#pragma unroll
-      for (int u = 0; u < KG; ++u) {
-        const int i = i0 + u * nth;
-        const int ic = i < tot ? i : tot - 1;
-        const int p = div_rcp(ic, nseg, rseg), j = ic - p * nseg;
-        oo[u] = i < tot ? (unsigned)((p * B.npw + j) * (int)sizeof(C)) : kOOB;
-        if (B.kin) {
-          bload(rpsi, oo[u], 0u, cv[u]);
-          bload(rk, (unsigned)(j * (int)sizeof(T)), 0u, kv[u]);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < KG; ++u) outv[u] = lds[lx[u]];
-#pragma unroll
-      for (int u = 0; u < KG; ++u) {
-        C out = outv[u];
-        if (B.kin) {  // H application: + 1/2|G+k|^2 psi; without it a plain masked forward FFT
-          out.x += kv[u] * cv[u].x;
-          out.y += kv[u] * cv[u].y;
-        }
-        if (B.write_h) {
-          if (B.accumulate) {
-            C q;
-            bload(rh, oo[u], 0u, q);
-            q.x += out.x;
-            q.y += out.y;
-            bstore(rh, oo[u], 0u, q);
-          } else {
-            bstore(rh, oo[u], 0u, out);
-          }
-        }
-        if (B.acc1) {
-          if (B.acc_wide) {
-            cplx<double> q;
-            const unsigned ow = oo[u] == kOOB ? kOOB : oo[u] << 1;
-            bload(ra1, ow, 0u, q);
-            q.x += (double)c1r * (double)out.x - (double)c1i * (double)out.y;
-            q.y += (double)c1r * (double)out.y + (double)c1i * (double)out.x;
-            bstore(ra1, ow, 0u, q);
-          } else {
-            C q;
-            bload(ra1, oo[u], 0u, q);
-            q.x += c1r * out.x - c1i * out.y;
-            q.y += c1r * out.y + c1i * out.x;
-            bstore(ra1, oo[u], 0u, q);
-          }
-        }
-        if (B.acc2) {
-          if (B.acc_wide) {
-            cplx<double> q;
-            const unsigned ow = oo[u] == kOOB ? kOOB : oo[u] << 1;
-            bload(ra2, ow, 0u, q);
-            q.x += (double)c2r * (double)out.x - (double)c2i * (double)out.y;
-            q.y += (double)c2r * (double)out.y + (double)c2i * (double)out.x;
-            bstore(ra2, ow, 0u, q);
-          } else {
-            C q;
-            bload(ra2, oo[u], 0u, q);
-            q.x += c2r * out.x - c2i * out.y;
-            q.y += c2r * out.y + c2i * out.x;
-            bstore(ra2, oo[u], 0u, q);
-          }
-        }
-      }
+      if (!kEarly || i0 != tid) gather_offsets<KG>(lx, pwl, i0, nth, tot, nseg, rseg, pstr);
+      gather_update<T, KG>(B, R, lds, lx, i0, nth, tot, nseg, rseg, c1r, c1i, c2r, c2i);
     }
     __syncthreads();
   }
@@ -1211,7 +933,6 @@ __global__ void __launch_bounds__(256) k_obs_combine(const ObsBlock *ob, int nbl
 // -------------------------------------------------------------------------------------------
 
 bool grid_supported(const int *n) {
-  if (n[1] != n[2]) return false;
   auto ok = [](int v) {
     switch (v) {
 #define CASE(S) case S:
@@ -1221,7 +942,7 @@ bool grid_supported(const int *n) {
       default: return false;
     }
   };
-  return ok(n[0]) && ok(n[1]) && n[0] <= 96;
+  return ok(n[0]) && ok(n[1]) && ok(n[2]) && n[0] <= 96;
 }
 
 void build_kb_tables(inqx_ham_s *h) {
@@ -1689,20 +1410,46 @@ template <typename T, int N0> static void set_x_attr(int device, int lds) {
   d = lds;
 }
 
+// N12 > 0: square planes N12 x N12 (one kernel per plane pass); N12 == 0:
+// planes n1 x n2 with n1 != n2 (row and column kernels, plane_rect.hip)
 template <typename T, int N0, int N12>
 static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev &hd, const double *cf) {
-  using Cfg = PlaneCfg<T, N12, N12>;
+  constexpr int NP = N12 > 0 ? N12 : 16;  // (square configuration; unused when N12 == 0)
+  using Cfg = PlaneCfg<T, NP, NP>;
   hipStream_t st = h->ctx->c.stream;
   const SphereBlock<T> *blk = ld.desc.as<SphereBlock<T>>();
   const int *pref_plane = reinterpret_cast<const int *>(ld.desc.as<char>() + ld.off_pref[0]);
   const int *pref_x = reinterpret_cast<const int *>(ld.desc.as<char>() + ld.off_pref[1]);
   const int *pref_red = reinterpret_cast<const int *>(ld.desc.as<char>() + ld.off_pref[2]);
-  set_plane_attr<T, N12, N12>(h->ctx->c.device);
+  if constexpr (N12 > 0) set_plane_attr<T, N12, N12>(h->ctx->c.device);
   const int cu = h->ctx->c.num_cu;
   const int grid_plane = ld.total_plane;  // one item per work-group
   const int plane_lds = ld.plane_batch * ld.plane_pstr * (int)sizeof(cplx<T>);
   Ctx &cx = h->ctx->c;
   const bool f64 = std::is_same<T, double>::value;
+  auto plane_inv = [&]() {
+    if constexpr (N12 > 0) {
+      ProfScope ps(cx, f64 ? "plane_inverse_f64" : "plane_inverse_f32");
+      hipLaunchKernelGGL((k_plane_inverse<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), plane_lds, st, blk,
+                         pref_plane, nblocks, ld.total_plane, ld.plane_batch, ld.plane_pstr);
+      INQX_HIP(hipGetLastError());
+    } else {
+      launch_rect_inverse<T>(cx, blk, pref_plane, nblocks, ld.total_plane, ld.plane_batch, ld.plane_pstr, hd.n1,
+                             hd.n2);
+    }
+  };
+  auto plane_fwd = [&](const double *c) {
+    if constexpr (N12 > 0) {
+      ProfScope ps(cx, f64 ? "plane_forward_f64" : "plane_forward_f32");
+      hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), plane_lds, st, blk,
+                         pref_plane, nblocks, ld.total_plane, ld.plane_batch, ld.plane_pstr, (T)c[0], (T)c[1],
+                         (T)c[2], (T)c[3]);
+      INQX_HIP(hipGetLastError());
+    } else {
+      launch_rect_forward<T>(cx, blk, pref_plane, nblocks, ld.total_plane, ld.plane_batch, ld.plane_pstr, hd.n1,
+                             hd.n2, c);
+    }
+  };
   if (ld.mode == 2) {  // real space -> sphere
     const int grid_l = ld.total_line;
     {
@@ -1711,20 +1458,11 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
                          ld.total_line, hd.n12, ld.ntile_line);
     }
     INQX_HIP(hipGetLastError());
-    {
-      ProfScope ps(cx, f64 ? "plane_forward_f64" : "plane_forward_f32");
-      hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), plane_lds, st, blk,
-                         pref_plane, nblocks, ld.total_plane, ld.plane_batch, ld.plane_pstr, (T)0, (T)0, (T)0, (T)0);
-    }
-    INQX_HIP(hipGetLastError());
+    const double zero[4] = {0.0, 0.0, 0.0, 0.0};
+    plane_fwd(zero);
     return;
   }
-  {
-  ProfScope ps(cx, f64 ? "plane_inverse_f64" : "plane_inverse_f32");
-  hipLaunchKernelGGL((k_plane_inverse<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), plane_lds, st, blk,
-                     pref_plane, nblocks, ld.total_plane, ld.plane_batch, ld.plane_pstr);
-  }
-  INQX_HIP(hipGetLastError());
+  plane_inv();
   if (ld.mode == 1) {  // sphere -> real space
     const int grid_l = ld.total_line;
     ProfScope ps(cx, f64 ? "line_to_real_f64" : "line_to_real_f32");
@@ -1756,20 +1494,21 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
                        ld.total_x, hd, ld.xl_apply);
   }
   INQX_HIP(hipGetLastError());
-  {
-    ProfScope ps(cx, f64 ? "plane_forward_f64" : "plane_forward_f32");
-    hipLaunchKernelGGL((k_plane_forward<T, N12, N12>), dim3(grid_plane), dim3(Cfg::THREADS), plane_lds, st, blk,
-                       pref_plane, nblocks, ld.total_plane, ld.plane_batch, ld.plane_pstr, (T)cf[0], (T)cf[1], (T)cf[2], (T)cf[3]);
-  }
-  INQX_HIP(hipGetLastError());
+  plane_fwd(cf);
 }
 
-// the pass templates for a grid n0 x n1 x n1: x-pass line length N0 and
-// plane size N12 x N12 are independent compile-time sizes (host code only:
-// every kernel is instantiated once per size)
+// the pass templates for a grid n0 x n1 x n2: x-pass line length N0 and the
+// plane size (N12 x N12 when n1 = n2, else the row and column kernels of
+// plane_rect.hip) are independent compile-time sizes (host code only: every
+// kernel is instantiated once per size)
 template <typename T, int N0>
-static void dispatch_n12(int n12, inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev &hd, const double *cf) {
-  switch (n12) {
+static void dispatch_n12(int n1, int n2, inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev &hd,
+                         const double *cf) {
+  if (n1 != n2) {
+    run_passes<T, N0, 0>(h, ld, nblocks, hd, cf);
+    return;
+  }
+  switch (n1) {
 #define CASE(S)                                     \
   case S:                                           \
     run_passes<T, N0, S>(h, ld, nblocks, hd, cf);   \
@@ -1781,12 +1520,12 @@ static void dispatch_n12(int n12, inqx_ham_s *h, LaunchData &ld, int nblocks, co
   }
 }
 template <typename T>
-static void dispatch_passes(int n0, int n12, inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev &hd,
+static void dispatch_passes(int n0, int n1, int n2, inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev &hd,
                             const double *cf) {
   switch (n0) {
-#define CASE(S)                                         \
-  case S:                                               \
-    dispatch_n12<T, S>(n12, h, ld, nblocks, hd, cf);    \
+#define CASE(S)                                             \
+  case S:                                                   \
+    dispatch_n12<T, S>(n1, n2, h, ld, nblocks, hd, cf);     \
     break;
     INQX_SIZES(CASE)
 #undef CASE
@@ -1869,19 +1608,8 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     INQX_CHECK(!g_capture_keep, INQX_ERR_STATE, "launch configuration first seen during graph capture");
     auto ld = std::make_shared<LaunchData>();
     std::vector<SphereBlock<T>> blk(nblocks);
-    const int P = PlaneCfg<T, 16, 16>::P;  // placeholder, replaced below by the real plane config
-    (void)P;
     std::vector<int> pp(nblocks + 1, 0), px(nblocks + 1, 0), pr(nblocks + 1, 0);
-    int plane_p = 1;
-    switch (g.n[1]) {
-#define CASE(S)                            \
-  case S:                                  \
-    plane_p = PlaneCfg<T, S, S>::P;        \
-    break;
-      INQX_SIZES(CASE)
-#undef CASE
-      default: break;
-    }
+    int plane_p = max_plane_batch(g.n, prec);
     // the launch shape's plane batch (autotuned), at most what the LDS budget allows
     if (h->plane_batch[prec] > 0) plane_p = std::min(plane_p, h->plane_batch[prec]);
     ld->plane_batch = plane_p;
@@ -2019,13 +1747,14 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) hd.lat[3 * r + c] = g.a[r][c];
   const double cf[4] = {opts->c1r, opts->c1i, opts->c2r, opts->c2i};
-  dispatch_passes<T>(g.n[0], g.n[1], h, ld, nblocks, hd, cf);
+  dispatch_passes<T>(g.n[0], g.n[1], g.n[2], h, ld, nblocks, hd, cf);
 }
 
 int dots_chunks(int npw) { return (npw + kDotChunk - 1) / kDotChunk; }
 
 // largest plane batch the plane passes of this grid and precision take
 int max_plane_batch(const int *n, int prec) {
+  if (n[1] != n[2]) return rect_plane_batch(prec, n[1], n[2]);
   switch (n[1]) {
 #define CASE(S) \
   case S:       \

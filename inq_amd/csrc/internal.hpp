@@ -333,6 +333,16 @@ void ensure_basis_device(inqx_ham_s *h, Basis &b);
 void ensure_workspace(inqx_ham_s *h, Basis &b, int nst, int prec);
 bool grid_supported(const int *n);
 int max_plane_batch(const int *n, int prec);
+// plane passes of a grid with n1 != n2 (plane_rect.hip): rows and columns in
+// separate kernels, each built for one line length, the planes passing through t1
+bool rect_plane_supported(int n1, int n2);
+int rect_plane_batch(int prec, int n1, int n2);
+template <typename T>
+void launch_rect_inverse(Ctx &cx, const SphereBlock<T> *blk, const int *pref, int nblocks, int total, int pb,
+                         int pstr, int n1, int n2);
+template <typename T>
+void launch_rect_forward(Ctx &cx, const SphereBlock<T> *blk, const int *pref, int nblocks, int total, int pb,
+                         int pstr, int n1, int n2, const double *cf);
 // linalg.hip
 void launch_overlap(Ctx &cx, int prec, int n1, int n2, long K, const void *A, const void *B, double vol, double *S,
                     DevMem &partial);

@@ -65,6 +65,41 @@ def test_grid_n0_differs_from_n1(ctx, dtype, n, lat):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,lat", [((24, 20, 30), (6.0, 5.0, 7.5)), ((20, 32, 16), (5.0, 8.0, 4.0))])
+def test_plane_n1_differs_from_n2(ctx, dtype, n, lat):
+    """planes that are not square (n1 != n2 runs the row and column kernels)
+    at k != 0 with three atoms: H on the sphere and H on a whole-cube basis
+    (every plane row holds sphere points) against the oracle, and the sphere
+    <-> real-space transforms against numpy's FFT"""
+    rng = np.random.default_rng(23)
+    L = np.diag(lat)
+    sysm = S.System(f"rect{n[1]}x{n[2]}", 191, L, n, rng.uniform(-0.5, 0.5, (3, 3)) @ L, 8.0, 5,
+                    v_sigma=0.6, kb_width=0.7, kb_radius=1.6)
+    k = (-0.3, 0.2, 0.35)
+    say("case: plane", n, np.dtype(dtype).name)
+    gh = gpu_ham(ctx, sysm)
+    oh = oracle_ham(sysm)
+    b, c, out = _run(ctx, gh, sysm, k, sysm.ecut, sysm.nst, dtype, 29)
+    ref = oh.apply_sphere(c.astype(dtype).astype(np.complex128), b.gindex(), k)
+    assert rel_err(out, ref) < TOL[dtype]
+    bw, cw, outw = _run(ctx, gh, sysm, k, 1e6, 3, dtype, 31)
+    assert bw.npw == sysm.N
+    refw = oh.apply_sphere(cw.astype(dtype).astype(np.complex128), bw.gindex(), k)
+    assert rel_err(outw, refw) < TOL[dtype]
+    # sphere -> real space (x 1/N) and back (unnormalised), both layouts
+    g = b.gindex()
+    cube = np.zeros((sysm.nst, sysm.N), dtype=np.complex128)
+    cube[:, g] = c.astype(dtype)
+    want = np.fft.ifftn(cube.reshape((sysm.nst,) + tuple(n)), axes=(1, 2, 3)).reshape(sysm.nst, sysm.N)
+    d = DeviceArray.from_numpy(ctx, c.astype(dtype))
+    for layout in (0, 1):
+        rs = gh.to_real(b, d, layout).numpy()
+        assert rel_err(rs if layout == 0 else rs.T, want) < TOL[dtype]
+        back = gh.to_sphere(b, DeviceArray.from_numpy(ctx, rs), sysm.nst, layout).numpy()
+        assert rel_err(back, c.astype(dtype)) < TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 def test_c1_diamond(ctx, dtype):
     sysm = S.config(1)
     say("case: C1 diamond 32^3", np.dtype(dtype).name)
