@@ -95,8 +95,7 @@ int inqx_stream_copy(inqx_ctx ctx, const void *src, void *dst, size_t bytes);
  * real-space grid (src/basis/real_space.hpp:39-50).  The sphere path takes any
  * axis length of 12, 16, 18, 20, 24, 28, 30, 32, 36, 40, 48, 54, 60, 64, 72, 80
  * or 96 on each axis independently (other lengths return INQX_ERR_UNSUPPORTED
- * from the calls that transform); the Poisson, Hartree and exchange calls also
- * need n1 = n2. */
+ * from the calls that transform). */
 int inqx_ham_create(inqx_ctx ctx, const double lattice[9], const int n[3], inqx_ham *out);
 int inqx_ham_destroy(inqx_ham ham);
 /* ks_hamiltonian::scalar_potential() (ks_hamiltonian.hpp:241-243): host array
@@ -343,7 +342,7 @@ int inqx_prop_records(inqx_prop prop, int *count, double *records, int cap);
  * density (inqx_ks_potential).  These steps run eagerly (use_graph is
  * rejected with INQX_ERR_UNSUPPORTED): the pass count depends on the density
  * change, read after each pass.  max_passes = 0 returns to fixed-H steps.
- * Grids of the Poisson sizes only (every axis a built size, n1 = n2). */
+ * Grids of built axis lengths only. */
 int inqx_prop_set_scf(inqx_prop prop, const double *vion, int max_passes, double threshold);
 /* State precision of an FP32 propagator: INQX_FP64 keeps a complex128 copy of
  * the states that the Taylor terms of the FP32 H applications accumulate into

@@ -1245,7 +1245,7 @@ int inqx_hartree(inqx_ham h, const double *density, double *vh, double *energy) 
   API_BEGIN
   INQX_CHECK(h && density && (vh || energy), INQX_ERR_ARG, "bad arguments");
   INQX_CHECK(!ctx->c.host_only, INQX_ERR_STATE, "host-only context");
-  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "Poisson solves need built grid sizes with n1 = n2");
+  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "Poisson solves need built grid sizes");
   const long N = h->g.N;
   const int nch = hartree_chunks(N);
   h->hartree_work.ensure(16 * (size_t)N);
@@ -1271,7 +1271,7 @@ int inqx_hartree_noalloc(inqx_ham h, const double *density, double *vh, double *
   INQX_CHECK(h->g.N <= kNoAllocMaxPoints && hartree_noalloc_bytes(h->g.N) <= kNoAllocScratchBytes, INQX_ERR_ARG,
              "grid of " + std::to_string(h->g.N) + " points exceeds the allocation-free Hartree limit of " +
                  std::to_string(kNoAllocMaxPoints) + " points");
-  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "Poisson solves need built grid sizes with n1 = n2");
+  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "Poisson solves need built grid sizes");
   const double *total = launch_hartree_noalloc(h, density, vh);
   if (energy) {
     double e = 0.0;
@@ -1342,7 +1342,7 @@ static void exchange_direct_impl(inqx_ham h, Basis &B, int nst, int prec, const 
                                  const std::vector<HfBlock> &blocks, double coeff, double scale, void *out,
                                  bool accumulate) {
   Ctx &cx = h->ctx->c;
-  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "exchange needs built grid sizes with n1 = n2");
+  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "exchange needs built grid sizes");
   // Repeated complex128 exchanges on identical inputs have differed in a few
   // output states (about 1e-3 relative) at 72^3 and 96^3; no cause in the code
   // has been shown, so the complex128 pair path with occupied states is not

@@ -8,71 +8,30 @@
 // |G+q|^2 < 1e-6, unnormalised inverse FFT) and accumulates
 // exxphi_i += factor occ_j hf_j v_ij.
 //
-// Here a launch covers a chunk of j's times all i, as three kernels over
+// Here a launch covers a chunk of j's times all i, as kernels over
 // real-space arrays in state-major layout [state][i0][i1][i2]:
 //
 //   pair_plane_fwd   per (j, plane i0, i): forms rho on the fly from the two
-//                    planes, 2-D forward FFT along i1 (registers, one column
-//                    per lane) then i2 (LDS rows), stores the plane of t
-//   pair_xline       per (j, i, 64 lines (i1,i2)): forward FFT along i0 in
+//                    planes, 2-D forward FFT along i1 (registers, a column
+//                    per lane pair) then i2 (LDS rows), stores the plane of t
+//   pair_xline       per (j, i, 128 lines (i1,i2)): forward FFT along i0 in
 //                    registers, Poisson kernel, inverse FFT along i0, in place
-//   pair_plane_inv   per (plane i0, i), looping over the chunk's j in order:
-//                    2-D inverse FFT, multiplies by coef_j hf_j and adds into
-//                    w_i (deterministic order; the w plane stays in cache
-//                    between consecutive j)
+//   pair_plane_inv   per (j, plane i0, i): 2-D inverse FFT of the plane of t
+//   pair_reduce      w_i += sum_j coef_j hf_j t_ji in j order (deterministic;
+//                    chunks of j give the same bits); pair_reduce_sym adds the
+//                    conjugate pairs' terms when hf = phi
 //
+// Planes with n1 != n2 run the 2-D transforms as a column kernel and a row
+// kernel in place in t (pair_cols, pair_rows) instead of pair_plane_fwd/inv.
 // Without hf the same kernels solve Poisson in place on a batch of densities.
-#include "internal.hpp"
+#include "plane_common.hpp"
 
 #include <algorithm>
 #include <cmath>
 
-#if defined(__HIP_DEVICE_COMPILE__)
-#define GAS __attribute__((address_space(1)))
-#else
-#define GAS
-#endif
-
 namespace inqx {
 
 namespace {
-
-template <typename T> __device__ __forceinline__ const GAS T *xg(const T *p) { return (const GAS T *)p; }
-template <typename T> __device__ __forceinline__ GAS T *xw(T *p) { return (GAS T *)p; }
-
-// raw buffer access to one complex value: per-lane byte offset in a VGPR and a
-// wave-uniform byte offset (a constant after unrolling) in an SGPR, so that a
-// line of loads needs a single address register
-// the base and size of a buffer resource are wave-uniform by construction;
-// readfirstlane makes that explicit so the resource stays in SGPRs
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, unsigned bytes) {
-  const unsigned long long a = (unsigned long long)base;
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
-  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
-  void *p = (void *)(((unsigned long long)hi << 32) | lo);
-  return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)__builtin_amdgcn_readfirstlane(bytes), 0x00020000);
-}
-__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<float> &v) {
-  const auto q = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-  v.x = __int_as_float(q[0]);
-  v.y = __int_as_float(q[1]);
-}
-__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<double> &v) {
-  const auto q = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
-  v.x = __longlong_as_double(((long long)(unsigned)q[1] << 32) | (unsigned)q[0]);
-  v.y = __longlong_as_double(((long long)(unsigned)q[3] << 32) | (unsigned)q[2]);
-}
-__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<float> v) {
-  typedef int i2 __attribute__((ext_vector_type(2)));
-  i2 q = {__float_as_int(v.x), __float_as_int(v.y)};
-  __builtin_amdgcn_raw_buffer_store_b64(q, r, voff, soff, 0);
-}
-__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, cplx<double> v) {
-  typedef int i4 __attribute__((ext_vector_type(4)));
-  const long long a = __double_as_longlong(v.x), b = __double_as_longlong(v.y);
-  i4 q = {(int)a, (int)(a >> 32), (int)b, (int)(b >> 32)};
-  __builtin_amdgcn_raw_buffer_store_b128(q, r, voff, soff, 0);
-}
 
 template <typename T> struct PairDev {
   const cplx<T> *phi;  // [nst][N]
@@ -111,7 +70,7 @@ __device__ __forceinline__ int pair_line(int tid) { return (tid >> 6) * 32 + (ti
 // symmetric mode: the pair (hf index j of chunk entry jj, state i) is skipped
 // when both are occupied and i > j (the potential of pair (i, j) stands for it)
 template <typename T> __device__ __forceinline__ bool pair_skipped(const PairDev<T> &d, int jj, int i) {
-  return d.occ && i > xg(d.jlist)[jj] && xg(d.occ)[i] != 0;
+  return d.occ && i > gp(d.jlist)[jj] && gp(d.occ)[i] != 0;
 }
 
 // FFT along i2 of row r of the LDS plane, split over the lane pair; rows >= N1 run on a clamped row and do not store
@@ -162,9 +121,9 @@ __global__ void __launch_bounds__(256) k_pair_plane_fwd(PairDev<T> d, int total)
     const int jj = r / d.n0;
     if (pair_skipped(d, jj, i)) return;  // uniform over the work-group
     const size_t plane = (size_t)i0 * NP;
-    const GAS C *ph = xg(d.phi) + (size_t)i * N + plane;
+    const GAS C *ph = gp(d.phi) + (size_t)i * N + plane;
     const GAS C *hf = nullptr;
-    if (d.hf) hf = xg(d.hf) + (size_t)xg(d.jlist)[jj] * N + plane;
+    if (d.hf) hf = gp(d.hf) + (size_t)gp(d.jlist)[jj] * N + plane;
     {
       // column z = line along i1: lower lanes y = 2n, upper y = 2n + 1
       const int z = line < N2 ? line : N2 - 1;
@@ -189,7 +148,7 @@ __global__ void __launch_bounds__(256) k_pair_plane_fwd(PairDev<T> d, int total)
     __syncthreads();
     lds_row_fft_split<C, N1, N2, -1>(lds, line, upper);
     __syncthreads();
-    GAS C *tp = xw(d.t) + ((size_t)jj * d.nst + i) * N + plane;
+    GAS C *tp = gw(d.t) + ((size_t)jj * d.nst + i) * N + plane;
     for (int e = tid; e < NP; e += Cfg::THREADS) {
       const int y = e / N2, z = e - y * N2;
       tp[e] = lds[y * Cfg::STRIDE + z];
@@ -284,7 +243,7 @@ __global__ void __launch_bounds__(256) k_pair_plane_inv(PairDev<T> d, int total)
     const int jj = r / d.n0;
     if (pair_skipped(d, jj, i)) return;  // uniform over the work-group
     const size_t plane = (size_t)i0 * NP;
-    GAS C *tp = xw(d.t) + ((size_t)jj * d.nst + i) * N + plane;
+    GAS C *tp = gw(d.t) + ((size_t)jj * d.nst + i) * N + plane;
     {
       const int z = line < N2 ? line : N2 - 1;
       C v[H1];
@@ -305,6 +264,122 @@ __global__ void __launch_bounds__(256) k_pair_plane_inv(PairDev<T> d, int total)
     }
     __syncthreads();
   }
+}
+
+// ---- planes with n1 != n2: the 2-D transforms of a pair plane as a column
+// kernel (built for N1, n2 at run time) and a row kernel (built for N2, n1 at
+// run time), both in place in t, a lane pair per line and 32 lines per wave
+// (a kernel per (n1, n2) pair would square the instantiations)
+
+// pair plane (chunk entry jj, plane i0, state i) of work-group wg
+__device__ __forceinline__ void pair_plane_item(int wg, int nst, int n0, int &jj, int &i0, int &i) {
+  i = wg % nst;
+  const int r = wg / nst;
+  i0 = r % n0;
+  jj = r / n0;
+}
+
+// FFT along i1 of every column of one pair plane; PROD: the input is
+// conj(hf_j) phi_i (phi_i alone without hf), formed on the fly as in
+// pair_plane_fwd, else the plane of t itself
+template <typename T, int N1, int SIGN, bool PROD>
+__global__ void __launch_bounds__(256) k_pair_cols(PairDev<T> d, int total) {
+  using C = cplx<T>;
+  constexpr int H1 = N1 / 2;
+  const int n2 = d.n2, NP = N1 * n2;
+  const long N = (long)d.n0 * NP;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int half = lane >= 32 ? 1 : 0;
+  const int wg = blockIdx.x;
+  if (wg >= total) return;
+  int jj, i0, i;
+  pair_plane_item(wg, d.nst, d.n0, jj, i0, i);
+  if (pair_skipped(d, jj, i)) return;  // uniform over the work-group
+  const size_t plane = (size_t)i0 * NP;
+  GAS C *tp = gw(d.t) + ((size_t)jj * d.nst + i) * N + plane;
+  const GAS C *ph = gp(d.phi) + (size_t)i * N + plane;
+  const GAS C *hf = (PROD && d.hf) ? gp(d.hf) + (size_t)gp(d.jlist)[jj] * N + plane : nullptr;
+  // the lanes past the last column repeat it and store nothing; they are in
+  // the same wave as its owner, whose loads all precede its stores
+  for (int g = wave; g * 32 < n2; g += (int)(blockDim.x >> 6)) {
+    const int zl = g * 32 + (lane & 31);
+    const bool ok = zl < n2;
+    const int z = ok ? zl : n2 - 1;
+    C v[H1];
+#pragma unroll
+    for (int n = 0; n < H1; ++n) {
+      const int e = (2 * n + half) * n2 + z;
+      if (PROD) {
+        const C a = ph[e];
+        if (hf) {
+          const C h = hf[e];
+          v[n] = C{h.x * a.x + h.y * a.y, h.x * a.y - h.y * a.x};  // conj(h) a
+        } else {
+          v[n] = a;
+        }
+      } else {
+        v[n] = tp[e];
+      }
+    }
+    fft_split<N1, SIGN, C>::dit(v, half != 0);
+    if (ok) {
+#pragma unroll
+      for (int k = 0; k < H1; ++k) tp[(k + half * H1) * n2 + z] = v[k];
+    }
+  }
+}
+
+// FFT along i2 of every row of one pair plane of t, in place
+template <typename T, int N2, int SIGN>
+__global__ void __launch_bounds__(256) k_pair_rows(PairDev<T> d, int total) {
+  using C = cplx<T>;
+  constexpr int H2 = N2 / 2;
+  const int n1 = d.n1, NP = n1 * N2;
+  const long N = (long)d.n0 * NP;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int half = lane >= 32 ? 1 : 0;
+  const int wg = blockIdx.x;
+  if (wg >= total) return;
+  int jj, i0, i;
+  pair_plane_item(wg, d.nst, d.n0, jj, i0, i);
+  if (pair_skipped(d, jj, i)) return;
+  GAS C *tp = gw(d.t) + ((size_t)jj * d.nst + i) * N + (size_t)i0 * NP;
+  for (int g = wave; g * 32 < n1; g += (int)(blockDim.x >> 6)) {
+    const int yl = g * 32 + (lane & 31);
+    const bool ok = yl < n1;
+    GAS C *row = tp + (size_t)(ok ? yl : n1 - 1) * N2;
+    C v[H2];
+#pragma unroll
+    for (int n = 0; n < H2; ++n) v[n] = row[2 * n + half];
+    fft_split<N2, SIGN, C>::dit(v, half != 0);
+    if (ok) {
+#pragma unroll
+      for (int k = 0; k < H2; ++k) row[k + half * H2] = v[k];
+    }
+  }
+}
+
+// the column then the row transforms of every pair plane (n1 != n2)
+template <typename T, int SIGN, bool PROD>
+void pair_planes_rect(Ctx &cx, const PairDev<T> &d, int total) {
+  switch (d.n1) {
+#define CASE(S)                                                                                          \
+  case S:                                                                                                \
+    hipLaunchKernelGGL((k_pair_cols<T, S, SIGN, PROD>), dim3(total), dim3(256), 0, cx.stream, d, total); \
+    break;
+    INQX_SIZES(CASE)
+#undef CASE
+  }
+  INQX_HIP(hipGetLastError());
+  switch (d.n2) {
+#define CASE(S)                                                                                  \
+  case S:                                                                                        \
+    hipLaunchKernelGGL((k_pair_rows<T, S, SIGN>), dim3(total), dim3(256), 0, cx.stream, d, total); \
+    break;
+    INQX_SIZES(CASE)
+#undef CASE
+  }
+  INQX_HIP(hipGetLastError());
 }
 
 // w_i += sum_jj c_j hf_j t[jj][i] over the chunk, added in jj order so that
@@ -332,9 +407,9 @@ __global__ void __launch_bounds__(256) k_pair_reduce(PairDev<T> d, long nvec) {
   constexpr int U = 4;
   const int i = (int)(blockIdx.x % (unsigned)d.nst);
   const long v0 = (long)(blockIdx.x / (unsigned)d.nst) * 256 * U + threadIdx.x;
-  GAS V *wv = xw(reinterpret_cast<V *>(d.w)) + (size_t)i * nvec;
-  const GAS V *tv = xg(reinterpret_cast<const V *>(d.t)) + (size_t)i * nvec;
-  const GAS V *hv = xg(reinterpret_cast<const V *>(d.hf));
+  GAS V *wv = gw(reinterpret_cast<V *>(d.w)) + (size_t)i * nvec;
+  const GAS V *tv = gp(reinterpret_cast<const V *>(d.t)) + (size_t)i * nvec;
+  const GAS V *hv = gp(reinterpret_cast<const V *>(d.hf));
   const size_t jstride = (size_t)d.nst * nvec;
   V acc[U];
 #pragma unroll
@@ -343,9 +418,9 @@ __global__ void __launch_bounds__(256) k_pair_reduce(PairDev<T> d, long nvec) {
     if (v < nvec) acc[u] = wv[v];
   }
   for (int jj = 0; jj < d.nj; ++jj) {
-    const int j = xg(d.jlist)[jj];
-    if (d.occ && i > j && xg(d.occ)[i] != 0) continue;  // pair not solved: k_pair_reduce_sym adds it
-    const T c = (T)xg(d.coef)[j];
+    const int j = gp(d.jlist)[jj];
+    if (d.occ && i > j && gp(d.occ)[i] != 0) continue;  // pair not solved: k_pair_reduce_sym adds it
+    const T c = (T)gp(d.coef)[j];
     V x[U], h[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -388,15 +463,15 @@ __global__ void __launch_bounds__(256) k_pair_reduce_sym(PairDev<T> d, long nvec
   constexpr int U = 4;
   const int jj = (int)(blockIdx.x % (unsigned)d.nj);
   const long v0 = (long)(blockIdx.x / (unsigned)d.nj) * 256 * U + threadIdx.x;
-  const int j = xg(d.jlist)[jj];
-  const GAS V *tv = xg(reinterpret_cast<const V *>(d.t)) + (size_t)jj * d.nst * nvec;
-  const GAS V *hv = xg(reinterpret_cast<const V *>(d.hf));
+  const int j = gp(d.jlist)[jj];
+  const GAS V *tv = gp(reinterpret_cast<const V *>(d.t)) + (size_t)jj * d.nst * nvec;
+  const GAS V *hv = gp(reinterpret_cast<const V *>(d.hf));
   V acc[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) acc[u] = V{};
   for (int s = 0; s < j; ++s) {
-    if (xg(d.occ)[s] == 0) continue;
-    const T c = (T)xg(d.coef)[s];
+    if (gp(d.occ)[s] == 0) continue;
+    const T c = (T)gp(d.coef)[s];
     V x[U], h[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -409,7 +484,7 @@ __global__ void __launch_bounds__(256) k_pair_reduce_sym(PairDev<T> d, long nvec
 #pragma unroll
     for (int u = 0; u < U; ++u) cmac_conj(acc[u], c, h[u], x[u]);
   }
-  GAS V *bv = xw(reinterpret_cast<V *>(d.wb)) + (size_t)j * nvec;
+  GAS V *bv = gw(reinterpret_cast<V *>(d.wb)) + (size_t)j * nvec;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const long v = v0 + u * 256;
@@ -417,9 +492,10 @@ __global__ void __launch_bounds__(256) k_pair_reduce_sym(PairDev<T> d, long nvec
   }
 }
 
-// S0: line length along i0; S: the plane size (n1 = n2)
+// S0: line length along i0; S > 0: square planes S x S (pair_plane_fwd /
+// pair_plane_inv); S == 0: planes n1 x n2 with n1 != n2 (column and row kernels)
 template <typename T, int S0, int S> void run_pairs(inqx_ham_s *h, const PairArgs &a) {
-  using Cfg = PairPlaneCfg<T, S, S>;
+  using Cfg = PairPlaneCfg<T, (S > 0 ? S : 16), (S > 0 ? S : 16)>;
   Ctx &cx = h->ctx->c;
   const Grid &g = h->g;
   PairDev<T> d;
@@ -441,22 +517,27 @@ template <typename T, int S0, int S> void run_pairs(inqx_ham_s *h, const PairArg
   for (int c = 0; c < 3; ++c) d.q[c] = (T)a.gshift[c];
   d.scal = (T)(4.0 * M_PI / (double)g.N);
   d.zero = (T)(a.zeroterm / (double)g.N);
-  static DeviceOnce attr;
-  if (Cfg::LDS > 65536 && !attr.test_and_set(cx.device)) {
-    INQX_HIP(hipFuncSetAttribute((const void *)k_pair_plane_fwd<T, S, S>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
-    INQX_HIP(hipFuncSetAttribute((const void *)k_pair_plane_inv<T, S, S>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
+  if constexpr (S > 0) {
+    static DeviceOnce attr;
+    if (Cfg::LDS > 65536 && !attr.test_and_set(cx.device)) {
+      INQX_HIP(hipFuncSetAttribute((const void *)k_pair_plane_fwd<T, S, S>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
+      INQX_HIP(hipFuncSetAttribute((const void *)k_pair_plane_inv<T, S, S>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
+    }
   }
   const bool f64 = std::is_same<T, double>::value;
   const int tot1 = d.nj * S0 * d.nst;
   {
     ProfScope ps(cx, f64 ? "pair_plane_fwd_f64" : "pair_plane_fwd_f32");
-    hipLaunchKernelGGL((k_pair_plane_fwd<T, S, S>), dim3(tot1), dim3(Cfg::THREADS), Cfg::LDS,
-                       cx.stream, d, tot1);
+    if constexpr (S > 0)
+      hipLaunchKernelGGL((k_pair_plane_fwd<T, S, S>), dim3(tot1), dim3(Cfg::THREADS), Cfg::LDS,
+                         cx.stream, d, tot1);
+    else
+      pair_planes_rect<T, -1, true>(cx, d, tot1);
   }
   INQX_HIP(hipGetLastError());
-  constexpr int ngrp = ((S * S + 31) / 32 + 3) / 4;
+  const int ngrp = ((d.n1 * d.n2 + 31) / 32 + 3) / 4;
   const int tot2 = d.nj * d.nst * ngrp;
   {
     ProfScope ps(cx, f64 ? "pair_xline_f64" : "pair_xline_f32");
@@ -466,8 +547,11 @@ template <typename T, int S0, int S> void run_pairs(inqx_ham_s *h, const PairArg
   INQX_HIP(hipGetLastError());
   {
     ProfScope ps(cx, f64 ? "pair_plane_inv_f64" : "pair_plane_inv_f32");
-    hipLaunchKernelGGL((k_pair_plane_inv<T, S, S>), dim3(tot1), dim3(Cfg::THREADS), Cfg::LDS,
-                       cx.stream, d, tot1);
+    if constexpr (S > 0)
+      hipLaunchKernelGGL((k_pair_plane_inv<T, S, S>), dim3(tot1), dim3(Cfg::THREADS), Cfg::LDS,
+                         cx.stream, d, tot1);
+    else
+      pair_planes_rect<T, +1, false>(cx, d, tot1);
   }
   INQX_HIP(hipGetLastError());
   if (d.hf) {
@@ -497,7 +581,7 @@ constexpr int kHChunk = 8192;
 
 __global__ void __launch_bounds__(256) k_real_to_cplx(const double *r, double2 *c, long N) {
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
-  if (p < N) xw(c)[p] = double2{xg(r)[p], 0.0};
+  if (p < N) gw(c)[p] = double2{gp(r)[p], 0.0};
 }
 
 // vh = Re(c); part[chunk] = sum over the chunk of rho vh (fixed tree order)
@@ -508,9 +592,9 @@ __global__ void __launch_bounds__(256) k_hartree_out(const double2 *c, const dou
   const long p0 = (long)blockIdx.x * kHChunk;
   double acc = 0.0;
   for (long p = p0 + tid; p < p0 + kHChunk && p < N; p += 256) {
-    const double v = xg(c)[p].x;
-    if (vh) xw(vh)[p] = v;
-    acc += xg(rho)[p] * v;
+    const double v = gp(c)[p].x;
+    if (vh) gw(vh)[p] = v;
+    acc += gp(rho)[p] * v;
   }
   red[tid] = acc;
   __syncthreads();
@@ -518,7 +602,7 @@ __global__ void __launch_bounds__(256) k_hartree_out(const double2 *c, const dou
     if (tid < h) red[tid] += red[tid + h];
     __syncthreads();
   }
-  if (tid == 0) xw(part)[blockIdx.x] = red[0];
+  if (tid == 0) gw(part)[blockIdx.x] = red[0];
 }
 }  // namespace
 
@@ -540,8 +624,8 @@ DeviceOnce g_scratch_once;
 __global__ void __launch_bounds__(64) k_sum_partials(const double *part, int n, double *out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     double e = 0.0;
-    for (int c = 0; c < n; ++c) e += xg(part)[c];
-    xw(out)[0] = e;
+    for (int c = 0; c < n; ++c) e += gp(part)[c];
+    gw(out)[0] = e;
   }
 }
 
@@ -610,23 +694,23 @@ namespace {
 __global__ void __launch_bounds__(256) k_add_vec4(float4 *w, const float4 *b, long n4) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i < n4) {
-    float4 x = xg(w)[i];
-    const float4 y = xg(b)[i];
+    float4 x = gp(w)[i];
+    const float4 y = gp(b)[i];
     x.x += y.x;
     x.y += y.y;
     x.z += y.z;
     x.w += y.w;
-    xw(w)[i] = x;
+    gw(w)[i] = x;
   }
 }
 __global__ void __launch_bounds__(256) k_add_dvec2(double2 *w, const double2 *b, long n2) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i < n2) {
-    double2 x = xg(w)[i];
-    const double2 y = xg(b)[i];
+    double2 x = gp(w)[i];
+    const double2 y = gp(b)[i];
     x.x += y.x;
     x.y += y.y;
-    xw(w)[i] = x;
+    gw(w)[i] = x;
   }
 }
 }  // namespace
@@ -647,7 +731,6 @@ void launch_add_inplace(Ctx &cx, int prec, void *w, const void *b, size_t bytes)
 }
 
 bool pair_grid_supported(const int *n) {
-  if (n[1] != n[2]) return false;
   auto ok = [](int v) {
     switch (v) {
 #define CASE(S) case S:
@@ -657,11 +740,15 @@ bool pair_grid_supported(const int *n) {
       default: return false;
     }
   };
-  return ok(n[0]) && ok(n[1]);
+  return ok(n[0]) && ok(n[1]) && ok(n[2]);
 }
 
 namespace {
 template <typename T, int S0> void pairs_n12(inqx_ham_s *h, const PairArgs &a) {
+  if (h->g.n[1] != h->g.n[2]) {
+    run_pairs<T, S0, 0>(h, a);
+    return;
+  }
   switch (h->g.n[1]) {
 #define CASE(S)                   \
   case S:                         \
@@ -677,7 +764,7 @@ template <typename T, int S0> void pairs_n12(inqx_ham_s *h, const PairArgs &a) {
 void launch_pairs(inqx_ham_s *h, int prec, const PairArgs &a) {
   const int *n = h->g.n;
   INQX_CHECK(pair_grid_supported(n), INQX_ERR_UNSUPPORTED,
-             "pair Poisson solves need built sizes with n1 = n2");
+             "pair Poisson solves need built grid sizes");
   INQX_CHECK(a.nst > 0 && a.nj > 0 && a.phi && a.t && a.w, INQX_ERR_ARG, "bad pair arguments");
   INQX_CHECK(!a.hf || (a.jlist && a.coef), INQX_ERR_ARG, "pair chunk without j list");
   INQX_CHECK(a.hf || a.t == a.w, INQX_ERR_ARG, "a plain Poisson solve works in place");

@@ -264,7 +264,7 @@ static void ks_update(inqx_ham_s *h, const double *vion, const double *den, KsWo
   Ctx &cx = h->ctx->c;
   const long N = h->g.N;
   const int ncomp = h->ncomp;
-  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "the Hartree solve needs built grid sizes with n1 = n2");
+  INQX_CHECK(pair_grid_supported(h->g.n), INQX_ERR_UNSUPPORTED, "the Hartree solve needs built grid sizes");
   INQX_CHECK(h->v64.p && h->v32.p && h->v64.bytes >= sizeof(double) * (size_t)N * ncomp, INQX_ERR_STATE,
              "set a potential (inqx_ham_set_potential) before updating it");
   const double *rho = den;
@@ -467,7 +467,7 @@ int inqx_prop_set_scf(inqx_prop p, const double *vion, int max_passes, double th
   INQX_CHECK(max_passes == 0 || vion, INQX_ERR_ARG, "the self-consistent step needs the ionic potential");
   INQX_CHECK(!p->exec, INQX_ERR_STATE, "the propagator already holds a captured graph of the fixed-H step");
   if (max_passes > 0) INQX_CHECK(pair_grid_supported(p->ham->g.n), INQX_ERR_UNSUPPORTED,
-                                 "the Hartree solve needs built grid sizes with n1 = n2");
+                                 "the Hartree solve needs built grid sizes");
   p->vion = max_passes > 0 ? vion : nullptr;
   p->nscf = max_passes;
   p->scf_threshold = threshold;

@@ -416,12 +416,15 @@ def test_c5_si216_exchange_subset(ctx):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-def test_noncubic_grid_poisson_exchange_hartree(ctx, dtype):
-    """a 24 x 16 x 16 grid (first axis longer): batched Poisson with shift and
-    zeroterm, exchange of states at k with occupied states at k and q, and the
-    Hartree potential, against the oracle"""
-    say("case: non-cubic 24x16x16 Poisson / exchange / Hartree", np.dtype(dtype).name)
-    lat2, ng2 = np.diag([9.0, 6.0, 6.0]), (24, 16, 16)
+@pytest.mark.parametrize("ng2,lens", [((24, 16, 16), (9.0, 6.0, 6.0)), ((16, 20, 24), (6.0, 7.5, 9.0))])
+def test_noncubic_grid_poisson_exchange_hartree(ctx, dtype, ng2, lens):
+    """grids that are not cubic (24 x 16 x 16: the first axis longer; 16 x 20 x
+    24: planes that are not square, column and row kernels): batched Poisson
+    with shift and zeroterm, exchange of states at k with occupied states at k
+    and q, and the Hartree potential, against the oracle"""
+    say("case: non-cubic Poisson / exchange / Hartree", ng2, np.dtype(dtype).name)
+    lat2 = np.diag(lens)
+    vol = float(np.prod(lens))
     n2 = int(np.prod(ng2))
     gh = KSHamiltonian(ctx, lat2, ng2)
     oh = orc.Ham(lat2, ng2)
@@ -434,7 +437,7 @@ def test_noncubic_grid_poisson_exchange_hartree(ctx, dtype):
     # exchange at two k-points
     k, q = (0.3, -0.2, 0.1), (-0.4, 0.5, 0.2)
     bk, bq = gh.basis(k, 5.0), gh.basis(q, 5.0)
-    vel = 54.0 * 6.0 / float(n2) ** 2
+    vel = vol / float(n2) ** 2
     phi = S.random_states(bk.npw, 3, 61, vel).astype(dtype)
     hq = S.random_states(bq.npw, 2, 62, vel).astype(dtype)
     ex = ExchangeOperator(gh, bk, exchange_coefficient=0.5)
@@ -452,9 +455,15 @@ def test_noncubic_grid_poisson_exchange_hartree(ctx, dtype):
                            np.array([2.0, 1.0]), exx_coeff=0.5, gshift=cart(q) - cart(k), zeroterm=0.0)
     ref = np.fft.fftn(np.ascontiguousarray(w.T).reshape((3,) + ng2), axes=(1, 2, 3)).reshape(3, n2)[:, gk]
     assert rel_err(got, ref) < TOL_X[dtype]
+    # hf = phi: each unordered pair of occupied states solved once
+    dp = DeviceArray.from_numpy(ctx, phi)
+    occ3 = np.array([2.0, 1.0, 0.0])
+    ek = ExchangeOperator(gh, bk, exchange_coefficient=0.5, zeroterm=0.2)
+    full = ek.direct(dp, DeviceArray.from_numpy(ctx, phi.copy()), occ3).numpy()
+    assert rel_err(ek.direct(dp, dp, occ3).numpy(), full) < TOL_X[dtype]
     # Hartree potential of a real density (complex128 path)
     dens = 0.01 + rng.random(n2) * 0.02
     vh, eh = gh.hartree(DeviceArray.from_numpy(ctx, dens))
     vref = oh.poisson_in_place(dens.astype(np.complex128)[:, None], zeroterm=0.0)[:, 0].real
     assert rel_err(vh.numpy(), vref) < 1e-11
-    assert eh == pytest.approx(0.5 * float(np.sum(dens * vref)) * 324.0 / n2, rel=1e-11)
+    assert eh == pytest.approx(0.5 * float(np.sum(dens * vref)) * vol / n2, rel=1e-11)
