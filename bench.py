@@ -10,7 +10,8 @@ comes from HIP events recorded around each launch on the library's stream.
 
 Secondary lines in the same JSON object (same seeded inputs):
   hbm_copy      plain streaming copy kernel, the measured HBM reference
-  c3_kpoints    Al4 fcc, 64 k-points x 2 spins (256 uneven blocks) in one batched launch
+  c3_kpoints    Al4 fcc, 64 k-points x 2 spins (128 uneven blocks, each spin reading its own
+                potential component) in one batched launch
   c4_tddft      ETRS steps on Si64 with a velocity-gauge kick, hipGraph replay,
                 FP32 vs FP64 dipole trace and energy drift
   c1_diamond    C8 diamond apply + overlap + Cholesky, GPU and CPU oracle
@@ -182,10 +183,13 @@ def c3_kpoints(ctx, steps):
     from inq_amd import systems as S
     sysm = S.config(3)
     gh = build(ctx, sysm)
+    # spin-polarised: a two-component potential [N][2], each block reading its spin's
+    v = S.local_potential(sysm)
+    gh.set_scalar_potential(np.stack([v, v + 0.3 * np.random.default_rng(5).standard_normal(v.shape)], axis=1), ncomp=2)
     vol = S.sphere_volume_element(sysm)
     bases, ins, outs = [], [], []
     for i, (k, spin, ec) in enumerate(sysm.blocks()):
-        b = gh.basis(k, ec, 0)
+        b = gh.basis(k, ec, spin)
         c = S.random_states(b.npw, sysm.nst, sysm.seed + i, vol).astype(np.complex64)
         bases.append(b)
         ins.append(DeviceArray.from_numpy(ctx, c))
@@ -202,7 +206,8 @@ def c3_kpoints(ctx, steps):
     ctx.sync()
     dt = (time.perf_counter() - t0) / steps
     nsa = sysm.nst * len(bases)
-    return {"blocks": len(bases), "nst": sysm.nst, "npw_min": int(min(npws)), "npw_max": int(max(npws)),
+    return {"blocks": len(bases), "spins": sorted({int(b.spin) for b in bases}), "nst": sysm.nst,
+            "npw_min": int(min(npws)), "npw_max": int(max(npws)),
             "ms_per_batch": dt * 1e3, "state_applies_per_s": nsa / dt, "dtype": "c64",
             "launch_shape": {"xpass_states_per_wg": shape[0], "planes_per_wg": shape[1], "tune_s": t_tune}}
 

@@ -133,17 +133,23 @@ def test_no_projectors_and_plane_waves(ctx):
 
 
 def test_c3_kpoint_batch(ctx):
-    """several uneven (k, spin) blocks in one batched launch"""
+    """several uneven (k, spin) blocks of both spins in one batched launch: a
+    two-component potential [N][2], each block's spin index selecting its
+    component, against the oracle with that component"""
     sysm = S.config(3)
-    say("case: C3 Al4 batch of 8 blocks, fp64 and fp32")
+    say("case: C3 Al4 batch of 8 blocks, both spins, fp64 and fp32")
+    v = S.local_potential(sysm)
+    v2 = np.stack([v, v + 0.3 * np.random.default_rng(5).standard_normal(v.shape)], axis=1)
     gh = gpu_ham(ctx, sysm)
-    oh = oracle_ham(sysm)
-    blocks = sysm.blocks()[:: 16][:8]
+    gh.set_scalar_potential(v2, ncomp=2)
+    oh = [oracle_ham(sysm, np.ascontiguousarray(v2[:, s])) for s in (0, 1)]
+    blocks = sysm.blocks()[:: 15][:8]
+    assert {spin for _, spin, _ in blocks} == {0, 1}
     vol = S.sphere_volume_element(sysm)
     for dtype in (np.complex128, np.complex64):
         bases, ins, outs, cs = [], [], [], []
         for i, (k, spin, ec) in enumerate(blocks):
-            b = gh.basis(k, ec, 0)
+            b = gh.basis(k, ec, spin)
             c = S.random_states(b.npw, sysm.nst, 100 + i, vol)
             bases.append(b)
             cs.append(c)
@@ -151,8 +157,11 @@ def test_c3_kpoint_batch(ctx):
             outs.append(DeviceArray(ctx, c.shape, dtype))
         gh.apply_batch(bases, ins, outs)
         for (k, spin, ec), b, c, o in zip(blocks, bases, cs, outs):
-            ref = oh.apply_sphere(c.astype(dtype).astype(np.complex128), b.gindex(), k)
-            assert rel_err(o.numpy(), ref) < TOL[dtype], (k, dtype)
+            ref = oh[spin].apply_sphere(c.astype(dtype).astype(np.complex128), b.gindex(), k)
+            assert rel_err(o.numpy(), ref) < TOL[dtype], (k, spin, dtype)
+            if spin == 1:  # the other component gives a different result
+                assert rel_err(o.numpy(), oh[0].apply_sphere(c.astype(dtype).astype(np.complex128),
+                                                             b.gindex(), k)) > 1e-3
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
