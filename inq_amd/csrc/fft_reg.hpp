@@ -281,4 +281,51 @@ template <int N, int SIGN, typename C> struct fft_split {
   }
 };
 
+// ---------------------------------------------------------------- lines split over four lanes
+// A line of length N (a multiple of 4) is shared by the lanes l, l + 16, l + 32
+// and l + 48 of a wave (part q = lane / 16), N/4 values each: a quarter of the
+// registers of fft_line.  Two radix-2 stages cross the lanes: parts q and q ^ 2
+// (32 lanes apart, v_permlane32_swap as in fft_split) and parts q and q ^ 1
+// (16 lanes apart, a lane shuffle within each half-wave).
+
+// the value of lane l ^ 16
+__device__ __forceinline__ float xor16(float v) { return __shfl_xor(v, 16); }
+__device__ __forceinline__ double xor16(double v) { return __shfl_xor(v, 16); }
+
+// output position of part q: the values of part q are X[k + (N/4) quad_rev(q)]
+__host__ __device__ constexpr int quad_rev(int q) { return ((q & 1) << 1) | (q >> 1); }
+
+template <int N, int SIGN, typename C> struct fft_quad {
+  static_assert(N % 4 == 0, "four-lane lines need a length divisible by 4");
+  static constexpr int Q = N / 4;
+  using T = decltype(C{}.x);
+  // decimation in time: part q holds x[4n + q] (n < Q); afterwards it holds
+  // X[k + Q quad_rev(q)] (k < Q)
+  static __device__ __forceinline__ void dit(C (&v)[Q], int q) {
+    fft_line<Q, SIGN, C>::run(v);  // Y_q[k], the transform of x[4n + q]
+    // parts q, q ^ 2 hold the even and odd samples of x[2m + (q & 1)]: their
+    // length-N/2 transform, E (q & 1 = 0) or O (q & 1 = 1), at k (q < 2) and k + Q (q >= 2)
+    const T sa = q >= 2 ? (T)-1 : (T)1;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      C a, b;
+      pair_vals(v[k], a, b);
+      const C t = twmul<N / 2, SIGN>(b, k);
+      v[k] = C{a.x + sa * t.x, a.y + sa * t.y};
+    }
+    // parts q, q ^ 1 hold E[m] and O[m], m = k + Q (q >> 1): X[m] = E + W^m O on
+    // the even part, X[m + N/2] = E - W^m O on the odd part
+    const bool odd = (q & 1) != 0, hi = q >= 2;
+    const T sb = odd ? (T)-1 : (T)1;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      const C w{xor16(v[k].x), xor16(v[k].y)};
+      const C e = odd ? w : v[k], o = odd ? v[k] : w;
+      C t = twmul<N, SIGN>(o, k);
+      if (hi) t = SIGN > 0 ? C{-t.y, t.x} : C{t.y, -t.x};  // W^(k + N/4) = W^k (SIGN i)
+      v[k] = C{e.x + sb * t.x, e.y + sb * t.y};
+    }
+  }
+};
+
 }  // namespace inqx

@@ -71,16 +71,18 @@ template <typename T> constexpr int xpass_threads() { return sizeof(T) == 4 ? 51
 // plane passes (shared helpers in plane_common.hpp)
 // -------------------------------------------------------------------------------------------
 template <typename T, int N1, int N2>
-__global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
-                                                       int total, int pb, int pstr) {
+__global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blocks,
+                                                                          const int *prefix, int nblocks,
+                                                                          int total, int pb, int pstr) {
   using C = cplx<T>;
   using Cfg = PlaneCfg<T, N1, N2>;
+  constexpr int S = Cfg::LANES, LPW = Cfg::LPW;  // lanes per line, lines per wave
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x, nth = blockDim.x;
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const bool upper = lane >= 32;
-  const int half = upper ? 1 : 0;
+  const int q = lane / LPW, ll = lane % LPW;  // part of the line, line of the wave
+  const int qo = line_part_out<S>(q);
   {  // one work item per work-group: the grid is the work count
     const int wg = blockIdx.x;
     if (wg >= total) return;
@@ -90,7 +92,7 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
     const V4<int> sg = ld4<int>(B.slot_seg, 2 * slot);
     const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
     const V4<int> sw = ld4<int>(B.slot_seg, 2 * slot + 1);
-    const int wlo = sw.x, nwin = sw.y;  // row window (complex64 layout)
+    const int wlo = sw.x, nwin = sw.y;  // row window of the plane in LDS
     const float rseg = 1.0f / (float)max(nseg, 1), rrow = 1.0f / (float)max(nrow, 1);
     const GAS int *pwl = gp(B.pw_lds) + seg0;
     const GAS int *rows = gp(B.rows) + r0;
@@ -98,7 +100,7 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
     // the first row-pass line's row index, loaded now so that its latency
     // overlaps the scatter
     const int nr = np * nrow;
-    const int lr0 = min(wave * 32 + (lane & 31), max(nr - 1, 0));
+    const int lr0 = min(wave * LPW + ll, max(nr - 1, 0));
     const int pr0 = div_rcp(lr0, nrow, rrow);
     const int yr0 = nr > 0 ? rows[lr0 - pr0 * nrow] : 0;
     // the planes' sphere points, KG per thread in one round (a whole
@@ -121,41 +123,42 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
       }
     }
     __syncthreads();
-    // rows along i2 (sphere rows only), a lane pair per row
-    for (int g = wave; g * 32 < nr; g += Cfg::WAVES) {
-      const int li = g * 32 + (lane & 31);
+    // rows along i2 (sphere rows only), S lanes per row
+    for (int g = wave; g * LPW < nr; g += Cfg::WAVES) {
+      const int li = g * LPW + ll;
       const bool ok = li < nr;
       const int lc = ok ? li : nr - 1;
       const int p = div_rcp(lc, nrow, rrow);
       const int y = g == wave ? yr0 : rows[lc - p * nrow];
-      plane_row_split<C, N2, +1>(lds + p * pstr + plane_row<T, N1>(y, wlo) * (N2 + 1), upper, ok);
+      plane_row_lanes<C, N2, +1, S>(lds + p * pstr + plane_row<N1>(y, wlo) * (N2 + 1), q, ok);
     }
     __syncthreads();
-    // columns along i1 to the plane slots, a lane pair per column; the np
+    // columns along i1 to the plane slots, S lanes per column; the np
     // planes of this work-group are slot `slot` of states s0 .. s0+np-1
     const size_t pstride = (size_t)B.nslot * (N1 * N2);  // between the states' planes
     const auto rt = make_rsrc(B.t1 + (size_t)s0 * pstride + (size_t)slot * (N1 * N2),
                               (unsigned)(((size_t)(np - 1) * pstride + N1 * N2) * sizeof(C)));
-    for (int g = wave; g * 32 < np * N2; g += Cfg::WAVES) {
-      const int li = g * 32 + (lane & 31);
+    for (int g = wave; g * LPW < np * N2; g += Cfg::WAVES) {
+      const int li = g * LPW + ll;
       const bool ok = li < np * N2;
       const int lc = ok ? li : np * N2 - 1;
       const int p = lc / N2, z = lc - p * N2;
-      // rows y = 2n + half; in the window layout rows outside the window are zero
+      // rows y = S n + q; in the window layout rows outside the window are zero
       // (row(y) = y + const within each half of the FFT order: one base per half)
-      const int bA = p * pstr + z + plane_row<T, N1>(half, wlo) * (N2 + 1);  // rows y < N1/2
-      const int bB = bA - (plane_window<T>() ? N1 * (N2 + 1) : 0);          // rows y >= N1/2
-      C v[N1 / 2];
+      const int bA = p * pstr + z + plane_row<N1>(q, wlo) * (N2 + 1);  // rows y < N1/2
+      const int bB = bA - N1 * (N2 + 1);                                 // rows y >= N1/2
+      C v[N1 / S];
 #pragma unroll
-      for (int n = 0; n < N1 / 2; ++n) {
-        const int y = 2 * n + half;
-        v[n] = row_in_plane<T, N1>(y, wlo, nwin) ? lds[(y < N1 / 2 ? bA : bB) + 2 * n * (N2 + 1)] : C{0, 0};
+      for (int n = 0; n < N1 / S; ++n) {
+        const int y = S * n + q;
+        v[n] = row_in_plane<N1>(y, wlo, nwin) ? lds[(y < N1 / 2 ? bA : bB) + S * n * (N2 + 1)] : C{0, 0};
       }
-      fft_split<N1, +1, C>::dit(v, upper);
-      // lanes past the last column store out of range (dropped), no branch
-      const unsigned vo = ok ? (unsigned)(((size_t)p * pstride + z + half * (N1 / 2) * N2) * sizeof(C)) : kOOB;
+      line_dit<N1, +1, C, S>(v, q);
+      // v[k] is row k + qo N1/S; lanes past the last column store out of range
+      // (dropped), no branch
+      const unsigned vo = ok ? (unsigned)(((size_t)p * pstride + z + qo * (N1 / S) * N2) * sizeof(C)) : kOOB;
 #pragma unroll
-      for (int k = 0; k < N1 / 2; ++k) bstore(rt, vo, (unsigned)(k * N2 * sizeof(C)), v[k]);
+      for (int k = 0; k < N1 / S; ++k) bstore(rt, vo, (unsigned)(k * N2 * sizeof(C)), v[k]);
     }
     __syncthreads();
   }
@@ -164,17 +167,19 @@ __global__ void __launch_bounds__(256) k_plane_inverse(const SphereBlock<T> *blo
 // complex64 up to 72-point lines: at most 168 VGPRs, three waves per SIMD
 // (longer lines would spill)
 template <typename T, int N1, int N2>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && N1 <= 72 ? 3 : 1)))
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && N1 <= 72 ? 3 : 1)))
 k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, int total, int pb, int pstr, T c1r,
                 T c1i, T c2r, T c2i) {
   using C = cplx<T>;
   using Cfg = PlaneCfg<T, N1, N2>;
+  constexpr int S = Cfg::LANES, LPW = Cfg::LPW;  // lanes per line, lines per wave
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x, nth = blockDim.x;
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const bool upper = lane >= 32;
-  const int half = upper ? 1 : 0;
+  const int q = lane / LPW, ll = lane % LPW;  // part of the line, line of the wave
+  const int qo = line_part_out<S>(q);
   {  // one work item per work-group: the grid is the work count
     const int wg = blockIdx.x;
     if (wg >= total) return;
@@ -185,38 +190,39 @@ k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, in
     const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
     if (nseg == 0) return;  // uniform: nothing of this plane is on the sphere
     const V4<int> sw = ld4<int>(B.slot_seg, 2 * slot + 1);
-    const int wlo = sw.x, nwin = sw.y;  // row window (complex64 layout)
+    const int wlo = sw.x, nwin = sw.y;  // row window of the plane in LDS
     const float rseg = 1.0f / (float)nseg, rrow = 1.0f / (float)nrow;
     const GAS int *pwl = gp(B.pw_lds) + seg0;
     const GAS int *rows = gp(B.rows) + r0;
     const int tot = np * nseg, nr = np * nrow;
     // the first row-pass line's row index, loaded now
-    const int lr0 = min(wave * 32 + (lane & 31), nr - 1);
+    const int lr0 = min(wave * LPW + ll, nr - 1);
     const int pr0 = div_rcp(lr0, nrow, rrow);
     const int yr0 = rows[lr0 - pr0 * nrow];
     // columns along i1 from the plane slots, then the sphere rows along i2,
-    // a lane pair per line; the np planes are slot `slot` of states s0..s0+np-1
+    // S lanes per line; the np planes are slot `slot` of states s0..s0+np-1
     const size_t pstride = (size_t)B.nslot * (N1 * N2);
     const auto rt = make_rsrc(B.t1 + (size_t)s0 * pstride + (size_t)slot * (N1 * N2),
                               (unsigned)(((size_t)(np - 1) * pstride + N1 * N2) * sizeof(C)));
-    for (int g = wave; g * 32 < np * N2; g += Cfg::WAVES) {
-      const int li = g * 32 + (lane & 31);
+    for (int g = wave; g * LPW < np * N2; g += Cfg::WAVES) {
+      const int li = g * LPW + ll;
       const bool ok = li < np * N2;
       const int lc = ok ? li : np * N2 - 1;
       const int p = lc / N2, z = lc - p * N2;
-      const unsigned vo = (unsigned)(((size_t)p * pstride + z + half * N2) * sizeof(C));
-      C v[N1 / 2];
+      const unsigned vo = (unsigned)(((size_t)p * pstride + z + q * N2) * sizeof(C));  // rows S n + q
+      C v[N1 / S];
 #pragma unroll
-      for (int n = 0; n < N1 / 2; ++n) bload(rt, vo, (unsigned)(2 * n * N2 * sizeof(C)), v[n]);
-      fft_split<N1, -1, C>::dit(v, upper);
-      // v[k] is row y = k + half N1/2: the lower lanes hold y < N1/2, the upper
-      // lanes y >= N1/2; in the window layout rows outside the window are not stored
-      const int sb = p * pstr + z + plane_row<T, N1>(half * (N1 / 2), wlo) * (N2 + 1);
+      for (int n = 0; n < N1 / S; ++n) bload(rt, vo, (unsigned)(S * n * N2 * sizeof(C)), v[n]);
+      line_dit<N1, -1, C, S>(v, q);
+      // v[k] is row y = k + qo N1/S (complex64: the lower lanes hold y < N1/2,
+      // the upper lanes y >= N1/2); in the window layout rows outside the window
+      // are not stored
+      const int sb = p * pstr + z + plane_row<N1>(qo * (N1 / S), wlo) * (N2 + 1);
       if (ok) {
 #pragma unroll
-        for (int k = 0; k < N1 / 2; ++k) {
-          const int y = k + half * (N1 / 2);
-          if (row_in_plane<T, N1>(y, wlo, nwin)) lds[sb + k * (N2 + 1)] = v[k];
+        for (int k = 0; k < N1 / S; ++k) {
+          const int y = k + qo * (N1 / S);
+          if (row_in_plane<N1>(y, wlo, nwin)) lds[sb + k * (N2 + 1)] = v[k];
         }
       }
     }
@@ -233,13 +239,13 @@ k_plane_forward(const SphereBlock<T> *blocks, const int *prefix, int nblocks, in
     constexpr bool kEarly = sizeof(T) == 8;
     int lx[KG];
     if constexpr (kEarly) gather_offsets<KG>(lx, pwl, tid, nth, tot, nseg, rseg, pstr);
-    for (int g = wave; g * 32 < nr; g += Cfg::WAVES) {
-      const int li = g * 32 + (lane & 31);
+    for (int g = wave; g * LPW < nr; g += Cfg::WAVES) {
+      const int li = g * LPW + ll;
       const bool ok = li < nr;
       const int lc = ok ? li : nr - 1;
       const int p = div_rcp(lc, nrow, rrow);
       const int y = g == wave ? yr0 : rows[lc - p * nrow];
-      plane_row_split<C, N2, -1>(lds + p * pstr + plane_row<T, N1>(y, wlo) * (N2 + 1), upper, ok);
+      plane_row_lanes<C, N2, -1, S>(lds + p * pstr + plane_row<N1>(y, wlo) * (N2 + 1), q, ok);
     }
     __syncthreads();
     for (int i0 = tid; i0 < tot; i0 += KG * nth) {
@@ -1154,24 +1160,21 @@ void build_kb_tables(inqx_ham_s *h) {
 void ensure_basis_device(inqx_ham_s *h, Basis &b) {
   hipStream_t st = h->ctx->c.stream;
   if (!b.d_pw_lw.p) {
-    // LDS offset of every sphere point in its plane: row window layout
-    // (complex64) and whole-plane layout (complex128)
+    // LDS offset of every sphere point in its plane (row window layout)
     const Grid &g = h->g;
     const int n2p = g.n[2] + 1;
-    std::vector<int> lw(b.npw), lf(b.npw);
+    std::vector<int> lw(b.npw);
     for (int sl = 0; sl < b.nslot; ++sl) {
       const int *ss = &b.slot_seg[8 * sl];
       for (int j = 0; j < ss[1]; ++j) {
         const long q = ss[0] + j;
         const int y = b.pw_yz[q] / g.n[2], z = b.pw_yz[q] - y * g.n[2];
         lw[q] = (Grid::sym(y, g.n[1]) - ss[4] + 1) * n2p + z;
-        lf[q] = y * n2p + z;
       }
     }
     upload(b.d_slot_seg, b.slot_seg, st);
     upload(b.d_rows, b.rows, st);
     upload(b.d_pw_lw, lw, st);
-    upload(b.d_pw_lf, lf, st);
     INQX_HIP(hipStreamSynchronize(st));
   }
   if (b.kin_A[0] != h->A[0] || b.kin_A[1] != h->A[1] || b.kin_A[2] != h->A[2]) {
@@ -1666,7 +1669,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
       d.dps = B.dps[prec].as<cplx<T>>();
       d.slot_seg = B.d_slot_seg.as<int>();
       d.rows = B.d_rows.as<int>();
-      d.pw_lds = plane_window<T>() ? B.d_pw_lw.as<int>() : B.d_pw_lf.as<int>();
+      d.pw_lds = B.d_pw_lw.as<int>();
       d.kin = mode == 2 ? nullptr : prec == INQX_FP64 ? (const T *)B.d_kin64.p : (const T *)B.d_kin32.p;
       d.rs = mode != 0 ? reinterpret_cast<cplx<T> *>(opts->rs[i]) : nullptr;
       d.rs_layout = opts->rs_layout;
@@ -1697,7 +1700,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     }
     int nwin = 1;
     for (int i = 0; i < nblocks; ++i) nwin = std::max(nwin, bases[i]->nwin_max);
-    ld->plane_pstr = (plane_window<T>() ? nwin + 2 : g.n[1]) * (g.n[2] + 1);
+    ld->plane_pstr = (nwin + 2) * (g.n[2] + 1);
     ld->total_plane = pp[nblocks];
     ld->total_x = px[nblocks];
     ld->total_line = px[nblocks];

@@ -249,7 +249,7 @@ struct Basis {
                               // (symmetric i1 of the plane's first row), window length, 0, 0
   int nwin_max = 0;           // longest row window of a plane
   std::vector<int> rows, pw_yz;
-  DevMem d_slot_seg, d_rows, d_pw_lw, d_pw_lf, d_kin64, d_kin32;
+  DevMem d_slot_seg, d_rows, d_pw_lw, d_kin64, d_kin32;
   DevMem d_g2h;  // (1/2)|G|^2 per sphere point, FP64, no k shift (preconditioner)
   DevMem d_gidx; // flattened FFT index per sphere point (layout conversions), on first use
   double kin_A[3] = {1e300, 0, 0};  // A for which d_kin* were built

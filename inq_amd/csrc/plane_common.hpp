@@ -86,27 +86,20 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned voff, 
 // a per-lane buffer offset beyond every record: loads return 0, stores are dropped
 constexpr unsigned kOOB = 0x80000000u;
 
-// Plane layout in LDS.  complex64 planes hold only their row window (the
-// symmetric range of i1 that holds sphere rows, nwin rows from wlo) at LDS
-// rows 1 .. nwin, so more work-groups fit per CU; complex128 plane passes are
-// limited by registers to one wave per SIMD, and their planes hold all N1
-// rows at LDS row i1.  plane_row(y) is the LDS row of a row y that holds
-// sphere points; row_in_plane(y) whether row y has an LDS row at all.
-template <typename T> constexpr bool plane_window() { return sizeof(T) == 4; }
+// Plane layout in LDS: a plane holds only its row window (the symmetric range
+// of i1 that holds sphere rows, nwin rows from wlo) at LDS rows 1 .. nwin, so
+// more work-groups fit per CU (51 of 72 rows at the 72^3 sphere: complex64
+// planes of 29 KB, complex128 planes of 62 KB, two work-groups per CU).
+// plane_row(y) is the LDS row of a row y that holds sphere points;
+// row_in_plane(y) whether row y has an LDS row at all.
 // (plane_row_n and row_in_plane_n take the plane's row count n at run time)
-template <typename T> __device__ __forceinline__ int plane_row_n(int y, int wlo, int n) {
-  if constexpr (plane_window<T>()) return (y >= (n + 1) / 2 ? y - n : y) - wlo + 1;
-  else return y;
+__device__ __forceinline__ int plane_row_n(int y, int wlo, int n) { return (y >= (n + 1) / 2 ? y - n : y) - wlo + 1; }
+__device__ __forceinline__ bool row_in_plane_n(int y, int wlo, int nwin, int n) {
+  return (unsigned)((y >= (n + 1) / 2 ? y - n : y) - wlo) < (unsigned)nwin;
 }
-template <typename T> __device__ __forceinline__ bool row_in_plane_n(int y, int wlo, int nwin, int n) {
-  if constexpr (plane_window<T>()) return (unsigned)((y >= (n + 1) / 2 ? y - n : y) - wlo) < (unsigned)nwin;
-  else return true;
-}
-template <typename T, int N> __device__ __forceinline__ int plane_row(int y, int wlo) {
-  return plane_row_n<T>(y, wlo, N);
-}
-template <typename T, int N> __device__ __forceinline__ bool row_in_plane(int y, int wlo, int nwin) {
-  return row_in_plane_n<T>(y, wlo, nwin, N);
+template <int N> __device__ __forceinline__ int plane_row(int y, int wlo) { return plane_row_n(y, wlo, N); }
+template <int N> __device__ __forceinline__ bool row_in_plane(int y, int wlo, int nwin) {
+  return row_in_plane_n(y, wlo, nwin, N);
 }
 
 // i / d for 0 <= i < 2^24 and d >= 1 from the float reciprocal rd = 1/d: the
@@ -135,22 +128,57 @@ __device__ __forceinline__ void plane_row_split(C *row, bool upper, bool ok) {
   }
 }
 
-// In both plane passes every row and column of a plane is split over a lane
-// pair (lane, lane + 32): a wave transforms 32 lines; P planes share a
-// work-group.  A plane occupies LDS only for its row window (the rows that hold
-// sphere points, 51 of 72 at the 72^3 sphere), so more work-groups fit per CU.
+// the transform of a line split over S lanes (part q of the line): input
+// x[S n + q], output X[k + (N/S) line_part_out<S>(q)]
+template <int N, int SIGN, typename C, int S>
+__device__ __forceinline__ void line_dit(C (&v)[N / S], int q) {
+  if constexpr (S == 2) fft_split<N, SIGN, C>::dit(v, q != 0);
+  else fft_quad<N, SIGN, C>::dit(v, q);
+}
+template <int S> __device__ __forceinline__ int line_part_out(int q) { return S == 2 ? q : quad_rev(q); }
+
+// FFT of one LDS row of N values split over S lanes; lanes with ok == false
+// transform a copy of a valid row and store nothing
+template <typename C, int N, int SIGN, int S>
+__device__ __forceinline__ void plane_row_lanes(C *row, int q, bool ok) {
+  if constexpr (S == 2) {
+    plane_row_split<C, N, SIGN>(row, q != 0, ok);
+  } else {
+    constexpr int M = N / S;
+    C v[M];
+#pragma unroll
+    for (int n = 0; n < M; ++n) v[n] = row[S * n + q];
+    line_dit<N, SIGN, C, S>(v, q);
+    __builtin_amdgcn_wave_barrier();
+    if (ok) {
+      const int o = M * line_part_out<S>(q);
+#pragma unroll
+      for (int k = 0; k < M; ++k) row[k + o] = v[k];
+    }
+  }
+}
+
+// In both plane passes every row and column of a plane is split over LANES
+// lanes of a wave; P planes share a work-group of at most 256 lanes.
+// complex64: a lane pair (lane, lane + 32), 32 lines per wave.  complex128, when
+// both plane axes divide by 4: four lanes (lane + 16 q, part q = 0..3), 16 lines
+// per wave; a quarter line per lane keeps the passes near 170 VGPRs, so two
+// work-groups share a CU (a lane pair per line needs ~300 VGPRs, one wave per
+// SIMD: plane_forward 337 -> 251 us, plane_inverse 265 -> 221 us at Si64).
 template <typename T, int N1, int N2> struct PlaneCfg {
   static_assert(N1 % 2 == 0 && N2 % 2 == 0, "split lines need even lengths");
+  static constexpr int LANES = (sizeof(T) == 8 && N1 % 4 == 0 && N2 % 4 == 0) ? 4 : 2;
+  static constexpr int LPW = 64 / LANES;  // lines per wave
   static constexpr int NL = (N1 > N2) ? N1 : N2;
-  // a plane in LDS (padded row stride N2+1): complex64 the rows of its row
-  // window (at most N1) and two more, complex128 all N1 rows
-  static constexpr int PLANE = (plane_window<T>() ? N1 + 2 : N1) * (N2 + 1);
+  // a plane in LDS (padded row stride N2+1): the rows of its row window (at
+  // most N1) and two more
+  static constexpr int PLANE = (N1 + 2) * (N2 + 1);
   static constexpr int PLANE_BYTES = PLANE * (int)sizeof(cplx<T>);
-  static constexpr int P0 = 128 / NL > 0 ? 128 / NL : 1;  // 128 lines per 256 threads
+  static constexpr int P0 = 128 / NL > 0 ? 128 / NL : 1;  // about 128 lines per work-group
   static constexpr int PL = (65536 / PLANE_BYTES) > 0 ? (65536 / PLANE_BYTES) : 1;
   static constexpr int P = (P0 < PL ? P0 : PL) > 0 ? (P0 < PL ? P0 : PL) : 1;
-  static constexpr int WAVES = (P * NL + 31) / 32 < 4 ? (P * NL + 31) / 32 : 4;
-  static constexpr int THREADS = 64 * WAVES;  // split lines
+  static constexpr int WAVES = (P * NL + LPW - 1) / LPW < 4 ? (P * NL + LPW - 1) / LPW : 4;
+  static constexpr int THREADS = 64 * WAVES;
   static constexpr int LDS = P * PLANE_BYTES;
 };
 

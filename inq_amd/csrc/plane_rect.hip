@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(256) k_rows_inverse(const SphereBlock<T> *bloc
   const V4<int> sg = ld4<int>(B.slot_seg, 2 * slot);
   const int seg0 = sg.x, nseg = sg.y, r0 = sg.z, nrow = sg.w;
   const V4<int> sw = ld4<int>(B.slot_seg, 2 * slot + 1);
-  const int wlo = sw.x, nwin = sw.y;  // row window (complex64 layout)
+  const int wlo = sw.x, nwin = sw.y;  // row window of the plane in LDS
   const float rseg = 1.0f / (float)max(nseg, 1), rrow = 1.0f / (float)max(nrow, 1);
   const GAS int *pwl = gp(B.pw_lds) + seg0;
   const GAS int *rows = gp(B.rows) + r0;
@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(256) k_rows_inverse(const SphereBlock<T> *bloc
     const int lc = ok ? li : nr - 1;
     const int p = div_rcp(lc, nrow, rrow);
     const int y = rows[lc - p * nrow];
-    plane_row_split<C, N2, +1>(lds + p * pstr + plane_row_n<T>(y, wlo, n1) * (N2 + 1), upper, ok);
+    plane_row_split<C, N2, +1>(lds + p * pstr + plane_row_n(y, wlo, n1) * (N2 + 1), upper, ok);
   }
   __syncthreads();
   // the whole planes to t1 (slot `slot` of states s0 .. s0+np-1), rows outside
@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(256) k_rows_inverse(const SphereBlock<T> *bloc
     const int p = div_rcp(e, n12, r12), r = e - p * n12;
     const int y = r / N2, z = r - y * N2;
     C v = C{0, 0};
-    if (row_in_plane_n<T>(y, wlo, nwin, n1)) v = lds[p * pstr + plane_row_n<T>(y, wlo, n1) * (N2 + 1) + z];
+    if (row_in_plane_n(y, wlo, nwin, n1)) v = lds[p * pstr + plane_row_n(y, wlo, n1) * (N2 + 1) + z];
     dst[(size_t)p * pstride + r] = v;
   }
 }
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(256) k_rows_forward(const SphereBlock<T> *bloc
     const int lr = e / N2, z = e - lr * N2;
     const int p = div_rcp(lr, nrow, rrow);
     const int y = rows[lr - p * nrow];
-    lds[p * pstr + plane_row_n<T>(y, wlo, n1) * (N2 + 1) + z] = src[(size_t)p * pstride + y * N2 + z];
+    lds[p * pstr + plane_row_n(y, wlo, n1) * (N2 + 1) + z] = src[(size_t)p * pstride + y * N2 + z];
   }
   __syncthreads();
   for (int g = wave; g * 32 < nr; g += nwave) {
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(256) k_rows_forward(const SphereBlock<T> *bloc
     const int lc = ok ? li : nr - 1;
     const int p = div_rcp(lc, nrow, rrow);
     const int y = rows[lc - p * nrow];
-    plane_row_split<C, N2, -1>(lds + p * pstr + plane_row_n<T>(y, wlo, n1) * (N2 + 1), upper, ok);
+    plane_row_split<C, N2, -1>(lds + p * pstr + plane_row_n(y, wlo, n1) * (N2 + 1), upper, ok);
   }
   __syncthreads();
   const GatherBufs R = gather_bufs(B, s0, seg0, np, nseg);
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(256) k_rows_forward(const SphereBlock<T> *bloc
 
 constexpr int kRectThreads = 256;
 
-// dynamic LDS up to the CU's 160 KB (a complex128 plane of 96 x 97 is 149 KB)
+// dynamic LDS up to the CU's 160 KB (a complex128 plane of 98 x 97 rows x columns is 152 KB)
 template <typename K> void allow_lds(K kernel, int device, DeviceOnce &once) {
   if (once.test_and_set(device)) return;
   INQX_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
@@ -230,8 +230,7 @@ bool rect_plane_supported(int n1, int n2) {
 
 int rect_plane_batch(int prec, int n1, int n2) {
   const int cs = prec == INQX_FP64 ? 16 : 8;
-  const int rows = prec == INQX_FP64 ? n1 : n1 + 2;  // complex64 planes: the row window and two more rows
-  const int plane = rows * (n2 + 1) * cs;
+  const int plane = (n1 + 2) * (n2 + 1) * cs;  // the row window (at most n1 rows) and two more
   const int p0 = std::max(1, 128 / std::max(n1, n2));  // about 128 lines per work-group
   return std::max(1, std::min(p0, 65536 / plane));
 }
