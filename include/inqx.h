@@ -92,10 +92,13 @@ int inqx_stream_copy(inqx_ctx ctx, const void *src, void *dst, size_t bytes);
 /* ------------------------------------------------------------ Hamiltonian */
 /* hamiltonian::ks_hamiltonian constructor (src/hamiltonian/ks_hamiltonian.hpp:92-102):
  * lattice rows are the cell vectors in bohr (src/systems/cell.hpp:36-52), n the
- * real-space grid (src/basis/real_space.hpp:39-50).  The sphere path takes any
- * axis length of 12, 16, 18, 20, 24, 28, 30, 32, 36, 40, 48, 54, 60, 64, 72, 80
- * or 96 on each axis independently (other lengths return INQX_ERR_UNSUPPORTED
- * from the calls that transform). */
+ * real-space grid (src/basis/real_space.hpp:39-50).  Each axis takes any of the
+ * lengths 12, 16, 18, 20, 24, 28, 30, 32, 36, 40, 48, 54, 60, 64, 72, 80, 96, 100,
+ * 108, 120 or 128, independently of the others (other lengths return
+ * INQX_ERR_UNSUPPORTED from the calls that transform).  A complex128 apply
+ * whose sphere's plane row window (rows x (n2 + 1) x 16 bytes) exceeds the
+ * 160 KB of LDS also returns INQX_ERR_UNSUPPORTED (a whole-cube basis from 100
+ * points; spheres of a plane-wave cutoff fit to 128^3). */
 int inqx_ham_create(inqx_ctx ctx, const double lattice[9], const int n[3], inqx_ham *out);
 int inqx_ham_destroy(inqx_ham ham);
 /* ks_hamiltonian::scalar_potential() (ks_hamiltonian.hpp:241-243): host array

@@ -744,15 +744,22 @@ bool pair_grid_supported(const int *n) {
 }
 
 namespace {
+// square planes whose LDS plane passes do not fit the CU's 160 KB (complex128
+// from 100 points) run the column and row kernels instead
+template <typename T, int S0, int S> void run_pairs_square(inqx_ham_s *h, const PairArgs &a) {
+  if constexpr (PairPlaneCfg<T, S, S>::LDS > 160 * 1024) run_pairs<T, S0, 0>(h, a);
+  else run_pairs<T, S0, S>(h, a);
+}
+
 template <typename T, int S0> void pairs_n12(inqx_ham_s *h, const PairArgs &a) {
   if (h->g.n[1] != h->g.n[2]) {
     run_pairs<T, S0, 0>(h, a);
     return;
   }
   switch (h->g.n[1]) {
-#define CASE(S)                   \
-  case S:                         \
-    run_pairs<T, S0, S>(h, a);    \
+#define CASE(S)                          \
+  case S:                                \
+    run_pairs_square<T, S0, S>(h, a);    \
     break;
     INQX_SIZES(CASE)
 #undef CASE

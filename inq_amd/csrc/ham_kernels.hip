@@ -63,9 +63,10 @@ struct HamDev {
   double lat[9];    // lattice vectors (rows), for positions r of grid points
 };
 
-// x-pass work-group size limit: up to 8 states for complex64; complex128
-// needs a wave per SIMD (VGPRs plus AGPRs) and takes at most 4
-template <typename T> constexpr int xpass_threads() { return sizeof(T) == 4 ? 512 : 256; }
+// x-pass work-group size limit: up to 8 states for complex64 lines up to 96
+// points; complex128 and longer lines need a wave per SIMD (VGPRs plus AGPRs)
+// and take at most 4
+template <typename T> constexpr int xpass_threads(int n0) { return sizeof(T) == 4 && n0 <= 96 ? 512 : 256; }
 
 // -------------------------------------------------------------------------------------------
 // plane passes (shared helpers in plane_common.hpp)
@@ -323,40 +324,48 @@ __device__ __forceinline__ void store_line_slots_split(const SphereBlock<T> &B, 
   }
 }
 
-// bit i0 of a line's 96-bit point mask, i0 = k + half N0/2 (k a constant)
-template <int N0>
-__device__ __forceinline__ bool mask_bit(const uint32_t (&m)[3], int k, bool upper) {
-  const int il = k, ih = k + N0 / 2;
-  const bool bl = (m[il >> 5] >> (il & 31)) & 1u;
-  const bool bh = (m[ih >> 5] >> (ih & 31)) & 1u;
-  return upper ? bh : bl;
-}
-// number of mask bits below i0 = N0/2
-template <int N0> __device__ __forceinline__ int mask_count_low(const uint32_t (&m)[3]) {
-  constexpr int H = N0 / 2;
-  int c = 0;
-#pragma unroll
-  for (int w = 0; w < 3; ++w) {
-    const int lo = w * 32;
-    if (H >= lo + 32) c += __popc(m[w]);
-    else if (H > lo) c += __popc(m[w] & ((1u << (H - lo)) - 1u));
+// a line's point mask (bit i0: the line holds a sphere point of the tile's KB
+// records at i0) and the number of the tile's points on its earlier lines.
+// The table holds one 16-byte record per line (mask words 0-2, count) for lines
+// up to 96 points, two (mask words 0-3, count) for longer lines.
+template <int N0> constexpr int mask_words() { return N0 > 96 ? 4 : 3; }
+template <int N0> struct LineMask {
+  uint32_t m[mask_words<N0>()];
+  int base;
+};
+template <int N0> __device__ __forceinline__ LineMask<N0> load_line_mask(const uint32_t *tab, int L) {
+  LineMask<N0> r;
+  if constexpr (N0 > 96) {
+    const V4<uint32_t> a = ld4<uint32_t>(tab, 2 * (size_t)L), b = ld4<uint32_t>(tab, 2 * (size_t)L + 1);
+    r.m[0] = a.x;
+    r.m[1] = a.y;
+    r.m[2] = a.z;
+    r.m[3] = a.w;
+    r.base = (int)b.x;
+  } else {
+    const V4<uint32_t> a = ld4<uint32_t>(tab, L);
+    r.m[0] = a.x;
+    r.m[1] = a.y;
+    r.m[2] = a.z;
+    r.base = (int)a.w;
   }
-  return c;
+  return r;
 }
 
 // the lane's half of a line's point mask as bits 0..N0/2-1 (bit k: i0 = k +
 // half N0/2 holds a sphere point), and the number of points of the line
-// before that half; m = the line's 96-bit mask, base = points of the tile's
-// earlier lines
+// before that half plus the tile's points on earlier lines
 template <int N0>
-__device__ __forceinline__ unsigned long long half_mask(const uint32_t (&m)[3], bool upper, int base, int &cnt0) {
+__device__ __forceinline__ unsigned long long half_mask(const LineMask<N0> &lm, bool upper, int &cnt0) {
   constexpr int H = N0 / 2;
-  static_assert(H <= 48, "lines up to 96 points");
-  const unsigned long long lo = (unsigned long long)m[0] | ((unsigned long long)m[1] << 32);  // bits 0..63
-  const unsigned long long keep = (1ull << H) - 1ull;
+  static_assert(H <= 64, "lines up to 128 points");
+  const unsigned long long lo = (unsigned long long)lm.m[0] | ((unsigned long long)lm.m[1] << 32);  // bits 0..63
+  unsigned long long hi = lm.m[2];                                                                 // bits 64..
+  if constexpr (N0 > 96) hi |= (unsigned long long)lm.m[3] << 32;
+  const unsigned long long keep = H >= 64 ? ~0ull : (1ull << (H & 63)) - 1ull;
   const unsigned long long ml = lo & keep;
-  const unsigned long long mu = ((lo >> H) | ((unsigned long long)m[2] << (64 - H))) & keep;
-  cnt0 = base + (upper ? __popcll(ml) : 0);
+  const unsigned long long mu = H >= 64 ? hi : ((lo >> (H & 63)) | (hi << ((64 - H) & 63))) & keep;
+  cnt0 = lm.base + (upper ? __popcll(ml) : 0);
   return upper ? mu : ml;
 }
 
@@ -396,7 +405,7 @@ struct XLds {
 // sums of a slot are then added in lane order (no atomics, the sums do not
 // depend on timing).
 template <typename T, int N0>
-__global__ void __launch_bounds__(xpass_threads<T>()) k_xpass_project(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
+__global__ void __launch_bounds__(xpass_threads<T>(N0)) k_xpass_project(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                        int total, HamDev hd, XLds xl) {
   using C = cplx<T>;
   constexpr int H = N0 / 2;
@@ -437,18 +446,16 @@ __global__ void __launch_bounds__(xpass_threads<T>()) k_xpass_project(const Sphe
     const bool active = s < B.nst;
     const bool valid = L < hd.n12;
     C v[H];
-    V4<uint32_t> w{0u, 0u, 0u, 0u};
     if (active) {
       // line loads and the line's point mask in flight together
       load_line_slots_split<T, N0>(B, hd.n12, s, valid ? L : hd.n12 - 1, upper, v);
-      w = ld4<uint32_t>(hd.line_mask, valid ? L : hd.n12 - 1);
+      const LineMask<N0> lm = load_line_mask<N0>(hd.line_mask, valid ? L : hd.n12 - 1);
       fft_split<N0, +1, C>::dit(v, upper);
       // every value goes out: the line's sphere points to their compact
       // slots, the others (and lanes past the last line) to the trash slot
       // pts[npts + 1], so the stores need no branches
-      const uint32_t m[3] = {w.x, w.y, w.z};
       int cnt;
-      const unsigned long long mh = valid ? half_mask<N0>(m, upper, (int)w.w, cnt) : 0ull;
+      const unsigned long long mh = valid ? half_mask<N0>(lm, upper, cnt) : 0ull;
       if (!valid) cnt = 0;
       const int trash = npts + 1;
 #pragma unroll
@@ -623,7 +630,7 @@ __global__ void __launch_bounds__(256) k_kb_reduce(const SphereBlock<T> *blocks,
 // (shared), its KB entry records (shared) and each state's D P of the tile's
 // slots, so no memory round trip sits between the two line transforms.
 template <typename T, int N0>
-__global__ void __launch_bounds__(xpass_threads<T>()) k_xpass_apply(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
+__global__ void __launch_bounds__(xpass_threads<T>(N0)) k_xpass_apply(const SphereBlock<T> *blocks, const int *prefix, int nblocks,
                                                      int total, HamDev hd, XLds xl) {
   using C = cplx<T>;
   constexpr int H = N0 / 2;
@@ -701,11 +708,11 @@ __global__ void __launch_bounds__(xpass_threads<T>()) k_xpass_apply(const Sphere
       }
     }
     C v[H];
-    V4<uint32_t> w{0u, 0u, 0u, 0u};
+    LineMask<N0> lm{};
     if (act) {
       load_line_slots_split<T, N0>(B, hd.n12, s, Lc, upper, v);
       if (npts > 0) {
-        w = ld4<uint32_t>(hd.line_mask, Lc);
+        lm = load_line_mask<N0>(hd.line_mask, Lc);
         // the zero slot pts[npts] and the trash slot pts[npts + 1]; the compact
         // points too unless every point has exactly one record (then the KB
         // loop below stores each point once)
@@ -801,9 +808,8 @@ __global__ void __launch_bounds__(xpass_threads<T>()) k_xpass_apply(const Sphere
       if (npts > 0) {
         // every value reads: its compact point, or the zero slot pts[npts]
         // (no branches, so the reads are in flight together)
-        const uint32_t m[3] = {w.x, w.y, w.z};
         int cnt;
-        const unsigned long long mh = valid ? half_mask<N0>(m, upper, (int)w.w, cnt) : 0ull;
+        const unsigned long long mh = valid ? half_mask<N0>(lm, upper, cnt) : 0ull;
         if (!valid) cnt = 0;
 #pragma unroll
         for (int k = 0; k < H; ++k) {
@@ -948,7 +954,7 @@ bool grid_supported(const int *n) {
       default: return false;
     }
   };
-  return ok(n[0]) && ok(n[1]) && ok(n[2]) && n[0] <= 96;
+  return ok(n[0]) && ok(n[1]) && ok(n[2]);
 }
 
 void build_kb_tables(inqx_ham_s *h) {
@@ -989,7 +995,10 @@ void build_kb_tables(inqx_ham_s *h) {
     }
   }
   // unique points of each tile sorted by (lane, i0) give the compact index
-  std::vector<uint32_t> mask((size_t)n12 * 4, 0u);
+  // line mask records: 4 words (mask 0-2, count) up to 96 points per line, 8
+  // (mask 0-3, count, padding) above
+  const int mw = g.n[0] > 96 ? 8 : 4, mcount = g.n[0] > 96 ? 4 : 3;
+  std::vector<uint32_t> mask((size_t)n12 * mw, 0u);
   std::vector<std::vector<std::pair<int, int>>> tile_pts(kb.ntile);
   for (auto &m : mem) tile_pts[m.tile].push_back({m.lane, m.i0});
   std::vector<int> tile_info((size_t)kb.ntile * 8, 0);
@@ -1002,8 +1011,8 @@ void build_kb_tables(inqx_ham_s *h) {
     int c = 0;
     for (size_t k = 0; k < v.size(); ++k) {
       const int L = t * kTile + v[k].first;
-      if (k == 0 || v[k].first != v[k - 1].first) mask[(size_t)L * 4 + 3] = (uint32_t)c;
-      mask[(size_t)L * 4 + (v[k].second >> 5)] |= 1u << (v[k].second & 31);
+      if (k == 0 || v[k].first != v[k - 1].first) mask[(size_t)L * mw + mcount] = (uint32_t)c;
+      mask[(size_t)L * mw + (v[k].second >> 5)] |= 1u << (v[k].second & 31);
       ++c;
     }
     tile_info[8 * t + 0] = (int)v.size();
@@ -1390,14 +1399,15 @@ __global__ void __launch_bounds__(256) k_line_from_real(const SphereBlock<T> *bl
   }
 }
 
+// dynamic LDS of the plane passes: up to the largest plane window of the
+// configuration (checked against the CU's 160 KB at each launch)
 template <typename T, int N1, int N2> static void set_plane_attr(int device) {
   static DeviceOnce once;
   using Cfg = PlaneCfg<T, N1, N2>;
   if (Cfg::LDS <= 65536 || once.test_and_set(device)) return;
-  INQX_HIP(hipFuncSetAttribute((const void *)k_plane_inverse<T, N1, N2>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
-  INQX_HIP(hipFuncSetAttribute((const void *)k_plane_forward<T, N1, N2>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, Cfg::LDS));
+  const int lds = std::min(Cfg::LDS, 160 * 1024);
+  INQX_HIP(hipFuncSetAttribute((const void *)k_plane_inverse<T, N1, N2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  INQX_HIP(hipFuncSetAttribute((const void *)k_plane_forward<T, N1, N2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
 }
 
 // the x-pass LDS depends on the KB tables: the attribute grows to the largest
@@ -1428,6 +1438,9 @@ static void run_passes(inqx_ham_s *h, LaunchData &ld, int nblocks, const HamDev 
   const int cu = h->ctx->c.num_cu;
   const int grid_plane = ld.total_plane;  // one item per work-group
   const int plane_lds = ld.plane_batch * ld.plane_pstr * (int)sizeof(cplx<T>);
+  INQX_CHECK(plane_lds <= 160 * 1024, INQX_ERR_UNSUPPORTED,
+             "the plane passes' row window does not fit the 160 KB of LDS in this precision (" +
+                 std::to_string(plane_lds) + " bytes)");
   Ctx &cx = h->ctx->c;
   const bool f64 = std::is_same<T, double>::value;
   auto plane_inv = [&]() {
@@ -1618,7 +1631,7 @@ void launch_apply(inqx_ham_s *h, int nblocks, Basis *const *bases, const int *ns
     ld->plane_batch = plane_p;
     // waves (states) per x-pass work-group within the LDS budget
     const int cs = (int)sizeof(cplx<T>);
-    int nw = std::max(1, std::min(xpass_threads<T>() / 64, h->xwaves[prec]));  // states (waves) per x-pass work-group
+    int nw = std::max(1, std::min(xpass_threads<T>(g.n[0]) / 64, h->xwaves[prec]));  // states (waves) per x-pass work-group
     ld->mode = mode;
     ld->observables = opts->observables;
     ld->ntile_line = (g.n[1] * g.n[2] + 31) / 32;

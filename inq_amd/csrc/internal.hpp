@@ -146,7 +146,7 @@ struct Projector {
 };
 
 // line lengths with compiled register FFTs (radix-4/2/3/5/7 stages)
-#define INQX_SIZES(X) X(12) X(16) X(18) X(20) X(24) X(28) X(30) X(32) X(36) X(40) X(48) X(54) X(60) X(64) X(72) X(80) X(96)
+#define INQX_SIZES(X) X(12) X(16) X(18) X(20) X(24) X(28) X(30) X(32) X(36) X(40) X(48) X(54) X(60) X(64) X(72) X(80) X(96) X(100) X(108) X(120) X(128)
 
 constexpr int kMaxNlm = 4;   // channels of one KB group (s + 3 p, padded)
 // A projector of nlm channels (up to l = 2: s + 3 p + 5 d = 9) is split into
@@ -174,7 +174,7 @@ struct KBTables {
   long nent = 0;
   // per line: bitmask of sphere points along i0 (3 words) and the line's
   // first compact point index inside its tile
-  DevMem line_mask;   // [n12] uint4: m0, m1, m2, base
+  DevMem line_mask;   // [n12] uint4 m0, m1, m2, base (n0 <= 96) or 2 x uint4 m0-m3, base, 0, 0, 0
   DevMem tile_info;   // [ntile] int4: npts, nslot_t, slot_base, ent_base ; [ntile] ent count in a 2nd int4
   DevMem lane_info;   // [ntile][64] int: project-pass run length | (slot + 1) << 8 | (first lane of slot) << 16
   DevMem slot_atom;   // [nslot] atom of each global slot

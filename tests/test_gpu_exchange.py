@@ -467,3 +467,50 @@ def test_noncubic_grid_poisson_exchange_hartree(ctx, dtype, ng2, lens):
     vref = oh.poisson_in_place(dens.astype(np.complex128)[:, None], zeroterm=0.0)[:, 0].real
     assert rel_err(vh.numpy(), vref) < 1e-11
     assert eh == pytest.approx(0.5 * float(np.sum(dens * vref)) * vol / n2, rel=1e-11)
+
+
+@pytest.mark.parametrize("ng2,lens", [((128, 128, 128), (12.0, 12.0, 12.0)), ((100, 120, 108), (10.0, 12.0, 10.8))])
+def test_grids_above_96_points_poisson_hartree(ctx, ng2, lens):
+    """batched Poisson (both precisions) and the Hartree potential on grids
+    with lines above 96 points: complex128 pair planes that do not fit the
+    160 KB of LDS run the column and row kernels; complex64 exchange at k"""
+    say("case: Poisson / Hartree / exchange above 96 points", ng2)
+    lat2 = np.diag(lens)
+    vol = float(np.prod(lens))
+    n2 = int(np.prod(ng2))
+    gh = KSHamiltonian(ctx, lat2, ng2)
+    oh = orc.Ham(lat2, ng2)
+    rng = np.random.default_rng(14)
+    rho = rng.standard_normal((2, n2)) + 1j * rng.standard_normal((2, n2))
+    q = (0.1, -0.2, 0.05)
+    ref = oh.poisson_in_place(np.ascontiguousarray(rho.T), gshift=q, zeroterm=0.3).T
+    for dtype in DTYPES:
+        got = gh.poisson(DeviceArray.from_numpy(ctx, rho.astype(dtype)), q, 0.3).numpy()
+        assert rel_err(got, ref) < TOL_T[dtype], dtype
+    dens = 0.01 + rng.random(n2) * 0.02
+    vh, eh = gh.hartree(DeviceArray.from_numpy(ctx, dens))
+    vref = oh.poisson_in_place(dens.astype(np.complex128)[:, None], zeroterm=0.0)[:, 0].real
+    assert rel_err(vh.numpy(), vref) < 1e-11
+    assert eh == pytest.approx(0.5 * float(np.sum(dens * vref)) * vol / n2, rel=1e-11)
+    # complex64 exchange of states at k with occupied states at q
+    k, kq = (0.3, -0.2, 0.1), (-0.4, 0.5, 0.2)
+    bk, bq = gh.basis(k, 8.0), gh.basis(kq, 8.0)
+    vel = vol / float(n2) ** 2
+    phi = S.random_states(bk.npw, 2, 63, vel).astype(np.complex64)
+    hq = S.random_states(bq.npw, 1, 64, vel).astype(np.complex64)
+    ex = ExchangeOperator(gh, bk, exchange_coefficient=0.5)
+    got = ex.direct_kpts(DeviceArray.from_numpy(ctx, phi),
+                         [(bq, DeviceArray.from_numpy(ctx, hq), np.array([2.0]), 0.0)]).numpy()
+
+    def rs(c, g):
+        cube = np.zeros((c.shape[0], n2), dtype=np.complex128)
+        cube[:, g] = c
+        return np.ascontiguousarray(np.fft.ifftn(cube.reshape((c.shape[0],) + ng2), axes=(1, 2, 3))
+                                    .reshape(c.shape[0], n2).T)
+    gk, gq = bk.gindex(), bq.gindex()
+    cart = lambda v: np.asarray(v, dtype=np.float64) @ np.linalg.inv(lat2).T
+    w = oh.exchange_direct(rs(phi.astype(np.complex128), gk), rs(hq.astype(np.complex128), gq),
+                           np.array([2.0]), exx_coeff=0.5, gshift=cart(kq) - cart(k), zeroterm=0.0)
+    ref = np.fft.fftn(np.ascontiguousarray(w.T).reshape((2,) + ng2), axes=(1, 2, 3)).reshape(2, n2)[:, gk]
+    assert rel_err(got, ref) < TOL_X[np.complex64]
+

@@ -100,6 +100,52 @@ def test_plane_n1_differs_from_n2(ctx, dtype, n, lat):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,lat", [((128, 128, 128), (12.0, 12.0, 12.0)), ((100, 120, 108), (10.0, 12.0, 10.8))])
+def test_grids_above_96_points(ctx, dtype, n, lat):
+    """line lengths above 96 (lines over 96 points use 128-bit point masks; the
+    complex64 x passes take 4 states per work-group there): H on the sphere at
+    k != 0 with three atoms against the oracle, and the sphere <-> real-space
+    transforms against numpy's FFT"""
+    rng = np.random.default_rng(29)
+    L = np.diag(lat)
+    sysm = S.System(f"big{n[0]}", 192, L, n, rng.uniform(-0.5, 0.5, (3, 3)) @ L, 20.0, 3,
+                    v_sigma=0.6, kb_width=0.7, kb_radius=1.6)
+    k = (0.25, -0.15, 0.3)
+    say("case: grid", n, np.dtype(dtype).name)
+    gh = gpu_ham(ctx, sysm)
+    oh = oracle_ham(sysm)
+    b, c, out = _run(ctx, gh, sysm, k, sysm.ecut, sysm.nst, dtype, 37)
+    ref = oh.apply_sphere(c.astype(dtype).astype(np.complex128), b.gindex(), k)
+    assert rel_err(out, ref) < TOL[dtype]
+    g = b.gindex()
+    cube = np.zeros((sysm.nst, sysm.N), dtype=np.complex128)
+    cube[:, g] = c.astype(dtype)
+    want = np.fft.ifftn(cube.reshape((sysm.nst,) + tuple(n)), axes=(1, 2, 3)).reshape(sysm.nst, sysm.N)
+    rs = gh.to_real(b, DeviceArray.from_numpy(ctx, c.astype(dtype)), 0).numpy()
+    assert rel_err(rs, want) < TOL[dtype]
+
+
+def test_complex128_plane_window_beyond_lds_is_a_status(ctx):
+    """a complex128 sphere whose plane row window does not fit the 160 KB of
+    LDS (a whole-cube basis at 128^3) is refused with INQX_ERR_UNSUPPORTED; the
+    complex64 apply of the same basis runs"""
+    from inq_amd import InqxError
+    sysm = S.small_config(seed=4, n=(128, 128, 128), natoms=0, nst=1, ecut=6.0)
+    say("case: complex128 plane window beyond LDS at 128^3")
+    from inq_amd import KSHamiltonian
+    gh = KSHamiltonian(ctx, sysm.lattice, sysm.n)
+    gh.set_scalar_potential(np.zeros(sysm.N))
+    b = gh.basis((0, 0, 0), 1e6)
+    c = np.zeros((1, b.npw), dtype=np.complex128)
+    c[0, 5] = 1.0
+    with pytest.raises(InqxError) as e:
+        gh(b, DeviceArray.from_numpy(ctx, c))
+    assert "status -3" in str(e.value)
+    out = gh(b, DeviceArray.from_numpy(ctx, c.astype(np.complex64))).numpy()
+    assert out.shape == c.shape
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 def test_c1_diamond(ctx, dtype):
     sysm = S.config(1)
     say("case: C1 diamond 32^3", np.dtype(dtype).name)
