@@ -241,6 +241,7 @@ void launch_rect_inverse(Ctx &cx, const SphereBlock<T> *blk, const int *pref, in
   INQX_CHECK(rect_plane_supported(n1, n2), INQX_ERR_UNSUPPORTED, "plane size not built for the sphere path");
   const bool f64 = std::is_same<T, double>::value;
   const int lds = pb * pstr * (int)sizeof(cplx<T>);
+  INQX_CHECK(lds <= 160 * 1024, INQX_ERR_UNSUPPORTED, "the plane passes' row window does not fit the 160 KB of LDS");
   {
     ProfScope ps(cx, f64 ? "plane_rows_inverse_f64" : "plane_rows_inverse_f32");
     switch (n2) {
@@ -273,6 +274,7 @@ void launch_rect_forward(Ctx &cx, const SphereBlock<T> *blk, const int *pref, in
   INQX_CHECK(rect_plane_supported(n1, n2), INQX_ERR_UNSUPPORTED, "plane size not built for the sphere path");
   const bool f64 = std::is_same<T, double>::value;
   const int lds = pb * pstr * (int)sizeof(cplx<T>);
+  INQX_CHECK(lds <= 160 * 1024, INQX_ERR_UNSUPPORTED, "the plane passes' row window does not fit the 160 KB of LDS");
   {
     ProfScope ps(cx, f64 ? "plane_cols_forward_f64" : "plane_cols_forward_f32");
     switch (n1) {
