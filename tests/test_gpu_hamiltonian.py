@@ -82,6 +82,12 @@ def test_plane_n1_differs_from_n2(ctx, dtype, n, lat):
     b, c, out = _run(ctx, gh, sysm, k, sysm.ecut, sysm.nst, dtype, 29)
     ref = oh.apply_sphere(c.astype(dtype).astype(np.complex128), b.gindex(), k)
     assert rel_err(out, ref) < TOL[dtype]
+    # the launch-shape autotuner on these planes (row and column kernels at
+    # every plane batch it tries) changes no result bit
+    shape, ms = gh.autotune_batch([b], [sysm.nst], dtype, None)
+    assert shape[1] >= 1 and ms > 0
+    d_in = DeviceArray.from_numpy(ctx, c.astype(dtype))
+    assert np.array_equal(gh(b, d_in).numpy(), out)
     bw, cw, outw = _run(ctx, gh, sysm, k, 1e6, 3, dtype, 31)
     assert bw.npw == sysm.N
     refw = oh.apply_sphere(cw.astype(dtype).astype(np.complex128), bw.gindex(), k)
