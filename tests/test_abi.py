@@ -89,6 +89,23 @@ def test_gsphere_matches_oracle(cid):
         assert np.max(np.abs(b.kinetic() - oh.gsphere_kinetic(k, g))) < 1e-12
 
 
+@pytest.mark.parametrize("n,lat", [((128, 128, 128), (12.0, 12.0, 12.0)), ((100, 120, 108), (10.0, 12.0, 10.8)),
+                                   ((24, 20, 30), (6.0, 5.0, 7.5))])
+def test_gsphere_on_large_and_noncubic_grids(n, lat):
+    """the G sphere and its kinetic factors on grids with lines above 96 points
+    and with planes that are not square, bit-exact against the oracle"""
+    from oracle import oracle as orc
+    L = np.diag(lat)
+    ctx = Context(device=-1)
+    gh = KSHamiltonian(ctx, L, n)
+    oh = orc.Ham(L, n)
+    for k, ec in (((0.0, 0.0, 0.0), 20.0), ((0.25, -0.15, 0.3), 12.0)):
+        b = gh.basis(k, ec)
+        g = b.gindex()
+        assert np.array_equal(g, oh.gsphere(k, ec))
+        assert np.max(np.abs(b.kinetic() - oh.gsphere_kinetic(k, g))) < 1e-12
+
+
 def test_potential_components_cover_every_basis_spin():
     """A basis with spin index s reads column s of the potential, so a potential
     with ncomp <= s is rejected (INQX_ERR_ARG), and so is the apply on such a
