@@ -157,22 +157,27 @@ __global__ void __launch_bounds__(256) k_pair_plane_fwd(PairDev<T> d, int total)
   }
 }
 
-// each line is split over a lane pair (lane, lane + 32): 32 lines per wave
+// lanes per line of the pair line pass: four for complex128 lines above 96
+// points (a quarter line per lane: the lane-pair form spills 57-308 VGPRs to
+// scratch there), else a lane pair
+template <typename T, int N0> constexpr int xline_lanes() { return sizeof(T) == 8 && N0 > 96 && N0 % 4 == 0 ? 4 : 2; }
+
+// each line is split over S lanes (lane pair (lane, lane + 32), or lane + 16 q):
+// 64 / S lines per wave
 // CUBE: n1 = n2 = N0 known at compile time (the row strides of the loads
 // become constants; 143 instead of 198 VGPRs at 96^3 complex64), otherwise
 // n1 and n2 are read at run time
 template <typename T, int N0, bool CUBE>
 __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
   using C = cplx<T>;
+  constexpr int S = xline_lanes<T, N0>(), LPW = 64 / S, M = N0 / S;
   const int N1 = CUBE ? N0 : d.n1, N2 = CUBE ? N0 : d.n2;
   const int NP = N1 * N2;
-  constexpr int H = N0 / 2;
-  const int NTILE = (NP + 31) / 32;
+  const int NTILE = (NP + LPW - 1) / LPW;
   const int NGRP = (NTILE + 3) / 4;
   const long N = (long)N0 * NP;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
-  const bool upper = lane >= 32;
-  const int half = upper ? 1 : 0;
+  const int q = lane / LPW, qo = line_part_out<S>(q);  // part of the line, its output position
   // one work item per work-group, the grid being the work count: a grid-stride loop
   // here made the complex128 96-point kernels spill to scratch
   {
@@ -182,24 +187,24 @@ __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
     const int tile = (wg - pair * NGRP) * 4 + wave;
     if (tile >= NTILE) return;  // wave-uniform; no barrier follows
     if (pair_skipped(d, pair / d.nst, pair % d.nst)) return;
-    const int L = tile * 32 + (lane & 31);
+    const int L = tile * LPW + lane % LPW;
     const int Lc = L < NP ? L : NP - 1;
     const auto rs = make_rsrc(d.t + (size_t)pair * N, (unsigned)(N * sizeof(C)));
-    const unsigned voff = (unsigned)((Lc + half * NP) * sizeof(C));
-    C v[H];
+    const unsigned voff = (unsigned)((Lc + q * NP) * sizeof(C));  // i0 = S n + q
+    C v[M];
 #pragma unroll
-    for (int n = 0; n < H; ++n) bload(rs, voff, (unsigned)(2 * n * NP * sizeof(C)), v[n]);
-    fft_split<N0, -1, C>::dit(v, upper);
+    for (int n = 0; n < M; ++n) bload(rs, voff, (unsigned)(S * n * NP * sizeof(C)), v[n]);
+    line_dit<N0, -1, C, S>(v, q);
     __builtin_amdgcn_sched_barrier(0);  // keep the kernel factors out of the FFT's live range
-    // frequency f = k + half H; G + q = m0 b0 + (m1 b1 + m2 b2 + q)
+    // frequency f = k + qo N0/S; G + q = m0 b0 + (m1 b1 + m2 b2 + q)
     const int i1 = Lc / N2, i2 = Lc - (Lc / N2) * N2;
     const T m1 = (T)(i1 >= (N1 + 1) / 2 ? i1 - N1 : i1), m2 = (T)(i2 >= (N2 + 1) / 2 ? i2 - N2 : i2);
     T u[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) u[c] = m1 * d.b[1][c] + m2 * d.b[2][c] + d.q[c];
 #pragma unroll
-    for (int k = 0; k < H; ++k) {
-      const int f = k + half * H;
+    for (int k = 0; k < M; ++k) {
+      const int f = k + qo * M;
       const T m0 = (T)(f >= (N0 + 1) / 2 ? f - N0 : f);
       const T g0 = m0 * d.b[0][0] + u[0], g1 = m0 * d.b[0][1] + u[1], g2 = m0 * d.b[0][2] + u[2];
       const T gg = g0 * g0 + g1 * g1 + g2 * g2;
@@ -208,10 +213,10 @@ __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
       v[k].y *= fct;
     }
     __builtin_amdgcn_sched_barrier(0);
-    fft_split<N0, +1, C>::dif(v, upper);
+    line_dif<N0, +1, C, S>(v, q);  // v[n]: i0 = S n + q
     if (L < NP) {
 #pragma unroll
-      for (int n = 0; n < H; ++n) bstore(rs, voff, (unsigned)(2 * n * NP * sizeof(C)), v[n]);
+      for (int n = 0; n < M; ++n) bstore(rs, voff, (unsigned)(S * n * NP * sizeof(C)), v[n]);
     }
   }
 }
@@ -537,7 +542,8 @@ template <typename T, int S0, int S> void run_pairs(inqx_ham_s *h, const PairArg
       pair_planes_rect<T, -1, true>(cx, d, tot1);
   }
   INQX_HIP(hipGetLastError());
-  const int ngrp = ((d.n1 * d.n2 + 31) / 32 + 3) / 4;
+  constexpr int xlpw = 64 / xline_lanes<T, S0>();  // lines per wave of the line pass
+  const int ngrp = ((d.n1 * d.n2 + xlpw - 1) / xlpw + 3) / 4;
   const int tot2 = d.nj * d.nst * ngrp;
   {
     ProfScope ps(cx, f64 ? "pair_xline_f64" : "pair_xline_f32");

@@ -326,6 +326,40 @@ template <int N, int SIGN, typename C> struct fft_quad {
       v[k] = C{e.x + sb * t.x, e.y + sb * t.y};
     }
   }
+  // decimation in frequency, the mirror of dit: part q holds x[m + Q quad_rev(q)]
+  // (m < Q); afterwards it holds X[4k + q] (k < Q).  With chunk p = p0 + 2 p1
+  // in part q (p0 = q >> 1, p1 = q & 1) and frequency t = t0 + 2 t1,
+  // X[4k + t] = sum_m W_Q^(mk) W_N^(mt) sum_p x[m + Q p] W_4^(pt), the inner sum
+  // taken over p1 (parts q, q ^ 1) and then over p0 (parts q, q ^ 2)
+  static __device__ __forceinline__ void dif(C (&v)[Q], int q) {
+    const bool odd = (q & 1) != 0, hi = q >= 2;
+    // u = x[m + Q p0] + (-1)^t0 x[m + Q (p0 + 2)], t0 = q & 1
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      const C w{xor16(v[k].x), xor16(v[k].y)};
+      const C a = odd ? w : v[k], b = odd ? v[k] : w;
+      v[k] = odd ? C{a.x - b.x, a.y - b.y} : C{a.x + b.x, a.y + b.y};
+    }
+    // z = u(p0 = 0) + (-1)^t1 W_4^t0 u(p0 = 1), t1 = q >> 1
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+      C a, b;
+      pair_vals(v[k], a, b);
+      if (odd) b = SIGN > 0 ? C{-b.y, b.x} : C{b.y, -b.x};  // W_4 = SIGN i
+      v[k] = hi ? C{a.x - b.x, a.y - b.y} : C{a.x + b.x, a.y + b.y};
+    }
+    // W_N^(m q), then the length-Q transform
+#pragma unroll
+    for (int m = 0; m < Q; ++m) {
+      const T c = q == 0 ? (T)1 : q == 1 ? (T)kTw<N>.c[m % N] : q == 2 ? (T)kTw<N>.c[(2 * m) % N] : (T)kTw<N>.c[(3 * m) % N];
+      const T sn = q == 0 ? (T)0
+                          : (T)SIGN * (q == 1   ? (T)kTw<N>.s[m % N]
+                                       : q == 2 ? (T)kTw<N>.s[(2 * m) % N]
+                                                : (T)kTw<N>.s[(3 * m) % N]);
+      v[m] = C{v[m].x * c - v[m].y * sn, v[m].x * sn + v[m].y * c};
+    }
+    fft_line<Q, SIGN, C>::run(v);
+  }
 };
 
 }  // namespace inqx

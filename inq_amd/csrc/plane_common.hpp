@@ -136,6 +136,13 @@ __device__ __forceinline__ void line_dit(C (&v)[N / S], int q) {
   else fft_quad<N, SIGN, C>::dit(v, q);
 }
 template <int S> __device__ __forceinline__ int line_part_out(int q) { return S == 2 ? q : quad_rev(q); }
+// the mirror: input X[k + (N/S) line_part_out<S>(q)] (line_dit's output order),
+// output x[S k + q] (line_dit's input order)
+template <int N, int SIGN, typename C, int S>
+__device__ __forceinline__ void line_dif(C (&v)[N / S], int q) {
+  if constexpr (S == 2) fft_split<N, SIGN, C>::dif(v, q != 0);
+  else fft_quad<N, SIGN, C>::dif(v, q);
+}
 
 // FFT of one LDS row of N values split over S lanes; lanes with ok == false
 // transform a copy of a valid row and store nothing

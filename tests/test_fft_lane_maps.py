@@ -9,16 +9,21 @@ here on numpy arrays (one row per part of the line):
                      combines parts q and q ^ 2 (the 32-lane exchange, twiddles
                      of length N/2) and parts q and q ^ 1 (the 16-lane exchange,
                      twiddles of length N, times SIGN i on parts 2 and 3); part
-                     q then holds X[k + (N/4) quad_rev(q)].
+                     q then holds X[k + (N/4) quad_rev(q)];
+  fft_quad<N>::dif   its mirror (the pair line pass's inverse): part q holds
+                     x[m + (N/4) quad_rev(q)]; the 16-lane exchange combines
+                     parts q and q ^ 1, the 32-lane exchange parts q and q ^ 2
+                     (times SIGN i on odd parts), then the twiddle W_N^(m q) and
+                     the quarter-length FFT; part q then holds X[4k + q].
 
-Both are checked against numpy's FFT in both directions for every line length
+All are checked against numpy's FFT in both directions for every line length
 the plane passes are built for, so a wrong exchange partner, twiddle sign or
 output order fails here, on the CPU, before any GPU run.
 """
 import numpy as np
 import pytest
 
-SIZES = [12, 16, 18, 20, 24, 28, 30, 32, 36, 40, 48, 54, 60, 64, 72, 80, 96]
+SIZES = [12, 16, 18, 20, 24, 28, 30, 32, 36, 40, 48, 54, 60, 64, 72, 80, 96, 100, 108, 120, 128]
 
 
 def dft(y, sign):
@@ -65,6 +70,26 @@ def quad_dit(x, sign):
     return out
 
 
+def quad_dif(parts, sign):
+    """parts[q] = x[m + Q quad_rev(q)], m < Q; returns parts[q] = X[4k + q]"""
+    qn = len(parts[0])
+    n = 4 * qn
+    m = np.arange(qn)
+    # parts q, q ^ 1 (the 16-lane exchange): u = x[m + Q p0] + (-1)^t0 x[m + Q (p0 + 2)]
+    u = []
+    for q in range(4):
+        own, w = parts[q], parts[q ^ 1]
+        u.append(w - own if q & 1 else own + w)
+    # parts q, q ^ 2 (the 32-lane exchange): lower value a, upper value b
+    z = []
+    for q in range(4):
+        a, b = u[q & 1], u[(q & 1) + 2]
+        if q & 1:
+            b = b * (1j * sign)
+        z.append(a - b if q >= 2 else a + b)
+    return [dft(z[q] * np.exp(sign * 2j * np.pi * m * q / n), sign) for q in range(4)]
+
+
 def reference(x, sign):
     return np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * len(x)
 
@@ -87,6 +112,27 @@ def test_four_lane_split(n, sign):
     qn = n // 4
     for q in range(4):
         assert np.allclose(parts[q], ref[np.arange(qn) + qn * quad_rev(q)]), q
+
+
+@pytest.mark.parametrize("n", [s for s in SIZES if s % 4 == 0])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_four_lane_split_dif(n, sign):
+    x = np.random.default_rng(n).standard_normal(n) + 1j * np.random.default_rng(n + 5).standard_normal(n)
+    qn = n // 4
+    parts = quad_dif([x[np.arange(qn) + qn * quad_rev(q)] for q in range(4)], sign)
+    ref = reference(x, sign)
+    for q in range(4):
+        assert np.allclose(parts[q], ref[q::4]), q
+
+
+def test_four_lane_round_trip():
+    """dit then dif of the opposite sign, as the pair line pass applies them,
+    returns N x in the input's lane order"""
+    n = 128
+    x = np.random.default_rng(11).standard_normal(n) + 1j * np.random.default_rng(12).standard_normal(n)
+    back = quad_dif(quad_dit(x, -1), 1)
+    for q in range(4):
+        assert np.allclose(back[q], n * x[q::4]), q
 
 
 def test_a_wrong_exchange_partner_fails():
