@@ -485,11 +485,12 @@ def test_grids_above_96_points_poisson_hartree(ctx, ng2, lens):
     q = (0.1, -0.2, 0.05)
     ref = oh.poisson_in_place(np.ascontiguousarray(rho.T), gshift=q, zeroterm=0.3).T
     for dtype in DTYPES:
-        rd = DeviceArray.from_numpy(ctx, rho.astype(dtype))
-        got = gh.poisson(rd, q, 0.3).numpy()
+        got = gh.poisson(DeviceArray.from_numpy(ctx, rho.astype(dtype)), q, 0.3).numpy()
         assert rel_err(got, ref) < TOL_T[dtype], dtype
-        # the line pass (four lanes per line for complex128 here) repeats bit for bit
-        assert np.array_equal(gh.poisson(rd, q, 0.3).numpy(), got), dtype
+        # the solve is in place: a fresh copy of rho; the line pass (four lanes
+        # per line for complex128 here) repeats bit for bit
+        again = gh.poisson(DeviceArray.from_numpy(ctx, rho.astype(dtype)), q, 0.3).numpy()
+        assert np.array_equal(again, got), dtype
     dens = 0.01 + rng.random(n2) * 0.02
     vh, eh = gh.hartree(DeviceArray.from_numpy(ctx, dens))
     vref = oh.poisson_in_place(dens.astype(np.complex128)[:, None], zeroterm=0.0)[:, 0].real
