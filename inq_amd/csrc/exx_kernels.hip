@@ -157,10 +157,14 @@ __global__ void __launch_bounds__(256) k_pair_plane_fwd(PairDev<T> d, int total)
   }
 }
 
-// lanes per line of the pair line pass: four for complex128 lines above 96
-// points (a quarter line per lane: the lane-pair form spills 57-308 VGPRs to
-// scratch there), else a lane pair
-template <typename T, int N0> constexpr int xline_lanes() { return sizeof(T) == 8 && N0 > 96 && N0 % 4 == 0 ? 4 : 2; }
+// lanes per line of the pair line pass: a quarter line per lane for
+// complex128 lines of 72, 80 and above 96 points, else a lane pair.  Measured
+// per 16 complex128 Poisson solves, lane pair vs four lanes: 72^3 53.9 vs
+// 48.3 us, 80^3 79.2 vs 61.9 us, 96^3 122.9 vs 135.1 us, 32-64 points within
+// 1 us; above 96 points the lane-pair form spills 13-308 VGPRs to scratch
+template <typename T, int N0> constexpr int xline_lanes() {
+  return sizeof(T) == 8 && N0 % 4 == 0 && N0 >= 72 && N0 != 96 ? 4 : 2;
+}
 
 // each line is split over S lanes (lane pair (lane, lane + 32), or lane + 16 q):
 // 64 / S lines per wave

@@ -286,7 +286,7 @@ def test_symmetric_pairs_match_full_pairs(ctx, dtype):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("n", [72, 96])
+@pytest.mark.parametrize("n", [72, 80, 96])
 def test_large_grid_paths(ctx, dtype, n):
     """the largest line lengths (their kernels run close to the register limit):
     transforms and Poisson at n^3 against numpy / the oracle, and the direct
