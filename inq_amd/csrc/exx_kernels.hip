@@ -166,8 +166,8 @@ template <typename T, int N0> constexpr int xline_lanes() {
   return sizeof(T) == 8 && N0 % 4 == 0 && N0 >= 72 && N0 != 96 ? 4 : 2;
 }
 
-// each line is split over S lanes (lane pair (lane, lane + 32), or lane + 16 q):
-// 64 / S lines per wave
+// each line is split over S lanes of a wave, a lane pair (l, l + 32) or the
+// four lanes l + 16 q (q < 4), so a wave covers 64 / S lines
 // CUBE: n1 = n2 = N0 known at compile time (the row strides of the loads
 // become constants; 143 instead of 198 VGPRs at 96^3 complex64), otherwise
 // n1 and n2 are read at run time
