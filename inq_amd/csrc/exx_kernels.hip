@@ -54,18 +54,22 @@ template <typename T> struct PairDev {
 };
 
 // a plane pass works on one N1 x N2 plane in LDS (rows padded to N2 + 1); every
-// row or column of the plane is split over a lane pair (lane, lane + 32), so
-// a wave covers 32 lines and the work-group has ceil(max(N1, N2) / 32) waves
+// row or column of the plane is split over LANES lanes of a wave (a lane pair
+// (l, l + 32), or the four lanes l + 16 q for complex128 planes from 72
+// points, whose planes fill the LDS of a CU: four lanes per line give that
+// one work-group twice the waves), so a wave covers 64 / LANES lines
 template <typename T, int N1, int N2> struct PairPlaneCfg {
   static constexpr int STRIDE = N2 + 1;
   static constexpr int NL = N1 > N2 ? N1 : N2;
-  static constexpr int THREADS = (NL + 31) / 32 * 64;
-  static_assert(THREADS <= 256, "plane pass sized for at most 128 lines");
+  static constexpr int LANES = sizeof(T) == 8 && N1 % 4 == 0 && N2 % 4 == 0 && NL >= 72 ? 4 : 2;
+  static constexpr int LPW = 64 / LANES;
+  static constexpr int THREADS = (NL + LPW - 1) / LPW * 64;
+  static_assert(THREADS <= 512, "plane pass sized for at most 8 waves");
+  // the launch bound: 256 keeps up to 512 VGPRs per lane-pair thread
+  static constexpr int MAXT = THREADS <= 256 ? 256 : 512;
   static constexpr int LDS = N1 * STRIDE * (int)sizeof(cplx<T>);
 };
-
-// line index of the lane pair and the pair half of this thread
-__device__ __forceinline__ int pair_line(int tid) { return (tid >> 6) * 32 + (tid & 31); }
+template <typename T, int N1, int N2> constexpr int pair_plane_maxt() { return PairPlaneCfg<T, N1, N2>::MAXT; }
 
 // symmetric mode: the pair (hf index j of chunk entry jj, state i) is skipped
 // when both are occupied and i > j (the potential of pair (i, j) stands for it)
@@ -73,20 +77,22 @@ template <typename T> __device__ __forceinline__ bool pair_skipped(const PairDev
   return d.occ && i > gp(d.jlist)[jj] && gp(d.occ)[i] != 0;
 }
 
-// FFT along i2 of row r of the LDS plane, split over the lane pair; rows >= N1 run on a clamped row and do not store
-template <typename C, int N1, int N2, int SIGN>
-__device__ __forceinline__ void lds_row_fft_split(C *lds, int r, bool upper) {
-  constexpr int H = N2 / 2;
+// FFT along i2 of row r of the LDS plane, split over S lanes (part q of the
+// row); rows >= N1 run on a clamped row and do not store
+template <typename C, int N1, int N2, int SIGN, int S>
+__device__ __forceinline__ void lds_row_fft_split(C *lds, int r, int q) {
+  constexpr int M = N2 / S;
   const int rc = r < N1 ? r : N1 - 1;
   C *row = lds + rc * (N2 + 1);
-  C v[H];
+  C v[M];
 #pragma unroll
-  for (int n = 0; n < H; ++n) v[n] = row[2 * n + (upper ? 1 : 0)];
-  fft_split<N2, SIGN, C>::dit(v, upper);
+  for (int n = 0; n < M; ++n) v[n] = row[S * n + q];
+  line_dit<N2, SIGN, C, S>(v, q);
   __builtin_amdgcn_wave_barrier();
   if (r < N1) {
+    const int o = M * line_part_out<S>(q);
 #pragma unroll
-    for (int k = 0; k < H; ++k) row[k + (upper ? H : 0)] = v[k];
+    for (int k = 0; k < M; ++k) row[k + o] = v[k];
   }
 }
 
@@ -98,18 +104,17 @@ __device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x
 __device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
 
 template <typename T, int N1, int N2>
-__global__ void __launch_bounds__(256) k_pair_plane_fwd(PairDev<T> d, int total) {
+__global__ void __launch_bounds__((pair_plane_maxt<T, N1, N2>())) k_pair_plane_fwd(PairDev<T> d, int total) {
   using C = cplx<T>;
   using Cfg = PairPlaneCfg<T, N1, N2>;
   constexpr int NP = N1 * N2;
-  constexpr int H1 = N1 / 2;
+  constexpr int S = Cfg::LANES, M1 = N1 / S;
   const long N = (long)d.n0 * NP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x;
-  const int line = pair_line(tid);
-  const bool upper = (tid & 63) >= 32;
-  const int half = upper ? 1 : 0;
+  const int line = (tid >> 6) * Cfg::LPW + (tid & 63) % Cfg::LPW;
+  const int q = (tid & 63) / Cfg::LPW, qo = line_part_out<S>(q);  // part of the line, its output position
   // one work item per work-group, the grid being the work count: a grid-stride loop
   // here made the complex128 96-point kernels spill to scratch
   {
@@ -125,28 +130,28 @@ __global__ void __launch_bounds__(256) k_pair_plane_fwd(PairDev<T> d, int total)
     const GAS C *hf = nullptr;
     if (d.hf) hf = gp(d.hf) + (size_t)gp(d.jlist)[jj] * N + plane;
     {
-      // column z = line along i1: lower lanes y = 2n, upper y = 2n + 1
+      // column z = line along i1: part q holds y = S n + q
       const int z = line < N2 ? line : N2 - 1;
-      C v[H1];
+      C v[M1];
       if (hf) {
 #pragma unroll
-        for (int n = 0; n < H1; ++n) {
-          const int y = 2 * n + half;
+        for (int n = 0; n < M1; ++n) {
+          const int y = S * n + q;
           const C a = ph[y * N2 + z], h = hf[y * N2 + z];
           v[n] = C{h.x * a.x + h.y * a.y, h.x * a.y - h.y * a.x};  // conj(h) a
         }
       } else {
 #pragma unroll
-        for (int n = 0; n < H1; ++n) v[n] = ph[(2 * n + half) * N2 + z];
+        for (int n = 0; n < M1; ++n) v[n] = ph[(S * n + q) * N2 + z];
       }
-      fft_split<N1, -1, C>::dit(v, upper);
+      line_dit<N1, -1, C, S>(v, q);
       if (line < N2) {
 #pragma unroll
-        for (int k = 0; k < H1; ++k) lds[(k + half * H1) * Cfg::STRIDE + z] = v[k];
+        for (int k = 0; k < M1; ++k) lds[(k + qo * M1) * Cfg::STRIDE + z] = v[k];
       }
     }
     __syncthreads();
-    lds_row_fft_split<C, N1, N2, -1>(lds, line, upper);
+    lds_row_fft_split<C, N1, N2, -1, S>(lds, line, q);
     __syncthreads();
     GAS C *tp = gw(d.t) + ((size_t)jj * d.nst + i) * N + plane;
     for (int e = tid; e < NP; e += Cfg::THREADS) {
@@ -229,18 +234,17 @@ __global__ void __launch_bounds__(256) k_pair_xline(PairDev<T> d, int total) {
 // the forward pass: the column transform reads t straight from memory, the
 // row transform runs from LDS.  k_pair_reduce then forms sum_j c_j hf_j t_j.
 template <typename T, int N1, int N2>
-__global__ void __launch_bounds__(256) k_pair_plane_inv(PairDev<T> d, int total) {
+__global__ void __launch_bounds__((pair_plane_maxt<T, N1, N2>())) k_pair_plane_inv(PairDev<T> d, int total) {
   using C = cplx<T>;
   using Cfg = PairPlaneCfg<T, N1, N2>;
   constexpr int NP = N1 * N2;
-  constexpr int H1 = N1 / 2;
+  constexpr int S = Cfg::LANES, M1 = N1 / S;
   const long N = (long)d.n0 * NP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C *lds = reinterpret_cast<C *>(smem_raw);
   const int tid = threadIdx.x;
-  const int line = pair_line(tid);
-  const bool upper = (tid & 63) >= 32;
-  const int half = upper ? 1 : 0;
+  const int line = (tid >> 6) * Cfg::LPW + (tid & 63) % Cfg::LPW;
+  const int q = (tid & 63) / Cfg::LPW, qo = line_part_out<S>(q);
   // one work item per work-group, the grid being the work count: a grid-stride loop
   // here made the complex128 96-point kernels spill to scratch
   {
@@ -255,17 +259,17 @@ __global__ void __launch_bounds__(256) k_pair_plane_inv(PairDev<T> d, int total)
     GAS C *tp = gw(d.t) + ((size_t)jj * d.nst + i) * N + plane;
     {
       const int z = line < N2 ? line : N2 - 1;
-      C v[H1];
+      C v[M1];
 #pragma unroll
-      for (int n = 0; n < H1; ++n) v[n] = tp[(2 * n + half) * N2 + z];
-      fft_split<N1, +1, C>::dit(v, upper);
+      for (int n = 0; n < M1; ++n) v[n] = tp[(S * n + q) * N2 + z];
+      line_dit<N1, +1, C, S>(v, q);
       if (line < N2) {
 #pragma unroll
-        for (int k = 0; k < H1; ++k) lds[(k + half * H1) * Cfg::STRIDE + z] = v[k];
+        for (int k = 0; k < M1; ++k) lds[(k + qo * M1) * Cfg::STRIDE + z] = v[k];
       }
     }
     __syncthreads();
-    lds_row_fft_split<C, N1, N2, +1>(lds, line, upper);
+    lds_row_fft_split<C, N1, N2, +1, S>(lds, line, q);
     __syncthreads();
     for (int e = tid; e < NP; e += Cfg::THREADS) {
       const int y = e / N2, z = e - y * N2;
